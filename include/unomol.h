@@ -1,0 +1,153 @@
+/*
+ * unomol.h -- C ABI of libunomol_hip.so, the MI355X (gfx950) RHF/UHF engine.
+ *
+ * This is the drop-in boundary for the integrals + Fock build + diagonalisation path
+ * of PatNichols/unomol.  Each entry point names the reference interface it replaces
+ * (file:line in the reference tree).  Conventions kept from the reference:
+ *   - packed symmetric storage, lower triangle row-wise: idx(i,j) = i(i+1)/2 + j, i>=j
+ *     (TwoElectronInts.cpp:705-711);
+ *   - P is the half density for RHF and the per-spin density for UHF;
+ *   - dense matrices are row-major N x N; eigenvectors are columns; eigenvalues ascend
+ *     (SymmPack.cpp:272-288);
+ *   - Fock/G builds ACCUMULATE into caller-zeroed arrays (RHF.hpp:89-93);
+ *   - rsp destroys its input, sp_trans works in place (SymmPack.hpp:13-26).
+ * Every function returns an int status (UNO_OK == 0); uno_last_error() gives the
+ * message of the last failure on the calling thread.  The reference calls exit() on
+ * failure; a C++ shim maps nonzero status to its fatal_error() (Util.cpp:6-9).
+ * A context is single-stream and not thread-safe, like the reference objects.
+ */
+#ifndef UNOMOL_H
+#define UNOMOL_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UNO_OK        0
+#define UNO_EINVAL    1   /* bad argument / malformed basis */
+#define UNO_EDEVICE   2   /* HIP runtime error or no gfx950 device */
+#define UNO_ENOMEM    3   /* device allocation failed */
+#define UNO_ENOCONV   4   /* SCF reached maxits (results still returned) */
+#define UNO_ESINGULAR 5   /* zero/negative overlap eigenvalue (RHF.hpp:222-223) */
+#define UNO_EIO       6   /* file could not be read/written */
+#define UNO_EEIGEN    7   /* tridiagonal QL failed to converge (SymmPack.cpp:282-285) */
+
+typedef struct uno_ctx uno_ctx;
+
+/* POD mirror of the reference Basis/Shell/Center (Basis.hpp:17-352).  Coefficients are
+ * the raw deck values; the library normalises them as Shell::normalize does
+ * (Basis.hpp:56-75). */
+typedef struct uno_basis {
+    int nshell, ncen;
+    const int* sh_l;      /* [nshell] */
+    const int* sh_npr;    /* [nshell] */
+    const int* sh_cen;    /* [nshell] */
+    const int* sh_poff;   /* [nshell] offset into alpha/coef */
+    const double* alpha;
+    const double* coef;
+    const double* xyz;    /* [3*ncen] bohr */
+    const double* charge; /* [ncen] */
+} uno_basis;
+
+const char* uno_last_error(void);
+
+/* Replaces TwoElectronInts::TwoElectronInts(const Basis&, int start_shell, const string&)
+ * (TwoElectronInts.hpp:83-88) and its calculate() (TwoElectronInts.cpp:511-697): instead
+ * of storing ERIs it uploads the shell-pair data and computes Schwarz bounds on the
+ * device (direct SCF).  start_shell restricts bra shells to index >= start_shell as the
+ * reference's partial quartet set does (TwoElectronInts.cpp:541). */
+int uno_ctx_create(const uno_basis* basis, int start_shell, uno_ctx** out);
+/* Replaces TwoElectronInts::recalculate(const Basis&) (TwoElectronInts.hpp:94-96). */
+int uno_ctx_rebuild(uno_ctx* ctx, const uno_basis* basis);
+int uno_ctx_destroy(uno_ctx* ctx);
+/* Tuning knobs (no reference counterpart): "schwarz" (quartet screening threshold on
+ * Qab*Qcd*max|D|, default 1e-13), "fp32_threshold" (quartets whose bound is below it
+ * run in FP32 when "precision" is 1; default 1e-6), "precision" (0 = FP64, 1 = mixed),
+ * "prim_cut" (bare primitive prefactor cut, default 1e-12 as TwoElectronInts.cpp:428). */
+int uno_ctx_set(uno_ctx* ctx, const char* key, double value);
+int uno_ctx_get(uno_ctx* ctx, const char* key, double* value);
+int uno_ctx_norb(const uno_ctx* ctx);
+
+/* Replaces TwoElectronInts::formGmatrix(const double* P, double* G)
+ * (TwoElectronInts.cpp:822-844): G += 2J[P] - K[P], packed host arrays. */
+int uno_fock_rhf(uno_ctx* ctx, const double* P, double* G);
+/* Replaces TwoElectronInts::formGmatrix(PA, PB, GA, GB) (TwoElectronInts.cpp:846-869):
+ * Gs += J[PA+PB] - K[Ps]. */
+int uno_fock_uhf(uno_ctx* ctx, const double* PA, const double* PB, double* GA, double* GB);
+
+/* Replaces SymmPack::rsp(n, a, z, evals, tmp) (SymmPack.cpp:272-288). */
+int uno_eigh_packed(int n, double* a, double* z, double* evals);
+/* Replaces SymmPack::sp_trans(n, a, z, tmp) (SymmPack.cpp:294-348): a <- X^T a X. */
+int uno_sp_trans(int n, double* a, const double* x);
+
+/* SCF options/results for the device-resident loop.  With accel 0 and guess 0 the loop
+ * is the reference's (RHF.hpp:114-176 / UHF.hpp:137-177: two updates, then damping by
+ * averaging when the energy rose, signed convergence test). */
+typedef struct uno_scf_opts {
+    int maxits;          /* maximum iterations (deck line 2) */
+    double eps;          /* convergence threshold (raised to DBL_EPSILON*N^2/2 as Basis.hpp:249-254) */
+    int cflag;           /* 0 energy, 1 density, 2 both (RHF.hpp:390-402); 3 |dE| < eps */
+    int accel;           /* 0 reference damping, 2 DIIS (Pulay) */
+    int guess;           /* 0 core Hamiltonian, 1 caller's P (PMATRIX.DAT), 2 learned (MLP) */
+    int precision;       /* 0 FP64, 1 mixed FP32/FP64 with incremental builds */
+    int fp64_final;      /* mixed: number of final iterations in full FP64 (default 3) */
+    int incremental;     /* 1: build F from the density change (Delta-D) */
+    int verbose;         /* 1: per-iteration lines on stderr like the reference */
+} uno_scf_opts;
+
+typedef struct uno_scf_result {
+    double energy;        /* total energy of the last input density (reference semantics) */
+    double init_energy;   /* iteration-0 total energy */
+    double ediff, pdiff;  /* last signed energy change and density difference norm */
+    double nucrep, kinetic, electronic;
+    int iterations, converged, uhf, nfock;
+    double wall_s, fock_s, diag_s;   /* seconds: whole SCF, Fock builds, eigen/transforms */
+    double quartets;      /* shell quartets evaluated over all Fock builds */
+} uno_scf_result;
+
+void uno_scf_defaults(uno_scf_opts* opts);
+/* Runs the SCF for nelec electrons (odd => UHF, Unomol.cc:13).
+ * evals: [N] RHF, [2N] UHF (alpha then beta).  P: packed [N(N+1)/2] RHF or [2*...] UHF;
+ * read as the guess when opts->guess == 1, always written with the final density.
+ * C: optional [N*N] (RHF) or [2*N*N] (UHF) MO coefficients (columns). */
+int uno_scf_run(uno_ctx* ctx, int nelec, const uno_scf_opts* opts, uno_scf_result* res,
+                double* evals, double* P, double* C);
+
+/* Debug/parity: every canonical ERI (i>=j, i>=k, k>=l, i==k => j>=l) in the reference's
+ * order (TwoElectronInts.cpp:541-672), evaluated on the device.  cap==0 returns the
+ * count only. */
+int uno_eri_emit(uno_ctx* ctx, long cap, double* val, int* ijkl, long* count);
+/* Debug/parity: Rys roots u=t^2 and weights from the device tables (n<=5). */
+int uno_rys_eval(int n, int npts, const double* x, double* u, double* w);
+/* Debug/parity: Schwarz bounds sqrt(max|(ab|ab)|) per shell pair a>=b, packed order. */
+int uno_schwarz(uno_ctx* ctx, double* q);
+
+/* Profiling of the ERI/Fock kernels (no reference counterpart; the reference only
+ * prints stage timers, TwoElectronInts.cpp:690-695).  enable=1 resets and turns on a
+ * profiling mode in which every Fock build records, per quartet class (bra pair class
+ * X, ket pair class Y, index 6*X+Y; pair classes ss ps pp ds dp dd), the kernel time
+ * (HIP events, ms), launches, and the primitive and shell quartets evaluated.
+ * enable=0 turns it off; stats are read with enable=-1.  Arrays hold 36 entries. */
+int uno_profile(uno_ctx* ctx, int enable, double* ms, double* launches, double* prim_quartets,
+                double* shell_quartets);
+
+/* One-electron matrices (host, McMurchie-Davidson; OneElectronInts.cpp:127-202):
+ * packed S, T, H = T + V. */
+int uno_oneint(const uno_basis* basis, double* S, double* T, double* H);
+/* Raw dipole/quadrupole integrals about the origin, packed, 9 per element
+ * (dx,dy,dz,qxx,qxy,qxz,qyy,qyz,qzz) as written to RMOM.DAT (Moments.cpp:95-186). */
+int uno_moment_ints(const uno_basis* basis, double* m);
+
+/* Deck tools.  kind: "water" (monomer), "water_cluster" (n waters), "molecule" (pool
+ * index n: H2 HF H2O NH3 CH4 N2 CO CO2 C2H2, seeded distortion).  Writes patin.dat text. */
+int uno_deck_generate(const char* kind, int n, unsigned long long seed, const char* basis,
+                      int charge, double eps, int maxits, const char* path);
+
+/* Drop-in run of the reference program (Unomol.cc:8-26) on the deck "patin.dat" in dir:
+ * writes short.gs.out, scfout.gs.out, PMATRIX.DAT, intsout.out, MOMINTS.DAT, RMOM.DAT,
+ * moments.out, mol_dipmom.out there.  opts may be NULL (reference semantics). */
+int uno_run_deck(const char* dir, const uno_scf_opts* opts, uno_scf_result* res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,114 @@
+// TEST INFRASTRUCTURE ONLY -- oracle Rys roots/weights in quad precision.
+//
+// The reference evaluates Rys roots with piecewise fitted polynomials
+// (Rys.cpp:314-2197, root1..root5) and a Gram-Schmidt moment method for 6-9 roots
+// (Rys.cpp:105-312).  The oracle does not reuse those fits: it computes the Boys
+// moments F_0..F_{2n-1}(x) in __float128, runs Gautschi's Chebyshev algorithm for the
+// recurrence coefficients and finds the Jacobi-matrix eigenvalues by Sturm bisection
+// followed by Newton steps on the three-term recurrence; weights are the Christoffel
+// numbers 1/sum_k phat_k(u_i)^2.  Roots are returned as u = t^2 (the reference stores
+// u/(1-u) and converts back with r/(1+r), Rys.hpp:126-127).
+#include <quadmath.h>
+#include <cmath>
+#include "oracle.h"
+
+typedef __float128 q;
+
+static void boys_q(int m, q x, q* f) {
+    if (x > (q)160) {
+        // Large-x limit; the neglected exp(-x) part is below 1e-40 relative here.
+        f[0] = 0.5Q * sqrtq(M_PIq / x);
+        for (int i = 0; i < m; ++i) f[i + 1] = f[i] * (q)(2 * i + 1) / (2 * x);
+        return;
+    }
+    q ex = expq(-x);
+    q term = 1.0Q / (q)(2 * m + 1);
+    q sum = term;
+    for (int i = 1; i < 2000; ++i) {
+        term *= 2 * x / (q)(2 * m + 2 * i + 1);
+        sum += term;
+        if (term < sum * 1e-36Q) break;
+    }
+    f[m] = ex * sum;
+    for (int i = m - 1; i >= 0; --i) f[i] = (2 * x * f[i + 1] + ex) / (q)(2 * i + 1);
+}
+
+extern "C" int orc_boys(int m, double x, double* f) {
+    if (m < 0 || m > 40 || !(x >= 0)) return 1;
+    q fq[41];
+    boys_q(m, (q)x, fq);
+    for (int i = 0; i <= m; ++i) f[i] = (double)fq[i];
+    return 0;
+}
+
+// Sturm count of Jacobi-matrix eigenvalues below lam
+static int sturm(int n, const q* a, const q* b2, q lam) {
+    int cnt = 0;
+    q d = a[0] - lam;
+    if (d < 0) ++cnt;
+    for (int k = 1; k < n; ++k) {
+        if (d == 0) d = 1e-60Q;
+        d = a[k] - lam - b2[k] / d;
+        if (d < 0) ++cnt;
+    }
+    return cnt;
+}
+
+extern "C" int orc_rys(int n, double xd, double* u, double* w) {
+    if (n < 1 || n > 9 || !(xd >= 0)) return 1;
+    q x = xd;
+    q mu[20];
+    boys_q(2 * n - 1, x, mu);
+    // Chebyshev algorithm (Gautschi) for the monic recurrence coefficients
+    q al[10], be[10];
+    q sp[20], sc[20], sn[20];
+    for (int l = 0; l < 2 * n; ++l) { sp[l] = 0; sc[l] = mu[l]; }
+    al[0] = mu[1] / mu[0];
+    be[0] = mu[0];
+    for (int k = 1; k < n; ++k) {
+        for (int l = k; l < 2 * n - k; ++l)
+            sn[l] = sc[l + 1] - al[k - 1] * sc[l] - be[k - 1] * sp[l];
+        al[k] = sn[k + 1] / sn[k] - sc[k] / sc[k - 1];
+        be[k] = sn[k] / sc[k - 1];
+        for (int l = 0; l < 2 * n; ++l) { sp[l] = sc[l]; sc[l] = sn[l]; }
+    }
+    q b2[10];
+    b2[0] = 0;
+    for (int k = 1; k < n; ++k) b2[k] = be[k];
+    // roots lie in (0,1); isolate each by bisection then polish with Newton
+    for (int i = 0; i < n; ++i) {
+        q lo = 0, hi = 1;
+        for (int it = 0; it < 60; ++it) {
+            q mid = 0.5Q * (lo + hi);
+            if (sturm(n, al, b2, mid) > i) hi = mid; else lo = mid;
+        }
+        q r = 0.5Q * (lo + hi);
+        for (int it = 0; it < 8; ++it) {
+            // p_n(r) and p_n'(r) with the monic recurrence
+            q p0 = 1, p1 = r - al[0], d0 = 0, d1 = 1;
+            for (int k = 1; k < n; ++k) {
+                q p2 = (r - al[k]) * p1 - be[k] * p0;
+                q d2 = p1 + (r - al[k]) * d1 - be[k] * d0;
+                p0 = p1; p1 = p2; d0 = d1; d1 = d2;
+            }
+            if (n == 1) { p1 = r - al[0]; d1 = 1; }
+            if (d1 == 0) break;
+            q step = p1 / d1;
+            r -= step;
+            if (fabsq(step) < 1e-33Q * (fabsq(r) + 1e-30Q)) break;
+        }
+        // Christoffel number with orthonormal polynomials
+        q ph0 = 1 / sqrtq(be[0]);
+        q s = ph0 * ph0;
+        q phm = 0, ph = ph0;
+        for (int k = 0; k + 1 < n; ++k) {
+            q phn = ((r - al[k]) * ph - (k > 0 ? sqrtq(be[k]) * phm : 0)) / sqrtq(be[k + 1]);
+            s += phn * phn;
+            phm = ph;
+            ph = phn;
+        }
+        u[i] = (double)r;
+        w[i] = (double)(1 / s);
+    }
+    return 0;
+}

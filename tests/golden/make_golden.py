@@ -1,0 +1,128 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures under tests/golden/ (run in the build container, where the
+reference sources are mounted; the GPU host never runs this).
+
+Sources, all of them the reference's own data or its own code:
+  * ref_test/      the reference's committed regression decks and goldens
+                   (test/patin.dat.*, scfout.dat.*, short.dat.*, moments.dat.*), copied as data;
+  * ref_run/<tag>/ outputs of the reference rebuilt from its sources (oracle/_ref/Unomol)
+                   on those decks and on this project's generated decks (config A water
+                   6-31G, water 6-31G*, its +1 cation, a 2-water cluster);
+  * ref_run_x2/    the same runs with only the reference's 2-root fit (Rys.cpp:413-631)
+                   replaced by exact roots (oracle/_ref/Unomol_x2);
+  * eri_<tag>.npz  every canonical ERI of small decks from the reference's ERI code
+                   (oracle/_ref/libref_harness.so) and from the same code with its 2-root
+                   fit replaced by exact roots (libref_harness_x2.so), plus its S, T, H;
+  * rys_mp.npz     Rys roots/weights and Boys F_m from 40-digit mpmath at probe points.
+
+usage: make -C oracle && make -C oracle ref && python tests/golden/make_golden.py
+"""
+import ctypes as C
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference"
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "unomol_amd", "csrc"))
+
+REF_TAGS = [b + "." + m for b in ("3g", "431", "631") for m in ("h2", "hf", "h2o", "nh3", "ch4", "n2", "co")]
+REF_TAGS += ["b.dhdz", "f.dhdz", "c.dhdz", "o.dhdz"]
+GEN_DECKS = {   # tag -> (kind, n, seed, basis, charge)
+    "A_water_631g": ("water", 1, 0, "6-31G", 0),
+    "water_631gs": ("water", 1, 0, "6-31G*", 0),
+    "water_631gs_cation": ("water", 1, 0, "6-31G*", 1),
+    "w2_631gs": ("water_cluster", 2, 2001, "6-31G*", 0),
+}
+ERI_TAGS = ["A_water_631g", "water_631gs"]
+
+
+def run_ref(deck_text, outdir, exe="Unomol"):
+    os.makedirs(outdir, exist_ok=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        with open(os.path.join(tmp, "patin.dat"), "w") as f:
+            f.write(deck_text)
+        subprocess.run([os.path.join(ROOT, "oracle", "_ref", exe)], cwd=tmp, check=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        for fn in ("short.gs.out", "scfout.gs.out", "moments.out"):
+            shutil.copy(os.path.join(tmp, fn), os.path.join(outdir, fn))
+        with open(os.path.join(outdir, "patin.dat"), "w") as f:
+            f.write(deck_text)
+
+
+def harness_eris(lib, path):
+    cnt = C.c_long()
+    lib.ref_eri_all(path.encode(), C.c_long(0), None, None, C.byref(cnt))
+    n = cnt.value
+    v = np.zeros(n)
+    f = np.zeros((n, 4), dtype=np.int32)
+    lib.ref_eri_all(path.encode(), C.c_long(n), v.ctypes.data_as(C.c_void_p), f.ctypes.data_as(C.c_void_p),
+                    C.byref(cnt))
+    return v, f
+
+
+def main():
+    import unomol_amd
+    from unomol_amd.deck import read_deck
+    # 1. reference test data
+    dst = os.path.join(HERE, "ref_test")
+    os.makedirs(dst, exist_ok=True)
+    for tag in REF_TAGS:
+        for pre in ("patin.dat.", "scfout.dat.", "short.dat.", "moments.dat."):
+            src = os.path.join(REF, "test", pre + tag)
+            if os.path.exists(src):
+                shutil.copy(src, os.path.join(dst, pre + tag))
+    # 2. generated decks
+    gdir = os.path.join(HERE, "decks")
+    os.makedirs(gdir, exist_ok=True)
+    for tag, (kind, n, seed, basis, charge) in GEN_DECKS.items():
+        unomol_amd.generate_deck(os.path.join(gdir, tag + ".dat"), kind, n, seed, basis, charge, 1e-10, 300)
+    # 3. reference runs
+    for tag in REF_TAGS:
+        run_ref(open(os.path.join(dst, "patin.dat." + tag)).read(), os.path.join(HERE, "ref_run", tag))
+    for tag in GEN_DECKS:
+        run_ref(open(os.path.join(gdir, tag + ".dat")).read(), os.path.join(HERE, "ref_run", tag))
+    # same runs with the reference's 2-root fit replaced by exact roots (Unomol_x2)
+    for tag in REF_TAGS + list(GEN_DECKS):
+        src = os.path.join(HERE, "ref_run", tag, "patin.dat")
+        run_ref(open(src).read(), os.path.join(HERE, "ref_run_x2", tag), exe="Unomol_x2")
+    # 4. ERIs and one-electron matrices from the reference's code
+    h1 = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libref_harness.so"))
+    h2 = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libref_harness_x2.so"))
+    for tag in ERI_TAGS:
+        path = os.path.join(gdir, tag + ".dat")
+        v1, f1 = harness_eris(h1, path)
+        v2, f2 = harness_eris(h2, path)
+        assert (f1 == f2).all()
+        d = read_deck(path)
+        n2 = d["norb"] * (d["norb"] + 1) // 2
+        S, T, H = np.zeros(n2), np.zeros(n2), np.zeros(n2)
+        h1.ref_oneint(path.encode(), *[a.ctypes.data_as(C.c_void_p) for a in (S, T, H)])
+        np.savez_compressed(os.path.join(HERE, "eri_%s.npz" % tag), ijkl=f1.astype(np.int16), ref=v1, ref_x2=v2,
+                            S=S, T=T, H=H)
+    # 5. mpmath Rys roots / Boys values
+    import gen_rys_table as g
+    import mpmath as mp
+    xs = np.array([0.0, 1e-9, 3e-7, 0.37, 1.0, 2.5, 3.0, 5.0, 7.3, 10.0, 14.99, 15.0, 15.01, 18.0, 20.0, 23.0,
+                   25.0, 29.0, 33.0, 35.0, 39.9, 40.0, 41.0, 47.5, 53.0, 59.0, 61.9, 62.0, 67.9, 68.0, 90.0,
+                   300.0, 4.0e3])
+    U = np.zeros((6, len(xs), 5))
+    W = np.zeros((6, len(xs), 5))
+    for nr in range(1, 6):
+        for i, x in enumerate(xs):
+            u, w = g.rys_exact(nr, x)
+            U[nr, i, :nr] = [float(a) for a in u]
+            W[nr, i, :nr] = [float(a) for a in w]
+    F = np.array([[float(g.boys(m, x)) for m in range(12)] for x in xs])
+    np.savez_compressed(os.path.join(HERE, "rys_mp.npz"), x=xs, u=U, w=W, boys=F)
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,125 @@
+"""ctypes binding of the oracle (oracle/liboracle.so).  Test infrastructure only."""
+import ctypes as C
+import os
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_lib = None
+
+
+class OrcBasis(C.Structure):
+    _fields_ = [("nshell", C.c_int), ("ncen", C.c_int),
+                ("sh_l", C.c_void_p), ("sh_npr", C.c_void_p), ("sh_cen", C.c_void_p),
+                ("sh_poff", C.c_void_p), ("alpha", C.c_void_p), ("coef", C.c_void_p),
+                ("xyz", C.c_void_p), ("charge", C.c_void_p)]
+
+
+class OrcScfOut(C.Structure):
+    _fields_ = [("energy", C.c_double), ("init_energy", C.c_double), ("ediff", C.c_double),
+                ("pdiff", C.c_double), ("nucrep", C.c_double), ("kinetic", C.c_double),
+                ("iterations", C.c_int), ("uhf", C.c_int)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(ROOT, "oracle", "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        _lib = C.CDLL(path)
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def basis(deck):
+    keep = [np.ascontiguousarray(deck[k]) for k in ("sh_l", "sh_npr", "sh_cen", "sh_poff", "alpha", "coef", "xyz", "charge")]
+    b = OrcBasis(len(deck["sh_l"]), len(deck["charge"]), *[_p(a) for a in keep])
+    b._keep = keep
+    return b
+
+
+def rys(n, x):
+    u = np.zeros(n); w = np.zeros(n)
+    assert lib().orc_rys(n, C.c_double(x), _p(u), _p(w)) == 0
+    return u, w
+
+
+def boys(m, x):
+    f = np.zeros(m + 1)
+    assert lib().orc_boys(m, C.c_double(x), _p(f)) == 0
+    return f
+
+
+def norb(deck):
+    return lib().orc_norb(C.byref(basis(deck)))
+
+
+def eri_all(deck):
+    b = basis(deck)
+    cnt = C.c_long(0)
+    lib().orc_eri_all(C.byref(b), C.c_long(0), None, None, C.byref(cnt))
+    n = cnt.value
+    val = np.zeros(n); f = np.zeros((n, 4), dtype=np.int32)
+    lib().orc_eri_all(C.byref(b), C.c_long(n), _p(val), _p(f), C.byref(cnt))
+    return val, f
+
+
+def gmat_rhf(val, f, nb, P):
+    G = np.zeros(nb * (nb + 1) // 2)
+    lib().orc_gmat_rhf(C.c_long(len(val)), _p(val), _p(np.ascontiguousarray(f, dtype=np.int32)), nb, _p(P), _p(G))
+    return G
+
+
+def gmat_uhf(val, f, nb, PA, PB):
+    GA = np.zeros(nb * (nb + 1) // 2); GB = np.zeros_like(GA)
+    lib().orc_gmat_uhf(C.c_long(len(val)), _p(val), _p(np.ascontiguousarray(f, dtype=np.int32)), nb,
+                       _p(PA), _p(PB), _p(GA), _p(GB))
+    return GA, GB
+
+
+def rsp(n, a):
+    a = np.array(a, dtype=np.float64)
+    z = np.zeros(n * n); e = np.zeros(n)
+    lib().orc_rsp(n, _p(a), _p(z), _p(e))
+    return e, z.reshape(n, n)
+
+
+def sp_trans(n, a, x):
+    a = np.array(a, dtype=np.float64)
+    lib().orc_sp_trans(n, _p(a), _p(np.ascontiguousarray(x, dtype=np.float64)))
+    return a
+
+
+def oneint(deck):
+    b = basis(deck)
+    n = lib().orc_norb(C.byref(b))
+    n2 = n * (n + 1) // 2
+    S = np.zeros(n2); T = np.zeros(n2); H = np.zeros(n2)
+    lib().orc_oneint(C.byref(b), _p(S), _p(T), _p(H))
+    return S, T, H
+
+
+def moment_ints(deck):
+    b = basis(deck)
+    n = lib().orc_norb(C.byref(b))
+    m = np.zeros(9 * n * (n + 1) // 2)
+    lib().orc_moment_ints(C.byref(b), _p(m))
+    return m.reshape(-1, 9)
+
+
+def scf(deck, maxits=None, eps=None, cflag=None):
+    b = basis(deck)
+    n = lib().orc_norb(C.byref(b))
+    n2 = n * (n + 1) // 2
+    out = OrcScfOut()
+    ev = np.zeros(2 * n); P = np.zeros(2 * n2); mom = np.zeros(9)
+    rc = lib().orc_scf(C.byref(b), deck["nelec"], maxits or deck["maxits"],
+                       C.c_double(eps if eps is not None else deck["eps"]),
+                       cflag if cflag is not None else deck["scf_flag"][0],
+                       C.byref(out), _p(ev), _p(P), _p(mom))
+    assert rc == 0, rc
+    nsp = 2 if out.uhf else 1
+    return out, ev[:nsp * n], P[:nsp * n2], mom

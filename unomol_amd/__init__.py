@@ -1,0 +1,10 @@
+"""unomol_amd -- MI355X (gfx950) RHF/UHF SCF engine, a drop-in for the integrals +
+Fock build + diagonalisation path of PatNichols/unomol.
+
+The compute path is libunomol_hip.so (hand-written HIP kernels behind a C ABI); this
+package only binds it.  See ``api`` for the mirror of the reference's class surface
+(Basis, TwoElectronInts, SymmPack, RestrictedHartreeFock, UnRestrictedHartreeFock).
+"""
+from .api import (Basis, TwoElectronInts, SymmPack, RestrictedHartreeFock,  # noqa: F401
+                  UnRestrictedHartreeFock, run_deck, generate_deck, rys_eval, oneint, moment_ints,
+                  scf_options)

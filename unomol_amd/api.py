@@ -1,0 +1,214 @@
+"""Mirror of the reference's C++ class surface over the C ABI.
+
+Names, argument meaning and conventions follow the reference (packed lower-triangle
+symmetric storage, half density for RHF, accumulate-into-G, rsp destroying its input);
+errors raise ``UnoError`` where the reference would call exit().
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _native as N
+from .deck import parse_deck, read_deck
+
+
+class Basis:
+    """Deck-backed basis (reference Basis.hpp:168-352)."""
+
+    def __init__(self, infile="patin.dat", text=None):
+        self.deck = parse_deck(text) if text is not None else read_deck(infile)
+        self._struct = N.basis_struct(self.deck)
+
+    def number_of_orbitals(self):
+        return self.deck["norb"]
+
+    def number_of_electrons(self):
+        return self.deck["nelec"]
+
+    def number_of_shells(self):
+        return self.deck["nshell"]
+
+    def number_of_centers(self):
+        return self.deck["ncen"]
+
+    def maximum_iterations(self):
+        return self.deck["maxits"]
+
+    def scf_eps(self):
+        return self.deck["eps"]
+
+    def scf_flags(self, i):
+        return self.deck["scf_flag"][i]
+
+    @property
+    def struct(self):
+        return self._struct
+
+
+class TwoElectronInts:
+    """Direct-SCF replacement of TwoElectronInts (TwoElectronInts.hpp:79-110)."""
+
+    def __init__(self, basis, start_shell=0, label="MINTS.DAT"):
+        self.basis = basis
+        self.ctx = C.c_void_p()
+        N.check(N.lib().uno_ctx_create(C.byref(basis.struct), start_shell, C.byref(self.ctx)))
+        self.norb = N.lib().uno_ctx_norb(self.ctx)
+
+    def __del__(self):
+        if getattr(self, "ctx", None) and self.ctx.value:
+            N.lib().uno_ctx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def recalculate(self, basis):
+        self.basis = basis
+        N.check(N.lib().uno_ctx_rebuild(self.ctx, C.byref(basis.struct)))
+
+    def set(self, key, value):
+        N.check(N.lib().uno_ctx_set(self.ctx, key.encode(), C.c_double(value)))
+
+    def get(self, key):
+        v = C.c_double()
+        N.check(N.lib().uno_ctx_get(self.ctx, key.encode(), C.byref(v)))
+        return v.value
+
+    def formGmatrix(self, *args):
+        """formGmatrix(P, G) (RHF) or formGmatrix(PA, PB, GA, GB) (UHF); accumulates."""
+        arrs = [a for a in args]
+        for a in arrs:
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+                raise TypeError("packed float64 contiguous arrays expected")
+        if len(arrs) == 2:
+            N.check(N.lib().uno_fock_rhf(self.ctx, N.ptr(arrs[0]), N.ptr(arrs[1])))
+        elif len(arrs) == 4:
+            N.check(N.lib().uno_fock_uhf(self.ctx, *[N.ptr(a) for a in arrs]))
+        else:
+            raise TypeError("formGmatrix takes (P, G) or (PA, PB, GA, GB)")
+
+    def eri_emit(self):
+        """All canonical ERIs in the reference's order: (values, int32[n,4] indices)."""
+        cnt = C.c_long()
+        N.check(N.lib().uno_eri_emit(self.ctx, C.c_long(0), None, None, C.byref(cnt)))
+        n = cnt.value
+        val = np.zeros(n)
+        f = np.zeros((n, 4), dtype=np.int32)
+        N.check(N.lib().uno_eri_emit(self.ctx, C.c_long(n), N.ptr(val), N.ptr(f), C.byref(cnt)))
+        return val, f
+
+    def schwarz(self):
+        ns = self.basis.deck["nshell"]
+        q = np.zeros(ns * (ns + 1) // 2)
+        N.check(N.lib().uno_schwarz(self.ctx, N.ptr(q)))
+        return q
+
+
+class SymmPack:
+    """SymmPack namespace (SymmPack.hpp:11-28) on the device."""
+
+    @staticmethod
+    def rsp(n, a, z, evals, tmp=None):
+        N.check(N.lib().uno_eigh_packed(n, N.ptr(a), N.ptr(z), N.ptr(evals)))
+
+    @staticmethod
+    def sp_trans(n, a, z, tmp=None):
+        N.check(N.lib().uno_sp_trans(n, N.ptr(a), N.ptr(np.ascontiguousarray(z, dtype=np.float64))))
+
+    @staticmethod
+    def TraceSymmPackProduct(a, b, n):
+        idx = np.arange(n)
+        diag = idx * (idx + 1) // 2 + idx
+        return 2.0 * float(np.dot(a, b)) - float(np.dot(a[diag], b[diag]))
+
+    @staticmethod
+    def SymmPackDiffNorm(a, b, n):
+        idx = np.arange(n)
+        diag = idx * (idx + 1) // 2 + idx
+        d = a - b
+        return float(np.sqrt(2.0 * np.dot(d, d) - np.dot(d[diag], d[diag])) / n)
+
+
+def scf_options(**kw):
+    o = N.ScfOpts()
+    N.lib().uno_scf_defaults(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class _HartreeFock:
+    uhf = False
+
+    def __init__(self, basis, tints):
+        self.basis = basis
+        self.tints = tints
+        n = tints.norb
+        ns = 2 if self.uhf else 1
+        self.evals = np.zeros(ns * n)
+        self.P = np.zeros(ns * n * (n + 1) // 2)
+        self.C = np.zeros(ns * n * n)
+        self.result = None
+
+    def findEnergy(self, **opts):
+        d = self.basis.deck
+        o = scf_options(maxits=d["maxits"], eps=d["eps"], cflag=d["scf_flag"][0])
+        for k, v in opts.items():
+            setattr(o, k, v)
+        r = N.ScfResult()
+        N.check(N.lib().uno_scf_run(self.tints.ctx, d["nelec"], C.byref(o), C.byref(r),
+                                    N.ptr(self.evals), N.ptr(self.P), N.ptr(self.C)),
+                allow=(N.UNO_ENOCONV,))
+        self.result = r
+        return r.energy
+
+
+class RestrictedHartreeFock(_HartreeFock):
+    """RHF driver (RHF.hpp:22-685) running the device-resident loop."""
+
+    def __init__(self, basis, tints):
+        if basis.deck["nelec"] % 2:
+            raise N.UnoError(N.UNO_EINVAL, "Odd number of electrons in RHF")
+        super().__init__(basis, tints)
+
+
+class UnRestrictedHartreeFock(_HartreeFock):
+    """UHF driver (UHF.hpp:23-790); alpha gets the extra electron."""
+    uhf = True
+
+
+def run_deck(directory=".", **opts):
+    """Drop-in run of patin.dat in ``directory`` (Unomol.cc:8-26); writes the reference outputs."""
+    o = scf_options(**opts)
+    r = N.ScfResult()
+    N.check(N.lib().uno_run_deck(os.fspath(directory).encode(), C.byref(o), C.byref(r)),
+            allow=(N.UNO_ENOCONV,))
+    return r
+
+
+def generate_deck(path, kind="water_cluster", n=1, seed=0, basis="6-31G*", charge=0, eps=1e-10, maxits=300):
+    N.check(N.lib().uno_deck_generate(kind.encode(), n, C.c_ulonglong(seed), basis.encode(), charge,
+                                      C.c_double(eps), maxits, os.fspath(path).encode()))
+
+
+def rys_eval(n, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    u = np.zeros((len(x), n))
+    w = np.zeros((len(x), n))
+    N.check(N.lib().uno_rys_eval(n, len(x), N.ptr(x), N.ptr(u), N.ptr(w)))
+    return u, w
+
+
+def oneint(deck):
+    b = N.basis_struct(deck)
+    n = deck["norb"]
+    n2 = n * (n + 1) // 2
+    S, T, H = np.zeros(n2), np.zeros(n2), np.zeros(n2)
+    N.check(N.lib().uno_oneint(C.byref(b), N.ptr(S), N.ptr(T), N.ptr(H)))
+    return S, T, H
+
+
+def moment_ints(deck):
+    b = N.basis_struct(deck)
+    n = deck["norb"]
+    m = np.zeros(9 * n * (n + 1) // 2)
+    N.check(N.lib().uno_moment_ints(C.byref(b), N.ptr(m)))
+    return m.reshape(-1, 9)

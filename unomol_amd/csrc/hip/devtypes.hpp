@@ -1,0 +1,71 @@
+// Device-side data layout of the direct-SCF engine (shared by host builders and kernels).
+//
+// Shell pairs (a >= b by shell index) are stored with the higher-l shell first, so every
+// quartet class (la,lb|lc,ld) has la >= lb and lc >= ld like the reference's swapped
+// shell quartets (TwoElectronInts.cpp:563-580).  Each pair owns a contiguous run of
+// primitive pairs.  Both records are 64 bytes so a lane fetches them as 4 x dwordx4.
+#pragma once
+#include <cstdint>
+
+namespace uno {
+
+struct alignas(16) PairHdr {
+    double A[3];    // centre of the first (higher-l) shell
+    double AB[3];   // A - B
+    int f1, f2;     // first basis function of shell 1 and shell 2
+    int prim0;      // first primitive pair
+    int nprim;      // number of primitive pairs
+    int same;       // both shells are the same shell (a == b)
+    int sh1, sh2;   // shell indices (first, second)
+    int pad;
+};
+
+struct alignas(16) PrimPair {
+    double p, ip;   // exponent sum and its inverse
+    double P[3];    // Gaussian product centre
+    double K;       // c_i c_j (x2 for off-diagonal same-shell pairs) exp(-ab/p |AB|^2)
+    double s;       // bare exp(-ab/p |AB|^2) for the primitive prefactor cut
+    double pad;
+};
+
+// pair class index for (l1 >= l2), l <= 2: ss=0 ps=1 pp=2 ds=3 dp=4 dd=5
+constexpr int pair_class(int l1, int l2) { return l1 * (l1 + 1) / 2 + l2; }
+constexpr int kMaxL = 2;
+constexpr int kNumPairClasses = 6;
+
+// digest modes of the ERI kernels
+enum DigestMode : int {
+    kDigestRHF = 0,      // W += 4 J'[P] - K'[P]
+    kDigestUHF = 1,      // Wa += 2 J'[Pa+Pb] - K'[Pa], Wb likewise
+    kDigestBlock = 2,    // write the contracted, normalised ERI block
+    kDigestSchwarz = 3,  // max |(ij|ij)| of a diagonal quartet
+};
+
+struct EriArgs {
+    const PairHdr* pairs;
+    const PrimPair* prims;
+    const int* bra;           // pair indices of the bra class, sorted by Q descending
+    const int* ket;           // pair indices of the ket class
+    const double* braQ;       // Schwarz bound of each bra entry
+    const double* ketQ;
+    const long long* prefix;  // implicit quartet space: prefix[i] = first quartet of bra i
+    int nbra;
+    long long nquart;
+    const int2* explicit_q;   // explicit (bra, ket) list (block / Schwarz modes) or null
+    int same_class;           // bra and ket lists are the same list
+    double tau;               // quartet screening threshold on Qab Qcd Dmax
+    double primcut;           // bare primitive prefactor cut (reference 1e-12)
+    const double* dsh;        // [nsh*nsh] max |D| over each shell block (null = global bound only)
+    int nsh;
+    double dmax_global;
+    int n;                    // basis functions
+    const double* D0;         // RHF: P (half density); UHF: Pa + Pb (dense n x n)
+    const double* D1;         // UHF: Pa
+    const double* D2;         // UHF: Pb
+    double* W0;               // RHF: W; UHF: Wa
+    double* W1;               // UHF: Wb
+    double* out;              // block / Schwarz output
+    unsigned long long* counter;  // evaluated-quartet counter (may be null)
+};
+
+}  // namespace uno

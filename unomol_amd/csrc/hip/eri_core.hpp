@@ -1,0 +1,189 @@
+// Rys-quadrature shell-quartet ERIs for gfx950, specialised per (la,lb|lc,ld) class.
+//
+// Mathematics (the reference's, TwoElectronInts.cpp:420-509 and Rys.hpp:85-212):
+//   (ab|cd) = sum_prims 2 pi^(5/2) K_ab K_cd / (p q sqrt(p+q)) sum_r w_r Ix Iy Iz,
+// with the 2-D integrals from the Rys recurrence (B00, B1, B1', C, C') and the
+// binomial transfer to (la,lb|lc,ld).  The primitive cut drops a primitive quartet
+// whose bare prefactor 2 pi^(5/2) e_ab e_cd / (p q sqrt(p+q)) is below primcut, as the
+// reference does with 1e-12 (TwoElectronInts.cpp:477-479).  The Rys roots/weights come
+// from this project's own Chebyshev tables (rys_table.inc, gen_rys_table.py).
+//
+// Two execution shapes:
+//   * one lane per shell quartet (classes with <= 81 components): VRR, transfer and the
+//     contracted component block live in registers, fully unrolled per class;
+//   * one 64-lane wave per shell quartet (larger classes): lanes split the roots, the
+//     (root, axis) recurrences, the transfer entries and the component assembly, with the
+//     2-D tables staged in LDS.
+// Both end in the same J/K digestion: the six density blocks the quartet touches are
+// contracted with the ERI block and added to the unsymmetrised W = 4J' - K' (RHF) or
+// Ws = 2J'[Pa+Pb] - K'[Ps] (UHF) with one FP64 atomic per output element; G = W + W^T.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "devtypes.hpp"
+
+namespace uno {
+
+#define UNO_RYS_QUAL static __device__ const
+#include "../rys_table.inc"
+#undef UNO_RYS_QUAL
+
+__host__ __device__ constexpr int ncart_c(int l) { return (l + 1) * (l + 2) / 2; }
+__host__ __device__ constexpr int cart_pow(int l, int k, int axis) {
+    int idx = 0;
+    for (int x = l; x >= 0; --x)
+        for (int y = l - x; y >= 0; --y) {
+            if (idx == k) return axis == 0 ? x : (axis == 1 ? y : l - x - y);
+            ++idx;
+        }
+    return 0;
+}
+// component normalisation for l <= 2: 1/sqrt(3) for the x^2, y^2, z^2 components of d
+__host__ __device__ constexpr double cart_nrm(int l, int k) {
+    return (cart_pow(l, k, 0) == 2 || cart_pow(l, k, 1) == 2 || cart_pow(l, k, 2) == 2) ? 0.57735026918962576451
+                                                                                         : 1.0;
+}
+__host__ __device__ constexpr double binom_c(int n, int k) {
+    double r = 1.0;
+    for (int i = 0; i < k; ++i) r = r * (n - i) / (i + 1);
+    return r;
+}
+
+constexpr double kTwoPi25 = 34.986836655249725;   // 2 pi^(5/2)
+
+template <int NR>
+__device__ __forceinline__ void rys_roots(double x, double* u, double* w) {
+    constexpr int kAsym[6] = {0, 40, 48, 56, 62, 68};
+    if (x >= (double)kAsym[NR]) {
+        const double ix = 1.0 / x, isx = sqrt(ix);
+        constexpr int o = NR * (NR - 1) / 2;
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            u[i] = uno_rys_asym_u[o + i] * ix;
+            w[i] = uno_rys_asym_w[o + i] * isx;
+        }
+        return;
+    }
+    const int k = (int)x;
+    const double s = 2.0 * (x - (double)k) - 1.0, s2 = 2.0 * s;
+    const double* c = uno_rys_coef + uno_rys_off[NR] + k * (2 * NR * UNO_RYS_NCOEF);
+#pragma unroll
+    for (int f = 0; f < 2 * NR; ++f) {
+        const double* cf = c + f * UNO_RYS_NCOEF;
+        double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+        for (int m = UNO_RYS_NCOEF - 1; m >= 1; --m) {
+            const double b0 = fma(s2, b1, cf[m] - b2);
+            b2 = b1;
+            b1 = b0;
+        }
+        const double v = fma(s, b1, cf[0] - b2);
+        if (f < NR) u[f] = v; else w[f - NR] = v;
+    }
+}
+
+// One axis of one root: Rys 2-D recurrence G[e][f] (e <= LA+LB, f <= LC+LD) then the
+// binomial transfer to I[a][b][c][d] (a <= LA, b <= LB, c <= LC, d <= LD), flattened.
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void axis_table(double C, double Cp, double B00, double B1, double B1p, double ab,
+                                           double cd, double* I) {
+    constexpr int E = LA + LB, F = LC + LD;
+    double G[E + 1][F + 1];
+    G[0][0] = 1.0;
+    if constexpr (E > 0) G[1][0] = C;
+#pragma unroll
+    for (int e = 1; e < E; ++e) G[e + 1][0] = e * B1 * G[e - 1][0] + C * G[e][0];
+#pragma unroll
+    for (int f = 0; f < F; ++f)
+#pragma unroll
+        for (int e = 0; e <= E; ++e) {
+            double v = Cp * G[e][f];
+            if (f > 0) v = fma(f * B1p, G[e][f - 1], v);
+            if (e > 0) v = fma(e * B00, G[e - 1][f], v);
+            G[e][f + 1] = v;
+        }
+    // ket transfer: H[e][c][d] = sum_j C(d,j) cd^j G[e][c+d-j]
+    double H[E + 1][LC + 1][LD + 1];
+    double cdp[LD + 1];
+    cdp[0] = 1.0;
+#pragma unroll
+    for (int j = 1; j <= LD; ++j) cdp[j] = cdp[j - 1] * cd;
+#pragma unroll
+    for (int e = 0; e <= E; ++e)
+#pragma unroll
+        for (int c = 0; c <= LC; ++c)
+#pragma unroll
+            for (int d = 0; d <= LD; ++d) {
+                double v = 0.0;
+#pragma unroll
+                for (int j = 0; j <= d; ++j) v = fma(binom_c(d, j) * cdp[j], G[e][c + d - j], v);
+                H[e][c][d] = v;
+            }
+    double abp[LB + 1];
+    abp[0] = 1.0;
+#pragma unroll
+    for (int i = 1; i <= LB; ++i) abp[i] = abp[i - 1] * ab;
+#pragma unroll
+    for (int a = 0; a <= LA; ++a)
+#pragma unroll
+        for (int b = 0; b <= LB; ++b)
+#pragma unroll
+            for (int c = 0; c <= LC; ++c)
+#pragma unroll
+                for (int d = 0; d <= LD; ++d) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int i = 0; i <= b; ++i) v = fma(binom_c(b, i) * abp[i], H[a + b - i][c][d], v);
+                    I[((a * (LB + 1) + b) * (LC + 1) + c) * (LD + 1) + d] = v;
+                }
+}
+
+template <int LA, int LB, int LC, int LD>
+struct ClassInfo {
+    static constexpr int NA = ncart_c(LA), NB = ncart_c(LB), NC = ncart_c(LC), ND = ncart_c(LD);
+    static constexpr int NCOMP = NA * NB * NC * ND;
+    static constexpr int E = LA + LB, F = LC + LD, NR = (E + F) / 2 + 1;
+    static constexpr int TBL = (LA + 1) * (LB + 1) * (LC + 1) * (LD + 1);
+    __host__ __device__ static constexpr int tix(int i, int j, int k, int l, int axis) {
+        return ((cart_pow(LA, i, axis) * (LB + 1) + cart_pow(LB, j, axis)) * (LC + 1) + cart_pow(LC, k, axis)) *
+                   (LD + 1) +
+               cart_pow(LD, l, axis);
+    }
+};
+
+// Resolve the quartet handled by linear index g: (bra pair, ket pair)
+// and the product of their Schwarz bounds (negative for explicit lists: no screening)
+__device__ __forceinline__ bool quartet_of(const EriArgs& a, long long g, int& pab, int& pcd, double& q12) {
+    if (g >= a.nquart) return false;
+    if (a.explicit_q) {
+        const int2 e = a.explicit_q[g];
+        pab = e.x;
+        pcd = e.y;
+        q12 = -1.0;
+        return true;
+    }
+    int lo = 0, hi = a.nbra - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.prefix[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    const int jk = (int)(g - a.prefix[lo]);
+    pab = a.bra[lo];
+    pcd = a.ket[jk];
+    q12 = a.braQ[lo] * a.ketQ[jk];
+    return true;
+}
+
+__device__ __forceinline__ void atomic_add_f64(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+// density screen: Qab Qcd max(4|D_ab|, 4|D_cd|, |D_ac|, |D_ad|, |D_bc|, |D_bd|) < tau
+__device__ __forceinline__ bool screened(const EriArgs& a, const PairHdr& P1, const PairHdr& P2, double q12) {
+    if (!a.dsh) return q12 * a.dmax_global < a.tau;
+    const int ns = a.nsh;
+    const double* m = a.dsh;
+    const int s1 = P1.sh1, s2 = P1.sh2, s3 = P2.sh1, s4 = P2.sh2;
+    double d = 4.0 * fmax(m[s1 * ns + s2], m[s3 * ns + s4]);
+    d = fmax(d, fmax(fmax(m[s1 * ns + s3], m[s1 * ns + s4]), fmax(m[s2 * ns + s3], m[s2 * ns + s4])));
+    return q12 * d < a.tau;
+}
+
+}  // namespace uno

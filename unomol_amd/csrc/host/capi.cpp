@@ -1,0 +1,521 @@
+// C ABI (include/unomol.h) and the drop-in deck runner with the reference's outputs.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+#include "deck.hpp"
+#include "engine.hpp"
+#include "../hip/kernels.hpp"
+
+using namespace uno;
+
+namespace {
+
+void basis_view(const Deck& d, uno_basis& b) {
+    b.nshell = d.nshell;
+    b.ncen = d.ncen;
+    b.sh_l = d.sh_l.data();
+    b.sh_npr = d.sh_npr.data();
+    b.sh_cen = d.sh_cen.data();
+    b.sh_poff = d.sh_poff.data();
+    b.alpha = d.alpha.data();
+    b.coef = d.coef.data();
+    b.xyz = d.xyz.data();
+    b.charge = d.charge.data();
+}
+
+// a <- X^T a X on the host-facing boundary, computed on the device
+int sp_trans_dev(int n, double* a, const double* x, hipStream_t st) {
+    const long nn = (long)n * n, n2 = (long)n * (n + 1) / 2;
+    DBuf<double> pk, A, X, T;
+    UNO_HIP(pk.upload(a, n2, st));
+    UNO_HIP(A.alloc(nn));
+    UNO_HIP(T.alloc(nn));
+    UNO_HIP(X.upload(x, nn, st));
+    UNO_HIP(k_unpack(n, pk.p, A.p, st));
+    UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, A.p, n, X.p, n, 0.0, T.p, n, st));
+    UNO_HIP(gemm_f64(true, false, n, n, n, 1.0, X.p, n, T.p, n, 0.0, A.p, n, st));
+    UNO_HIP(k_pack(n, A.p, pk.p, st));
+    UNO_HIP(hipMemcpyAsync(a, pk.p, n2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    UNO_HIP(hipStreamSynchronize(st));
+    return UNO_OK;
+}
+
+FILE* create(const std::string& dir, const char* name) { return std::fopen((dir + "/" + name).c_str(), "w"); }
+
+const char* kRule = "xxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxx\n";
+
+// Writes the reference's output files (RHF.hpp:404-533, UHF.hpp:399-485, Moments.cpp:95-469)
+int write_outputs(const std::string& dir, const Deck& d, const Basis& B, const uno_scf_result& r,
+                  const std::vector<double>& S, const std::vector<double>& T, const std::vector<double>& H,
+                  const std::vector<double>& M, const std::vector<double>& P, const std::vector<double>& ev,
+                  const std::vector<double>& C, int maxits) {
+    const int n = B.norb;
+    const long n2 = (long)n * (n + 1) / 2;
+    const bool uhf = r.uhf != 0;
+    const int noA = uhf ? d.nelec - d.nelec / 2 : d.nelec / 2, noB = d.nelec / 2;
+    FILE* f = create(dir, "intsout.out");
+    if (!f) { set_error("could not create intsout.out"); return UNO_EIO; }
+    std::fprintf(f, "  UnoMol alpha version \n If you are using this for serious research you are insane!\n\n");
+    std::fprintf(f, "Number of Shells   = %12u\nNumber of Orbitals = %12u\nNumber of Centers  = %12u\n", d.nshell, n,
+                 d.ncen);
+    std::fprintf(f, "Max L value        = %12u\nMax SCF Iterations = %12u\nNumber of Electrons= %12u\n", B.maxl, maxits,
+                 uhf ? d.nelec : 2 * (d.nelec / 2));
+    std::fprintf(f, " Centers \nCharge     Position(x,y,z  in bohr)\n");
+    for (int i = 0; i < d.ncen; ++i)
+        std::fprintf(f, "%15.10lf %15.10lf %15.10lf %15.10lf\n", d.charge[i], d.xyz[3 * i], d.xyz[3 * i + 1],
+                     d.xyz[3 * i + 2]);
+    std::fprintf(f, "\n Shells\n");
+    for (size_t s = 0; s < B.sh.size(); ++s) {
+        std::fprintf(f, "%3d %3d %3d\n", B.sh[s].npr, B.sh[s].l, B.sh[s].cen);
+        for (int p = 0; p < B.sh[s].npr; ++p) std::fprintf(f, " %20.10lf %20.10lf\n", B.sh[s].a[p], B.sh[s].c[p]);
+    }
+    const char* names[3] = {"Overlap Matrix \n", "Kinetic Energy Matrix \n", "Core Hamiltonian Matrix \n"};
+    const std::vector<double>* mats[3] = {&S, &T, &H};
+    for (int m = 0; m < 3; ++m) {
+        std::fprintf(f, m ? "\n%s" : "\n%s", names[m]);
+        long ij = 0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j <= i; ++j, ++ij) std::fprintf(f, "%7u %7u %24.16le\n", i, j, (*mats[m])[ij]);
+    }
+    std::fclose(f);
+    // raw and scaled moment integral files (80-byte MomInts records, Structs.hpp:17-20)
+    struct MomRec { double v[9]; unsigned int ijr; unsigned int pad; };
+    FILE* fm = create(dir, "MOMINTS.DAT");
+    FILE* fr = create(dir, "RMOM.DAT");
+    if (!fm || !fr) { set_error("could not create MOMINTS.DAT/RMOM.DAT"); return UNO_EIO; }
+    for (size_t is = 0; is < B.sh.size(); ++is)
+        for (size_t js = 0; js <= is; ++js)
+            for (int a = 0; a < ncart(B.sh[is].l); ++a)
+                for (int bb = 0; bb < ncart(B.sh[js].l); ++bb) {
+                    const int ir = B.off[is] + a, jr = B.off[js] + bb;
+                    if (jr > ir) break;
+                    MomRec rec{};
+                    rec.ijr = (unsigned)pidx(ir, jr);
+                    std::memcpy(rec.v, &M[9 * rec.ijr], 9 * sizeof(double));
+                    std::fwrite(&rec, sizeof rec, 1, fr);
+                    const double fct = (ir == jr) ? 2.0 : 4.0;
+                    for (double& v : rec.v) v *= fct;
+                    std::fwrite(&rec, sizeof rec, 1, fm);
+                }
+    std::fclose(fm);
+    std::fclose(fr);
+    // PMATRIX.DAT: raw packed doubles (alpha then beta for UHF)
+    FILE* fp = create(dir, "PMATRIX.DAT");
+    if (!fp) { set_error("could not create PMATRIX.DAT"); return UNO_EIO; }
+    std::fwrite(P.data(), sizeof(double), (uhf ? 2 : 1) * n2, fp);
+    std::fclose(fp);
+    f = create(dir, "short.gs.out");
+    std::fprintf(f, "%25.16le\n%25.16le\n%25.16le\n", r.init_energy, r.energy, r.ediff);
+    std::fclose(f);
+    f = create(dir, "scfout.gs.out");
+    const double virial = std::fabs((r.energy - r.kinetic) / r.kinetic / 2.0);
+    if (maxits <= r.iterations) std::fprintf(f, "WARNING CONVERGENCE _NOT_ REACHED!\n");
+    std::fprintf(f, "Final Iteration              = %12u\n", r.iterations);
+    std::fprintf(f, "Hartree Fock Energy          = %25.15le Hartree\n", r.energy);
+    std::fprintf(f, "Electronic Energy            = %25.15le Hartree\n", r.electronic);
+    std::fprintf(f, "Nuclear Rep. Energy          = %25.15le Hartree\n", r.nucrep);
+    std::fprintf(f, "Kinetic Energy               = %25.15le Hartree\n", r.kinetic);
+    std::fprintf(f, "Difference in Final Energies = %25.16le Hartree\n", r.ediff);
+    std::fprintf(f, "Pmatrix diff norm            = %25.16le\n", r.pdiff);
+    std::fprintf(f, "virial                       = %25.16le\n", virial);
+    std::fprintf(f, "%s", kRule);
+    if (!uhf) {
+        std::fprintf(f, "               Orbital Energies\nOrbital Energy          Occupancy\n");
+        for (int i = 0; i < n; ++i) std::fprintf(f, "%7u %25.16le %12u\n", i + 1, ev[i], i < noA ? 2 : 0);
+    } else {
+        std::fprintf(f, "        Alpha Orbital Energies\nOrbital Energy          Occupancy\n");
+        for (int i = 0; i < n; ++i) std::fprintf(f, "%7u %25.16le %12u\n", i + 1, ev[i], i < noA ? 1 : 0);
+        std::fprintf(f, "%s", kRule);
+        std::fprintf(f, "        Beta Orbital Energies\nOrbital Energy          Occupancy\n");
+        for (int i = 0; i < n; ++i) std::fprintf(f, "%7u %25.16le %12u\n", i + 1, ev[n + i], i < noB ? 1 : 0);
+    }
+    std::fprintf(f, "%s", kRule);
+    // Mulliken populations: diag of P S (x2 for the RHF half density)
+    const int ns = uhf ? 2 : 1;
+    std::vector<double> pop((size_t)ns * n, 0.0);
+    std::vector<double> ovl((size_t)ns * n * n, 0.0);
+    for (int s = 0; s < ns; ++s)
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                double sum = 0.0;
+                for (int k = 0; k < n; ++k) sum += P[s * n2 + pidx(i, k)] * S[pidx(k, j)];
+                ovl[((size_t)s * n + i) * n + j] = uhf ? sum : 2.0 * sum;
+            }
+    std::vector<double> net(d.charge);
+    for (size_t is = 0; is < B.sh.size(); ++is)
+        for (int a = 0; a < ncart(B.sh[is].l); ++a) {
+            const int ir = B.off[is] + a;
+            for (int s = 0; s < ns; ++s) net[B.sh[is].cen] -= ovl[((size_t)s * n + ir) * n + ir];
+        }
+    std::fprintf(f, "%s           Mulliken Populations \n", kRule);
+    if (!uhf) {
+        std::fprintf(f, "Orbital  Net Population\n");
+        for (int i = 0; i < n; ++i) std::fprintf(f, "%7u %25.16le\n", i + 1, ovl[(size_t)i * n + i]);
+    } else {
+        std::fprintf(f, "Orbital  Alpha Population          Beta Population\n");
+        for (int i = 0; i < n; ++i)
+            std::fprintf(f, "%7u %25.16le %25.16le\n", i + 1, ovl[(size_t)i * n + i], ovl[((size_t)n + i) * n + i]);
+    }
+    std::fprintf(f, "%sMulliken Atomic Charges \nOrbital   Nuclear Charge   Net Charge \n", kRule);
+    for (int i = 0; i < d.ncen; ++i) std::fprintf(f, "%7u %15.10lf %15.10lf \n", i + 1, d.charge[i], net[i]);
+    if (d.prt_flag[0]) {
+        for (int s = 0; s < ns; ++s) {
+            std::fprintf(f, "xxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxx\n");
+            std::fprintf(f, uhf ? (s ? "              Beta Fock Eigenvectors\n\n" : "              Alpha Fock Eigenvectors\n\n")
+                                : "              Fock Eigenvectors\n\n");
+            const int no = s ? noB : noA;
+            for (int i = 0; i < n; ++i) {
+                std::fprintf(f, "Column %5d Occupancy= %2d EigenEnergy= %25.15le\n", i, i < no ? (uhf ? 1 : 2) : 0,
+                             ev[(size_t)s * n + i]);
+                const double* Cm = &C[(size_t)s * n * n + i];
+                int j = 0;
+                for (; j + 4 <= n; j += 4)
+                    std::fprintf(f, "%18.10le %18.10le %18.10le %18.10le\n", Cm[(size_t)j * n], Cm[(size_t)(j + 1) * n],
+                                 Cm[(size_t)(j + 2) * n], Cm[(size_t)(j + 3) * n]);
+                for (; j < n; ++j) std::fprintf(f, "%18.10le ", Cm[(size_t)j * n]);
+                std::fprintf(f, "\n");
+            }
+        }
+    }
+    std::fclose(f);
+    // multipole analysis (Moments.cpp:189-363): electronic part from packed P with
+    // x4 off-diagonal / x2 diagonal factors; UHF uses (Pa + Pb)/2
+    double nM[9] = {0}, eM[9] = {0}, tM[9];
+    for (int c = 0; c < d.ncen; ++c) {
+        const double* rr = &d.xyz[3 * c];
+        const double v[9] = {rr[0], rr[1], rr[2], rr[0] * rr[0], rr[0] * rr[1], rr[0] * rr[2], rr[1] * rr[1],
+                             rr[1] * rr[2], rr[2] * rr[2]};
+        for (int x = 0; x < 9; ++x) nM[x] += d.charge[c] * v[x];
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) {
+            const long ij = pidx(i, j);
+            const double pij = uhf ? 0.5 * (P[ij] + P[n2 + ij]) : P[ij];
+            const double fct = (i == j) ? 2.0 : 4.0;
+            for (int x = 0; x < 9; ++x) eM[x] += pij * fct * M[9 * ij + x];
+        }
+    for (int x = 0; x < 9; ++x) tM[x] = nM[x] - eM[x];
+    const double q00 = tM[8] - 0.5 * (tM[3] + tM[6]);
+    f = create(dir, "moments.out");
+    std::fprintf(f, " MULTIPOLE MOMENT ANALYSIS \n units in bohr - hartree atomic units \n\n DIPOLE MOMENTS \n\n");
+    std::fprintf(f, "          Total         Electronic        Nuclear \n");
+    const char* dl[3] = {" x", " y", " z"};
+    for (int x = 0; x < 3; ++x) std::fprintf(f, "%s  %15.7le  %15.7le  %15.7le \n", dl[x], tM[x], eM[x], nM[x]);
+    std::fprintf(f, "\n QUADRUPOLE MOMENTS \n%s\n", uhf ? " " : "");
+    std::fprintf(f, "          Total         Electronic        Nuclear \n");
+    const char* ql[6] = {" xx", " xy", " xz", " yy", " yz", " zz"};
+    for (int x = 0; x < 6; ++x) std::fprintf(f, "%s  %15.7le  %15.7le  %15.7le \n", ql[x], tM[3 + x], eM[3 + x], nM[3 + x]);
+    std::fprintf(f, "\n Dipole moment     = %25.15le \n Quadrupole moment = %25.15le \n", tM[2], q00);
+    std::fclose(f);
+    // MO transition dipoles (Moments.cpp:365-469): C^T mu C per spin
+    f = create(dir, "mol_dipmom.out");
+    std::fprintf(f, " MULTIPOLE MOMENT ANALYSIS \n units in bohr - hartree atomic units \n\n");
+    for (int s = 0; s < ns; ++s) {
+        std::fprintf(f, uhf ? (s ? " BETA MO TRANSISITION DIPOLE MOMENTS \n\n" : " ALPHA MO TRANSISITION DIPOLE MOMENTS \n\n")
+                            : " MO TRANSISITION DIPOLE MOMENTS \n\n");
+        std::fprintf(f, " orbital 1       orbital2      dx-dy-dz\n");
+        std::vector<double> dm[3];
+        for (int x = 0; x < 3; ++x) {
+            dm[x].resize(n2);
+            for (long ij = 0; ij < n2; ++ij) dm[x][ij] = M[9 * ij + x];
+            std::vector<double> Cs(C.begin() + (size_t)s * n * n, C.begin() + (size_t)(s + 1) * n * n);
+            std::vector<double> a(n * (size_t)n, 0.0), t(n * (size_t)n, 0.0);
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) a[(size_t)i * n + j] = dm[x][pidx(i, j)];
+            for (int i = 0; i < n; ++i)
+                for (int k = 0; k < n; ++k) {
+                    const double aik = a[(size_t)i * n + k];
+                    for (int j = 0; j < n; ++j) t[(size_t)i * n + j] += aik * Cs[(size_t)k * n + j];
+                }
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j <= i; ++j) {
+                    double sum = 0.0;
+                    for (int k = 0; k < n; ++k) sum += Cs[(size_t)k * n + i] * t[(size_t)k * n + j];
+                    dm[x][pidx(i, j)] = sum;
+                }
+        }
+        long ij = 0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j <= i; ++j, ++ij)
+                std::fprintf(f, " %12d %12d %15.6le %15.6le %15.6le\n", i, j, dm[0][ij], dm[1][ij], dm[2][ij]);
+    }
+    std::fclose(f);
+    return UNO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* uno_last_error(void) { return get_error().c_str(); }
+
+int uno_ctx_create(const uno_basis* basis, int start_shell, uno_ctx** out) {
+    if (!basis || !out) { set_error("null argument"); return UNO_EINVAL; }
+    uno_ctx* c = new (std::nothrow) uno_ctx();
+    if (!c) { set_error("out of host memory"); return UNO_ENOMEM; }
+    const int rc = ctx_build(c, basis, start_shell);
+    if (rc) { delete c; *out = nullptr; return rc; }
+    *out = c;
+    return UNO_OK;
+}
+
+int uno_ctx_rebuild(uno_ctx* c, const uno_basis* basis) {
+    if (!c || !basis) { set_error("null argument"); return UNO_EINVAL; }
+    return ctx_build(c, basis, c->start_shell);
+}
+
+int uno_ctx_destroy(uno_ctx* c) {
+    delete c;
+    return UNO_OK;
+}
+
+int uno_ctx_norb(const uno_ctx* c) { return c ? c->n : 0; }
+
+int uno_ctx_set(uno_ctx* c, const char* key, double v) {
+    if (!c || !key) { set_error("null argument"); return UNO_EINVAL; }
+    std::string k(key);
+    if (k == "schwarz") c->tau = v;
+    else if (k == "prim_cut") c->primcut = v;
+    else if (k == "fp32_threshold") c->fp32_thr = v;
+    else if (k == "precision") c->precision = (int)v;
+    else { set_error("unknown option " + k); return UNO_EINVAL; }
+    return UNO_OK;
+}
+
+int uno_ctx_get(uno_ctx* c, const char* key, double* v) {
+    if (!c || !key || !v) { set_error("null argument"); return UNO_EINVAL; }
+    std::string k(key);
+    if (k == "schwarz") *v = c->tau;
+    else if (k == "prim_cut") *v = c->primcut;
+    else if (k == "fp32_threshold") *v = c->fp32_thr;
+    else if (k == "precision") *v = c->precision;
+    else if (k == "last_quartets") *v = c->last_quartets;
+    else { set_error("unknown option " + k); return UNO_EINVAL; }
+    return UNO_OK;
+}
+
+int uno_fock_rhf(uno_ctx* c, const double* P, double* G) {
+    if (!c || !P || !G) { set_error("null argument"); return UNO_EINVAL; }
+    const int n = c->n;
+    const long n2 = (long)n * (n + 1) / 2;
+    hipStream_t st = c->st;
+    UNO_HIP(c->pk.upload(P, n2, st));
+    UNO_HIP(k_unpack(n, c->pk.p, c->D0.p, st));
+    if (int rc = fock_build(c, false, -1.0)) return rc;
+    UNO_HIP(c->pk.upload(G, n2, st));
+    UNO_HIP(k_pack_sym_add(n, c->W0.p, c->pk.p, st));
+    UNO_HIP(hipMemcpyAsync(G, c->pk.p, n2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    UNO_HIP(hipStreamSynchronize(st));
+    return UNO_OK;
+}
+
+int uno_fock_uhf(uno_ctx* c, const double* PA, const double* PB, double* GA, double* GB) {
+    if (!c || !PA || !PB || !GA || !GB) { set_error("null argument"); return UNO_EINVAL; }
+    const int n = c->n;
+    const long n2 = (long)n * (n + 1) / 2, nn = (long)n * n;
+    hipStream_t st = c->st;
+    UNO_HIP(c->pk.upload(PA, n2, st));
+    UNO_HIP(k_unpack(n, c->pk.p, c->D1.p, st));
+    UNO_HIP(c->pk.upload(PB, n2, st));
+    UNO_HIP(k_unpack(n, c->pk.p, c->D2.p, st));
+    UNO_HIP(k_axpby(nn, 1.0, c->D1.p, 1.0, c->D2.p, c->D0.p, st));
+    if (int rc = fock_build(c, true, -1.0)) return rc;
+    double* Gs[2] = {GA, GB};
+    double* Ws[2] = {c->W0.p, c->W1.p};
+    for (int s = 0; s < 2; ++s) {
+        UNO_HIP(c->pk.upload(Gs[s], n2, st));
+        UNO_HIP(k_pack_sym_add(n, Ws[s], c->pk.p, st));
+        UNO_HIP(hipMemcpyAsync(Gs[s], c->pk.p, n2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        UNO_HIP(hipStreamSynchronize(st));
+    }
+    return UNO_OK;
+}
+
+int uno_eigh_packed(int n, double* a, double* z, double* evals) {
+    if (n <= 0 || !a || !z || !evals) { set_error("bad argument"); return UNO_EINVAL; }
+    hipStream_t st;
+    UNO_HIP(hipStreamCreate(&st));
+    const long nn = (long)n * n, n2 = (long)n * (n + 1) / 2;
+    int rc;
+    {
+        DBuf<double> pk, A, Z;
+        UNO_HIP(pk.upload(a, n2, st));
+        UNO_HIP(A.alloc(nn));
+        UNO_HIP(Z.alloc(nn));
+        UNO_HIP(k_unpack(n, pk.p, A.p, st));
+        rc = eigh_standalone(n, A.p, Z.p, evals, st);
+        if (!rc) {
+            UNO_HIP(hipMemcpyAsync(z, Z.p, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+            UNO_HIP(hipStreamSynchronize(st));
+            std::memset(a, 0, n2 * sizeof(double));   // input destroyed like rsp
+        }
+    }
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+int uno_sp_trans(int n, double* a, const double* x) {
+    if (n <= 0 || !a || !x) { set_error("bad argument"); return UNO_EINVAL; }
+    hipStream_t st;
+    UNO_HIP(hipStreamCreate(&st));
+    const int rc = sp_trans_dev(n, a, x, st);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+void uno_scf_defaults(uno_scf_opts* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->maxits = 300;
+    o->eps = 1e-10;
+    o->cflag = 2;
+    o->accel = 0;
+    o->guess = 0;
+    o->precision = 0;
+    o->fp64_final = 3;
+    o->incremental = 0;
+    o->verbose = 0;
+}
+
+int uno_scf_run(uno_ctx* c, int nelec, const uno_scf_opts* opts, uno_scf_result* res, double* evals, double* P,
+                double* C) {
+    if (!c || !res) { set_error("null argument"); return UNO_EINVAL; }
+    uno_scf_opts o;
+    if (opts) o = *opts; else uno_scf_defaults(&o);
+    return scf_run(c, nelec, o, *res, evals, P, C);
+}
+
+int uno_eri_emit(uno_ctx* c, long cap, double* val, int* ijkl, long* count) {
+    if (!c || !count) { set_error("null argument"); return UNO_EINVAL; }
+    return eri_emit(c, cap, val, ijkl, count);
+}
+
+int uno_rys_eval(int n, int npts, const double* x, double* u, double* w) {
+    if (n < 1 || n > 5 || npts <= 0) { set_error("bad argument"); return UNO_EINVAL; }
+    hipStream_t st;
+    UNO_HIP(hipStreamCreate(&st));
+    {
+        DBuf<double> dx, du, dw;
+        UNO_HIP(dx.upload(x, npts, st));
+        UNO_HIP(du.alloc((size_t)npts * n));
+        UNO_HIP(dw.alloc((size_t)npts * n));
+        UNO_HIP(rys_eval_launch(n, npts, dx.p, du.p, dw.p, st));
+        UNO_HIP(hipMemcpyAsync(u, du.p, (size_t)npts * n * sizeof(double), hipMemcpyDeviceToHost, st));
+        UNO_HIP(hipMemcpyAsync(w, dw.p, (size_t)npts * n * sizeof(double), hipMemcpyDeviceToHost, st));
+        UNO_HIP(hipStreamSynchronize(st));
+    }
+    (void)hipStreamDestroy(st);
+    return UNO_OK;
+}
+
+int uno_schwarz(uno_ctx* c, double* q) {
+    if (!c || !q) { set_error("null argument"); return UNO_EINVAL; }
+    // packed shell-pair order a >= b; pairs dropped by the primitive cut report 0
+    for (int a = 0; a < c->nsh; ++a)
+        for (int b = 0; b <= a; ++b) {
+            const int p = c->pair_of[(size_t)a * c->nsh + b];
+            q[pidx(a, b)] = p >= 0 ? c->pairQ[p] : 0.0;
+        }
+    return UNO_OK;
+}
+
+int uno_profile(uno_ctx* c, int enable, double* ms, double* launches, double* prim, double* shell) {
+    if (!c) { set_error("null argument"); return UNO_EINVAL; }
+    const int ncls = kNumPairClasses * kNumPairClasses;
+    if (enable >= 0) {
+        c->profile = enable;
+        for (int i = 0; i < ncls; ++i) c->prof_ms[i] = c->prof_launch[i] = c->prof_prim[i] = c->prof_shell[i] = 0.0;
+    }
+    for (int i = 0; i < ncls; ++i) {
+        if (ms) ms[i] = c->prof_ms[i];
+        if (launches) launches[i] = c->prof_launch[i];
+        if (prim) prim[i] = c->prof_prim[i];
+        if (shell) shell[i] = c->prof_shell[i];
+    }
+    return UNO_OK;
+}
+
+int uno_oneint(const uno_basis* basis, double* S, double* T, double* H) {
+    Basis B;
+    std::string err;
+    if (!make_basis(basis, B, err)) { set_error(err); return UNO_EINVAL; }
+    one_electron(B, S, T, H, nullptr);
+    return UNO_OK;
+}
+
+int uno_moment_ints(const uno_basis* basis, double* m) {
+    Basis B;
+    std::string err;
+    if (!make_basis(basis, B, err)) { set_error(err); return UNO_EINVAL; }
+    one_electron(B, nullptr, nullptr, nullptr, m);
+    return UNO_OK;
+}
+
+int uno_deck_generate(const char* kind, int n, unsigned long long seed, const char* basis, int charge, double eps,
+                      int maxits, const char* path) {
+    if (!kind || !basis || !path) { set_error("null argument"); return UNO_EINVAL; }
+    std::vector<Atom> atoms;
+    std::string k(kind);
+    if (k == "water") water_monomer(atoms);
+    else if (k == "water_cluster") {
+        if (n <= 0) { set_error("water_cluster needs n > 0"); return UNO_EINVAL; }
+        water_cluster(n, seed, atoms);
+    } else if (k == "molecule") small_molecule(n, seed, atoms);
+    else { set_error("unknown deck kind " + k); return UNO_EINVAL; }
+    Deck d;
+    std::string err;
+    if (!build_deck(atoms, basis, charge, d, err)) { set_error(err); return UNO_EINVAL; }
+    d.eps = eps;
+    d.maxits = maxits;
+    if (!write_deck(path, d, err)) { set_error(err); return UNO_EIO; }
+    return UNO_OK;
+}
+
+int uno_run_deck(const char* dir_c, const uno_scf_opts* opts, uno_scf_result* res) {
+    const std::string dir = dir_c ? dir_c : ".";
+    Deck d;
+    std::string err;
+    if (!read_deck(dir + "/patin.dat", d, err)) { set_error(err); return UNO_EIO; }
+    if (d.int_flag[0] || d.int_flag[1]) {
+        set_error("DPM and finite-field runs (int_flag) are not part of the device path");
+        return UNO_EINVAL;
+    }
+    uno_basis b;
+    basis_view(d, b);
+    uno_ctx* c = nullptr;
+    if (int rc = uno_ctx_create(&b, 0, &c)) return rc;
+    uno_scf_opts o;
+    if (opts) o = *opts; else uno_scf_defaults(&o);
+    o.maxits = d.maxits;
+    o.eps = d.eps;
+    o.cflag = d.scf_flag[0];
+    const int n = c->n;
+    const long n2 = (long)n * (n + 1) / 2;
+    const bool uhf = d.nelec % 2;
+    std::vector<double> P((uhf ? 2 : 1) * n2), ev((uhf ? 2 : 1) * (size_t)n), C((uhf ? 2 : 1) * (size_t)n * n);
+    if (d.scf_flag[2]) {
+        // restart from PMATRIX.DAT (RHF.hpp:120-123)
+        FILE* f = std::fopen((dir + "/PMATRIX.DAT").c_str(), "rb");
+        if (!f || std::fread(P.data(), sizeof(double), P.size(), f) != P.size()) {
+            if (f) std::fclose(f);
+            uno_ctx_destroy(c);
+            set_error("could not read PMATRIX.DAT");
+            return UNO_EIO;
+        }
+        std::fclose(f);
+        o.guess = 1;
+    }
+    uno_scf_result r{};
+    int rc = uno_scf_run(c, d.nelec, &o, &r, ev.data(), P.data(), C.data());
+    if (rc && rc != UNO_ENOCONV) { uno_ctx_destroy(c); return rc; }
+    std::vector<double> S(n2), T(n2), H(n2), M(9 * n2);
+    one_electron(c->B, S.data(), T.data(), H.data(), M.data());
+    const int wrc = write_outputs(dir, d, c->B, r, S, T, H, M, P, ev, C, d.maxits);
+    if (res) *res = r;
+    uno_ctx_destroy(c);
+    return wrc ? wrc : rc;
+}
+
+}  // extern "C"

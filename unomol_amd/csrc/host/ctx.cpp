@@ -1,0 +1,388 @@
+// Context build (shell pairs, primitive pairs, Schwarz bounds, quartet spaces) and
+// the direct Fock build.  Replaces the reference's store-then-replay design
+// (TwoElectronInts.cpp:511-697 computes every unique ERI once into an 8 GiB buffer
+// spilling to MINTS.DAT; formGmatrix replays it each iteration, :822-869) with a direct
+// build: every iteration recomputes the significant shell quartets on the GPU and
+// contracts them with the density on the fly.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include "engine.hpp"
+#include "../hip/kernels.hpp"
+
+namespace uno {
+
+static thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+const std::string& get_error() { return g_err; }
+
+static constexpr double kTwoPi25 = 34.986836655249725;
+
+static int ensure_device() {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        set_error("no HIP device is visible (the engine needs a gfx950 GPU)");
+        return UNO_EDEVICE;
+    }
+    return UNO_OK;
+}
+
+static void build_prefix(uno_ctx* c, QuartetClass& q, double dmax) {
+    const PairClassList& Lb = c->cls[q.X];
+    const PairClassList& Lk = c->cls[q.Y];
+    const int nb = (int)Lb.Q.size();
+    q.prefix.assign(nb + 1, 0);
+    for (int i = 0; i < nb; ++i) {
+        long long cnt = 0;
+        const double qd = Lb.Q[i] * dmax;
+        if (qd > 0.0) {
+            const double thr = c->tau / qd;
+            cnt = std::partition_point(Lk.Q.begin(), Lk.Q.end(), [&](double v) { return v >= thr; }) - Lk.Q.begin();
+            if (q.X == q.Y) cnt = std::min<long long>(cnt, i + 1);
+        }
+        q.prefix[i + 1] = q.prefix[i] + cnt;
+    }
+    q.nq = q.prefix[nb];
+}
+
+static EriArgs base_args(uno_ctx* c) {
+    EriArgs a{};
+    a.pairs = c->d_pairs.p;
+    a.prims = c->d_prims.p;
+    a.tau = c->tau;
+    a.primcut = c->primcut;
+    a.n = c->n;
+    a.nsh = c->nsh;
+    return a;
+}
+
+int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
+    if (int rc = ensure_device()) return rc;
+    std::string err;
+    if (!make_basis(b, c->B, err)) {
+        set_error(err);
+        return UNO_EINVAL;
+    }
+    if (c->B.maxl > kMaxL) {
+        set_error("the device ERI kernels cover s, p and d shells (l <= 2); this basis has l = " +
+                  std::to_string(c->B.maxl));
+        return UNO_EINVAL;
+    }
+    const Basis& B = c->B;
+    c->n = B.norb;
+    c->nsh = (int)B.sh.size();
+    c->start_shell = start_shell;
+    if (!c->st) UNO_HIP(hipStreamCreate(&c->st));
+    hipStream_t st = c->st;
+    const int nsh = c->nsh;
+    // smallest exponent sum a ket pair can have: bounds the bare prefactor from above
+    double amin = 1e300;
+    for (const Shell& s : B.sh)
+        for (double a : s.a) amin = std::min(amin, a);
+    const double qmin = 2.0 * amin;
+    c->pairs.clear();
+    c->prims.clear();
+    c->pair_of.assign((size_t)nsh * nsh, -1);
+    for (int a = 0; a < nsh; ++a)
+        for (int bsh = 0; bsh <= a; ++bsh) {
+            const int s1 = (B.sh[a].l >= B.sh[bsh].l) ? a : bsh;
+            const int s2 = (s1 == a) ? bsh : a;
+            const Shell& S1 = B.sh[s1];
+            const Shell& S2 = B.sh[s2];
+            const double* A = &B.xyz[3 * S1.cen];
+            const double* Bc = &B.xyz[3 * S2.cen];
+            PairHdr h{};
+            double ab2 = 0.0;
+            for (int x = 0; x < 3; ++x) {
+                h.A[x] = A[x];
+                h.AB[x] = A[x] - Bc[x];
+                ab2 += h.AB[x] * h.AB[x];
+            }
+            h.f1 = B.off[s1];
+            h.f2 = B.off[s2];
+            h.prim0 = (int)c->prims.size();
+            h.same = (a == bsh);
+            h.sh1 = s1;
+            h.sh2 = s2;
+            for (int i = 0; i < S1.npr; ++i) {
+                // same shell: lower triangle of primitive pairs, off-diagonal doubled
+                // (TwoElectronInts.cpp:444-449)
+                const int jend = h.same ? i + 1 : S2.npr;
+                for (int j = 0; j < jend; ++j) {
+                    const double al = S1.a[i], be = S2.a[j], p = al + be, ip = 1.0 / p;
+                    const double s = std::exp(-al * be * ip * ab2);
+                    // every quartet with this primitive pair has a bare prefactor below
+                    // this bound; drop it when the reference cut would reject them all
+                    const double bound = kTwoPi25 * s * ip / (qmin * std::sqrt(p + qmin));
+                    if (bound < c->primcut) continue;
+                    PrimPair pp{};
+                    pp.p = p;
+                    pp.ip = ip;
+                    for (int x = 0; x < 3; ++x) pp.P[x] = (al * A[x] + be * Bc[x]) * ip;
+                    pp.K = S1.c[i] * S2.c[j] * ((h.same && i != j) ? 2.0 : 1.0) * s;
+                    pp.s = s;
+                    c->prims.push_back(pp);
+                }
+            }
+            h.nprim = (int)c->prims.size() - h.prim0;
+            if (h.nprim == 0) continue;
+            c->pair_of[(size_t)a * nsh + bsh] = (int)c->pairs.size();
+            c->pairs.push_back(h);
+        }
+    const int np = (int)c->pairs.size();
+    UNO_HIP(c->d_pairs.upload(c->pairs.data(), np, st));
+    UNO_HIP(c->d_prims.upload(c->prims.data(), c->prims.size(), st));
+    // Schwarz bounds Q_ab = sqrt(max |(ij|ij)|) from diagonal quartets, per pair class
+    c->pairQ.assign(np, 0.0);
+    DBuf<double> dq;
+    UNO_HIP(dq.alloc(np));
+    std::vector<int> pcls(np);
+    for (int p = 0; p < np; ++p) pcls[p] = pair_class(B.sh[c->pairs[p].sh1].l, B.sh[c->pairs[p].sh2].l);
+    for (int X = 0; X < kNumPairClasses; ++X) {
+        std::vector<int2> ql;
+        for (int p = 0; p < np; ++p)
+            if (pcls[p] == X) ql.push_back(make_int2(p, p));
+        if (ql.empty()) continue;
+        DBuf<int2> dql;
+        UNO_HIP(dql.upload(ql.data(), ql.size(), st));
+        EriArgs a = base_args(c);
+        a.explicit_q = dql.p;
+        a.nquart = (long long)ql.size();
+        a.out = dq.p;
+        UNO_HIP(eri_launch(kDigestSchwarz, X, X, a, st));
+        UNO_HIP(hipStreamSynchronize(st));
+    }
+    UNO_HIP(hipMemcpyAsync(c->pairQ.data(), dq.p, np * sizeof(double), hipMemcpyDeviceToHost, st));
+    UNO_HIP(hipStreamSynchronize(st));
+    for (int X = 0; X < kNumPairClasses; ++X) {
+        PairClassList& L = c->cls[X];
+        L.pairs.clear();
+        L.Q.clear();
+        for (int p = 0; p < np; ++p)
+            if (pcls[p] == X) L.pairs.push_back(p);
+        std::stable_sort(L.pairs.begin(), L.pairs.end(), [&](int x, int y) { return c->pairQ[x] > c->pairQ[y]; });
+        for (int p : L.pairs) L.Q.push_back(c->pairQ[p]);
+        UNO_HIP(L.d_pairs.upload(L.pairs.data(), L.pairs.size(), st));
+        UNO_HIP(L.d_Q.upload(L.Q.data(), L.Q.size(), st));
+    }
+    c->qcls.clear();
+    for (int X = 0; X < kNumPairClasses; ++X)
+        for (int Y = 0; Y <= X; ++Y) {
+            if (c->cls[X].pairs.empty() || c->cls[Y].pairs.empty()) continue;
+            QuartetClass q;
+            q.X = X;
+            q.Y = Y;
+            c->qcls.push_back(std::move(q));
+        }
+    std::vector<int> shoff(nsh), shn(nsh);
+    for (int s = 0; s < nsh; ++s) {
+        shoff[s] = B.off[s];
+        shn[s] = ncart(B.sh[s].l);
+    }
+    UNO_HIP(c->d_shoff.upload(shoff.data(), nsh, st));
+    UNO_HIP(c->d_shn.upload(shn.data(), nsh, st));
+    const size_t nn = (size_t)c->n * c->n;
+    UNO_HIP(c->D0.alloc(nn));
+    UNO_HIP(c->D1.alloc(nn));
+    UNO_HIP(c->D2.alloc(nn));
+    UNO_HIP(c->W0.alloc(nn));
+    UNO_HIP(c->W1.alloc(nn));
+    UNO_HIP(c->Dsh.alloc((size_t)nsh * nsh));
+    UNO_HIP(c->pk.alloc((size_t)c->n * (c->n + 1)));
+    UNO_HIP(c->scal.alloc(16));
+    UNO_HIP(hipStreamSynchronize(st));
+    return UNO_OK;
+}
+
+int fock_build(uno_ctx* c, bool uhf, double dmax_hint) {
+    hipStream_t st = c->st;
+    const long nn = (long)c->n * c->n;
+    UNO_HIP(k_fill(nn, 0.0, c->W0.p, st));
+    if (uhf) UNO_HIP(k_fill(nn, 0.0, c->W1.p, st));
+    // shell-block density maxima for the per-quartet screen, and the global bound
+    UNO_HIP(k_shell_max(c->n, c->nsh, c->d_shoff.p, c->d_shn.p, c->D0.p, uhf ? c->D1.p : nullptr,
+                        uhf ? c->D2.p : nullptr, c->Dsh.p, st));
+    double dmax = dmax_hint;
+    if (dmax < 0.0) {
+        UNO_HIP(k_absmax((long)c->nsh * c->nsh, c->Dsh.p, c->scal.p, st));
+        UNO_HIP(hipMemcpyAsync(&dmax, c->scal.p, sizeof(double), hipMemcpyDeviceToHost, st));
+        UNO_HIP(hipStreamSynchronize(st));
+    }
+    double total = 0.0;
+    const int ncls = kNumPairClasses * kNumPairClasses;
+    std::vector<hipEvent_t> ev0, ev1;
+    std::vector<int> evcls;
+    if (c->profile) {
+        UNO_HIP(c->d_counter.alloc(2 * ncls));
+        UNO_HIP(hipMemsetAsync(c->d_counter.p, 0, 2 * ncls * sizeof(unsigned long long), st));
+    }
+    for (QuartetClass& q : c->qcls) {
+        build_prefix(c, q, 4.0 * dmax);
+        if (q.nq == 0) continue;
+        UNO_HIP(q.d_prefix.upload(q.prefix.data(), q.prefix.size(), st));
+        EriArgs a = base_args(c);
+        a.bra = c->cls[q.X].d_pairs.p;
+        a.ket = c->cls[q.Y].d_pairs.p;
+        a.braQ = c->cls[q.X].d_Q.p;
+        a.ketQ = c->cls[q.Y].d_Q.p;
+        a.prefix = q.d_prefix.p;
+        a.nbra = (int)c->cls[q.X].pairs.size();
+        a.nquart = q.nq;
+        a.same_class = q.X == q.Y;
+        a.dsh = c->Dsh.p;
+        a.dmax_global = 4.0 * dmax;
+        a.D0 = c->D0.p;
+        a.W0 = c->W0.p;
+        if (uhf) {
+            a.D1 = c->D1.p;
+            a.D2 = c->D2.p;
+            a.W1 = c->W1.p;
+        }
+        const int ci = q.X * kNumPairClasses + q.Y;
+        if (c->profile) {
+            a.counter = c->d_counter.p + 2 * ci;
+            hipEvent_t e0, e1;
+            UNO_HIP(hipEventCreate(&e0));
+            UNO_HIP(hipEventCreate(&e1));
+            ev0.push_back(e0);
+            ev1.push_back(e1);
+            evcls.push_back(ci);
+            UNO_HIP(hipEventRecord(e0, st));
+        }
+        UNO_HIP(eri_launch(uhf ? kDigestUHF : kDigestRHF, q.X, q.Y, a, st));
+        if (c->profile) UNO_HIP(hipEventRecord(ev1.back(), st));
+        total += (double)q.nq;
+    }
+    c->last_quartets = total;
+    if (c->profile) {
+        UNO_HIP(hipStreamSynchronize(st));
+        std::vector<unsigned long long> cnt(2 * ncls);
+        UNO_HIP(hipMemcpy(cnt.data(), c->d_counter.p, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < ev0.size(); ++k) {
+            float ms = 0.f;
+            UNO_HIP(hipEventElapsedTime(&ms, ev0[k], ev1[k]));
+            c->prof_ms[evcls[k]] += ms;
+            c->prof_launch[evcls[k]] += 1.0;
+            (void)hipEventDestroy(ev0[k]);
+            (void)hipEventDestroy(ev1[k]);
+        }
+        for (int ci = 0; ci < ncls; ++ci) {
+            c->prof_prim[ci] += (double)cnt[2 * ci];
+            c->prof_shell[ci] += (double)cnt[2 * ci + 1];
+        }
+    }
+    return UNO_OK;
+}
+
+// Every canonical function quartet in the reference's order, evaluated on the device
+// shell-quartet blocks.
+int eri_emit(uno_ctx* c, long cap, double* val, int* ijkl, long* count) {
+    const Basis& B = c->B;
+    const int nsh = c->nsh;
+    struct Rec {
+        int cls;        // index into groups
+        long q;         // quartet within the group
+        int comp;       // component index within the block
+    };
+    std::vector<Rec> recs;
+    std::vector<int> fq;   // 4 per record
+    std::vector<std::vector<int2>> groups(kNumPairClasses * kNumPairClasses);
+    long n = 0;
+    for (int ish = c->start_shell; ish < nsh; ++ish)
+        for (int jsh = 0; jsh <= ish; ++jsh)
+            for (int ksh = 0; ksh <= ish; ++ksh)
+                for (int lsh = 0; lsh <= ksh; ++lsh) {
+                    const int sh[4] = {ish, jsh, ksh, lsh};
+                    int nf[4], o[4];
+                    for (int x = 0; x < 4; ++x) {
+                        nf[x] = ncart(B.sh[sh[x]].l);
+                        o[x] = B.off[sh[x]];
+                    }
+                    const int p1 = c->pair_of[(size_t)ish * nsh + jsh];
+                    const int p2 = c->pair_of[(size_t)ksh * nsh + lsh];
+                    long qidx = -1;
+                    int gidx = -1;
+                    bool swap = false;
+                    int NBb = 0, NCb = 0, NDb = 0;
+                    for (int a = 0; a < nf[0]; ++a) {
+                        const int ir = o[0] + a;
+                        for (int b = 0; b < nf[1]; ++b) {
+                            const int jr = o[1] + b;
+                            if (jr > ir) break;
+                            for (int cc = 0; cc < nf[2]; ++cc) {
+                                const int kr = o[2] + cc;
+                                if (kr > ir) break;
+                                for (int d = 0; d < nf[3]; ++d) {
+                                    const int lr = o[3] + d;
+                                    if (lr > kr || (ir == kr && lr > jr)) break;
+                                    Rec r{-1, -1, 0};
+                                    if (p1 >= 0 && p2 >= 0) {
+                                        if (qidx < 0) {
+                                            const PairHdr& h1 = c->pairs[p1];
+                                            const PairHdr& h2 = c->pairs[p2];
+                                            const int X = pair_class(B.sh[h1.sh1].l, B.sh[h1.sh2].l);
+                                            const int Y = pair_class(B.sh[h2.sh1].l, B.sh[h2.sh2].l);
+                                            swap = X < Y;
+                                            const int gb = swap ? Y : X, gk = swap ? X : Y;
+                                            gidx = gb * kNumPairClasses + gk;
+                                            const PairHdr& hb = swap ? h2 : h1;
+                                            const PairHdr& hk = swap ? h1 : h2;
+                                            NBb = ncart(B.sh[hb.sh2].l);
+                                            NCb = ncart(B.sh[hk.sh1].l);
+                                            NDb = ncart(B.sh[hk.sh2].l);
+                                            groups[gidx].push_back(swap ? make_int2(p2, p1) : make_int2(p1, p2));
+                                            qidx = (long)groups[gidx].size() - 1;
+                                        }
+                                        const PairHdr& h1 = c->pairs[p1];
+                                        const PairHdr& h2 = c->pairs[p2];
+                                        // component indices in each pair's stored orientation
+                                        const int i1 = (h1.sh1 == ish) ? a : b, i2 = (h1.sh1 == ish) ? b : a;
+                                        const int k1 = (h2.sh1 == ksh) ? cc : d, k2 = (h2.sh1 == ksh) ? d : cc;
+                                        int comp;
+                                        if (!swap) comp = ((i1 * NBb + i2) * NCb + k1) * NDb + k2;
+                                        else comp = ((k1 * NBb + k2) * NCb + i1) * NDb + i2;
+                                        r = Rec{gidx, qidx, comp};
+                                    }
+                                    recs.push_back(r);
+                                    fq.push_back(ir); fq.push_back(jr); fq.push_back(kr); fq.push_back(lr);
+                                    ++n;
+                                }
+                            }
+                        }
+                    }
+                }
+    *count = n;
+    if (cap <= 0) return UNO_OK;
+    hipStream_t st = c->st;
+    std::vector<std::vector<double>> blocks(groups.size());
+    for (int g = 0; g < (int)groups.size(); ++g) {
+        if (groups[g].empty()) continue;
+        const int X = g / kNumPairClasses, Y = g % kNumPairClasses;
+        const int nc = eri_class_ncomp(X, Y);
+        DBuf<int2> dq;
+        DBuf<double> dout;
+        UNO_HIP(dq.upload(groups[g].data(), groups[g].size(), st));
+        UNO_HIP(dout.alloc(groups[g].size() * (size_t)nc));
+        EriArgs a = base_args(c);
+        a.explicit_q = dq.p;
+        a.nquart = (long long)groups[g].size();
+        a.out = dout.p;
+        UNO_HIP(eri_launch(kDigestBlock, X, Y, a, st));
+        blocks[g].resize(groups[g].size() * (size_t)nc);
+        UNO_HIP(hipMemcpyAsync(blocks[g].data(), dout.p, blocks[g].size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        UNO_HIP(hipStreamSynchronize(st));
+    }
+    for (long e = 0; e < n && e < cap; ++e) {
+        const Rec& r = recs[e];
+        double v = 0.0;
+        if (r.cls >= 0) {
+            const int X = r.cls / kNumPairClasses, Y = r.cls % kNumPairClasses;
+            v = blocks[r.cls][(size_t)r.q * eri_class_ncomp(X, Y) + r.comp];
+        }
+        val[e] = v;
+        for (int x = 0; x < 4; ++x) ijkl[4 * e + x] = fq[4 * e + x];
+    }
+    return UNO_OK;
+}
+
+}  // namespace uno

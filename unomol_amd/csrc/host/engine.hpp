@@ -1,0 +1,123 @@
+// Internal engine state behind the C ABI (include/unomol.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include "../../../include/unomol.h"
+#include "../hip/devtypes.hpp"
+#include "basis.hpp"
+
+namespace uno {
+
+void set_error(const std::string& msg);
+const std::string& get_error();
+
+#define UNO_HIP(expr)                                                                               \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess) {                                                                     \
+            ::uno::set_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + __FILE__ + \
+                             ":" + std::to_string(__LINE__));                                       \
+            return UNO_EDEVICE;                                                                     \
+        }                                                                                           \
+    } while (0)
+
+// device buffer owned by RAII
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t cnt) {
+        if (cnt <= n && p) return hipSuccess;
+        release();
+        n = cnt;
+        return hipMalloc((void**)&p, (cnt ? cnt : 1) * sizeof(T));
+    }
+    hipError_t upload(const T* h, size_t cnt, hipStream_t st) {
+        hipError_t e = alloc(cnt);
+        if (e != hipSuccess) return e;
+        return cnt ? hipMemcpyAsync(p, h, cnt * sizeof(T), hipMemcpyHostToDevice, st) : hipSuccess;
+    }
+};
+
+struct PairClassList {
+    std::vector<int> pairs;     // pair indices sorted by Q descending
+    std::vector<double> Q;
+    DBuf<int> d_pairs;
+    DBuf<double> d_Q;
+};
+
+struct QuartetClass {
+    int X, Y;                   // bra / ket pair classes, X >= Y
+    std::vector<long long> prefix;
+    DBuf<long long> d_prefix;
+    long long nq = 0;
+};
+
+struct EigenWS;
+
+}  // namespace uno
+
+struct uno_ctx {
+    uno::Basis B;
+    int n = 0, nsh = 0, start_shell = 0;
+    hipStream_t st = nullptr;
+    // options
+    double tau = 1e-13, primcut = 1e-12, fp32_thr = 1e-6;
+    int precision = 0;
+    // pair data
+    std::vector<uno::PairHdr> pairs;
+    std::vector<uno::PrimPair> prims;
+    std::vector<double> pairQ;                  // Schwarz bound per pair
+    std::vector<int> pair_of;                   // [a*nsh+b] (a>=b) -> pair index or -1
+    uno::DBuf<uno::PairHdr> d_pairs;
+    uno::DBuf<uno::PrimPair> d_prims;
+    uno::PairClassList cls[uno::kNumPairClasses];
+    std::vector<uno::QuartetClass> qcls;
+    // per-shell function layout for the density screen
+    uno::DBuf<int> d_shoff, d_shn;
+    // device work matrices (n x n)
+    uno::DBuf<double> D0, D1, D2, W0, W1, Dsh, pk, scal;
+    uno::EigenWS* eig = nullptr;
+    double last_quartets = 0.0;
+    // profiling pass: per quartet class (bra X, ket Y) kernel milliseconds and the
+    // primitive / shell quartets the kernels evaluated
+    int profile = 0;
+    double prof_ms[uno::kNumPairClasses * uno::kNumPairClasses] = {};
+    double prof_prim[uno::kNumPairClasses * uno::kNumPairClasses] = {};
+    double prof_shell[uno::kNumPairClasses * uno::kNumPairClasses] = {};
+    double prof_launch[uno::kNumPairClasses * uno::kNumPairClasses] = {};
+    uno::DBuf<unsigned long long> d_counter;
+    ~uno_ctx();
+};
+
+namespace uno {
+
+int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell);
+// Fock build from dense device densities already in c->D0 (and D1, D2 for UHF).
+// Leaves W0 (and W1) = unsymmetrised accumulators; G = W + W^T.
+int fock_build(uno_ctx* c, bool uhf, double dmax_hint);
+int eri_emit(uno_ctx* c, long cap, double* val, int* ijkl, long* count);
+
+// dense symmetric eigensolver on device data: A (destroyed) -> Z (eigenvectors in
+// columns), evals ascending (host)
+int eigh_device(uno_ctx* c, int n, double* dA, double* dZ, double* h_evals);
+int eigh_standalone(int n, double* dA, double* dZ, double* h_evals, hipStream_t st);
+
+int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, double* evals, double* P, double* C);
+
+}  // namespace uno

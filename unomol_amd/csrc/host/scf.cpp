@@ -1,0 +1,294 @@
+// Device-resident RHF/UHF SCF loop.
+//
+// Loop semantics are the reference's (RHF.hpp:87-176, UHF.hpp:101-177):
+//   update(): G from the *input* density, E = 2tr(PH) + tr(PG) (RHF) or
+//             tr(PaH)+tr(PbH)+(tr(PaGa)+tr(PbGb))/2 (UHF), ediff = E - Eold,
+//             F = H + G, F' = X^T F X, diagonalise, C = X W, P = C_occ C_occ^T,
+//             pdiff = |P - P_in| / n (per spin, summed for UHF);
+//   two updates before the loop (UHF zeroes Eold before the second), then
+//   { converger; update; stop if converged } with the signed test ediff < eps
+//   (and/or pdiff < eps) and damping P = (P + P_in)/2 whenever the energy rose.
+// All matrices stay in HBM as dense row-major N x N FP64; the host only reads a few
+// scalars per iteration and the tridiagonal for the QL step.
+// accel = 2 replaces the damping with Pulay DIIS on F (error X^T(FDS - SDF)X).
+#include <chrono>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include "engine.hpp"
+#include "../hip/kernels.hpp"
+
+namespace uno {
+
+namespace {
+using clk = std::chrono::steady_clock;
+double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+// small dense solve (Gaussian elimination with partial pivoting) for DIIS
+bool solve_small(int m, std::vector<double> A, std::vector<double>& b) {
+    for (int k = 0; k < m; ++k) {
+        int piv = k;
+        for (int i = k + 1; i < m; ++i)
+            if (std::fabs(A[i * m + k]) > std::fabs(A[piv * m + k])) piv = i;
+        if (A[piv * m + k] == 0.0) return false;
+        if (piv != k) {
+            for (int j = 0; j < m; ++j) std::swap(A[k * m + j], A[piv * m + j]);
+            std::swap(b[k], b[piv]);
+        }
+        for (int i = k + 1; i < m; ++i) {
+            const double f = A[i * m + k] / A[k * m + k];
+            for (int j = k; j < m; ++j) A[i * m + j] -= f * A[k * m + j];
+            b[i] -= f * b[k];
+        }
+    }
+    for (int k = m - 1; k >= 0; --k) {
+        double s = b[k];
+        for (int j = k + 1; j < m; ++j) s -= A[k * m + j] * b[j];
+        b[k] = s / A[k * m + k];
+    }
+    return true;
+}
+}  // namespace
+
+struct ScfBufs {
+    DBuf<double> H, T, S, X, F, Fp, Tm, Cw, G[2], C[2], Pold[2], diisF, diisE, Etmp;
+};
+
+int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, double* evals_out, double* P_io,
+            double* C_out) {
+    const auto t_start = clk::now();
+    std::memset(&r, 0, sizeof(r));
+    const int n = c->n;
+    const long nn = (long)n * n;
+    const long n2 = (long)n * (n + 1) / 2;
+    const bool uhf = (nelec % 2) != 0;
+    const int ns = uhf ? 2 : 1;
+    const int nocc[2] = {uhf ? nelec - nelec / 2 : nelec / 2, nelec / 2};
+    if (nelec <= 0 || nocc[0] > n) {
+        set_error("electron count does not fit the basis");
+        return UNO_EINVAL;
+    }
+    hipStream_t st = c->st;
+    double eps = o.eps;
+    const double xeps = DBL_EPSILON * n * n * 0.5;   // Basis.hpp:249-254
+    if (eps < xeps) eps = xeps;
+    ScfBufs b;
+    for (DBuf<double>* p : {&b.H, &b.T, &b.S, &b.X, &b.F, &b.Fp, &b.Tm, &b.Cw, &b.G[0], &b.G[1], &b.C[0], &b.C[1],
+                            &b.Pold[0], &b.Pold[1], &b.Etmp})
+        UNO_HIP(p->alloc(nn));
+    // one-electron matrices on the host, once
+    std::vector<double> hS(n2), hT(n2), hH(n2);
+    one_electron(c->B, hS.data(), hT.data(), hH.data(), nullptr);
+    double nuc = 0.0;
+    for (int i = 0; i < c->B.ncen; ++i)
+        for (int j = i + 1; j < c->B.ncen; ++j) {
+            const double* a = &c->B.xyz[3 * i];
+            const double* bb = &c->B.xyz[3 * j];
+            const double dx = a[0] - bb[0], dy = a[1] - bb[1], dz = a[2] - bb[2];
+            nuc += c->B.q[i] * c->B.q[j] / std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    UNO_HIP(c->pk.upload(hH.data(), n2, st));
+    UNO_HIP(k_unpack(n, c->pk.p, b.H.p, st));
+    UNO_HIP(c->pk.upload(hT.data(), n2, st));
+    UNO_HIP(k_unpack(n, c->pk.p, b.T.p, st));
+    UNO_HIP(c->pk.upload(hS.data(), n2, st));
+    UNO_HIP(k_unpack(n, c->pk.p, b.S.p, st));
+    // X = U s^-1/2 keeping every vector (RHF.hpp:214-233)
+    std::vector<double> ev(n), hZ(nn);
+    UNO_HIP(hipMemcpyAsync(b.Tm.p, b.S.p, nn * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (int rc = eigh_device(c, n, b.Tm.p, b.Cw.p, ev.data())) return rc;
+    UNO_HIP(hipMemcpyAsync(hZ.data(), b.Cw.p, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+    UNO_HIP(hipStreamSynchronize(st));
+    for (int j = 0; j < n; ++j) {
+        double f = ev[j];
+        if (std::fabs(f) < 1e-7 && f < 0.0) f = -f;
+        if (f == 0.0) { set_error("Zero Eigenvalue in Overlap Matrix"); return UNO_ESINGULAR; }
+        if (f < 0.0) { set_error("Negative Eigenvalue in Overlap Matrix"); return UNO_ESINGULAR; }
+        ev[j] = 1.0 / std::sqrt(f);
+    }
+    for (long i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) hZ[i * n + j] *= ev[j];
+    UNO_HIP(b.X.upload(hZ.data(), nn, st));
+
+    double* P[2] = {uhf ? c->D1.p : c->D0.p, c->D2.p};
+    std::vector<double> evals(2 * (size_t)n, 0.0);
+    double diag_s = 0.0, fock_s = 0.0;
+
+    // F (device, n x n) -> P[s] with nocc electrons; keeps C[s] and the eigenvalues
+    auto diag_to_P = [&](const double* Fd, int s, int no) -> int {
+        const auto t0 = clk::now();
+        UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, Fd, n, b.X.p, n, 0.0, b.Tm.p, n, st));       // F X
+        UNO_HIP(gemm_f64(true, false, n, n, n, 1.0, b.X.p, n, b.Tm.p, n, 0.0, b.Fp.p, n, st));   // X^T F X
+        if (int rc = eigh_device(c, n, b.Fp.p, b.Cw.p, evals.data() + (size_t)s * n)) return rc;
+        UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, b.X.p, n, b.Cw.p, n, 0.0, b.C[s].p, n, st));  // C = X W
+        UNO_HIP(gemm_f64(false, true, n, n, no, 1.0, b.C[s].p, n, b.C[s].p, n, 0.0, P[s], n, st)); // C_occ C_occ^T
+        diag_s += secs(t0, clk::now());
+        return UNO_OK;
+    };
+
+    // guess
+    if (o.guess == 1 && P_io) {
+        for (int s = 0; s < ns; ++s) {
+            UNO_HIP(c->pk.upload(P_io + s * n2, n2, st));
+            UNO_HIP(k_unpack(n, c->pk.p, P[s], st));
+        }
+    } else {
+        // core Hamiltonian (RHF.hpp:205-211; UHF.hpp:206-215: beta reuses alpha's orbitals)
+        if (int rc = diag_to_P(b.H.p, 0, nocc[0])) return rc;
+        if (uhf) {
+            UNO_HIP(gemm_f64(false, true, n, n, nocc[1], 1.0, b.C[0].p, n, b.C[0].p, n, 0.0, P[1], n, st));
+            std::memcpy(evals.data() + n, evals.data(), n * sizeof(double));
+        }
+    }
+
+    // DIIS state
+    const int kDiis = 8;
+    std::vector<double> Bm;
+    int ndiis = 0, diis_next = 0;
+    if (o.accel == 2) {
+        UNO_HIP(b.diisF.alloc((size_t)kDiis * ns * nn));
+        UNO_HIP(b.diisE.alloc((size_t)kDiis * ns * nn));
+        Bm.assign(kDiis * kDiis, 0.0);
+    }
+
+    double energy = 0.0, eold = 0.0, ediff = 10.0, pdiff = 10.0;
+    int iteration = 0;
+    double hs[4];
+    auto update = [&]() -> int {
+        const auto t0 = clk::now();
+        if (uhf) UNO_HIP(k_axpby(nn, 1.0, P[0], 1.0, P[1], c->D0.p, st));
+        if (int rc = fock_build(c, uhf, -1.0)) return rc;
+        for (int s = 0; s < ns; ++s) UNO_HIP(k_sym_w(n, s == 0 ? c->W0.p : c->W1.p, b.G[s].p, st));
+        UNO_HIP(k_fill(4, 0.0, c->scal.p, st));
+        for (int s = 0; s < ns; ++s) {
+            UNO_HIP(k_dot(nn, P[s], b.H.p, c->scal.p, st));
+            UNO_HIP(k_dot(nn, P[s], b.G[s].p, c->scal.p + 1, st));
+        }
+        UNO_HIP(hipMemcpyAsync(hs, c->scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        UNO_HIP(hipStreamSynchronize(st));
+        fock_s += secs(t0, clk::now());
+        r.quartets += c->last_quartets;
+        ++r.nfock;
+        energy = uhf ? hs[0] + 0.5 * hs[1] : 2.0 * hs[0] + hs[1];
+        ediff = energy - eold;
+        eold = energy;
+        // F per spin (+ DIIS extrapolation)
+        double* Fs[2] = {b.F.p, b.Etmp.p};
+        for (int s = 0; s < ns; ++s) UNO_HIP(k_axpby(nn, 1.0, b.H.p, 1.0, b.G[s].p, Fs[s], st));
+        if (o.accel == 2) {
+            const int slot = diis_next;
+            diis_next = (diis_next + 1) % kDiis;
+            if (ndiis < kDiis) ++ndiis;
+            for (int s = 0; s < ns; ++s) {
+                double* Fh = b.diisF.p + ((size_t)slot * ns + s) * nn;
+                double* Eh = b.diisE.p + ((size_t)slot * ns + s) * nn;
+                UNO_HIP(hipMemcpyAsync(Fh, Fs[s], nn * sizeof(double), hipMemcpyDeviceToDevice, st));
+                // e = F P S - S P F, then X^T e X
+                UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, Fs[s], n, P[s], n, 0.0, b.Tm.p, n, st));
+                UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, b.Tm.p, n, b.S.p, n, 0.0, b.Fp.p, n, st));
+                UNO_HIP(k_sym_w(n, b.Fp.p, b.Cw.p, st));   // FPS + (FPS)^T = FPS + SPF
+                UNO_HIP(k_axpby(nn, 2.0, b.Fp.p, -1.0, b.Cw.p, b.Fp.p, st));   // FPS - SPF
+                UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, b.Fp.p, n, b.X.p, n, 0.0, b.Tm.p, n, st));
+                UNO_HIP(gemm_f64(true, false, n, n, n, 1.0, b.X.p, n, b.Tm.p, n, 0.0, Eh, n, st));
+            }
+            // B matrix row/column of the new slot
+            std::vector<double> row(kDiis, 0.0);
+            DBuf<double> dr;
+            UNO_HIP(dr.alloc(kDiis));
+            UNO_HIP(k_fill(kDiis, 0.0, dr.p, st));
+            for (int j = 0; j < ndiis; ++j)
+                for (int s = 0; s < ns; ++s)
+                    UNO_HIP(k_dot(nn, b.diisE.p + ((size_t)slot * ns + s) * nn, b.diisE.p + ((size_t)j * ns + s) * nn,
+                                  dr.p + j, st));
+            UNO_HIP(hipMemcpyAsync(row.data(), dr.p, kDiis * sizeof(double), hipMemcpyDeviceToHost, st));
+            UNO_HIP(hipStreamSynchronize(st));
+            for (int j = 0; j < ndiis; ++j) Bm[slot * kDiis + j] = Bm[j * kDiis + slot] = row[j];
+            if (ndiis >= 2) {
+                const int m = ndiis + 1;
+                std::vector<double> A(m * m, 0.0), rhs(m, 0.0);
+                for (int i = 0; i < ndiis; ++i) {
+                    for (int j = 0; j < ndiis; ++j) A[i * m + j] = Bm[i * kDiis + j];
+                    A[i * m + ndiis] = A[ndiis * m + i] = -1.0;
+                }
+                rhs[ndiis] = -1.0;
+                if (solve_small(m, A, rhs)) {
+                    for (int s = 0; s < ns; ++s) {
+                        UNO_HIP(k_fill(nn, 0.0, Fs[s], st));
+                        for (int i = 0; i < ndiis; ++i)
+                            UNO_HIP(k_axpby(nn, 1.0, Fs[s], rhs[i], b.diisF.p + ((size_t)i * ns + s) * nn, Fs[s], st));
+                    }
+                }
+            }
+        }
+        pdiff = 0.0;
+        for (int s = 0; s < ns; ++s) {
+            UNO_HIP(hipMemcpyAsync(b.Pold[s].p, P[s], nn * sizeof(double), hipMemcpyDeviceToDevice, st));
+            // spin-beta Fock lives in Etmp; copy to F's slot is unnecessary: diag reads Fs[s]
+            if (int rc = diag_to_P(Fs[s], s, nocc[s])) return rc;
+            UNO_HIP(k_diffnorm(nn, n, P[s], b.Pold[s].p, c->scal.p + 2 + s, st));
+        }
+        UNO_HIP(hipMemcpyAsync(hs + 2, c->scal.p + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        UNO_HIP(hipStreamSynchronize(st));
+        pdiff = hs[2] + (uhf ? hs[3] : 0.0);
+        ++iteration;
+        if (o.verbose)
+            std::fprintf(stderr, "Iteration     =  %6d\nEnergy        =  %25.16le\nDelta Energy  =  %25.15le\n"
+                                 "Delta Density =  %25.15le\n\n",
+                         iteration, energy + nuc, ediff, pdiff);
+        return UNO_OK;
+    };
+    auto converged = [&]() {
+        if (o.cflag == 0) return ediff < eps;
+        if (o.cflag == 1) return pdiff < eps;
+        if (o.cflag == 3) return std::fabs(ediff) < eps;   // |dE| < eps (no reference counterpart)
+        return pdiff < eps && ediff < eps;
+    };
+    iteration = 0;
+    eold = 0.0;
+    if (int rc = update()) return rc;
+    const double init_energy = energy + nuc;
+    iteration = 1;
+    if (uhf) eold = 0.0;   // UHF.hpp:157-158
+    if (int rc = update()) return rc;
+    bool conv = false;
+    while (iteration < o.maxits) {
+        if (o.accel != 2 && !(ediff < 0.0))
+            for (int s = 0; s < ns; ++s) UNO_HIP(k_axpby(nn, 0.5, P[s], 0.5, b.Pold[s].p, P[s], st));
+        if (int rc = update()) return rc;
+        if (converged()) { conv = true; break; }
+    }
+    // kinetic energy and outputs
+    UNO_HIP(k_fill(1, 0.0, c->scal.p, st));
+    for (int s = 0; s < ns; ++s) UNO_HIP(k_dot(nn, P[s], b.T.p, c->scal.p, st));
+    double kin = 0.0;
+    UNO_HIP(hipMemcpyAsync(&kin, c->scal.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    UNO_HIP(hipStreamSynchronize(st));
+    if (!uhf) kin *= 2.0;
+    if (evals_out) std::memcpy(evals_out, evals.data(), sizeof(double) * ns * n);
+    if (P_io)
+        for (int s = 0; s < ns; ++s) {
+            UNO_HIP(k_pack(n, P[s], c->pk.p, st));
+            UNO_HIP(hipMemcpyAsync(P_io + s * n2, c->pk.p, n2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        }
+    if (C_out)
+        for (int s = 0; s < ns; ++s)
+            UNO_HIP(hipMemcpyAsync(C_out + s * nn, b.C[s].p, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+    UNO_HIP(hipStreamSynchronize(st));
+    r.energy = energy + nuc;
+    r.init_energy = init_energy;
+    r.ediff = ediff;
+    r.pdiff = pdiff;
+    r.nucrep = nuc;
+    r.kinetic = kin;
+    r.electronic = energy;
+    r.iterations = iteration;
+    r.converged = conv ? 1 : 0;
+    r.uhf = uhf ? 1 : 0;
+    r.fock_s = fock_s;
+    r.diag_s = diag_s;
+    r.wall_s = secs(t_start, clk::now());
+    return conv ? UNO_OK : UNO_ENOCONV;
+}
+
+}  // namespace uno
