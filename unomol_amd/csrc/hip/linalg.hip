@@ -159,7 +159,7 @@ hipError_t k_absmax(long len, const double* x, double* out, hipStream_t st) {
 // FP64 MFMA GEMM: 64x64 block tile, 4 waves each owning a 32x32 quarter made of
 // 2x2 tiles of v_mfma_f64_16x16x4_f64.  Operand layout of that instruction:
 // A(16x4): lane l holds A[l%16][l/16]; B(4x16): lane l holds B[l/16][l%16];
-// D(16x16): lane l holds D[4*(l/16)+r][l%16] in register r.
+// D(16x16): lane l holds D[4*r + l/16][l%16] in register r (pinned by an on-device probe).
 // ---------------------------------------------------------------------------
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, double alpha, const double* __restrict__ A,
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, double alpha,
         for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gi = bi + wi + ta * 16 + 4 * (lane >> 4) + r;
+                const int gi = bi + wi + ta * 16 + 4 * r + (lane >> 4);
                 const int gj = bj + wj + tb * 16 + (lane & 15);
                 if (gi < M && gj < N) {
                     double* c = C + (long)gi * ldc + gj;
@@ -383,7 +383,7 @@ __global__ void k_apply_sweeps(int n, double* Q, int nsweep, const int* __restri
     for (int q = threadIdx.x; q < nr * n; q += blockDim.x) Q[(long)r0 * n + q] = srow[q];
 }
 hipError_t apply_sweeps(int n, double* Q, int nsweep, const int* sweeps, const double* cs, hipStream_t st) {
-    int rpb = (int)((150 * 1024) / (8 * (long)n));
+    int rpb = (int)((60 * 1024) / (8 * (long)n));
     if (rpb > 64) rpb = 64;
     if (rpb < 1) rpb = 1;
     const size_t lds = (size_t)rpb * n * sizeof(double);

@@ -478,10 +478,11 @@ int uno_run_deck(const char* dir_c, const uno_scf_opts* opts, uno_scf_result* re
     Deck d;
     std::string err;
     if (!read_deck(dir + "/patin.dat", d, err)) { set_error(err); return UNO_EIO; }
-    if (d.int_flag[0] || d.int_flag[1]) {
-        set_error("DPM and finite-field runs (int_flag) are not part of the device path");
-        return UNO_EINVAL;
-    }
+    if (d.int_flag[0] || d.int_flag[1])
+        // the ground-state SCF and its outputs are produced; the extra SCFs the reference
+        // runs afterwards (finite field, RHF.hpp:235-271; DPM scan, RHF.hpp:292-388) are not
+        std::fprintf(stderr, "note: int_flag (DPM %d, finite field %d) requested; only the ground-state SCF "
+                             "is run by this engine\n", d.int_flag[0], d.int_flag[1]);
     uno_basis b;
     basis_view(d, b);
     uno_ctx* c = nullptr;

@@ -149,6 +149,10 @@ int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
         a.explicit_q = dql.p;
         a.nquart = (long long)ql.size();
         a.out = dq.p;
+        // bounds from the uncut integrals: with the primitive cut a pair of distant
+        // tight primitives can lose its whole diagonal quartet while its quartets with
+        // diffuse partners survive the cut, so a cut diagonal would under-bound them
+        a.primcut = 0.0;
         UNO_HIP(eri_launch(kDigestSchwarz, X, X, a, st));
         UNO_HIP(hipStreamSynchronize(st));
     }
