@@ -58,6 +58,10 @@ def flops_per_shell_quartet_digest(la, lb, lc, ld):
     return 6 * 2 * ncomp + ncomp
 
 
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
 def cpu_baseline(deck_path, budget_s=15.0):
     """Reference ERI code (oracle/_ref/libref_harness.so) on a random sample of canonical
     shell quartets of the deck, one host core; extrapolated to one full ERI pass."""
@@ -90,6 +94,9 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--fock-only", action="store_true",
+                    help="hardware-counter runs: context build, 2 SCF updates, one profiled Fock "
+                         "build and a 1 GiB calibration copy; prints the per-class table")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures one MI355X (--gpus 1); "
@@ -105,6 +112,9 @@ def main():
     basis = unomol_amd.Basis(deck)
     nelec = basis.deck["nelec"]
     opts = dict(maxits=200, eps=args.eps, cflag=3, accel=2, guess=0)
+    if args.fock_only:
+        opts["maxits"] = 2
+        args.warmup, args.steps = 0, 1
 
     def one_scf():
         t0 = time.perf_counter()
@@ -114,12 +124,15 @@ def main():
         dt = time.perf_counter() - t0
         return t, hf, e, dt
 
-    for _ in range(args.warmup):
-        one_scf()
+    log("deck: %d functions, %d electrons" % (basis.number_of_orbitals(), nelec))
+    for i in range(args.warmup):
+        _, _, e, dt = one_scf()
+        log("warmup %d: E=%.10f in %.2f s" % (i, e, dt))
     results = []
     t_all = time.perf_counter()
     for _ in range(args.steps):
         t, hf, e, dt = one_scf()
+        log("step: E=%.10f in %.2f s" % (e, dt))
         results.append((t, hf, e, dt))
     wall = time.perf_counter() - t_all
     t, hf, energy, dt = results[-1]
@@ -158,6 +171,14 @@ def main():
                     "frac": round(dom["tflops"] / FP64_PEAK_TFLOPS, 5), "traffic": None,
                     "all_eri_kernels": {"ms": round(tot_ms, 3), "tflops": round(tot_fl / (tot_ms * 1e-3) / 1e12, 4),
                                         "frac": round(tot_fl / (tot_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 5)}}
+    if args.fock_only:
+        ms = C.c_double()
+        N.check(N.lib().uno_copy_probe(C.c_ulonglong(1 << 30), 1, C.byref(ms)))
+        print(json.dumps({"fock_only": True, "copy_probe_bytes": 1 << 30, "copy_ms": ms.value,
+                          "copy_GBps": 2 * (1 << 30) / (ms.value * 1e-3) / 1e9, "roofline": roofline,
+                          "eri_classes": classes}))
+        return
+    log("cpu baseline")
     cpu = None if args.no_cpu_baseline else cpu_baseline(deck)
     out = {
         "metric": "SCF time-to-converge (|dE|<1e-8 Eh) + Fock builds/s, FP64 GPU",

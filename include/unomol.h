@@ -64,6 +64,10 @@ int uno_ctx_destroy(uno_ctx* ctx);
  * run in FP32 when "precision" is 1; default 1e-6), "precision" (0 = FP64, 1 = mixed),
  * "prim_cut" (bare primitive prefactor cut, default 1e-12 as TwoElectronInts.cpp:428). */
 int uno_ctx_set(uno_ctx* ctx, const char* key, double value);
+/* Reads the knobs above and, read-only: "last_quartets" (shell quartets evaluated by the
+ * last Fock build), "pairs" (significant shell pairs), "pair_runs" (runs of the pair
+ * array: one per angular pair class and primitive-count bucket, at most 36) and
+ * "quartet_launches" (ERI launches per Fock build). */
 int uno_ctx_get(uno_ctx* ctx, const char* key, double* value);
 int uno_ctx_norb(const uno_ctx* ctx);
 
@@ -129,6 +133,11 @@ int uno_schwarz(uno_ctx* ctx, double* q);
  * enable=0 turns it off; stats are read with enable=-1.  Arrays hold 36 entries. */
 int uno_profile(uno_ctx* ctx, int enable, double* ms, double* launches, double* prim_quartets,
                 double* shell_quartets);
+
+/* Measurement helper (no reference counterpart): a plain device copy of `bytes`
+ * (read once, written once) repeated reps times; *ms gets the total kernel time.  Used
+ * to calibrate hardware traffic counters and as an HBM bandwidth reference. */
+int uno_copy_probe(unsigned long long bytes, int reps, double* ms);
 
 /* One-electron matrices (host, McMurchie-Davidson; OneElectronInts.cpp:127-202):
  * packed S, T, H = T + V. */

@@ -132,6 +132,20 @@ def test_fock_uhf_vs_oracle(tag):
     assert np.abs(GB - RB).max() < 1e-11
 
 
+def test_pair_runs_grouped(tmp_path):
+    """The pair array of a water cluster splits into at most one run per angular pair
+    class and primitive-count bucket, and every run pair (X >= Y) is one ERI launch."""
+    say("pair runs of a 16-water cluster")
+    deck = str(tmp_path / "w16.dat")
+    unomol_amd.generate_deck(deck, "water_cluster", 16, 5, "6-31G*", 0, 1e-8, 50)
+    b = unomol_amd.Basis(deck)
+    t = unomol_amd.TwoElectronInts(b)
+    runs = int(t.get("pair_runs"))
+    assert 1 <= runs <= 36
+    assert int(t.get("quartet_launches")) == runs * (runs + 1) // 2
+    assert int(t.get("pairs")) > 0
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 13, 64, 200])
 def test_eigh_vs_numpy(n):
     say("eigh n=%d" % n)

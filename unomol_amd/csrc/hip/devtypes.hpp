@@ -44,11 +44,14 @@ enum DigestMode : int {
 struct EriArgs {
     const PairHdr* pairs;
     const PrimPair* prims;
-    const int* bra;           // pair indices of the bra class, sorted by Q descending
-    const int* ket;           // pair indices of the ket class
-    const double* braQ;       // Schwarz bound of each bra entry
-    const double* ketQ;
-    const long long* prefix;  // implicit quartet space: prefix[i] = first quartet of bra i
+    // implicit quartet space of one launch: bra pairs [bra_off, bra_off+nbra) and ket pairs
+    // from ket_off, both runs of the pair array sorted by Schwarz bound (descending);
+    // prefix[i] = first quartet of bra i, blk[b] = bra holding quartet b*qpb
+    int bra_off, ket_off;
+    const double* pairQ;      // Schwarz bound of every pair, in pair-array order
+    const long long* prefix;
+    const int* blk;
+    int qpb;                  // quartets per block of the launch
     int nbra;
     long long nquart;
     const int2* explicit_q;   // explicit (bra, ket) list (block / Schwarz modes) or null

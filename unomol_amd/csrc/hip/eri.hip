@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) void k_eri_lane(const EriArgs a) {
 // one 64-lane wave (one block) per shell quartet
 // ---------------------------------------------------------------------------
 template <int LA, int LB, int LC, int LD, int MODE>
-__global__ __launch_bounds__(64) void k_eri_wave(const EriArgs a) {
+__device__ __forceinline__ void wave_one(const EriArgs& a, const long long g) {
     using CI = ClassInfo<LA, LB, LC, LD>;
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND, NCOMP = CI::NCOMP, NR = CI::NR;
     constexpr int E = CI::E, F = CI::F, TBL = CI::TBL;
@@ -227,7 +227,6 @@ __global__ __launch_bounds__(64) void k_eri_wave(const EriArgs a) {
     __shared__ double sI[NR * 3][TBL];
     __shared__ double sE[NCOMP];
     const int lane = threadIdx.x;
-    const long long g = blockIdx.x;
     int pab, pcd;
     double q12;
     if (!quartet_of(a, g, pab, pcd, q12)) return;
@@ -441,18 +440,33 @@ __global__ __launch_bounds__(64) void k_eri_wave(const EriArgs a) {
     }
 }
 
+// a block (one wave) walks qpb consecutive quartets of the launch
+template <int LA, int LB, int LC, int LD, int MODE>
+__global__ __launch_bounds__(64) void k_eri_wave(const EriArgs a) {
+    const long long g0 = (long long)blockIdx.x * a.qpb;
+    for (int q = 0; q < a.qpb; ++q) {
+        wave_one<LA, LB, LC, LD, MODE>(a, g0 + q);
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launcher: dispatch on the class and digest mode
 // ---------------------------------------------------------------------------
+// lane kernels: one quartet per lane, 256 per block; wave kernels: 8 per block
+static constexpr int quartets_per_block(int ncomp) { return ncomp <= 81 ? 256 : 8; }
+
 template <int LA, int LB, int LC, int LD, int MODE>
 static hipError_t launch_cls(const EriArgs& a, hipStream_t st) {
     using CI = ClassInfo<LA, LB, LC, LD>;
     if (a.nquart <= 0) return hipSuccess;
+    EriArgs b = a;
+    b.qpb = quartets_per_block(CI::NCOMP);
+    const long long nb = (a.nquart + b.qpb - 1) / b.qpb;
     if constexpr (CI::NCOMP <= 81) {
-        const long long nb = (a.nquart + 255) / 256;
-        hipLaunchKernelGGL((k_eri_lane<LA, LB, LC, LD, MODE>), dim3((unsigned)nb), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_eri_lane<LA, LB, LC, LD, MODE>), dim3((unsigned)nb), dim3(256), 0, st, b);
     } else {
-        hipLaunchKernelGGL((k_eri_wave<LA, LB, LC, LD, MODE>), dim3((unsigned)a.nquart), dim3(64), 0, st, a);
+        hipLaunchKernelGGL((k_eri_wave<LA, LB, LC, LD, MODE>), dim3((unsigned)nb), dim3(64), 0, st, b);
     }
     return hipGetLastError();
 }
@@ -498,6 +512,8 @@ int eri_class_ncomp(int cb, int ck) {
     static const int nab[6] = {1, 3, 9, 6, 18, 36};
     return nab[cb] * nab[ck];
 }
+
+int eri_quartets_per_block(int cb, int ck) { return quartets_per_block(eri_class_ncomp(cb, ck)); }
 
 // ---------------------------------------------------------------------------
 // Rys-table evaluation for the parity tests

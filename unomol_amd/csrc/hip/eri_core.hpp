@@ -161,15 +161,17 @@ __device__ __forceinline__ bool quartet_of(const EriArgs& a, long long g, int& p
         q12 = -1.0;
         return true;
     }
-    int lo = 0, hi = a.nbra - 1;
+    // the block table brackets the bra of every quartet of this block
+    const long long b = g / a.qpb;
+    int lo = a.blk[b], hi = a.blk[b + 1];
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (a.prefix[mid] <= g) lo = mid; else hi = mid - 1;
     }
     const int jk = (int)(g - a.prefix[lo]);
-    pab = a.bra[lo];
-    pcd = a.ket[jk];
-    q12 = a.braQ[lo] * a.ketQ[jk];
+    pab = a.bra_off + lo;
+    pcd = a.ket_off + jk;
+    q12 = a.pairQ[pab] * a.pairQ[pcd];
     return true;
 }
 

@@ -8,6 +8,8 @@ namespace uno {
 // ERIs (eri.hip)
 hipError_t eri_launch(int mode, int bra_class, int ket_class, const EriArgs& a, hipStream_t st);
 int eri_class_ncomp(int bra_class, int ket_class);
+// quartets one block of the class kernel walks (the block table granularity)
+int eri_quartets_per_block(int bra_class, int ket_class);
 hipError_t rys_eval_launch(int n, int npts, const double* x, double* u, double* w, hipStream_t st);
 
 // dense helpers (linalg.hip); matrices are row-major n x n
@@ -33,19 +35,28 @@ hipError_t gemm_f64(bool ta, bool tb, int m, int n, int k, double alpha, const d
 // eigensolver pieces (linalg.hip)
 struct EigWork {
     double* A;      // n x n working copy (destroyed)
-    double* Q;      // n x n accumulated reflectors -> eigenvectors
-    double* V;      // n x n reflector vectors (row k = v_k)
+    double* Q;      // n x n accumulated reflectors
+    double* V;      // n x n reflector vectors (row k = v_k, zero outside its support)
     double* tau;    // n
     double* p;      // n
     double* d;      // n  diagonal of T
     double* e;      // n  off-diagonal of T
     double* scal;   // 4 scalars
+    double* evals;  // n  eigenvalues of T (ascending)
+    double* Y;      // n x n eigenvectors of T, one per row
+    double* wk;     // 5 n^2 inverse-iteration work
+    double* wyS;    // 32 x 32
+    double* wyT;    // 32 x 32
+    double* wy1;    // 32 x n
+    double* wy2;    // 32 x n
 };
 hipError_t householder_tridiag(int n, EigWork& w, hipStream_t st);
 hipError_t accumulate_q(int n, EigWork& w, hipStream_t st);
-// apply recorded QL sweeps to the rows of Q: sweeps[s] = {lo, hi, first rotation}
-hipError_t apply_sweeps(int n, double* Q, int nsweep, const int* sweeps, const double* cs, hipStream_t st);
-// Z[:, j] = Q[:, perm[j]]
-hipError_t permute_cols(int n, const double* Q, const int* perm, double* Z, hipStream_t st);
+// nclusters < 0: bisection for w.evals; otherwise inverse iteration into w.Y followed by
+// Gram-Schmidt inside the given clusters ({first, end} pairs, device memory)
+hipError_t tridiag_eigen(int n, EigWork& w, int nclusters, const int* clusters, hipStream_t st);
+
+// streaming copy of `bytes` (read + write), reps times; total milliseconds in *ms
+hipError_t copy_probe(size_t bytes, int reps, float* ms, hipStream_t st);
 
 }  // namespace uno

@@ -144,10 +144,13 @@ void hermite_coulomb(double R[9][9][9], int L, double p, const double pc[3], dou
 
 void one_electron(const Basis& B, double* S, double* T, double* H, double* M) {
     const int ns = (int)B.sh.size();
-    Hermite1D hx, hy, hz;
-    double R[9][9][9];
+    // shell pairs are independent and write disjoint packed elements: spread them over
+    // the host cores (dynamic schedule, rows of the triangle differ in length)
+#pragma omp parallel for schedule(dynamic, 1)
     for (int is = 0; is < ns; ++is)
         for (int js = 0; js <= is; ++js) {
+            Hermite1D hx, hy, hz;
+            double R[9][9][9];
             const Shell& A = B.sh[is];
             const Shell& Bs = B.sh[js];
             const double* ra = &B.xyz[3 * A.cen];

@@ -293,6 +293,9 @@ int uno_ctx_get(uno_ctx* c, const char* key, double* v) {
     else if (k == "fp32_threshold") *v = c->fp32_thr;
     else if (k == "precision") *v = c->precision;
     else if (k == "last_quartets") *v = c->last_quartets;
+    else if (k == "pair_runs") *v = (double)c->cls.size();
+    else if (k == "quartet_launches") *v = (double)c->qcls.size();
+    else if (k == "pairs") *v = (double)c->pairs.size();
     else { set_error("unknown option " + k); return UNO_EINVAL; }
     return UNO_OK;
 }
@@ -434,6 +437,14 @@ int uno_profile(uno_ctx* c, int enable, double* ms, double* launches, double* pr
         if (prim) prim[i] = c->prof_prim[i];
         if (shell) shell[i] = c->prof_shell[i];
     }
+    return UNO_OK;
+}
+
+int uno_copy_probe(unsigned long long bytes, int reps, double* ms) {
+    if (!ms || reps <= 0 || bytes < 32) { set_error("bad argument"); return UNO_EINVAL; }
+    float t = 0.f;
+    UNO_HIP(copy_probe((size_t)bytes, reps, &t, nullptr));
+    *ms = t;
     return UNO_OK;
 }
 

@@ -158,21 +158,64 @@ int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
     }
     UNO_HIP(hipMemcpyAsync(c->pairQ.data(), dq.p, np * sizeof(double), hipMemcpyDeviceToHost, st));
     UNO_HIP(hipStreamSynchronize(st));
-    for (int X = 0; X < kNumPairClasses; ++X) {
-        PairClassList& L = c->cls[X];
-        L.pairs.clear();
-        L.Q.clear();
-        for (int p = 0; p < np; ++p)
-            if (pcls[p] == X) L.pairs.push_back(p);
-        std::stable_sort(L.pairs.begin(), L.pairs.end(), [&](int x, int y) { return c->pairQ[x] > c->pairQ[y]; });
-        for (int p : L.pairs) L.Q.push_back(c->pairQ[p]);
-        UNO_HIP(L.d_pairs.upload(L.pairs.data(), L.pairs.size(), st));
-        UNO_HIP(L.d_Q.upload(L.Q.data(), L.Q.size(), st));
+    // Reorder the pair and primitive arrays into runs of (angular class, primitive-count
+    // bucket), each sorted by Schwarz bound: a launch then walks contiguous memory, and the
+    // lanes of a wave see alike primitive loops.
+    // run key of every pair, taken before the arrays are permuted
+    std::vector<int> runkey(np);
+    for (int p = 0; p < np; ++p) runkey[p] = pcls[p] * kNumBuckets + prim_bucket(c->pairs[p].nprim);
+    std::vector<int> order(np);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        if (runkey[x] != runkey[y]) return runkey[x] < runkey[y];
+        return c->pairQ[x] > c->pairQ[y];
+    });
+    std::vector<PairHdr> pairs2;
+    std::vector<PrimPair> prims2;
+    std::vector<double> q2;
+    std::vector<int> newidx(np);
+    for (int k = 0; k < np; ++k) {
+        PairHdr h = c->pairs[order[k]];
+        const int p0 = h.prim0;
+        h.prim0 = (int)prims2.size();
+        for (int q = 0; q < h.nprim; ++q) prims2.push_back(c->prims[p0 + q]);
+        pairs2.push_back(h);
+        q2.push_back(c->pairQ[order[k]]);
+        newidx[order[k]] = k;
     }
+    for (int& p : c->pair_of)
+        if (p >= 0) p = newidx[p];
+    c->pairs.swap(pairs2);
+    c->prims.swap(prims2);
+    c->pairQ.swap(q2);
+    UNO_HIP(c->d_pairs.upload(c->pairs.data(), np, st));
+    UNO_HIP(c->d_prims.upload(c->prims.data(), c->prims.size(), st));
+    UNO_HIP(c->d_pairQ.upload(c->pairQ.data(), np, st));
+    c->cls.clear();
+    for (int k = 0; k < np;) {
+        const int r = runkey[order[k]];
+        PairClassList L;
+        L.lcls = r / kNumBuckets;
+        L.bucket = r % kNumBuckets;
+        L.off = k;
+        while (k < np && runkey[order[k]] == r) L.Q.push_back(c->pairQ[k++]);
+        L.count = k - L.off;
+        c->cls.push_back(std::move(L));
+    }
+    // one run per (angular class, bucket) at most, each sorted by bound
+    if ((int)c->cls.size() > kNumPairClasses * kNumBuckets) {
+        set_error("internal error: pair runs are not grouped (" + std::to_string(c->cls.size()) + " runs)");
+        return UNO_EINVAL;
+    }
+    for (const PairClassList& L : c->cls)
+        for (size_t i = 1; i < L.Q.size(); ++i)
+            if (L.Q[i] > L.Q[i - 1]) {
+                set_error("internal error: pair run is not sorted by Schwarz bound");
+                return UNO_EINVAL;
+            }
     c->qcls.clear();
-    for (int X = 0; X < kNumPairClasses; ++X)
+    for (int X = 0; X < (int)c->cls.size(); ++X)
         for (int Y = 0; Y <= X; ++Y) {
-            if (c->cls[X].pairs.empty() || c->cls[Y].pairs.empty()) continue;
             QuartetClass q;
             q.X = X;
             q.Y = Y;
@@ -223,14 +266,30 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint) {
     for (QuartetClass& q : c->qcls) {
         build_prefix(c, q, 4.0 * dmax);
         if (q.nq == 0) continue;
+        const PairClassList& LX = c->cls[q.X];
+        const PairClassList& LY = c->cls[q.Y];
+        const int qpb = eri_quartets_per_block(LX.lcls, LY.lcls);
+        // bra holding the first quartet of every block: each lane then searches a short range
+        const long long nblk = (q.nq + qpb - 1) / qpb;
+        q.blk.resize(nblk + 1);
+        {
+            int i = 0;
+            for (long long b = 0; b <= nblk; ++b) {
+                const long long g = std::min(b * qpb, q.nq - 1);
+                while (q.prefix[i + 1] <= g) ++i;
+                q.blk[b] = i;
+            }
+        }
         UNO_HIP(q.d_prefix.upload(q.prefix.data(), q.prefix.size(), st));
+        UNO_HIP(q.d_blk.upload(q.blk.data(), q.blk.size(), st));
         EriArgs a = base_args(c);
-        a.bra = c->cls[q.X].d_pairs.p;
-        a.ket = c->cls[q.Y].d_pairs.p;
-        a.braQ = c->cls[q.X].d_Q.p;
-        a.ketQ = c->cls[q.Y].d_Q.p;
+        a.bra_off = LX.off;
+        a.ket_off = LY.off;
+        a.pairQ = c->d_pairQ.p;
         a.prefix = q.d_prefix.p;
-        a.nbra = (int)c->cls[q.X].pairs.size();
+        a.blk = q.d_blk.p;
+        a.qpb = qpb;
+        a.nbra = LX.count;
         a.nquart = q.nq;
         a.same_class = q.X == q.Y;
         a.dsh = c->Dsh.p;
@@ -242,7 +301,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint) {
             a.D2 = c->D2.p;
             a.W1 = c->W1.p;
         }
-        const int ci = q.X * kNumPairClasses + q.Y;
+        const int ci = LX.lcls * kNumPairClasses + LY.lcls;
         if (c->profile) {
             a.counter = c->d_counter.p + 2 * ci;
             hipEvent_t e0, e1;
@@ -253,7 +312,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint) {
             evcls.push_back(ci);
             UNO_HIP(hipEventRecord(e0, st));
         }
-        UNO_HIP(eri_launch(uhf ? kDigestUHF : kDigestRHF, q.X, q.Y, a, st));
+        UNO_HIP(eri_launch(uhf ? kDigestUHF : kDigestRHF, LX.lcls, LY.lcls, a, st));
         if (c->profile) UNO_HIP(hipEventRecord(ev1.back(), st));
         total += (double)q.nq;
     }

@@ -54,19 +54,28 @@ struct DBuf {
     }
 };
 
+// A run of the pair array: one angular pair class (ss ps pp ds dp dd) and one bucket of
+// primitive-pair counts, sorted by Schwarz bound (descending).  Bucketing by primitive
+// count keeps the loop trip counts of the lanes of a wave alike.
 struct PairClassList {
-    std::vector<int> pairs;     // pair indices sorted by Q descending
-    std::vector<double> Q;
-    DBuf<int> d_pairs;
-    DBuf<double> d_Q;
+    int lcls = 0, bucket = 0;
+    int off = 0, count = 0;
+    std::vector<double> Q;      // host copy of the bounds of the run
 };
 
 struct QuartetClass {
-    int X, Y;                   // bra / ket pair classes, X >= Y
+    int X, Y;                   // indices into the runs, X >= Y (bra run, ket run)
     std::vector<long long> prefix;
+    std::vector<int> blk;
     DBuf<long long> d_prefix;
+    DBuf<int> d_blk;
     long long nq = 0;
 };
+
+constexpr int kNumBuckets = 6;
+inline int prim_bucket(int nprim) {
+    return nprim <= 1 ? 0 : nprim <= 3 ? 1 : nprim <= 6 ? 2 : nprim <= 9 ? 3 : nprim <= 18 ? 4 : 5;
+}
 
 struct EigenWS;
 
@@ -86,8 +95,9 @@ struct uno_ctx {
     std::vector<int> pair_of;                   // [a*nsh+b] (a>=b) -> pair index or -1
     uno::DBuf<uno::PairHdr> d_pairs;
     uno::DBuf<uno::PrimPair> d_prims;
-    uno::PairClassList cls[uno::kNumPairClasses];
+    std::vector<uno::PairClassList> cls;   // runs ordered by (angular class, bucket)
     std::vector<uno::QuartetClass> qcls;
+    uno::DBuf<double> d_pairQ;
     // per-shell function layout for the density screen
     uno::DBuf<int> d_shoff, d_shn;
     // device work matrices (n x n)
