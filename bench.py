@@ -143,7 +143,7 @@ def main():
     classes = []
     if not args.no_profile:
         # profiling pass: one Fock build on the converged density with per-class HIP events
-        ms = np.zeros(36); launches = np.zeros(36); prim = np.zeros(36); shell = np.zeros(36)
+        ms = np.zeros(72); launches = np.zeros(72); prim = np.zeros(72); shell = np.zeros(72)
         N.check(N.lib().uno_profile(t.ctx, 1, None, None, None, None))
         n = t.norb
         G = np.zeros_like(hf.P)

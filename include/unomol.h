@@ -33,9 +33,10 @@ extern "C" {
 
 typedef struct uno_ctx uno_ctx;
 
-/* POD mirror of the reference Basis/Shell/Center (Basis.hpp:17-352).  Coefficients are
- * the raw deck values; the library normalises them as Shell::normalize does
- * (Basis.hpp:56-75). */
+/* POD mirror of the reference Basis/Shell/Center (Basis.hpp:17-352).  With normalized 0
+ * the coefficients are the raw deck values and the library normalises them as
+ * Shell::normalize does (Basis.hpp:56-75); with normalized 1 they are taken as already
+ * normalised, as a reference Basis holds them after its constructor (Basis.hpp:246). */
 typedef struct uno_basis {
     int nshell, ncen;
     const int* sh_l;      /* [nshell] */
@@ -46,6 +47,7 @@ typedef struct uno_basis {
     const double* coef;
     const double* xyz;    /* [3*ncen] bohr */
     const double* charge; /* [ncen] */
+    int normalized;       /* 0: raw deck coefficients, 1: already normalised */
 } uno_basis;
 
 const char* uno_last_error(void);
@@ -61,11 +63,12 @@ int uno_ctx_rebuild(uno_ctx* ctx, const uno_basis* basis);
 int uno_ctx_destroy(uno_ctx* ctx);
 /* Tuning knobs (no reference counterpart): "schwarz" (quartet screening threshold on
  * Qab*Qcd*max|D|, default 1e-13), "fp32_threshold" (quartets whose bound is below it
- * run in FP32 when "precision" is 1; default 1e-6), "precision" (0 = FP64, 1 = mixed),
+ * run in FP32 when "precision" is 1; default 1e-6), "precision" (0 = FP64, 1 = mixed, for the
+ * builds of uno_fock_rhf/uno_fock_uhf; uno_scf_run takes its own option),
  * "prim_cut" (bare primitive prefactor cut, default 1e-12 as TwoElectronInts.cpp:428). */
 int uno_ctx_set(uno_ctx* ctx, const char* key, double value);
 /* Reads the knobs above and, read-only: "last_quartets" (shell quartets evaluated by the
- * last Fock build), "pairs" (significant shell pairs), "pair_runs" (runs of the pair
+ * last Fock build), "last_quartets32" (of which in FP32), "pairs" (significant shell pairs), "pair_runs" (runs of the pair
  * array: one per angular pair class and primitive-count bucket, at most 36) and
  * "quartet_launches" (ERI launches per Fock build). */
 int uno_ctx_get(uno_ctx* ctx, const char* key, double* value);
@@ -91,8 +94,10 @@ typedef struct uno_scf_opts {
     double eps;          /* convergence threshold (raised to DBL_EPSILON*N^2/2 as Basis.hpp:249-254) */
     int cflag;           /* 0 energy, 1 density, 2 both (RHF.hpp:390-402); 3 |dE| < eps */
     int accel;           /* 0 reference damping, 2 DIIS (Pulay) */
-    int guess;           /* 0 core Hamiltonian, 1 caller's P (PMATRIX.DAT), 2 learned (MLP) */
-    int precision;       /* 0 FP64, 1 mixed FP32/FP64 with incremental builds */
+    int guess;           /* 0 core Hamiltonian, 1 caller's P (PMATRIX.DAT), 2 learned (the
+                            network set by uno_ctx_set_guess) */
+    int precision;       /* 0 FP64, 1 mixed: FP32 for quartets with Schwarz bound below
+                            "fp32_threshold" until |dE| < 1e-5, then FP64 */
     int fp64_final;      /* mixed: number of final iterations in full FP64 (default 3) */
     int incremental;     /* 1: build F from the density change (Delta-D) */
     int verbose;         /* 1: per-iteration lines on stderr like the reference */
@@ -106,6 +111,9 @@ typedef struct uno_scf_result {
     int iterations, converged, uhf, nfock;
     double wall_s, fock_s, diag_s;   /* seconds: whole SCF, Fock builds, eigen/transforms */
     double quartets;      /* shell quartets evaluated over all Fock builds */
+    double quartets32;    /* of which evaluated in FP32 (mixed precision) */
+    int nfull;            /* full (not incremental) Fock builds */
+    double guess_s;       /* seconds spent on the learned guess (guess = 2) */
 } uno_scf_result;
 
 void uno_scf_defaults(uno_scf_opts* opts);
@@ -122,6 +130,9 @@ int uno_scf_run(uno_ctx* ctx, int nelec, const uno_scf_opts* opts, uno_scf_resul
 int uno_eri_emit(uno_ctx* ctx, long cap, double* val, int* ijkl, long* count);
 /* Debug/parity: Rys roots u=t^2 and weights from the device tables (n<=5). */
 int uno_rys_eval(int n, int npts, const double* x, double* u, double* w);
+/* Debug/parity: the same from the single-precision tables of the FP32 quartets
+ * (x rounded to float; results widened to double). */
+int uno_rys_eval_f32(int n, int npts, const double* x, double* u, double* w);
 /* Debug/parity: Schwarz bounds sqrt(max|(ab|ab)|) per shell pair a>=b, packed order. */
 int uno_schwarz(uno_ctx* ctx, double* q);
 
@@ -130,7 +141,8 @@ int uno_schwarz(uno_ctx* ctx, double* q);
  * profiling mode in which every Fock build records, per quartet class (bra pair class
  * X, ket pair class Y, index 6*X+Y; pair classes ss ps pp ds dp dd), the kernel time
  * (HIP events, ms), launches, and the primitive and shell quartets evaluated.
- * enable=0 turns it off; stats are read with enable=-1.  Arrays hold 36 entries. */
+ * enable=0 turns it off; stats are read with enable=-1.  Arrays hold 72 entries: the
+ * 36 FP64 class pairs, then the same 36 for the FP32 launches of mixed builds. */
 int uno_profile(uno_ctx* ctx, int enable, double* ms, double* launches, double* prim_quartets,
                 double* shell_quartets);
 
@@ -155,6 +167,54 @@ int uno_deck_generate(const char* kind, int n, unsigned long long seed, const ch
  * writes short.gs.out, scfout.gs.out, PMATRIX.DAT, intsout.out, MOMINTS.DAT, RMOM.DAT,
  * moments.out, mol_dipmom.out there.  opts may be NULL (reference semantics). */
 int uno_run_deck(const char* dir, const uno_scf_opts* opts, uno_scf_result* res);
+
+/* ---- Learned initial density (no reference counterpart; the reference's guess is
+ * the core Hamiltonian, RHF.hpp:205-211).  A multilayer perceptron maps atom-pair
+ * descriptors to density blocks in the first atom's local frame; it is trained and
+ * evaluated on the GPU with BF16-input / FP32-accumulate matrix-core kernels.
+ * Pairs (A <= B) within 10 bohr; descriptors have 128 floats, blocks 256 (entry
+ * (mu, nu) of the atoms' Cartesian functions at 16 mu + nu); H, C, N, O, F with s, p, d
+ * shells and at most 16 functions per atom. */
+typedef struct uno_mlp uno_mlp;
+
+/* in_dim, width, out_dim: positive multiples of 64; hidden_layers SiLU layers of width
+ * width and a linear output layer.  Weights LeCun-normal from seed, biases zero. */
+int uno_mlp_create(int in_dim, int width, int hidden_layers, int out_dim, unsigned long long seed,
+                   uno_mlp** out);
+int uno_mlp_destroy(uno_mlp* m);
+int uno_mlp_nparams(const uno_mlp* m, long* n);
+/* ndims in: capacity of dims; out: number of layer widths (input, hidden..., output) */
+int uno_mlp_dims(const uno_mlp* m, int* ndims, int* dims);
+/* FP32 parameters, per layer W [in x out] row-major then b [out] */
+int uno_mlp_get_params(uno_mlp* m, float* params);
+int uno_mlp_set_params(uno_mlp* m, const float* params);
+/* Y [n x out] = network(X [n x in]), host arrays */
+int uno_mlp_forward(uno_mlp* m, long n, const float* X, float* Y);
+/* masked mean squared error sum M (Y - T)^2 / sum M over n <= 65536 rows and its
+ * gradient with respect to every parameter (same order as the parameters) */
+int uno_mlp_loss_grad(uno_mlp* m, long n, const float* X, const float* T, const float* M, float* loss,
+                      float* grad);
+/* Adam (beta 0.9/0.999) with a cosine-decayed rate from lr over `steps` minibatches of
+ * `batch` rows (a multiple of 64) drawn by epoch shuffles from seed; *loss = masked MSE
+ * of the final network over all n rows */
+int uno_mlp_train(uno_mlp* m, long n, const float* X, const float* T, const float* M, int steps, int batch,
+                  float lr, unsigned long long seed, float* loss);
+int uno_mlp_save(uno_mlp* m, const char* path);
+int uno_mlp_load(const char* path, uno_mlp** out);
+
+/* Descriptors [count x 128] and atom pairs [count x 2] of a basis (cap 0: count only). */
+int uno_guess_features(const uno_basis* basis, long cap, float* X, int* pairs, long* count);
+/* Training samples from a packed (half) density: descriptors, target blocks in the
+ * pair frames and their masks [count x 256] (cap 0: count only). */
+int uno_guess_samples(const uno_basis* basis, const double* P, long cap, float* X, float* Y, float* M,
+                      long* count);
+/* Packed density from blocks Y [count x 256] of the pairs [count x 2]. */
+int uno_guess_assemble(const uno_basis* basis, long count, const int* pairs, const float* Y, double* P);
+/* Network used by uno_scf_run with guess = 2 (not owned; NULL clears). */
+int uno_ctx_set_guess(uno_ctx* ctx, uno_mlp* model);
+/* Debug/parity: the (ncart x ncart) map of one shell's Cartesian functions under a
+ * rotation Q of the molecule (row-major 3x3; l <= 2): phi'_mu = sum_nu M_mu,nu phi_nu. */
+int uno_shell_rotation(int l, const double* Q, double* M);
 
 #ifdef __cplusplus
 }

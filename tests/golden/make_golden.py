@@ -16,6 +16,7 @@ Sources, all of them the reference's own data or its own code:
   * rys_mp.npz     Rys roots/weights and Boys F_m from 40-digit mpmath at probe points.
 
 usage: make -C oracle && make -C oracle ref && python tests/golden/make_golden.py
+       python tests/golden/make_golden.py --config-b    (the 16-water runs only, a few minutes)
 """
 import ctypes as C
 import os
@@ -41,6 +42,14 @@ GEN_DECKS = {   # tag -> (kind, n, seed, basis, charge)
     "w2_631gs": ("water_cluster", 2, 2001, "6-31G*", 0),
 }
 ERI_TAGS = ["A_water_631g", "water_631gs"]
+# the 16-water 6-31G* cluster (304 functions), RHF and its +1 UHF doublet, run for three
+# SCF iterations from the core-Hamiltonian guess: the reference's damped SCF does not
+# converge on this cluster within 300 iterations, so parity is taken on the trajectory
+B_DECKS = {
+    "B_w16_631gs": ("water_cluster", 16, 16001, "6-31G*", 0),
+    "B_w16_631gs_cation": ("water_cluster", 16, 16001, "6-31G*", 1),
+}
+B_MAXITS = 3
 
 
 def run_ref(deck_text, outdir, exe="Unomol"):
@@ -67,7 +76,21 @@ def harness_eris(lib, path):
     return v, f
 
 
+def make_config_b():
+    import unomol_amd
+    gdir = os.path.join(HERE, "decks")
+    os.makedirs(gdir, exist_ok=True)
+    for tag, (kind, n, seed, basis, charge) in B_DECKS.items():
+        path = os.path.join(gdir, tag + ".dat")
+        unomol_amd.generate_deck(path, kind, n, seed, basis, charge, 1e-10, B_MAXITS)
+        run_ref(open(path).read(), os.path.join(HERE, "ref_run_x2", tag), exe="Unomol_x2")
+        print("config B reference run done:", tag, flush=True)
+
+
 def main():
+    if "--config-b" in sys.argv:
+        make_config_b()
+        return
     import unomol_amd
     from unomol_amd.deck import read_deck
     # 1. reference test data

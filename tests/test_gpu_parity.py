@@ -224,3 +224,169 @@ def test_scf_diis_same_energy(tmp_path):
     assert hf.result.converged
     assert abs(e - e_ref) < 1e-9
     assert hf.result.iterations < 40
+
+
+def test_rys_fp32_tables_vs_mpmath(golden):
+    say("fp32 rys tables vs mpmath")
+    z = np.load(os.path.join(golden, "rys_mp.npz"))
+    for n in range(1, 6):
+        u, w = unomol_amd.rys_eval(n, z["x"], fp32=True)
+        np.testing.assert_allclose(u, z["u"][n, :, :n], rtol=2e-6)
+        np.testing.assert_allclose(w, z["w"][n, :, :n], rtol=2e-6)
+
+
+def test_fock_fp32_quartets_vs_fp64():
+    """FP32 quartet kernels against the FP64 ones (pinned to the oracle above): every
+    quartet in FP32 stays within single-precision error of the FP64 G, and the mixed
+    build (FP32 only below the Schwarz bound 1e-6) within 1e-9."""
+    say("fock fp32 / mixed vs fp64 w2_631gs")
+    path = R.deck_path("w2_631gs")
+    deck = read_deck(path)
+    n = deck["norb"]
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((n, n)) * 0.05
+    P = _pack(A + A.T + np.eye(n))
+    b, t = _tints(path)
+    G64 = np.zeros_like(P)
+    t.formGmatrix(P, G64)
+    assert t.get("last_quartets32") == 0
+    t.set("precision", 1)
+    t.set("fp32_threshold", 1e30)      # every quartet in FP32
+    Gall = np.zeros_like(P)
+    t.formGmatrix(P, Gall)
+    assert t.get("last_quartets32") == t.get("last_quartets")
+    assert np.abs(Gall - G64).max() < 2e-6 * np.abs(G64).max()
+    t.set("fp32_threshold", 1e-6)
+    Gmix = np.zeros_like(P)
+    t.formGmatrix(P, Gmix)
+    assert 0 < t.get("last_quartets32") < t.get("last_quartets")
+    assert np.abs(Gmix - G64).max() < 1e-9
+
+
+def test_scf_incremental_same_energy():
+    say("scf incremental builds w2_631gs")
+    path = R.deck_path("w2_631gs")
+    b, t = _tints(path)
+    e_full = unomol_amd.RestrictedHartreeFock(b, t).findEnergy(accel=2, eps=1e-11, cflag=3)
+    hf = unomol_amd.RestrictedHartreeFock(b, t)
+    e = hf.findEnergy(accel=2, eps=1e-11, cflag=3, incremental=1)
+    assert hf.result.converged
+    assert hf.result.nfull < hf.result.nfock
+    assert abs(e - e_full) < 1e-9
+
+
+def test_scf_mixed_precision_energy(tmp_path):
+    """Mixed precision + incremental builds on a 4-water 6-31G* cluster: the final energy
+    is within 1e-8 Eh of the plain FP64 SCF, with FP32 quartets used on the way."""
+    say("scf mixed precision 4-water 6-31G*")
+    deck = str(tmp_path / "w4.dat")
+    unomol_amd.generate_deck(deck, "water_cluster", 4, 4001, "6-31G*", 0, 1e-9, 100)
+    b = unomol_amd.Basis(deck)
+    t = unomol_amd.TwoElectronInts(b)
+    ref = unomol_amd.RestrictedHartreeFock(b, t)
+    e64 = ref.findEnergy(accel=2, eps=1e-9, cflag=3)
+    assert ref.result.converged
+    hf = unomol_amd.RestrictedHartreeFock(b, t)
+    e = hf.findEnergy(accel=2, eps=1e-9, cflag=3, precision=1, incremental=1)
+    r = hf.result
+    assert r.converged
+    assert r.quartets32 > 0
+    assert abs(e - e64) < 1e-8
+    ev64 = ref.evals.copy()
+    np.testing.assert_allclose(hf.evals, ev64, atol=1e-6)
+
+
+B_TAGS = [t for t in ("B_w16_631gs", "B_w16_631gs_cation")
+          if os.path.exists(os.path.join(R.GOLDEN, "ref_run_x2", t, "short.gs.out"))]
+
+
+@pytest.mark.parametrize("tag", B_TAGS)
+def test_config_b_trajectory_vs_reference(tmp_path, tag):
+    """16-water 6-31G* cluster (304 functions), RHF and the +1 UHF doublet: three SCF
+    iterations from the core-Hamiltonian guess with the reference's damping, against the
+    reference (exact 2-root quadrature) on the same deck."""
+    say("config B trajectory " + tag)
+    src = os.path.join(R.GOLDEN, "ref_run_x2", tag)
+    (tmp_path / "patin.dat").write_text(open(os.path.join(src, "patin.dat")).read())
+    unomol_amd.run_deck(tmp_path)
+    ref = R.read_short(os.path.join(src, "short.gs.out"))
+    got = R.read_short(str(tmp_path / "short.gs.out"))
+    assert abs(got["init"] - ref["init"]) < 1e-9
+    assert abs(got["energy"] - ref["energy"]) < 1e-9
+    assert abs(got["ediff"] - ref["ediff"]) < 1e-9
+    mref = R.read_scfout(os.path.join(src, "scfout.gs.out"))
+    mine = R.read_scfout(str(tmp_path / "scfout.gs.out"))
+    np.testing.assert_allclose(mine["evals"], mref["evals"], atol=1e-9)
+
+
+def _mlp_oracle():
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    import mlp_oracle
+    return mlp_oracle
+
+
+def test_mlp_forward_and_gradient_vs_oracle():
+    """BF16 matrix-core forward/backward of the guess network against the numpy oracle
+    with the same bf16 rounding points (float32 accumulation order is the only
+    difference)."""
+    say("mlp forward / gradient vs oracle")
+    MO = _mlp_oracle()
+    net = unomol_amd.MLP(128, 128, 3, 256, seed=1)
+    rng = np.random.default_rng(8)
+    X = rng.standard_normal((200, 128)).astype(np.float32)
+    T = (0.1 * rng.standard_normal((200, 256))).astype(np.float32)
+    M = (rng.uniform(size=(200, 256)) > 0.4).astype(np.float32)
+    p = net.get_params()
+    Y = net.forward(X)
+    Yo = MO.forward(p, net.dims, X)[0]
+    assert np.abs(Y - Yo).max() < 2e-2 * np.abs(Yo).max()
+    assert np.linalg.norm(Y - Yo) < 2e-3 * np.linalg.norm(Yo)
+    loss, g = net.loss_grad(X, T, M)
+    lo, go = MO.loss_grad(p, net.dims, X, T, M)
+    assert abs(loss - lo) < 1e-3 * abs(lo)
+    assert np.linalg.norm(g - go) < 1e-2 * np.linalg.norm(go)
+    o = 0
+    for l in range(len(net.dims) - 1):
+        nw = net.dims[l] * net.dims[l + 1]
+        gw, gwo = g[o:o + nw], go[o:o + nw]
+        assert np.linalg.norm(gw - gwo) < 2e-2 * np.linalg.norm(gwo)
+        o += nw + net.dims[l + 1]
+
+
+def test_mlp_training_reduces_loss():
+    say("mlp training on a smooth synthetic map")
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((4096, 128)).astype(np.float32)
+    R = rng.standard_normal((128, 256)).astype(np.float32) / np.sqrt(128)
+    T = np.tanh(X @ R).astype(np.float32)
+    M = np.ones_like(T)
+    net = unomol_amd.MLP(128, 128, 3, 256, seed=2)
+    l0, _ = net.loss_grad(X[:1024], T[:1024], M[:1024])
+    loss = net.train(X, T, M, steps=400, batch=512, lr=3e-3, seed=3)
+    assert loss < 0.25 * l0
+
+
+def test_learned_guess_fewer_iterations(tmp_path):
+    """A network trained on densities of small clusters (2-6 waters) converges a held-out
+    8-water cluster to the core-Hamiltonian result (1e-8 Eh) in fewer iterations."""
+    say("learned guess: training set")
+    from unomol_amd import guess_train as GT
+    X, Y, M = GT.make_training_set(n_clusters=24, min_w=2, max_w=6, seed=500)
+    say("learned guess: training on %d samples" % len(X))
+    net, loss = GT.train_guess(X, Y, M, steps=800, batch=512)
+    deck = str(tmp_path / "w8.dat")
+    unomol_amd.generate_deck(deck, "water_cluster", 8, 8123, "6-31G*", 0, 1e-9, 100)
+    b = unomol_amd.Basis(deck)
+    t = unomol_amd.TwoElectronInts(b)
+    ref = unomol_amd.RestrictedHartreeFock(b, t)
+    e0 = ref.findEnergy(accel=2, eps=1e-9, cflag=3)
+    unomol_amd.set_guess_model(t, net)
+    hf = unomol_amd.RestrictedHartreeFock(b, t)
+    e1 = hf.findEnergy(accel=2, eps=1e-9, cflag=3, guess=2)
+    say("learned guess: %d vs %d iterations (core-H), loss %.2e" % (hf.result.iterations, ref.result.iterations,
+                                                                   loss))
+    assert ref.result.converged and hf.result.converged
+    assert abs(e1 - e0) < 1e-8
+    assert hf.result.iterations < ref.result.iterations
+    # the guess itself is far closer than the core Hamiltonian's
+    assert abs(hf.result.init_energy - e0) < 0.2 * abs(ref.result.init_energy - e0)

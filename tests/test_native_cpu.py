@@ -79,3 +79,163 @@ def test_deck_errors(tmp_path):
         unomol_amd.run_deck(tmp_path)   # no patin.dat there
     with pytest.raises(N.UnoError):
         unomol_amd.generate_deck(tmp_path / "x.dat", "molecule", 0, 1, "cc-pVDZ", 0)
+
+
+def _read_inc_array(text, name, dtype):
+    body = re.search(r"%s\[\d+\] = \{(.*?)\};" % name, text, re.S).group(1)
+    vals = []
+    for line in body.split("\n"):
+        line = line.strip()
+        if line and not line.startswith("//"):
+            vals += [float(v.rstrip("f")) for v in line.split(",") if v.strip()]
+    return np.array(vals, dtype=dtype)
+
+
+def test_fp32_rys_table_vs_mpmath(golden):
+    """The single-precision Rys table of the FP32 quartets, evaluated in float32 as the
+    kernels do (Clenshaw on the degree-6 series of each unit interval, Gauss-Hermite
+    limit beyond), against 40-digit mpmath roots and weights."""
+    src = os.path.join(ROOT, "unomol_amd", "csrc")
+    t64 = open(os.path.join(src, "rys_table.inc")).read()
+    t32 = open(os.path.join(src, "rys_table_f32.inc")).read()
+    xasym = [int(v) for v in re.search(r"uno_rys_xasym\[\d+\] = \{(.*?)\}", t64).group(1).split(",")]
+    off = [int(v) for v in re.search(r"uno_rys_off_f\[\d+\] = \{(.*?)\}", t32).group(1).split(",")]
+    nc = int(re.search(r"#define UNO_RYS_NCOEF_F (\d+)", t32).group(1))
+    coef = _read_inc_array(t32, "uno_rys_coef_f", np.float32)
+    au = _read_inc_array(t32, "uno_rys_asym_u_f", np.float32)
+    aw = _read_inc_array(t32, "uno_rys_asym_w_f", np.float32)
+    z = np.load(os.path.join(golden, "rys_mp.npz"))
+    f32 = np.float32
+    for n in range(1, 6):
+        for i, x64 in enumerate(z["x"]):
+            x = f32(x64)
+            if x >= xasym[n]:
+                o = n * (n - 1) // 2
+                u = au[o:o + n] / x
+                w = aw[o:o + n] / np.sqrt(x)
+            else:
+                k = int(x)
+                s = f32(2) * (x - f32(k)) - f32(1)
+                vals = []
+                for fi in range(2 * n):
+                    c = coef[off[n] + (k * 2 * n + fi) * nc: off[n] + (k * 2 * n + fi + 1) * nc]
+                    b1, b2 = f32(0), f32(0)
+                    for m in range(nc - 1, 0, -1):
+                        b1, b2 = f32(2) * s * b1 + (c[m] - b2), b1
+                    vals.append(s * b1 + (c[0] - b2))
+                u, w = np.array(vals[:n]), np.array(vals[n:])
+            # float32 evaluation: a few ulp on top of the 1.7e-8 truncation bound; the
+            # mpmath reference is taken at x rounded to float like the argument
+            np.testing.assert_allclose(u, z["u"][n, i, :n], rtol=2e-6 if x64 != float(x) else 1e-6)
+            np.testing.assert_allclose(w, z["w"][n, i, :n], rtol=2e-6 if x64 != float(x) else 1e-6)
+
+
+def _rot(seed):
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    if np.linalg.det(Q) < 0:
+        Q[:, 0] = -Q[:, 0]
+    return Q
+
+
+def _rotated_deck(d, Q, shift=(0.0, 0.0, 0.0)):
+    r = dict(d)
+    r["xyz"] = (d["xyz"].reshape(-1, 3) @ Q.T + np.asarray(shift)).copy()
+    return r
+
+
+def _block_rotation(d, Q):
+    """Block-diagonal map of all basis functions (shell order) under rotation Q."""
+    n = d["norb"]
+    M = np.zeros((n, n))
+    o = 0
+    for l in d["sh_l"]:
+        nc = (l + 1) * (l + 2) // 2
+        M[o:o + nc, o:o + nc] = unomol_amd.shell_rotation(int(l), Q)
+        o += nc
+    return M
+
+
+def _dense(p, n):
+    A = np.zeros((n, n))
+    A[np.tril_indices(n)] = p
+    return A + np.tril(A, -1).T
+
+
+def test_shell_rotation_is_a_representation():
+    Q1, Q2 = _rot(1), _rot(2)
+    for l in range(3):
+        M1, M2 = unomol_amd.shell_rotation(l, Q1), unomol_amd.shell_rotation(l, Q2)
+        nc = M1.shape[0]
+        np.testing.assert_allclose(M1 @ unomol_amd.shell_rotation(l, Q1.T), np.eye(nc), atol=1e-13)
+        # one of the two composition orders holds for a representation of SO(3)
+        M12 = unomol_amd.shell_rotation(l, Q1 @ Q2)
+        assert min(np.abs(M12 - M1 @ M2).max(), np.abs(M12 - M2 @ M1).max()) < 1e-13
+    np.testing.assert_allclose(unomol_amd.shell_rotation(1, Q1), Q1.T, atol=1e-15)
+
+
+@pytest.mark.parametrize("deck_tag", ["water_631gs", "631.nh3"])
+def test_shell_rotation_maps_overlap(golden, deck_tag):
+    """Overlap of a rotated molecule: S = M S' M^T with M from shell_rotation (pins the
+    Cartesian d normalisation and component order against the one-electron code)."""
+    path = os.path.join(golden, "decks", deck_tag + ".dat")
+    if not os.path.exists(path):
+        path = os.path.join(golden, "ref_run", deck_tag, "patin.dat")
+    d = read_deck(path)
+    Q = _rot(3)
+    n = d["norb"]
+    S = _dense(unomol_amd.oneint(d)[0], n)
+    Sr = _dense(unomol_amd.oneint(_rotated_deck(d, Q, (0.3, -1.1, 2.0)))[0], n)
+    M = _block_rotation(d, Q)
+    np.testing.assert_allclose(S, M @ Sr @ M.T, atol=1e-12)
+
+
+def _w4(tmp_path):
+    p = tmp_path / "w4.dat"
+    unomol_amd.generate_deck(p, "water_cluster", 4, 77, "6-31G*", 0)
+    return read_deck(p)
+
+
+def test_guess_samples_assemble_round_trip(tmp_path):
+    """Blocks taken from a density in the pair frames and assembled back give the density
+    on every atom pair within the cutoff (and zero beyond it)."""
+    d = _w4(tmp_path)
+    n = d["norb"]
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((n, n))
+    P = (A + A.T)[np.tril_indices(n)]
+    X, Y, M = unomol_amd.guess_samples(d, P)
+    Xf, pairs = unomol_amd.guess_features(d)
+    np.testing.assert_array_equal(X, Xf)
+    assert X.shape == (len(pairs), 128) and Y.shape == (len(pairs), 256)
+    Pb = unomol_amd.guess_assemble(d, pairs, Y)
+    Dp, Db = _dense(P, n), _dense(Pb, n)
+    # function -> atom
+    atom = np.concatenate([[c] * ((l + 1) * (l + 2) // 2) for l, c in zip(d["sh_l"], d["sh_cen"])])
+    near = np.zeros((n, n), dtype=bool)
+    for a, b in pairs:
+        near[np.ix_(atom == a, atom == b)] = True
+        near[np.ix_(atom == b, atom == a)] = True
+    assert np.abs(Db - Dp)[near].max() < 1e-5 * np.abs(Dp).max()
+    assert np.all(Db[~near] == 0.0)
+    # every mask row covers exactly n_A x n_B entries
+    nf = np.bincount(atom, minlength=d["ncen"])
+    np.testing.assert_array_equal(M.sum(axis=1), nf[pairs[:, 0]] * nf[pairs[:, 1]])
+
+
+def test_guess_descriptors_and_blocks_rotation_invariant(tmp_path):
+    """Rotating and translating the cluster leaves the descriptors and the frame blocks of
+    the correspondingly rotated density unchanged: the learned guess is covariant."""
+    d = _w4(tmp_path)
+    n = d["norb"]
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((n, n))
+    D = A + A.T
+    Q = _rot(6)
+    dr = _rotated_deck(d, Q, (1.5, 0.2, -0.7))
+    M = _block_rotation(d, Q)
+    Dr = M.T @ D @ M
+    X, Y, _ = unomol_amd.guess_samples(d, D[np.tril_indices(n)])
+    Xr, Yr, _ = unomol_amd.guess_samples(dr, Dr[np.tril_indices(n)])
+    np.testing.assert_allclose(Xr, X, atol=2e-5)
+    np.testing.assert_allclose(Yr, Y, atol=2e-5 * np.abs(Y).max())

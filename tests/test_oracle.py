@@ -2,6 +2,7 @@
 to what the mathematics fixes.  No GPU is used."""
 import math
 import os
+import sys
 
 import numpy as np
 import pytest
@@ -9,6 +10,8 @@ import pytest
 import oracle_ffi as O
 import refdata as R
 from unomol_amd.deck import read_deck
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 @pytest.fixture(scope="module")
@@ -204,3 +207,35 @@ def test_oracle_scf_degenerate_start(tag):
     short = R.read_short(os.path.join(R.GOLDEN, "ref_run_x2", tag, "short.gs.out"))
     assert out.iterations < d["maxits"]
     assert out.energy > short["energy"] - 1e-9
+
+
+def test_mlp_oracle_gradient_finite_differences():
+    """The MLP oracle's backward pass (float64 mode) against central differences of its
+    own loss: pins the formulas the GPU gradient test relies on."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mlp_oracle as MO
+    rng = np.random.default_rng(0)
+    dims = [6, 5, 5, 4]
+    npar = sum(dims[l] * dims[l + 1] + dims[l + 1] for l in range(len(dims) - 1))
+    p = rng.standard_normal(npar) * 0.5
+    X = rng.standard_normal((7, 6))
+    T = rng.standard_normal((7, 4))
+    M = (rng.uniform(size=(7, 4)) > 0.3).astype(np.float64)
+    _, g = MO.loss_grad(p, dims, X, T, M, bf16=False)
+    h = 1e-6
+    for k in range(0, npar, 3):
+        e = np.zeros(npar)
+        e[k] = h
+        lp, _ = MO.loss_grad(p + e, dims, X, T, M, bf16=False)
+        lm, _ = MO.loss_grad(p - e, dims, X, T, M, bf16=False)
+        assert abs((lp - lm) / (2 * h) - g[k]) < 1e-7 * max(1.0, abs(g[k]))
+
+
+def test_bf16_rounding_ties_to_even():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mlp_oracle as MO
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -9, -2.5, 3.0e-3], dtype=np.float32)
+    r = MO.bf16_round(x)
+    np.testing.assert_array_equal(r[:4], np.array([1.0, 1.0, 1.0 + 2 ** -6, 1.0], dtype=np.float32))
+    assert r[4] == np.float32(-2.5)
+    assert abs(r[5] - 3.0e-3) <= 3.0e-3 * 2 ** -8

@@ -25,7 +25,7 @@ class Basis(C.Structure):
     _fields_ = [("nshell", C.c_int), ("ncen", C.c_int),
                 ("sh_l", C.c_void_p), ("sh_npr", C.c_void_p), ("sh_cen", C.c_void_p),
                 ("sh_poff", C.c_void_p), ("alpha", C.c_void_p), ("coef", C.c_void_p),
-                ("xyz", C.c_void_p), ("charge", C.c_void_p)]
+                ("xyz", C.c_void_p), ("charge", C.c_void_p), ("normalized", C.c_int)]
 
 
 class ScfOpts(C.Structure):
@@ -39,7 +39,8 @@ class ScfResult(C.Structure):
                 ("pdiff", C.c_double), ("nucrep", C.c_double), ("kinetic", C.c_double),
                 ("electronic", C.c_double), ("iterations", C.c_int), ("converged", C.c_int),
                 ("uhf", C.c_int), ("nfock", C.c_int), ("wall_s", C.c_double), ("fock_s", C.c_double),
-                ("diag_s", C.c_double), ("quartets", C.c_double)]
+                ("diag_s", C.c_double), ("quartets", C.c_double), ("quartets32", C.c_double),
+                ("nfull", C.c_int), ("guess_s", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -50,8 +51,12 @@ _lib = None
 # every symbol include/unomol.h declares
 EXPORTS = ["uno_last_error", "uno_ctx_create", "uno_ctx_rebuild", "uno_ctx_destroy", "uno_ctx_set",
            "uno_ctx_get", "uno_ctx_norb", "uno_fock_rhf", "uno_fock_uhf", "uno_eigh_packed",
-           "uno_sp_trans", "uno_scf_defaults", "uno_scf_run", "uno_eri_emit", "uno_rys_eval",
-           "uno_schwarz", "uno_profile", "uno_copy_probe", "uno_oneint", "uno_moment_ints", "uno_deck_generate", "uno_run_deck"]
+           "uno_sp_trans", "uno_scf_defaults", "uno_scf_run", "uno_eri_emit", "uno_rys_eval", "uno_rys_eval_f32",
+           "uno_schwarz", "uno_profile", "uno_copy_probe", "uno_oneint", "uno_moment_ints", "uno_deck_generate", "uno_run_deck",
+           "uno_mlp_create", "uno_mlp_destroy", "uno_mlp_nparams", "uno_mlp_dims", "uno_mlp_get_params",
+           "uno_mlp_set_params", "uno_mlp_forward", "uno_mlp_loss_grad", "uno_mlp_train", "uno_mlp_save",
+           "uno_mlp_load", "uno_guess_features", "uno_guess_samples", "uno_guess_assemble", "uno_ctx_set_guess",
+           "uno_shell_rotation"]
 
 
 def lib():

@@ -189,11 +189,12 @@ def generate_deck(path, kind="water_cluster", n=1, seed=0, basis="6-31G*", charg
                                       C.c_double(eps), maxits, os.fspath(path).encode()))
 
 
-def rys_eval(n, x):
+def rys_eval(n, x, fp32=False):
     x = np.ascontiguousarray(x, dtype=np.float64)
     u = np.zeros((len(x), n))
     w = np.zeros((len(x), n))
-    N.check(N.lib().uno_rys_eval(n, len(x), N.ptr(x), N.ptr(u), N.ptr(w)))
+    f = N.lib().uno_rys_eval_f32 if fp32 else N.lib().uno_rys_eval
+    N.check(f(n, len(x), N.ptr(x), N.ptr(u), N.ptr(w)))
     return u, w
 
 
@@ -212,3 +213,117 @@ def moment_ints(deck):
     m = np.zeros(9 * n * (n + 1) // 2)
     N.check(N.lib().uno_moment_ints(C.byref(b), N.ptr(m)))
     return m.reshape(-1, 9)
+
+
+# ---- learned density guess -------------------------------------------------------------
+GUESS_IN, GUESS_OUT = 128, 256
+
+
+def shell_rotation(l, Q):
+    """(ncart x ncart) map of a shell's Cartesian functions under a rotation Q."""
+    nc = (l + 1) * (l + 2) // 2
+    M = np.zeros((nc, nc))
+    N.check(N.lib().uno_shell_rotation(l, N.ptr(np.ascontiguousarray(Q, dtype=np.float64)), N.ptr(M)))
+    return M
+
+
+def guess_features(deck):
+    """Descriptors [count x 128] and atom pairs [count x 2] of the pairs within 10 bohr."""
+    b = N.basis_struct(deck)
+    cnt = C.c_long()
+    N.check(N.lib().uno_guess_features(C.byref(b), C.c_long(0), None, None, C.byref(cnt)))
+    X = np.zeros((cnt.value, GUESS_IN), dtype=np.float32)
+    pairs = np.zeros((cnt.value, 2), dtype=np.int32)
+    N.check(N.lib().uno_guess_features(C.byref(b), C.c_long(cnt.value), N.ptr(X), N.ptr(pairs), C.byref(cnt)))
+    return X, pairs
+
+
+def guess_samples(deck, P):
+    """Training samples from a packed (half) density: descriptors, blocks, masks."""
+    b = N.basis_struct(deck)
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    cnt = C.c_long()
+    N.check(N.lib().uno_guess_samples(C.byref(b), N.ptr(P), C.c_long(0), None, None, None, C.byref(cnt)))
+    n = cnt.value
+    X = np.zeros((n, GUESS_IN), dtype=np.float32)
+    Y = np.zeros((n, GUESS_OUT), dtype=np.float32)
+    M = np.zeros((n, GUESS_OUT), dtype=np.float32)
+    N.check(N.lib().uno_guess_samples(C.byref(b), N.ptr(P), C.c_long(n), N.ptr(X), N.ptr(Y), N.ptr(M),
+                                      C.byref(cnt)))
+    return X, Y, M
+
+
+def guess_assemble(deck, pairs, Y):
+    """Packed density from predicted blocks of the given atom pairs."""
+    b = N.basis_struct(deck)
+    n = deck["norb"]
+    P = np.zeros(n * (n + 1) // 2)
+    pairs = np.ascontiguousarray(pairs, dtype=np.int32)
+    Y = np.ascontiguousarray(Y, dtype=np.float32)
+    N.check(N.lib().uno_guess_assemble(C.byref(b), C.c_long(len(pairs)), N.ptr(pairs), N.ptr(Y), N.ptr(P)))
+    return P
+
+
+class MLP:
+    """Density-guess network on the GPU (BF16 matrix-core training and inference)."""
+
+    def __init__(self, in_dim=GUESS_IN, width=128, hidden_layers=3, out_dim=GUESS_OUT, seed=0, path=None):
+        self.h = C.c_void_p()
+        if path is not None:
+            N.check(N.lib().uno_mlp_load(os.fspath(path).encode(), C.byref(self.h)))
+        else:
+            N.check(N.lib().uno_mlp_create(in_dim, width, hidden_layers, out_dim, C.c_ulonglong(seed),
+                                           C.byref(self.h)))
+        nd = C.c_int(16)
+        dims = (C.c_int * 16)()
+        N.check(N.lib().uno_mlp_dims(self.h, C.byref(nd), dims))
+        self.dims = [dims[i] for i in range(nd.value)]
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            N.lib().uno_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def nparams(self):
+        n = C.c_long()
+        N.check(N.lib().uno_mlp_nparams(self.h, C.byref(n)))
+        return n.value
+
+    def get_params(self):
+        p = np.zeros(self.nparams(), dtype=np.float32)
+        N.check(N.lib().uno_mlp_get_params(self.h, N.ptr(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        N.check(N.lib().uno_mlp_set_params(self.h, N.ptr(p)))
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((len(X), self.dims[-1]), dtype=np.float32)
+        N.check(N.lib().uno_mlp_forward(self.h, C.c_long(len(X)), N.ptr(X), N.ptr(Y)))
+        return Y
+
+    def loss_grad(self, X, T, M):
+        X, T, M = [np.ascontiguousarray(a, dtype=np.float32) for a in (X, T, M)]
+        loss = C.c_float()
+        g = np.zeros(self.nparams(), dtype=np.float32)
+        N.check(N.lib().uno_mlp_loss_grad(self.h, C.c_long(len(X)), N.ptr(X), N.ptr(T), N.ptr(M), C.byref(loss),
+                                          N.ptr(g)))
+        return loss.value, g
+
+    def train(self, X, T, M, steps=2000, batch=1024, lr=2e-3, seed=0):
+        X, T, M = [np.ascontiguousarray(a, dtype=np.float32) for a in (X, T, M)]
+        loss = C.c_float()
+        N.check(N.lib().uno_mlp_train(self.h, C.c_long(len(X)), N.ptr(X), N.ptr(T), N.ptr(M), steps, batch,
+                                      C.c_float(lr), C.c_ulonglong(seed), C.byref(loss)))
+        return loss.value
+
+    def save(self, path):
+        N.check(N.lib().uno_mlp_save(self.h, os.fspath(path).encode()))
+
+
+def set_guess_model(tints, mlp):
+    """Network used by findEnergy(guess=2) on this context (kept alive by the caller)."""
+    N.check(N.lib().uno_ctx_set_guess(tints.ctx, mlp.h if mlp is not None else None))
+    tints._guess_model = mlp

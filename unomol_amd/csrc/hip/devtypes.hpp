@@ -51,6 +51,8 @@ struct EriArgs {
     const double* pairQ;      // Schwarz bound of every pair, in pair-array order
     const long long* prefix;
     const int* blk;
+    const int* kstart;        // first ket of every bra's range (null: 0), FP32 ranges
+                              // start after the bra's FP64 kets
     int qpb;                  // quartets per block of the launch
     int nbra;
     long long nquart;

@@ -11,44 +11,43 @@ namespace uno {
 // shared digestion: the contracted block e[c] (already normalised and scaled by the
 // permutational factor) is contracted with the six density blocks it touches.
 // ---------------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD, int MODE>
-__device__ __forceinline__ void digest_regs(const EriArgs& a, const PairHdr& P1, const PairHdr& P2,
-                                            const double* e) {
+template <int LA, int LB, int LC, int LD, int MODE, typename T>
+__device__ __forceinline__ void digest_regs(const EriArgs& a, const PairHdr& P1, const PairHdr& P2, const T* e) {
     using CI = ClassInfo<LA, LB, LC, LD>;
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND;
     const int n = a.n;
     const int fa = P1.f1, fb = P1.f2, fc = P2.f1, fd = P2.f2;
-    constexpr double jf = (MODE == kDigestRHF) ? 4.0 : 2.0;
+    constexpr T jf = (MODE == kDigestRHF) ? 4 : 2;
     const double* DJ = a.D0;
     // Coulomb: W_ab += jf sum_cd e D_cd ; W_cd += jf sum_ab e D_ab
     {
-        double dcd[NC * ND], dab[NA * NB];
+        T dcd[NC * ND], dab[NA * NB];
 #pragma unroll
         for (int k = 0; k < NC; ++k)
 #pragma unroll
-            for (int l = 0; l < ND; ++l) dcd[k * ND + l] = DJ[(long)(fc + k) * n + fd + l];
+            for (int l = 0; l < ND; ++l) dcd[k * ND + l] = (T)DJ[(long)(fc + k) * n + fd + l];
 #pragma unroll
         for (int i = 0; i < NA; ++i)
 #pragma unroll
-            for (int j = 0; j < NB; ++j) dab[i * NB + j] = DJ[(long)(fa + i) * n + fb + j];
+            for (int j = 0; j < NB; ++j) dab[i * NB + j] = (T)DJ[(long)(fa + i) * n + fb + j];
 #pragma unroll
         for (int i = 0; i < NA; ++i)
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
-                double s = 0.0;
+                T s = 0;
 #pragma unroll
                 for (int kl = 0; kl < NC * ND; ++kl) s = fma(e[(i * NB + j) * NC * ND + kl], dcd[kl], s);
-                atomic_add_f64(a.W0 + (long)(fa + i) * n + fb + j, jf * s);
-                if (MODE == kDigestUHF) atomic_add_f64(a.W1 + (long)(fa + i) * n + fb + j, jf * s);
+                atomic_add_f64(a.W0 + (long)(fa + i) * n + fb + j, (double)(jf * s));
+                if (MODE == kDigestUHF) atomic_add_f64(a.W1 + (long)(fa + i) * n + fb + j, (double)(jf * s));
             }
 #pragma unroll
         for (int kl = 0; kl < NC * ND; ++kl) {
-            double s = 0.0;
+            T s = 0;
 #pragma unroll
             for (int ij = 0; ij < NA * NB; ++ij) s = fma(e[ij * NC * ND + kl], dab[ij], s);
             const int k = kl / ND, l = kl % ND;
-            atomic_add_f64(a.W0 + (long)(fc + k) * n + fd + l, jf * s);
-            if (MODE == kDigestUHF) atomic_add_f64(a.W1 + (long)(fc + k) * n + fd + l, jf * s);
+            atomic_add_f64(a.W0 + (long)(fc + k) * n + fd + l, (double)(jf * s));
+            if (MODE == kDigestUHF) atomic_add_f64(a.W1 + (long)(fc + k) * n + fd + l, (double)(jf * s));
         }
     }
     // exchange, once per spin density: W_ac -= sum_bd e D_bd, W_ad -= sum_bc e D_bc,
@@ -57,64 +56,64 @@ __device__ __forceinline__ void digest_regs(const EriArgs& a, const PairHdr& P1,
     for (int sp = 0; sp < (MODE == kDigestUHF ? 2 : 1); ++sp) {
         const double* DK = (MODE == kDigestUHF) ? (sp == 0 ? a.D1 : a.D2) : a.D0;
         double* W = (sp == 0) ? a.W0 : a.W1;
-        double dbd[NB * ND], dbc[NB * NC], dad[NA * ND], dac[NA * NC];
+        T dbd[NB * ND], dbc[NB * NC], dad[NA * ND], dac[NA * NC];
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
 #pragma unroll
-            for (int l = 0; l < ND; ++l) dbd[j * ND + l] = DK[(long)(fb + j) * n + fd + l];
+            for (int l = 0; l < ND; ++l) dbd[j * ND + l] = (T)DK[(long)(fb + j) * n + fd + l];
 #pragma unroll
-            for (int k = 0; k < NC; ++k) dbc[j * NC + k] = DK[(long)(fb + j) * n + fc + k];
+            for (int k = 0; k < NC; ++k) dbc[j * NC + k] = (T)DK[(long)(fb + j) * n + fc + k];
         }
 #pragma unroll
         for (int i = 0; i < NA; ++i) {
 #pragma unroll
-            for (int l = 0; l < ND; ++l) dad[i * ND + l] = DK[(long)(fa + i) * n + fd + l];
+            for (int l = 0; l < ND; ++l) dad[i * ND + l] = (T)DK[(long)(fa + i) * n + fd + l];
 #pragma unroll
-            for (int k = 0; k < NC; ++k) dac[i * NC + k] = DK[(long)(fa + i) * n + fc + k];
+            for (int k = 0; k < NC; ++k) dac[i * NC + k] = (T)DK[(long)(fa + i) * n + fc + k];
         }
 #pragma unroll
         for (int i = 0; i < NA; ++i)
 #pragma unroll
             for (int k = 0; k < NC; ++k) {
-                double s = 0.0;
+                T s = 0;
 #pragma unroll
                 for (int j = 0; j < NB; ++j)
 #pragma unroll
                     for (int l = 0; l < ND; ++l) s = fma(e[((i * NB + j) * NC + k) * ND + l], dbd[j * ND + l], s);
-                atomic_add_f64(W + (long)(fa + i) * n + fc + k, -s);
+                atomic_add_f64(W + (long)(fa + i) * n + fc + k, (double)(-s));
             }
 #pragma unroll
         for (int i = 0; i < NA; ++i)
 #pragma unroll
             for (int l = 0; l < ND; ++l) {
-                double s = 0.0;
+                T s = 0;
 #pragma unroll
                 for (int j = 0; j < NB; ++j)
 #pragma unroll
                     for (int k = 0; k < NC; ++k) s = fma(e[((i * NB + j) * NC + k) * ND + l], dbc[j * NC + k], s);
-                atomic_add_f64(W + (long)(fa + i) * n + fd + l, -s);
+                atomic_add_f64(W + (long)(fa + i) * n + fd + l, (double)(-s));
             }
 #pragma unroll
         for (int j = 0; j < NB; ++j)
 #pragma unroll
             for (int k = 0; k < NC; ++k) {
-                double s = 0.0;
+                T s = 0;
 #pragma unroll
                 for (int i = 0; i < NA; ++i)
 #pragma unroll
                     for (int l = 0; l < ND; ++l) s = fma(e[((i * NB + j) * NC + k) * ND + l], dad[i * ND + l], s);
-                atomic_add_f64(W + (long)(fb + j) * n + fc + k, -s);
+                atomic_add_f64(W + (long)(fb + j) * n + fc + k, (double)(-s));
             }
 #pragma unroll
         for (int j = 0; j < NB; ++j)
 #pragma unroll
             for (int l = 0; l < ND; ++l) {
-                double s = 0.0;
+                T s = 0;
 #pragma unroll
                 for (int i = 0; i < NA; ++i)
 #pragma unroll
                     for (int k = 0; k < NC; ++k) s = fma(e[((i * NB + j) * NC + k) * ND + l], dac[i * NC + k], s);
-                atomic_add_f64(W + (long)(fb + j) * n + fd + l, -s);
+                atomic_add_f64(W + (long)(fb + j) * n + fd + l, (double)(-s));
             }
     }
 }
@@ -122,7 +121,7 @@ __device__ __forceinline__ void digest_regs(const EriArgs& a, const PairHdr& P1,
 // ---------------------------------------------------------------------------
 // one lane per shell quartet
 // ---------------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD, int MODE>
+template <int LA, int LB, int LC, int LD, int MODE, typename T>
 __global__ __launch_bounds__(256) void k_eri_lane(const EriArgs a) {
     using CI = ClassInfo<LA, LB, LC, LD>;
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND, NCOMP = CI::NCOMP, NR = CI::NR;
@@ -133,39 +132,44 @@ __global__ __launch_bounds__(256) void k_eri_lane(const EriArgs a) {
     const PairHdr P1 = a.pairs[pab];
     const PairHdr P2 = a.pairs[pcd];
     if (MODE <= kDigestUHF && q12 >= 0.0 && screened(a, P1, P2, q12)) return;
-    double e[NCOMP];
+    T e[NCOMP];
 #pragma unroll
-    for (int c = 0; c < NCOMP; ++c) e[c] = 0.0;
+    for (int c = 0; c < NCOMP; ++c) e[c] = 0;
     unsigned long long nprimq = 0;
+    const T ab[3] = {(T)P1.AB[0], (T)P1.AB[1], (T)P1.AB[2]};
+    const T cd[3] = {(T)P2.AB[0], (T)P2.AB[1], (T)P2.AB[2]};
     for (int ip = 0; ip < P1.nprim; ++ip) {
         const PrimPair pp = a.prims[P1.prim0 + ip];
-        const double PA[3] = {pp.P[0] - P1.A[0], pp.P[1] - P1.A[1], pp.P[2] - P1.A[2]};
+        // coordinate differences in double, then rounded to the compute type
+        const T PA[3] = {(T)(pp.P[0] - P1.A[0]), (T)(pp.P[1] - P1.A[1]), (T)(pp.P[2] - P1.A[2])};
+        const T p = (T)pp.p, ip_ = (T)pp.ip, sp = (T)pp.s, Kp = (T)pp.K;
         for (int iq = 0; iq < P2.nprim; ++iq) {
             const PrimPair qq = a.prims[P2.prim0 + iq];
-            const double t = pp.p + qq.p, it = 1.0 / t, rt = sqrt(it);
-            const double bare = kTwoPi25 * pp.s * qq.s * pp.ip * qq.ip * rt;
-            if (bare < a.primcut) continue;
+            const T q = (T)qq.p, iq_ = (T)qq.ip;
+            const T it = (T)1 / (p + q), rt = sqrt(it);
+            const T bare = (T)kTwoPi25 * sp * (T)qq.s * ip_ * iq_ * rt;
+            if (bare < (T)a.primcut) continue;
             ++nprimq;
-            const double pref = kTwoPi25 * pp.K * qq.K * pp.ip * qq.ip * rt;
-            const double PQ[3] = {pp.P[0] - qq.P[0], pp.P[1] - qq.P[1], pp.P[2] - qq.P[2]};
-            const double QC[3] = {qq.P[0] - P2.A[0], qq.P[1] - P2.A[1], qq.P[2] - P2.A[2]};
-            const double x = pp.p * qq.p * it * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-            double u[NR], w[NR];
-            rys_roots<NR>(x, u, w);
+            const T pref = (T)kTwoPi25 * Kp * (T)qq.K * ip_ * iq_ * rt;
+            const T PQ[3] = {(T)(pp.P[0] - qq.P[0]), (T)(pp.P[1] - qq.P[1]), (T)(pp.P[2] - qq.P[2])};
+            const T QC[3] = {(T)(qq.P[0] - P2.A[0]), (T)(qq.P[1] - P2.A[1]), (T)(qq.P[2] - P2.A[2])};
+            const T x = p * q * it * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+            T u[NR], w[NR];
+            rys_roots<NR, T>(x, u, w);
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
-                const double fff = u[r] * it;
-                const double B00 = 0.5 * fff;
-                const double B1 = (0.5 - B00 * qq.p) * pp.ip;
-                const double B1p = (0.5 - B00 * pp.p) * qq.ip;
-                double Ix[CI::TBL], Iy[CI::TBL], Iz[CI::TBL];
-                axis_table<LA, LB, LC, LD>(PA[0] - qq.p * PQ[0] * fff, QC[0] + pp.p * PQ[0] * fff, B00, B1, B1p,
-                                           P1.AB[0], P2.AB[0], Ix);
-                axis_table<LA, LB, LC, LD>(PA[1] - qq.p * PQ[1] * fff, QC[1] + pp.p * PQ[1] * fff, B00, B1, B1p,
-                                           P1.AB[1], P2.AB[1], Iy);
-                axis_table<LA, LB, LC, LD>(PA[2] - qq.p * PQ[2] * fff, QC[2] + pp.p * PQ[2] * fff, B00, B1, B1p,
-                                           P1.AB[2], P2.AB[2], Iz);
-                const double wr = w[r] * pref;
+                const T fff = u[r] * it;
+                const T B00 = (T)0.5 * fff;
+                const T B1 = ((T)0.5 - B00 * q) * ip_;
+                const T B1p = ((T)0.5 - B00 * p) * iq_;
+                T Ix[CI::TBL], Iy[CI::TBL], Iz[CI::TBL];
+                axis_table<LA, LB, LC, LD, T>(PA[0] - q * PQ[0] * fff, QC[0] + p * PQ[0] * fff, B00, B1, B1p, ab[0],
+                                              cd[0], Ix);
+                axis_table<LA, LB, LC, LD, T>(PA[1] - q * PQ[1] * fff, QC[1] + p * PQ[1] * fff, B00, B1, B1p, ab[1],
+                                              cd[1], Iy);
+                axis_table<LA, LB, LC, LD, T>(PA[2] - q * PQ[2] * fff, QC[2] + p * PQ[2] * fff, B00, B1, B1p, ab[2],
+                                              cd[2], Iz);
+                const T wr = w[r] * pref;
 #pragma unroll
                 for (int i = 0; i < NA; ++i)
 #pragma unroll
@@ -185,8 +189,8 @@ __global__ __launch_bounds__(256) void k_eri_lane(const EriArgs a) {
         atomicAdd(a.counter + 1, 1ull);
     }
     // normalisation (AuxFunctions.hpp:49-64) and permutational factor
-    double f = 1.0;
-    if (MODE <= kDigestUHF) f = (P1.same ? 0.5 : 1.0) * (P2.same ? 0.5 : 1.0) * (pab == pcd ? 0.5 : 1.0);
+    T f = 1;
+    if (MODE <= kDigestUHF) f = (P1.same ? (T)0.5 : (T)1) * (P2.same ? (T)0.5 : (T)1) * (pab == pcd ? (T)0.5 : (T)1);
 #pragma unroll
     for (int i = 0; i < NA; ++i)
 #pragma unroll
@@ -195,37 +199,38 @@ __global__ __launch_bounds__(256) void k_eri_lane(const EriArgs a) {
             for (int k = 0; k < NC; ++k)
 #pragma unroll
                 for (int l = 0; l < ND; ++l)
-                    e[((i * NB + j) * NC + k) * ND + l] *= f * cart_nrm(LA, i) * cart_nrm(LB, j) * cart_nrm(LC, k) * cart_nrm(LD, l);
+                    e[((i * NB + j) * NC + k) * ND + l] *=
+                        f * (T)(cart_nrm(LA, i) * cart_nrm(LB, j) * cart_nrm(LC, k) * cart_nrm(LD, l));
     if constexpr (MODE == kDigestBlock) {
 #pragma unroll
-        for (int c = 0; c < NCOMP; ++c) a.out[g * NCOMP + c] = e[c];
+        for (int c = 0; c < NCOMP; ++c) a.out[g * NCOMP + c] = (double)e[c];
     } else if constexpr (MODE == kDigestSchwarz) {
         double m = 0.0;
 #pragma unroll
         for (int i = 0; i < NA; ++i)
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
-                if constexpr (LA == LC && LB == LD) m = fmax(m, fabs(e[((i * NB + j) * NC + i) * ND + j]));
+                if constexpr (LA == LC && LB == LD) m = fmax(m, fabs((double)e[((i * NB + j) * NC + i) * ND + j]));
             }
         a.out[pab] = sqrt(m);
     } else {
-        digest_regs<LA, LB, LC, LD, MODE>(a, P1, P2, e);
+        digest_regs<LA, LB, LC, LD, MODE, T>(a, P1, P2, e);
     }
 }
 
 // ---------------------------------------------------------------------------
 // one 64-lane wave (one block) per shell quartet
 // ---------------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD, int MODE>
+template <int LA, int LB, int LC, int LD, int MODE, typename T>
 __device__ __forceinline__ void wave_one(const EriArgs& a, const long long g) {
     using CI = ClassInfo<LA, LB, LC, LD>;
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND, NCOMP = CI::NCOMP, NR = CI::NR;
     constexpr int E = CI::E, F = CI::F, TBL = CI::TBL;
     constexpr int PER = (NCOMP + 63) / 64;
-    __shared__ double sUW[2 * NR];
-    __shared__ double sG[NR * 3][(E + 1) * (F + 1)];
-    __shared__ double sI[NR * 3][TBL];
-    __shared__ double sE[NCOMP];
+    __shared__ T sUW[2 * NR];
+    __shared__ T sG[NR * 3][(E + 1) * (F + 1)];
+    __shared__ T sI[NR * 3][TBL];
+    __shared__ T sE[NCOMP];
     const int lane = threadIdx.x;
     int pab, pcd;
     double q12;
@@ -235,10 +240,10 @@ __device__ __forceinline__ void wave_one(const EriArgs& a, const long long g) {
     if (MODE <= kDigestUHF && q12 >= 0.0 && screened(a, P1, P2, q12)) return;   // uniform per block
     // each lane owns components lane + 64 m; precompute their table indices
     int ix[PER], iy[PER], iz[PER];
-    double acc[PER];
+    T acc[PER];
 #pragma unroll
     for (int m = 0; m < PER; ++m) {
-        acc[m] = 0.0;
+        acc[m] = 0;
         const int c = lane + 64 * m;
         const int cc = c < NCOMP ? c : 0;
         const int l = cc % ND, k = (cc / ND) % NC, j = (cc / (ND * NC)) % NB, i = cc / (ND * NC * NB);
@@ -268,42 +273,43 @@ __device__ __forceinline__ void wave_one(const EriArgs& a, const long long g) {
         const PrimPair pp = a.prims[P1.prim0 + ip];
         for (int iq = 0; iq < P2.nprim; ++iq) {
             const PrimPair qq = a.prims[P2.prim0 + iq];
-            const double t = pp.p + qq.p, it = 1.0 / t, rt = sqrt(it);
-            const double bare = kTwoPi25 * pp.s * qq.s * pp.ip * qq.ip * rt;
-            if (bare < a.primcut) continue;   // uniform across the block
+            const T p = (T)pp.p, q = (T)qq.p, ip_ = (T)pp.ip, iq_ = (T)qq.ip;
+            const T it = (T)1 / (p + q), rt = sqrt(it);
+            const T bare = (T)kTwoPi25 * (T)pp.s * (T)qq.s * ip_ * iq_ * rt;
+            if (bare < (T)a.primcut) continue;   // uniform across the block
             ++nprimq;
-            const double pref = kTwoPi25 * pp.K * qq.K * pp.ip * qq.ip * rt;
-            const double PQ[3] = {pp.P[0] - qq.P[0], pp.P[1] - qq.P[1], pp.P[2] - qq.P[2]};
-            const double x = pp.p * qq.p * it * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+            const T pref = (T)kTwoPi25 * (T)pp.K * (T)qq.K * ip_ * iq_ * rt;
+            const T PQ[3] = {(T)(pp.P[0] - qq.P[0]), (T)(pp.P[1] - qq.P[1]), (T)(pp.P[2] - qq.P[2])};
+            const T x = p * q * it * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
             if (lane == 0) {
-                double u[NR], w[NR];
-                rys_roots<NR>(x, u, w);
+                T u[NR], w[NR];
+                rys_roots<NR, T>(x, u, w);
                 for (int r = 0; r < NR; ++r) { sUW[r] = u[r]; sUW[NR + r] = w[r] * pref; }
             }
             __syncthreads();
             if (lane < 3 * NR) {
                 // VRR for (root, axis) = (lane / 3, lane % 3)
                 const int r = lane / 3, ax = lane % 3;
-                const double fff = sUW[r] * it;
-                const double B00 = 0.5 * fff, B1 = (0.5 - B00 * qq.p) * pp.ip, B1p = (0.5 - B00 * pp.p) * qq.ip;
-                const double C = (pp.P[ax] - P1.A[ax]) - qq.p * PQ[ax] * fff;
-                const double Cp = (qq.P[ax] - P2.A[ax]) + pp.p * PQ[ax] * fff;
-                double* G = sG[lane];
-                double Gl[E + 1][F + 1];
-                Gl[0][0] = 1.0;
+                const T fff = sUW[r] * it;
+                const T B00 = (T)0.5 * fff, B1 = ((T)0.5 - B00 * q) * ip_, B1p = ((T)0.5 - B00 * p) * iq_;
+                const T C = (T)(pp.P[ax] - P1.A[ax]) - q * PQ[ax] * fff;
+                const T Cp = (T)(qq.P[ax] - P2.A[ax]) + p * PQ[ax] * fff;
+                T* G = sG[lane];
+                T Gl[E + 1][F + 1];
+                Gl[0][0] = 1;
                 if (E > 0) Gl[1][0] = C;
 #pragma unroll
-                for (int e = 1; e < E; ++e) Gl[e + 1][0] = e * B1 * Gl[e - 1][0] + C * Gl[e][0];
+                for (int e = 1; e < E; ++e) Gl[e + 1][0] = (T)e * B1 * Gl[e - 1][0] + C * Gl[e][0];
 #pragma unroll
                 for (int f = 0; f < F; ++f)
 #pragma unroll
                     for (int e = 0; e <= E; ++e) {
-                        double v = Cp * Gl[e][f];
-                        if (f > 0) v = fma(f * B1p, Gl[e][f - 1], v);
-                        if (e > 0) v = fma(e * B00, Gl[e - 1][f], v);
+                        T v = Cp * Gl[e][f];
+                        if (f > 0) v = fma((T)f * B1p, Gl[e][f - 1], v);
+                        if (e > 0) v = fma((T)e * B00, Gl[e - 1][f], v);
                         Gl[e][f + 1] = v;
                     }
-                const double wz = (ax == 2) ? sUW[NR + r] : 1.0;   // weight x prefactor folded into z
+                const T wz = (ax == 2) ? sUW[NR + r] : (T)1;   // weight x prefactor folded into z
 #pragma unroll
                 for (int e = 0; e <= E; ++e)
 #pragma unroll
@@ -316,16 +322,16 @@ __device__ __forceinline__ void wave_one(const EriArgs& a, const long long g) {
                 const int d = t2 % (LD + 1), c = (t2 / (LD + 1)) % (LC + 1), b = (t2 / ((LD + 1) * (LC + 1))) % (LB + 1),
                           aa = t2 / ((LD + 1) * (LC + 1) * (LB + 1));
                 const int ax = ra % 3;
-                const double ab = P1.AB[ax], cd = P2.AB[ax];
-                const double* G = sG[ra];
-                double v = 0.0, abp = 1.0;
+                const T ab = (T)P1.AB[ax], cd = (T)P2.AB[ax];
+                const T* G = sG[ra];
+                T v = 0, abp = 1;
                 for (int i = 0; i <= b; ++i) {
-                    double cdp = 1.0, h = 0.0;
+                    T cdp = 1, h = 0;
                     for (int j = 0; j <= d; ++j) {
-                        h = fma(binom_c(d, j) * cdp, G[(aa + b - i) * (F + 1) + c + d - j], h);
+                        h = fma((T)binom_c(d, j) * cdp, G[(aa + b - i) * (F + 1) + c + d - j], h);
                         cdp *= cd;
                     }
-                    v = fma(binom_c(b, i) * abp, h, v);
+                    v = fma((T)binom_c(b, i) * abp, h, v);
                     abp *= ab;
                 }
                 sI[ra][t2] = v;
@@ -334,7 +340,7 @@ __device__ __forceinline__ void wave_one(const EriArgs& a, const long long g) {
 #pragma unroll
             for (int m = 0; m < PER; ++m) {
                 if (lane + 64 * m < NCOMP) {
-                    double s = acc[m];
+                    T s = acc[m];
 #pragma unroll
                     for (int r = 0; r < NR; ++r) s = fma(sI[3 * r][ix[m]] * sI[3 * r + 1][iy[m]], sI[3 * r + 2][iz[m]], s);
                     acc[m] = s;
@@ -347,14 +353,14 @@ __device__ __forceinline__ void wave_one(const EriArgs& a, const long long g) {
         atomicAdd(a.counter, nprimq);
         atomicAdd(a.counter + 1, 1ull);
     }
-    double f = 1.0;
-    if (MODE <= kDigestUHF) f = (P1.same ? 0.5 : 1.0) * (P2.same ? 0.5 : 1.0) * (pab == pcd ? 0.5 : 1.0);
+    T f = 1;
+    if (MODE <= kDigestUHF) f = (P1.same ? (T)0.5 : (T)1) * (P2.same ? (T)0.5 : (T)1) * (pab == pcd ? (T)0.5 : (T)1);
 #pragma unroll
     for (int m = 0; m < PER; ++m) {
         const int c = lane + 64 * m;
         if (c < NCOMP) {
             const int l = c % ND, k = (c / ND) % NC, j = (c / (ND * NC)) % NB, i = c / (ND * NC * NB);
-            double cn = 1.0;
+            T cn = 1;
             // component normalisation
             {
                 int idx = 0;
@@ -371,37 +377,37 @@ __device__ __forceinline__ void wave_one(const EriArgs& a, const long long g) {
     }
     __syncthreads();
     if constexpr (MODE == kDigestBlock) {
-        for (int c = lane; c < NCOMP; c += 64) a.out[g * NCOMP + c] = sE[c];
+        for (int c = lane; c < NCOMP; c += 64) a.out[g * NCOMP + c] = (double)sE[c];
         return;
     } else if constexpr (MODE == kDigestSchwarz) {
         if (lane == 0) {
             double m = 0.0;
             for (int i = 0; i < NA; ++i)
-                for (int j = 0; j < NB; ++j) m = fmax(m, fabs(sE[((i * NB + j) * NC + i) * ND + j]));
+                for (int j = 0; j < NB; ++j) m = fmax(m, fabs((double)sE[((i * NB + j) * NC + i) * ND + j]));
             a.out[pab] = sqrt(m);
         }
         return;
     } else {
         const int n = a.n;
         const int fa = P1.f1, fb = P1.f2, fc = P2.f1, fd = P2.f2;
-        constexpr double jf = (MODE == kDigestRHF) ? 4.0 : 2.0;
+        constexpr T jf = (MODE == kDigestRHF) ? 4 : 2;
         // Coulomb outputs: NA*NB (bra) + NC*ND (ket)
         for (int o = lane; o < NA * NB + NC * ND; o += 64) {
-            double s = 0.0;
+            T s = 0;
             if (o < NA * NB) {
                 const int i = o / NB, j = o % NB;
                 for (int k = 0; k < NC; ++k)
                     for (int l = 0; l < ND; ++l)
-                        s = fma(sE[((i * NB + j) * NC + k) * ND + l], a.D0[(long)(fc + k) * n + fd + l], s);
-                atomic_add_f64(a.W0 + (long)(fa + i) * n + fb + j, jf * s);
-                if (MODE == kDigestUHF) atomic_add_f64(a.W1 + (long)(fa + i) * n + fb + j, jf * s);
+                        s = fma(sE[((i * NB + j) * NC + k) * ND + l], (T)a.D0[(long)(fc + k) * n + fd + l], s);
+                atomic_add_f64(a.W0 + (long)(fa + i) * n + fb + j, (double)(jf * s));
+                if (MODE == kDigestUHF) atomic_add_f64(a.W1 + (long)(fa + i) * n + fb + j, (double)(jf * s));
             } else {
                 const int kl = o - NA * NB, k = kl / ND, l = kl % ND;
                 for (int i = 0; i < NA; ++i)
                     for (int j = 0; j < NB; ++j)
-                        s = fma(sE[((i * NB + j) * NC + k) * ND + l], a.D0[(long)(fa + i) * n + fb + j], s);
-                atomic_add_f64(a.W0 + (long)(fc + k) * n + fd + l, jf * s);
-                if (MODE == kDigestUHF) atomic_add_f64(a.W1 + (long)(fc + k) * n + fd + l, jf * s);
+                        s = fma(sE[((i * NB + j) * NC + k) * ND + l], (T)a.D0[(long)(fa + i) * n + fb + j], s);
+                atomic_add_f64(a.W0 + (long)(fc + k) * n + fd + l, (double)(jf * s));
+                if (MODE == kDigestUHF) atomic_add_f64(a.W1 + (long)(fc + k) * n + fd + l, (double)(jf * s));
             }
         }
         constexpr int NK = NA * NC + NA * ND + NB * NC + NB * ND;
@@ -409,31 +415,31 @@ __device__ __forceinline__ void wave_one(const EriArgs& a, const long long g) {
             const double* DK = (MODE == kDigestUHF) ? (sp == 0 ? a.D1 : a.D2) : a.D0;
             double* W = sp == 0 ? a.W0 : a.W1;
             for (int o = lane; o < NK; o += 64) {
-                double s = 0.0;
+                T s = 0;
                 if (o < NA * NC) {                       // (a,c) <- D_bd
                     const int i = o / NC, k = o % NC;
                     for (int j = 0; j < NB; ++j)
                         for (int l = 0; l < ND; ++l)
-                            s = fma(sE[((i * NB + j) * NC + k) * ND + l], DK[(long)(fb + j) * n + fd + l], s);
-                    atomic_add_f64(W + (long)(fa + i) * n + fc + k, -s);
+                            s = fma(sE[((i * NB + j) * NC + k) * ND + l], (T)DK[(long)(fb + j) * n + fd + l], s);
+                    atomic_add_f64(W + (long)(fa + i) * n + fc + k, (double)(-s));
                 } else if (o < NA * NC + NA * ND) {      // (a,d) <- D_bc
                     const int q = o - NA * NC, i = q / ND, l = q % ND;
                     for (int j = 0; j < NB; ++j)
                         for (int k = 0; k < NC; ++k)
-                            s = fma(sE[((i * NB + j) * NC + k) * ND + l], DK[(long)(fb + j) * n + fc + k], s);
-                    atomic_add_f64(W + (long)(fa + i) * n + fd + l, -s);
+                            s = fma(sE[((i * NB + j) * NC + k) * ND + l], (T)DK[(long)(fb + j) * n + fc + k], s);
+                    atomic_add_f64(W + (long)(fa + i) * n + fd + l, (double)(-s));
                 } else if (o < NA * NC + NA * ND + NB * NC) {   // (b,c) <- D_ad
                     const int q = o - NA * NC - NA * ND, j = q / NC, k = q % NC;
                     for (int i = 0; i < NA; ++i)
                         for (int l = 0; l < ND; ++l)
-                            s = fma(sE[((i * NB + j) * NC + k) * ND + l], DK[(long)(fa + i) * n + fd + l], s);
-                    atomic_add_f64(W + (long)(fb + j) * n + fc + k, -s);
+                            s = fma(sE[((i * NB + j) * NC + k) * ND + l], (T)DK[(long)(fa + i) * n + fd + l], s);
+                    atomic_add_f64(W + (long)(fb + j) * n + fc + k, (double)(-s));
                 } else {                                 // (b,d) <- D_ac
                     const int q = o - NA * NC - NA * ND - NB * NC, j = q / ND, l = q % ND;
                     for (int i = 0; i < NA; ++i)
                         for (int k = 0; k < NC; ++k)
-                            s = fma(sE[((i * NB + j) * NC + k) * ND + l], DK[(long)(fa + i) * n + fc + k], s);
-                    atomic_add_f64(W + (long)(fb + j) * n + fd + l, -s);
+                            s = fma(sE[((i * NB + j) * NC + k) * ND + l], (T)DK[(long)(fa + i) * n + fc + k], s);
+                    atomic_add_f64(W + (long)(fb + j) * n + fd + l, (double)(-s));
                 }
             }
         }
@@ -441,11 +447,11 @@ __device__ __forceinline__ void wave_one(const EriArgs& a, const long long g) {
 }
 
 // a block (one wave) walks qpb consecutive quartets of the launch
-template <int LA, int LB, int LC, int LD, int MODE>
+template <int LA, int LB, int LC, int LD, int MODE, typename T>
 __global__ __launch_bounds__(64) void k_eri_wave(const EriArgs a) {
     const long long g0 = (long long)blockIdx.x * a.qpb;
     for (int q = 0; q < a.qpb; ++q) {
-        wave_one<LA, LB, LC, LD, MODE>(a, g0 + q);
+        wave_one<LA, LB, LC, LD, MODE, T>(a, g0 + q);
         __syncthreads();
     }
 }
@@ -456,7 +462,7 @@ __global__ __launch_bounds__(64) void k_eri_wave(const EriArgs a) {
 // lane kernels: one quartet per lane, 256 per block; wave kernels: 8 per block
 static constexpr int quartets_per_block(int ncomp) { return ncomp <= 81 ? 256 : 8; }
 
-template <int LA, int LB, int LC, int LD, int MODE>
+template <int LA, int LB, int LC, int LD, int MODE, typename T = double>
 static hipError_t launch_cls(const EriArgs& a, hipStream_t st) {
     using CI = ClassInfo<LA, LB, LC, LD>;
     if (a.nquart <= 0) return hipSuccess;
@@ -464,18 +470,18 @@ static hipError_t launch_cls(const EriArgs& a, hipStream_t st) {
     b.qpb = quartets_per_block(CI::NCOMP);
     const long long nb = (a.nquart + b.qpb - 1) / b.qpb;
     if constexpr (CI::NCOMP <= 81) {
-        hipLaunchKernelGGL((k_eri_lane<LA, LB, LC, LD, MODE>), dim3((unsigned)nb), dim3(256), 0, st, b);
+        hipLaunchKernelGGL((k_eri_lane<LA, LB, LC, LD, MODE, T>), dim3((unsigned)nb), dim3(256), 0, st, b);
     } else {
-        hipLaunchKernelGGL((k_eri_wave<LA, LB, LC, LD, MODE>), dim3((unsigned)nb), dim3(64), 0, st, b);
+        hipLaunchKernelGGL((k_eri_wave<LA, LB, LC, LD, MODE, T>), dim3((unsigned)nb), dim3(64), 0, st, b);
     }
     return hipGetLastError();
 }
 
-template <int MODE>
+template <int MODE, typename T>
 static hipError_t launch_mode(int cb, int ck, const EriArgs& a, hipStream_t st) {
     // pair classes: 0 ss, 1 ps, 2 pp, 3 ds, 4 dp, 5 dd
 #define UNO_Q(X, Y, LA, LB, LC, LD) \
-    if (cb == X && ck == Y) return launch_cls<LA, LB, LC, LD, MODE>(a, st);
+    if (cb == X && ck == Y) return launch_cls<LA, LB, LC, LD, MODE, T>(a, st);
     UNO_Q(0, 0, 0, 0, 0, 0)
     UNO_Q(1, 0, 1, 0, 0, 0) UNO_Q(1, 1, 1, 0, 1, 0)
     UNO_Q(2, 0, 1, 1, 0, 0) UNO_Q(2, 1, 1, 1, 1, 0) UNO_Q(2, 2, 1, 1, 1, 1)
@@ -488,11 +494,18 @@ static hipError_t launch_mode(int cb, int ck, const EriArgs& a, hipStream_t st) 
     return hipErrorInvalidValue;
 }
 
-hipError_t eri_launch(int mode, int cb, int ck, const EriArgs& a, hipStream_t st) {
+hipError_t eri_launch(int mode, int cb, int ck, const EriArgs& a, hipStream_t st, bool fp32) {
+    if (fp32) {
+        switch (mode) {
+            case kDigestRHF: return launch_mode<kDigestRHF, float>(cb, ck, a, st);
+            case kDigestUHF: return launch_mode<kDigestUHF, float>(cb, ck, a, st);
+            default: return hipErrorInvalidValue;   // block and Schwarz modes are FP64 only
+        }
+    }
     switch (mode) {
-        case kDigestRHF: return launch_mode<kDigestRHF>(cb, ck, a, st);
-        case kDigestUHF: return launch_mode<kDigestUHF>(cb, ck, a, st);
-        case kDigestBlock: return launch_mode<kDigestBlock>(cb, ck, a, st);
+        case kDigestRHF: return launch_mode<kDigestRHF, double>(cb, ck, a, st);
+        case kDigestUHF: return launch_mode<kDigestUHF, double>(cb, ck, a, st);
+        case kDigestBlock: return launch_mode<kDigestBlock, double>(cb, ck, a, st);
         default: break;
     }
     // Schwarz: diagonal classes only
@@ -518,26 +531,31 @@ int eri_quartets_per_block(int cb, int ck) { return quartets_per_block(eri_class
 // ---------------------------------------------------------------------------
 // Rys-table evaluation for the parity tests
 // ---------------------------------------------------------------------------
-template <int NR>
+template <int NR, typename T>
 __global__ void k_rys_eval(int npts, const double* x, double* u, double* w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npts) return;
-    double uu[NR], ww[NR];
-    rys_roots<NR>(x[i], uu, ww);
-    for (int r = 0; r < NR; ++r) { u[i * NR + r] = uu[r]; w[i * NR + r] = ww[r]; }
+    T uu[NR], ww[NR];
+    rys_roots<NR, T>((T)x[i], uu, ww);
+    for (int r = 0; r < NR; ++r) { u[i * NR + r] = (double)uu[r]; w[i * NR + r] = (double)ww[r]; }
 }
 
-hipError_t rys_eval_launch(int n, int npts, const double* x, double* u, double* w, hipStream_t st) {
+template <typename T>
+static hipError_t rys_eval_t(int n, int npts, const double* x, double* u, double* w, hipStream_t st) {
     const dim3 g((npts + 255) / 256), b(256);
     switch (n) {
-        case 1: hipLaunchKernelGGL(k_rys_eval<1>, g, b, 0, st, npts, x, u, w); break;
-        case 2: hipLaunchKernelGGL(k_rys_eval<2>, g, b, 0, st, npts, x, u, w); break;
-        case 3: hipLaunchKernelGGL(k_rys_eval<3>, g, b, 0, st, npts, x, u, w); break;
-        case 4: hipLaunchKernelGGL(k_rys_eval<4>, g, b, 0, st, npts, x, u, w); break;
-        case 5: hipLaunchKernelGGL(k_rys_eval<5>, g, b, 0, st, npts, x, u, w); break;
+        case 1: hipLaunchKernelGGL((k_rys_eval<1, T>), g, b, 0, st, npts, x, u, w); break;
+        case 2: hipLaunchKernelGGL((k_rys_eval<2, T>), g, b, 0, st, npts, x, u, w); break;
+        case 3: hipLaunchKernelGGL((k_rys_eval<3, T>), g, b, 0, st, npts, x, u, w); break;
+        case 4: hipLaunchKernelGGL((k_rys_eval<4, T>), g, b, 0, st, npts, x, u, w); break;
+        case 5: hipLaunchKernelGGL((k_rys_eval<5, T>), g, b, 0, st, npts, x, u, w); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t rys_eval_launch(int n, int npts, const double* x, double* u, double* w, hipStream_t st, bool fp32) {
+    return fp32 ? rys_eval_t<float>(n, npts, x, u, w, st) : rys_eval_t<double>(n, npts, x, u, w, st);
 }
 
 }  // namespace uno

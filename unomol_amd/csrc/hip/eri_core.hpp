@@ -25,6 +25,7 @@ namespace uno {
 
 #define UNO_RYS_QUAL static __device__ const
 #include "../rys_table.inc"
+#include "../rys_table_f32.inc"
 #undef UNO_RYS_QUAL
 
 __host__ __device__ constexpr int ncart_c(int l) { return (l + 1) * (l + 2) / 2; }
@@ -50,61 +51,71 @@ __host__ __device__ constexpr double binom_c(int n, int k) {
 
 constexpr double kTwoPi25 = 34.986836655249725;   // 2 pi^(5/2)
 
-template <int NR>
-__device__ __forceinline__ void rys_roots(double x, double* u, double* w) {
+// Rys roots u = t^2 and weights for argument x: Clenshaw on the per-interval Chebyshev
+// series (degree 12 in double, degree 6 in float), Gauss-Hermite limit for large x
+template <int NR, typename T>
+__device__ __forceinline__ void rys_roots(T x, T* u, T* w) {
     constexpr int kAsym[6] = {0, 40, 48, 56, 62, 68};
-    if (x >= (double)kAsym[NR]) {
-        const double ix = 1.0 / x, isx = sqrt(ix);
+    constexpr bool f32 = sizeof(T) == 4;
+    if (x >= (T)kAsym[NR]) {
+        const T ix = (T)1 / x, isx = sqrt(ix);
         constexpr int o = NR * (NR - 1) / 2;
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
-            u[i] = uno_rys_asym_u[o + i] * ix;
-            w[i] = uno_rys_asym_w[o + i] * isx;
+            if constexpr (f32) {
+                u[i] = uno_rys_asym_u_f[o + i] * ix;
+                w[i] = uno_rys_asym_w_f[o + i] * isx;
+            } else {
+                u[i] = uno_rys_asym_u[o + i] * ix;
+                w[i] = uno_rys_asym_w[o + i] * isx;
+            }
         }
         return;
     }
+    constexpr int NC = f32 ? UNO_RYS_NCOEF_F : UNO_RYS_NCOEF;
     const int k = (int)x;
-    const double s = 2.0 * (x - (double)k) - 1.0, s2 = 2.0 * s;
-    const double* c = uno_rys_coef + uno_rys_off[NR] + k * (2 * NR * UNO_RYS_NCOEF);
+    const T s = (T)2 * (x - (T)k) - (T)1, s2 = (T)2 * s;
+    const T* c;
+    if constexpr (f32) c = uno_rys_coef_f + uno_rys_off_f[NR] + k * (2 * NR * NC);
+    else c = uno_rys_coef + uno_rys_off[NR] + k * (2 * NR * NC);
 #pragma unroll
     for (int f = 0; f < 2 * NR; ++f) {
-        const double* cf = c + f * UNO_RYS_NCOEF;
-        double b1 = 0.0, b2 = 0.0;
+        const T* cf = c + f * NC;
+        T b1 = 0, b2 = 0;
 #pragma unroll
-        for (int m = UNO_RYS_NCOEF - 1; m >= 1; --m) {
-            const double b0 = fma(s2, b1, cf[m] - b2);
+        for (int m = NC - 1; m >= 1; --m) {
+            const T b0 = fma(s2, b1, cf[m] - b2);
             b2 = b1;
             b1 = b0;
         }
-        const double v = fma(s, b1, cf[0] - b2);
+        const T v = fma(s, b1, cf[0] - b2);
         if (f < NR) u[f] = v; else w[f - NR] = v;
     }
 }
 
 // One axis of one root: Rys 2-D recurrence G[e][f] (e <= LA+LB, f <= LC+LD) then the
 // binomial transfer to I[a][b][c][d] (a <= LA, b <= LB, c <= LC, d <= LD), flattened.
-template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void axis_table(double C, double Cp, double B00, double B1, double B1p, double ab,
-                                           double cd, double* I) {
+template <int LA, int LB, int LC, int LD, typename T>
+__device__ __forceinline__ void axis_table(T C, T Cp, T B00, T B1, T B1p, T ab, T cd, T* I) {
     constexpr int E = LA + LB, F = LC + LD;
-    double G[E + 1][F + 1];
-    G[0][0] = 1.0;
+    T G[E + 1][F + 1];
+    G[0][0] = 1;
     if constexpr (E > 0) G[1][0] = C;
 #pragma unroll
-    for (int e = 1; e < E; ++e) G[e + 1][0] = e * B1 * G[e - 1][0] + C * G[e][0];
+    for (int e = 1; e < E; ++e) G[e + 1][0] = (T)e * B1 * G[e - 1][0] + C * G[e][0];
 #pragma unroll
     for (int f = 0; f < F; ++f)
 #pragma unroll
         for (int e = 0; e <= E; ++e) {
-            double v = Cp * G[e][f];
-            if (f > 0) v = fma(f * B1p, G[e][f - 1], v);
-            if (e > 0) v = fma(e * B00, G[e - 1][f], v);
+            T v = Cp * G[e][f];
+            if (f > 0) v = fma((T)f * B1p, G[e][f - 1], v);
+            if (e > 0) v = fma((T)e * B00, G[e - 1][f], v);
             G[e][f + 1] = v;
         }
     // ket transfer: H[e][c][d] = sum_j C(d,j) cd^j G[e][c+d-j]
-    double H[E + 1][LC + 1][LD + 1];
-    double cdp[LD + 1];
-    cdp[0] = 1.0;
+    T H[E + 1][LC + 1][LD + 1];
+    T cdp[LD + 1];
+    cdp[0] = 1;
 #pragma unroll
     for (int j = 1; j <= LD; ++j) cdp[j] = cdp[j - 1] * cd;
 #pragma unroll
@@ -113,13 +124,13 @@ __device__ __forceinline__ void axis_table(double C, double Cp, double B00, doub
         for (int c = 0; c <= LC; ++c)
 #pragma unroll
             for (int d = 0; d <= LD; ++d) {
-                double v = 0.0;
+                T v = 0;
 #pragma unroll
-                for (int j = 0; j <= d; ++j) v = fma(binom_c(d, j) * cdp[j], G[e][c + d - j], v);
+                for (int j = 0; j <= d; ++j) v = fma((T)binom_c(d, j) * cdp[j], G[e][c + d - j], v);
                 H[e][c][d] = v;
             }
-    double abp[LB + 1];
-    abp[0] = 1.0;
+    T abp[LB + 1];
+    abp[0] = 1;
 #pragma unroll
     for (int i = 1; i <= LB; ++i) abp[i] = abp[i - 1] * ab;
 #pragma unroll
@@ -130,9 +141,9 @@ __device__ __forceinline__ void axis_table(double C, double Cp, double B00, doub
             for (int c = 0; c <= LC; ++c)
 #pragma unroll
                 for (int d = 0; d <= LD; ++d) {
-                    double v = 0.0;
+                    T v = 0;
 #pragma unroll
-                    for (int i = 0; i <= b; ++i) v = fma(binom_c(b, i) * abp[i], H[a + b - i][c][d], v);
+                    for (int i = 0; i <= b; ++i) v = fma((T)binom_c(b, i) * abp[i], H[a + b - i][c][d], v);
                     I[((a * (LB + 1) + b) * (LC + 1) + c) * (LD + 1) + d] = v;
                 }
 }
@@ -170,7 +181,7 @@ __device__ __forceinline__ bool quartet_of(const EriArgs& a, long long g, int& p
     }
     const int jk = (int)(g - a.prefix[lo]);
     pab = a.bra_off + lo;
-    pcd = a.ket_off + jk;
+    pcd = a.ket_off + (a.kstart ? a.kstart[lo] : 0) + jk;
     q12 = a.pairQ[pab] * a.pairQ[pcd];
     return true;
 }

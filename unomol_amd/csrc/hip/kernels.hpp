@@ -6,17 +6,19 @@
 namespace uno {
 
 // ERIs (eri.hip)
-hipError_t eri_launch(int mode, int bra_class, int ket_class, const EriArgs& a, hipStream_t st);
+// fp32: evaluate the quartets in single precision (RHF/UHF digestion only), accumulating in FP64
+hipError_t eri_launch(int mode, int bra_class, int ket_class, const EriArgs& a, hipStream_t st, bool fp32 = false);
 int eri_class_ncomp(int bra_class, int ket_class);
 // quartets one block of the class kernel walks (the block table granularity)
 int eri_quartets_per_block(int bra_class, int ket_class);
-hipError_t rys_eval_launch(int n, int npts, const double* x, double* u, double* w, hipStream_t st);
+hipError_t rys_eval_launch(int n, int npts, const double* x, double* u, double* w, hipStream_t st, bool fp32 = false);
 
 // dense helpers (linalg.hip); matrices are row-major n x n
 hipError_t k_unpack(int n, const double* packed, double* dense, hipStream_t st);       // symmetric fill
 hipError_t k_pack(int n, const double* dense, double* packed, hipStream_t st);         // lower triangle
 hipError_t k_pack_sym_add(int n, const double* W, double* packed, hipStream_t st);     // packed += (W+W^T)_lower
-hipError_t k_sym_w(int n, const double* W, double* G, hipStream_t st);                 // G = W + W^T
+// G = W + W^T, or G += W + W^T when accumulating
+hipError_t k_sym_w(int n, const double* W, double* G, hipStream_t st, bool accumulate = false);
 hipError_t k_axpby(long len, double a, const double* x, double b, const double* y, double* z, hipStream_t st);
 hipError_t k_fill(long len, double v, double* x, hipStream_t st);
 // out[0] += sum_ij A_ij B_ij (full matrices, deterministic single-block reduction)

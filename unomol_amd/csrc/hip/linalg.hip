@@ -49,15 +49,17 @@ hipError_t k_pack_sym_add(int n, const double* W, double* packed, hipStream_t st
     return hipGetLastError();
 }
 
-__global__ void k_sym_w_(int n, const double* __restrict__ W, double* __restrict__ G) {
+__global__ void k_sym_w_(int n, const double* __restrict__ W, double* __restrict__ G, int acc) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)n * n) return;
     const int i = (int)(t / n), j = (int)(t % n);
-    G[t] = W[t] + W[(long)j * n + i];
+    const double v = W[t] + W[(long)j * n + i];
+    G[t] = acc ? G[t] + v : v;
 }
-hipError_t k_sym_w(int n, const double* W, double* G, hipStream_t st) {
+hipError_t k_sym_w(int n, const double* W, double* G, hipStream_t st, bool accumulate) {
     const long len = (long)n * n;
-    hipLaunchKernelGGL(k_sym_w_, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, n, W, G);
+    hipLaunchKernelGGL(k_sym_w_, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, n, W, G,
+                       accumulate ? 1 : 0);
     return hipGetLastError();
 }
 

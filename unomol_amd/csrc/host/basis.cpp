@@ -66,7 +66,8 @@ bool make_basis(const uno_basis* b, Basis& B, std::string& err) {
             return false;
         }
         const double sc = 1.0 / std::sqrt(ov);
-        for (int i = 0; i < x.npr; ++i) x.c[i] *= sc * std::sqrt(std::pow(2.0 * x.a[i], lp) / pi32);
+        if (!b->normalized)
+            for (int i = 0; i < x.npr; ++i) x.c[i] *= sc * std::sqrt(std::pow(2.0 * x.a[i], lp) / pi32);
         B.off.push_back(off);
         off += ncart(x.l);
         if (x.l > B.maxl) B.maxl = x.l;

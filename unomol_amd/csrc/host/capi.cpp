@@ -24,6 +24,7 @@ void basis_view(const Deck& d, uno_basis& b) {
     b.coef = d.coef.data();
     b.xyz = d.xyz.data();
     b.charge = d.charge.data();
+    b.normalized = 0;
 }
 
 // a <- X^T a X on the host-facing boundary, computed on the device
@@ -293,6 +294,7 @@ int uno_ctx_get(uno_ctx* c, const char* key, double* v) {
     else if (k == "fp32_threshold") *v = c->fp32_thr;
     else if (k == "precision") *v = c->precision;
     else if (k == "last_quartets") *v = c->last_quartets;
+    else if (k == "last_quartets32") *v = c->last_quartets32;
     else if (k == "pair_runs") *v = (double)c->cls.size();
     else if (k == "quartet_launches") *v = (double)c->qcls.size();
     else if (k == "pairs") *v = (double)c->pairs.size();
@@ -307,7 +309,7 @@ int uno_fock_rhf(uno_ctx* c, const double* P, double* G) {
     hipStream_t st = c->st;
     UNO_HIP(c->pk.upload(P, n2, st));
     UNO_HIP(k_unpack(n, c->pk.p, c->D0.p, st));
-    if (int rc = fock_build(c, false, -1.0)) return rc;
+    if (int rc = fock_build(c, false, -1.0, c->precision == 1)) return rc;
     UNO_HIP(c->pk.upload(G, n2, st));
     UNO_HIP(k_pack_sym_add(n, c->W0.p, c->pk.p, st));
     UNO_HIP(hipMemcpyAsync(G, c->pk.p, n2 * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -325,7 +327,7 @@ int uno_fock_uhf(uno_ctx* c, const double* PA, const double* PB, double* GA, dou
     UNO_HIP(c->pk.upload(PB, n2, st));
     UNO_HIP(k_unpack(n, c->pk.p, c->D2.p, st));
     UNO_HIP(k_axpby(nn, 1.0, c->D1.p, 1.0, c->D2.p, c->D0.p, st));
-    if (int rc = fock_build(c, true, -1.0)) return rc;
+    if (int rc = fock_build(c, true, -1.0, c->precision == 1)) return rc;
     double* Gs[2] = {GA, GB};
     double* Ws[2] = {c->W0.p, c->W1.p};
     for (int s = 0; s < 2; ++s) {
@@ -395,22 +397,33 @@ int uno_eri_emit(uno_ctx* c, long cap, double* val, int* ijkl, long* count) {
     return eri_emit(c, cap, val, ijkl, count);
 }
 
-int uno_rys_eval(int n, int npts, const double* x, double* u, double* w) {
-    if (n < 1 || n > 5 || npts <= 0) { set_error("bad argument"); return UNO_EINVAL; }
+static int rys_eval_on(hipStream_t st, int n, int npts, const double* x, double* u, double* w, bool fp32) {
+    DBuf<double> dx, du, dw;
+    UNO_HIP(dx.upload(x, npts, st));
+    UNO_HIP(du.alloc((size_t)npts * n));
+    UNO_HIP(dw.alloc((size_t)npts * n));
+    UNO_HIP(rys_eval_launch(n, npts, dx.p, du.p, dw.p, st, fp32));
+    UNO_HIP(hipMemcpyAsync(u, du.p, (size_t)npts * n * sizeof(double), hipMemcpyDeviceToHost, st));
+    UNO_HIP(hipMemcpyAsync(w, dw.p, (size_t)npts * n * sizeof(double), hipMemcpyDeviceToHost, st));
+    UNO_HIP(hipStreamSynchronize(st));
+    return UNO_OK;
+}
+
+static int rys_eval_impl(int n, int npts, const double* x, double* u, double* w, bool fp32) {
+    if (n < 1 || n > 5 || npts <= 0 || !x || !u || !w) { set_error("bad argument"); return UNO_EINVAL; }
     hipStream_t st;
     UNO_HIP(hipStreamCreate(&st));
-    {
-        DBuf<double> dx, du, dw;
-        UNO_HIP(dx.upload(x, npts, st));
-        UNO_HIP(du.alloc((size_t)npts * n));
-        UNO_HIP(dw.alloc((size_t)npts * n));
-        UNO_HIP(rys_eval_launch(n, npts, dx.p, du.p, dw.p, st));
-        UNO_HIP(hipMemcpyAsync(u, du.p, (size_t)npts * n * sizeof(double), hipMemcpyDeviceToHost, st));
-        UNO_HIP(hipMemcpyAsync(w, dw.p, (size_t)npts * n * sizeof(double), hipMemcpyDeviceToHost, st));
-        UNO_HIP(hipStreamSynchronize(st));
-    }
+    const int rc = rys_eval_on(st, n, npts, x, u, w, fp32);
     (void)hipStreamDestroy(st);
-    return UNO_OK;
+    return rc;
+}
+
+int uno_rys_eval(int n, int npts, const double* x, double* u, double* w) {
+    return rys_eval_impl(n, npts, x, u, w, false);
+}
+
+int uno_rys_eval_f32(int n, int npts, const double* x, double* u, double* w) {
+    return rys_eval_impl(n, npts, x, u, w, true);
 }
 
 int uno_schwarz(uno_ctx* c, double* q) {
@@ -426,7 +439,7 @@ int uno_schwarz(uno_ctx* c, double* q) {
 
 int uno_profile(uno_ctx* c, int enable, double* ms, double* launches, double* prim, double* shell) {
     if (!c) { set_error("null argument"); return UNO_EINVAL; }
-    const int ncls = kNumPairClasses * kNumPairClasses;
+    const int ncls = kProfSlots;
     if (enable >= 0) {
         c->profile = enable;
         for (int i = 0; i < ncls; ++i) c->prof_ms[i] = c->prof_launch[i] = c->prof_prim[i] = c->prof_shell[i] = 0.0;
@@ -437,6 +450,70 @@ int uno_profile(uno_ctx* c, int enable, double* ms, double* launches, double* pr
         if (prim) prim[i] = c->prof_prim[i];
         if (shell) shell[i] = c->prof_shell[i];
     }
+    return UNO_OK;
+}
+
+int uno_shell_rotation(int l, const double* Q, double* M) {
+    if (l < 0 || l > 2 || !Q || !M) { set_error("bad argument (0 <= l <= 2)"); return UNO_EINVAL; }
+    shell_rotation(l, Q, M);
+    return UNO_OK;
+}
+
+int uno_guess_features(const uno_basis* b, long cap, float* X, int* pairs, long* count) {
+    if (!b || !count) { set_error("null argument"); return UNO_EINVAL; }
+    Basis B;
+    std::string err;
+    if (!make_basis(b, B, err)) { set_error(err); return UNO_EINVAL; }
+    std::vector<float> x;
+    std::vector<int> pr;
+    if (int rc = guess_features(B, x, pr, err)) { set_error(err); return rc; }
+    *count = (long)pr.size() / 2;
+    if (cap == 0) return UNO_OK;
+    if (cap < *count || !X || !pairs) { set_error("capacity too small"); return UNO_EINVAL; }
+    std::memcpy(X, x.data(), x.size() * sizeof(float));
+    std::memcpy(pairs, pr.data(), pr.size() * sizeof(int));
+    return UNO_OK;
+}
+
+int uno_guess_samples(const uno_basis* b, const double* P, long cap, float* X, float* Y, float* M, long* count) {
+    if (!b || !P || !count) { set_error("null argument"); return UNO_EINVAL; }
+    Basis B;
+    std::string err;
+    if (!make_basis(b, B, err)) { set_error(err); return UNO_EINVAL; }
+    const int n = B.norb;
+    std::vector<double> D((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) D[(size_t)i * n + j] = P[pidx(i, j)];
+    std::vector<float> x, y, m;
+    if (int rc = guess_samples(B, D.data(), x, y, m, err)) { set_error(err); return rc; }
+    *count = (long)x.size() / kGuessInDim;
+    if (cap == 0) return UNO_OK;
+    if (cap < *count || !X || !Y || !M) { set_error("capacity too small"); return UNO_EINVAL; }
+    std::memcpy(X, x.data(), x.size() * sizeof(float));
+    std::memcpy(Y, y.data(), y.size() * sizeof(float));
+    std::memcpy(M, m.data(), m.size() * sizeof(float));
+    return UNO_OK;
+}
+
+int uno_guess_assemble(const uno_basis* b, long count, const int* pairs, const float* Y, double* P) {
+    if (!b || !pairs || !Y || !P || count < 0) { set_error("null argument"); return UNO_EINVAL; }
+    Basis B;
+    std::string err;
+    if (!make_basis(b, B, err)) { set_error(err); return UNO_EINVAL; }
+    for (long k = 0; k < 2 * count; ++k)
+        if (pairs[k] < 0 || pairs[k] >= B.ncen) { set_error("atom index out of range"); return UNO_EINVAL; }
+    std::vector<int> pr(pairs, pairs + 2 * count);
+    std::vector<double> D;
+    if (int rc = guess_assemble(B, pr, Y, D, err)) { set_error(err); return rc; }
+    const int n = B.norb;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) P[pidx(i, j)] = D[(size_t)i * n + j];
+    return UNO_OK;
+}
+
+int uno_ctx_set_guess(uno_ctx* c, uno_mlp* model) {
+    if (!c) { set_error("null argument"); return UNO_EINVAL; }
+    c->guess_model = model;
     return UNO_OK;
 }
 

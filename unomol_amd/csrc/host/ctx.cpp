@@ -27,22 +27,52 @@ static int ensure_device() {
     return UNO_OK;
 }
 
-static void build_prefix(uno_ctx* c, QuartetClass& q, double dmax) {
+// Quartet spaces of one launch.  Both runs are sorted by Schwarz bound (descending), so
+// for bra i the kets passing Qi Qj dmax >= tau form a prefix of the ket run, and within
+// it the kets with Qi Qj >= fp32_thr (FP64 in a mixed build) form a shorter prefix.
+static void build_prefix(uno_ctx* c, QuartetClass& q, double dmax, double fp32_thr) {
     const PairClassList& Lb = c->cls[q.X];
     const PairClassList& Lk = c->cls[q.Y];
     const int nb = (int)Lb.Q.size();
-    q.prefix.assign(nb + 1, 0);
+    const bool mixed = fp32_thr > 0.0;
+    q.s64.prefix.assign(nb + 1, 0);
+    q.s32.prefix.assign(mixed ? nb + 1 : 1, 0);
+    q.s32.kstart.assign(mixed ? nb : 0, 0);
     for (int i = 0; i < nb; ++i) {
-        long long cnt = 0;
+        long long cnt = 0, c64 = 0;
         const double qd = Lb.Q[i] * dmax;
         if (qd > 0.0) {
             const double thr = c->tau / qd;
             cnt = std::partition_point(Lk.Q.begin(), Lk.Q.end(), [&](double v) { return v >= thr; }) - Lk.Q.begin();
             if (q.X == q.Y) cnt = std::min<long long>(cnt, i + 1);
         }
-        q.prefix[i + 1] = q.prefix[i] + cnt;
+        c64 = cnt;
+        if (mixed && cnt > 0) {
+            const double t32 = fp32_thr / Lb.Q[i];
+            c64 = std::partition_point(Lk.Q.begin(), Lk.Q.begin() + cnt, [&](double v) { return v >= t32; }) -
+                  Lk.Q.begin();
+        }
+        q.s64.prefix[i + 1] = q.s64.prefix[i] + c64;
+        if (mixed) {
+            q.s32.prefix[i + 1] = q.s32.prefix[i] + (cnt - c64);
+            q.s32.kstart[i] = (int)c64;
+        }
     }
-    q.nq = q.prefix[nb];
+    q.s64.nq = q.s64.prefix[nb];
+    q.s32.nq = mixed ? q.s32.prefix[nb] : 0;
+}
+
+// block table: bra holding the first quartet of every block, so that each lane's bra
+// search covers a short range
+static void build_blocks(QuartetSpace& s, int qpb) {
+    const long long nblk = (s.nq + qpb - 1) / qpb;
+    s.blk.resize(nblk + 1);
+    int i = 0;
+    for (long long b = 0; b <= nblk; ++b) {
+        const long long g = std::min(b * qpb, s.nq - 1);
+        while (s.prefix[i + 1] <= g) ++i;
+        s.blk[b] = i;
+    }
 }
 
 static EriArgs base_args(uno_ctx* c) {
@@ -241,7 +271,7 @@ int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
     return UNO_OK;
 }
 
-int fock_build(uno_ctx* c, bool uhf, double dmax_hint) {
+int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
     hipStream_t st = c->st;
     const long nn = (long)c->n * c->n;
     UNO_HIP(k_fill(nn, 0.0, c->W0.p, st));
@@ -255,71 +285,66 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint) {
         UNO_HIP(hipMemcpyAsync(&dmax, c->scal.p, sizeof(double), hipMemcpyDeviceToHost, st));
         UNO_HIP(hipStreamSynchronize(st));
     }
-    double total = 0.0;
-    const int ncls = kNumPairClasses * kNumPairClasses;
+    double total = 0.0, total32 = 0.0;
     std::vector<hipEvent_t> ev0, ev1;
     std::vector<int> evcls;
     if (c->profile) {
-        UNO_HIP(c->d_counter.alloc(2 * ncls));
-        UNO_HIP(hipMemsetAsync(c->d_counter.p, 0, 2 * ncls * sizeof(unsigned long long), st));
+        UNO_HIP(c->d_counter.alloc(2 * kProfSlots));
+        UNO_HIP(hipMemsetAsync(c->d_counter.p, 0, 2 * kProfSlots * sizeof(unsigned long long), st));
     }
     for (QuartetClass& q : c->qcls) {
-        build_prefix(c, q, 4.0 * dmax);
-        if (q.nq == 0) continue;
+        build_prefix(c, q, 4.0 * dmax, mixed ? c->fp32_thr : 0.0);
         const PairClassList& LX = c->cls[q.X];
         const PairClassList& LY = c->cls[q.Y];
         const int qpb = eri_quartets_per_block(LX.lcls, LY.lcls);
-        // bra holding the first quartet of every block: each lane then searches a short range
-        const long long nblk = (q.nq + qpb - 1) / qpb;
-        q.blk.resize(nblk + 1);
-        {
-            int i = 0;
-            for (long long b = 0; b <= nblk; ++b) {
-                const long long g = std::min(b * qpb, q.nq - 1);
-                while (q.prefix[i + 1] <= g) ++i;
-                q.blk[b] = i;
+        for (int prec = 0; prec < 2; ++prec) {
+            QuartetSpace& sp = prec ? q.s32 : q.s64;
+            if (sp.nq == 0) continue;
+            build_blocks(sp, qpb);
+            UNO_HIP(sp.d_prefix.upload(sp.prefix.data(), sp.prefix.size(), st));
+            UNO_HIP(sp.d_blk.upload(sp.blk.data(), sp.blk.size(), st));
+            if (prec) UNO_HIP(sp.d_kstart.upload(sp.kstart.data(), sp.kstart.size(), st));
+            EriArgs a = base_args(c);
+            a.bra_off = LX.off;
+            a.ket_off = LY.off;
+            a.pairQ = c->d_pairQ.p;
+            a.prefix = sp.d_prefix.p;
+            a.blk = sp.d_blk.p;
+            a.kstart = prec ? sp.d_kstart.p : nullptr;
+            a.qpb = qpb;
+            a.nbra = LX.count;
+            a.nquart = sp.nq;
+            a.same_class = q.X == q.Y;
+            a.dsh = c->Dsh.p;
+            a.dmax_global = 4.0 * dmax;
+            a.D0 = c->D0.p;
+            a.W0 = c->W0.p;
+            if (uhf) {
+                a.D1 = c->D1.p;
+                a.D2 = c->D2.p;
+                a.W1 = c->W1.p;
             }
+            const int ci = prec * kNumPairClasses * kNumPairClasses + LX.lcls * kNumPairClasses + LY.lcls;
+            if (c->profile) {
+                a.counter = c->d_counter.p + 2 * ci;
+                hipEvent_t e0, e1;
+                UNO_HIP(hipEventCreate(&e0));
+                UNO_HIP(hipEventCreate(&e1));
+                ev0.push_back(e0);
+                ev1.push_back(e1);
+                evcls.push_back(ci);
+                UNO_HIP(hipEventRecord(e0, st));
+            }
+            UNO_HIP(eri_launch(uhf ? kDigestUHF : kDigestRHF, LX.lcls, LY.lcls, a, st, prec == 1));
+            if (c->profile) UNO_HIP(hipEventRecord(ev1.back(), st));
+            (prec ? total32 : total) += (double)sp.nq;
         }
-        UNO_HIP(q.d_prefix.upload(q.prefix.data(), q.prefix.size(), st));
-        UNO_HIP(q.d_blk.upload(q.blk.data(), q.blk.size(), st));
-        EriArgs a = base_args(c);
-        a.bra_off = LX.off;
-        a.ket_off = LY.off;
-        a.pairQ = c->d_pairQ.p;
-        a.prefix = q.d_prefix.p;
-        a.blk = q.d_blk.p;
-        a.qpb = qpb;
-        a.nbra = LX.count;
-        a.nquart = q.nq;
-        a.same_class = q.X == q.Y;
-        a.dsh = c->Dsh.p;
-        a.dmax_global = 4.0 * dmax;
-        a.D0 = c->D0.p;
-        a.W0 = c->W0.p;
-        if (uhf) {
-            a.D1 = c->D1.p;
-            a.D2 = c->D2.p;
-            a.W1 = c->W1.p;
-        }
-        const int ci = LX.lcls * kNumPairClasses + LY.lcls;
-        if (c->profile) {
-            a.counter = c->d_counter.p + 2 * ci;
-            hipEvent_t e0, e1;
-            UNO_HIP(hipEventCreate(&e0));
-            UNO_HIP(hipEventCreate(&e1));
-            ev0.push_back(e0);
-            ev1.push_back(e1);
-            evcls.push_back(ci);
-            UNO_HIP(hipEventRecord(e0, st));
-        }
-        UNO_HIP(eri_launch(uhf ? kDigestUHF : kDigestRHF, LX.lcls, LY.lcls, a, st));
-        if (c->profile) UNO_HIP(hipEventRecord(ev1.back(), st));
-        total += (double)q.nq;
     }
-    c->last_quartets = total;
+    c->last_quartets = total + total32;
+    c->last_quartets32 = total32;
     if (c->profile) {
         UNO_HIP(hipStreamSynchronize(st));
-        std::vector<unsigned long long> cnt(2 * ncls);
+        std::vector<unsigned long long> cnt(2 * kProfSlots);
         UNO_HIP(hipMemcpy(cnt.data(), c->d_counter.p, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
         for (size_t k = 0; k < ev0.size(); ++k) {
             float ms = 0.f;
@@ -329,7 +354,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint) {
             (void)hipEventDestroy(ev0[k]);
             (void)hipEventDestroy(ev1[k]);
         }
-        for (int ci = 0; ci < ncls; ++ci) {
+        for (int ci = 0; ci < kProfSlots; ++ci) {
             c->prof_prim[ci] += (double)cnt[2 * ci];
             c->prof_shell[ci] += (double)cnt[2 * ci + 1];
         }

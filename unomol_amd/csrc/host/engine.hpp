@@ -63,14 +63,25 @@ struct PairClassList {
     std::vector<double> Q;      // host copy of the bounds of the run
 };
 
-struct QuartetClass {
-    int X, Y;                   // indices into the runs, X >= Y (bra run, ket run)
+// One quartet space of a launch: per bra, a contiguous range of kets (prefix counts, the
+// first ket of each range when it does not start at 0) and the block table.
+struct QuartetSpace {
     std::vector<long long> prefix;
-    std::vector<int> blk;
+    std::vector<int> kstart, blk;
     DBuf<long long> d_prefix;
-    DBuf<int> d_blk;
+    DBuf<int> d_kstart, d_blk;
     long long nq = 0;
 };
+
+struct QuartetClass {
+    int X, Y;                   // indices into the runs, X >= Y (bra run, ket run)
+    QuartetSpace s64;           // FP64 quartets (all of them unless the build is mixed)
+    QuartetSpace s32;           // mixed builds: quartets whose Schwarz bound is below fp32_thr
+};
+
+// profiling slots: 36 class pairs (bra pair class X, ket pair class Y, 6 X + Y) for the
+// FP64 launches, then 36 for the FP32 launches of mixed builds
+constexpr int kProfSlots = 2 * kNumPairClasses * kNumPairClasses;
 
 constexpr int kNumBuckets = 6;
 inline int prim_bucket(int nprim) {
@@ -80,6 +91,8 @@ inline int prim_bucket(int nprim) {
 struct EigenWS;
 
 }  // namespace uno
+
+struct uno_mlp;
 
 struct uno_ctx {
     uno::Basis B;
@@ -103,15 +116,16 @@ struct uno_ctx {
     // device work matrices (n x n)
     uno::DBuf<double> D0, D1, D2, W0, W1, Dsh, pk, scal;
     uno::EigenWS* eig = nullptr;
-    double last_quartets = 0.0;
+    double last_quartets = 0.0, last_quartets32 = 0.0;
     // profiling pass: per quartet class (bra X, ket Y) kernel milliseconds and the
     // primitive / shell quartets the kernels evaluated
     int profile = 0;
-    double prof_ms[uno::kNumPairClasses * uno::kNumPairClasses] = {};
-    double prof_prim[uno::kNumPairClasses * uno::kNumPairClasses] = {};
-    double prof_shell[uno::kNumPairClasses * uno::kNumPairClasses] = {};
-    double prof_launch[uno::kNumPairClasses * uno::kNumPairClasses] = {};
+    double prof_ms[uno::kProfSlots] = {};
+    double prof_prim[uno::kProfSlots] = {};
+    double prof_shell[uno::kProfSlots] = {};
+    double prof_launch[uno::kProfSlots] = {};
     uno::DBuf<unsigned long long> d_counter;
+    uno_mlp* guess_model = nullptr;   // not owned: the network of guess = 2
     ~uno_ctx();
 };
 
@@ -119,8 +133,9 @@ namespace uno {
 
 int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell);
 // Fock build from dense device densities already in c->D0 (and D1, D2 for UHF).
-// Leaves W0 (and W1) = unsymmetrised accumulators; G = W + W^T.
-int fock_build(uno_ctx* c, bool uhf, double dmax_hint);
+// Leaves W0 (and W1) = unsymmetrised accumulators; G = W + W^T.  mixed: quartets whose
+// Schwarz bound Qab Qcd is below c->fp32_thr are evaluated in FP32.
+int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed = false);
 int eri_emit(uno_ctx* c, long cap, double* val, int* ijkl, long* count);
 
 // dense symmetric eigensolver on device data: A (destroyed) -> Z (eigenvectors in
@@ -129,5 +144,22 @@ int eigh_device(uno_ctx* c, int n, double* dA, double* dZ, double* h_evals);
 int eigh_standalone(int n, double* dA, double* dZ, double* h_evals, hipStream_t st);
 
 int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, double* evals, double* P, double* C);
+
+// learned density guess (guess.cpp): descriptors of the atom pairs within the cutoff
+// (kGuessIn floats each; pairs as (A, B), A <= B), target blocks and masks from a dense
+// density (kGuessOut floats each), and a dense density from predicted blocks
+constexpr int kGuessInDim = 128, kGuessOutDim = 256;
+int guess_features(const Basis& B, std::vector<float>& X, std::vector<int>& pairs, std::string& err);
+int guess_samples(const Basis& B, const double* P, std::vector<float>& X, std::vector<float>& Y,
+                  std::vector<float>& M, std::string& err);
+int guess_assemble(const Basis& B, const std::vector<int>& pairs, const float* Y, std::vector<double>& P,
+                   std::string& err);
+void shell_rotation(int l, const double* Q, double* M);
+
+}  // namespace uno
+
+extern "C" int uno_mlp_forward(uno_mlp* m, long n, const float* X, float* Y);
+
+namespace uno {
 
 }  // namespace uno

@@ -11,6 +11,15 @@
 // All matrices stay in HBM as dense row-major N x N FP64; the host only reads a few
 // scalars per iteration and the tridiagonal for the QL step.
 // accel = 2 replaces the damping with Pulay DIIS on F (error X^T(FDS - SDF)X).
+//
+// Incremental builds (incremental = 1): G is linear in the density, so after a full
+// build G[P_k] = G[P_ref] + G[P_k - P_ref]; the change is small late in the SCF and the
+// density screen then drops most quartets.  A full build is redone every kRebuild
+// incremental builds and whenever the precision changes.
+// Mixed precision (precision = 1): Fock builds evaluate quartets with Schwarz bound
+// Qab Qcd below fp32_threshold in FP32 (accumulating in FP64) until |dE| falls below
+// kMixedSwitch or the convergence test passes; then a full FP64 build starts the final
+// phase, and convergence is accepted only after fp64_final FP64 iterations.
 #include <chrono>
 #include <cfloat>
 #include <cmath>
@@ -52,8 +61,11 @@ bool solve_small(int m, std::vector<double> A, std::vector<double>& b) {
 }  // namespace
 
 struct ScfBufs {
-    DBuf<double> H, T, S, X, F, Fp, Tm, Cw, G[2], C[2], Pold[2], diisF, diisE, Etmp;
+    DBuf<double> H, T, S, X, F, Fp, Tm, Cw, G[2], C[2], Pold[2], diisF, diisE, Etmp, P[2], Pref[2];
 };
+
+constexpr int kRebuild = 10;            // incremental builds between full rebuilds
+constexpr double kMixedSwitch = 1e-5;   // |dE| (Eh) ending the mixed-precision phase
 
 int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, double* evals_out, double* P_io,
             double* C_out) {
@@ -69,14 +81,24 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
         set_error("electron count does not fit the basis");
         return UNO_EINVAL;
     }
+    if (o.guess == 2 && !c->guess_model) {
+        set_error("guess 2 needs a density-guess network (uno_ctx_set_guess)");
+        return UNO_EINVAL;
+    }
+    if (o.precision < 0 || o.precision > 1 || o.incremental < 0 || o.incremental > 1 || o.fp64_final < 0) {
+        set_error("bad precision / incremental / fp64_final option");
+        return UNO_EINVAL;
+    }
     hipStream_t st = c->st;
     double eps = o.eps;
     const double xeps = DBL_EPSILON * n * n * 0.5;   // Basis.hpp:249-254
     if (eps < xeps) eps = xeps;
     ScfBufs b;
     for (DBuf<double>* p : {&b.H, &b.T, &b.S, &b.X, &b.F, &b.Fp, &b.Tm, &b.Cw, &b.G[0], &b.G[1], &b.C[0], &b.C[1],
-                            &b.Pold[0], &b.Pold[1], &b.Etmp})
+                            &b.Pold[0], &b.Pold[1], &b.Etmp, &b.P[0], &b.P[1]})
         UNO_HIP(p->alloc(nn));
+    if (o.incremental)
+        for (int s = 0; s < ns; ++s) UNO_HIP(b.Pref[s].alloc(nn));
     // one-electron matrices on the host, once
     std::vector<double> hS(n2), hT(n2), hH(n2);
     one_electron(c->B, hS.data(), hT.data(), hH.data(), nullptr);
@@ -111,7 +133,7 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
         for (int j = 0; j < n; ++j) hZ[i * n + j] *= ev[j];
     UNO_HIP(b.X.upload(hZ.data(), nn, st));
 
-    double* P[2] = {uhf ? c->D1.p : c->D0.p, c->D2.p};
+    double* P[2] = {b.P[0].p, b.P[1].p};
     std::vector<double> evals(2 * (size_t)n, 0.0);
     double diag_s = 0.0, fock_s = 0.0;
 
@@ -128,7 +150,31 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
     };
 
     // guess
-    if (o.guess == 1 && P_io) {
+    if (o.guess == 2) {
+        // learned: pair descriptors (host) -> network (GPU) -> blocks assembled in the lab
+        // frame; UHF spins get the closed-shell density scaled to their occupations
+        const auto t0 = clk::now();
+        std::vector<float> X, Y;
+        std::vector<int> pairs;
+        std::string err;
+        if (int rc = guess_features(c->B, X, pairs, err)) { set_error(err); return rc; }
+        const long np = (long)pairs.size() / 2;
+        Y.assign((size_t)np * kGuessOutDim, 0.0f);
+        if (np > 0)
+            if (int rc = uno_mlp_forward(c->guess_model, np, X.data(), Y.data())) return rc;
+        std::vector<double> D;
+        if (int rc = guess_assemble(c->B, pairs, Y.data(), D, err)) { set_error(err); return rc; }
+        for (int s = 0; s < ns; ++s) {
+            const double f = uhf ? (double)nocc[s] / (0.5 * nelec) : 1.0;
+            if (f != 1.0)
+                for (double& v : D) v *= f;
+            UNO_HIP(hipMemcpyAsync(P[s], D.data(), nn * sizeof(double), hipMemcpyHostToDevice, st));
+            UNO_HIP(hipStreamSynchronize(st));
+            if (f != 1.0)
+                for (double& v : D) v /= f;
+        }
+        r.guess_s = secs(t0, clk::now());
+    } else if (o.guess == 1 && P_io) {
         for (int s = 0; s < ns; ++s) {
             UNO_HIP(c->pk.upload(P_io + s * n2, n2, st));
             UNO_HIP(k_unpack(n, c->pk.p, P[s], st));
@@ -155,11 +201,31 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
     double energy = 0.0, eold = 0.0, ediff = 10.0, pdiff = 10.0;
     int iteration = 0;
     double hs[4];
+    // build control: full vs incremental, mixed vs FP64 phase
+    bool need_full = true;
+    int nincr = 0;
+    int phase = o.precision == 1 ? 0 : 1;   // 0 mixed, 1 FP64
+    int fp64_updates = 0;
     auto update = [&]() -> int {
         const auto t0 = clk::now();
-        if (uhf) UNO_HIP(k_axpby(nn, 1.0, P[0], 1.0, P[1], c->D0.p, st));
-        if (int rc = fock_build(c, uhf, -1.0)) return rc;
-        for (int s = 0; s < ns; ++s) UNO_HIP(k_sym_w(n, s == 0 ? c->W0.p : c->W1.p, b.G[s].p, st));
+        const bool full = need_full || !o.incremental || nincr >= kRebuild;
+        const bool mixed = phase == 0;
+        // densities of this build: P, or its change since the last build (G is linear in P)
+        double* Din[2] = {uhf ? c->D1.p : c->D0.p, c->D2.p};
+        for (int s = 0; s < ns; ++s) {
+            if (full) UNO_HIP(hipMemcpyAsync(Din[s], P[s], nn * sizeof(double), hipMemcpyDeviceToDevice, st));
+            else UNO_HIP(k_axpby(nn, 1.0, P[s], -1.0, b.Pref[s].p, Din[s], st));
+        }
+        if (uhf) UNO_HIP(k_axpby(nn, 1.0, c->D1.p, 1.0, c->D2.p, c->D0.p, st));
+        if (int rc = fock_build(c, uhf, -1.0, mixed)) return rc;
+        for (int s = 0; s < ns; ++s) {
+            UNO_HIP(k_sym_w(n, s == 0 ? c->W0.p : c->W1.p, b.G[s].p, st, !full));
+            if (o.incremental)
+                UNO_HIP(hipMemcpyAsync(b.Pref[s].p, P[s], nn * sizeof(double), hipMemcpyDeviceToDevice, st));
+        }
+        if (full) { need_full = false; nincr = 0; ++r.nfull; } else ++nincr;
+        if (!mixed) ++fp64_updates;
+        r.quartets32 += c->last_quartets32;
         UNO_HIP(k_fill(4, 0.0, c->scal.p, st));
         for (int s = 0; s < ns; ++s) {
             UNO_HIP(k_dot(nn, P[s], b.H.p, c->scal.p, st));
@@ -238,11 +304,23 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
                          iteration, energy + nuc, ediff, pdiff);
         return UNO_OK;
     };
-    auto converged = [&]() {
+    auto test = [&]() {
         if (o.cflag == 0) return ediff < eps;
         if (o.cflag == 1) return pdiff < eps;
         if (o.cflag == 3) return std::fabs(ediff) < eps;   // |dE| < eps (no reference counterpart)
         return pdiff < eps && ediff < eps;
+    };
+    auto converged = [&]() {
+        const bool pass = test();
+        if (phase == 0) {
+            // leave the mixed phase: the next build is a full FP64 one
+            if (pass || std::fabs(ediff) < kMixedSwitch) {
+                phase = 1;
+                need_full = true;
+            }
+            return false;
+        }
+        return pass && fp64_updates >= (o.precision == 1 ? std::max(o.fp64_final, 1) : 0);
     };
     iteration = 0;
     eold = 0.0;
