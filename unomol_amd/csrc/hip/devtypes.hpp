@@ -53,6 +53,11 @@ struct EriArgs {
     const int* blk;
     const int* kstart;        // first ket of every bra's range (null: 0), FP32 ranges
                               // start after the bra's FP64 kets
+    // bra-major launches: one wave per (bra, chunk of 64 kets); wpre[i] = first wave
+    // task of bra i, tblk[b] = bra of task 64 b
+    const long long* wpre;
+    const int* tblk;
+    long long ntask;
     int qpb;                  // quartets per block of the launch
     int nbra;
     long long nquart;

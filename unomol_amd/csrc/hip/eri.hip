@@ -119,40 +119,33 @@ __device__ __forceinline__ void digest_regs(const EriArgs& a, const PairHdr& P1,
 }
 
 // ---------------------------------------------------------------------------
-// one lane per shell quartet
+// contracted (un-normalised) ERI block of one shell quartet in one lane; returns the
+// primitive quartets kept by the cut.  Ket primitives outside, bra primitives inside.
 // ---------------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD, int MODE, typename T>
-__global__ __launch_bounds__(256) void k_eri_lane(const EriArgs a) {
+template <int LA, int LB, int LC, int LD, typename T>
+__device__ __forceinline__ unsigned eri_block(const EriArgs& a, const PairHdr& P1, const PairHdr& P2, T* e) {
     using CI = ClassInfo<LA, LB, LC, LD>;
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND, NCOMP = CI::NCOMP, NR = CI::NR;
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    int pab, pcd;
-    double q12;
-    if (!quartet_of(a, g, pab, pcd, q12)) return;
-    const PairHdr P1 = a.pairs[pab];
-    const PairHdr P2 = a.pairs[pcd];
-    if (MODE <= kDigestUHF && q12 >= 0.0 && screened(a, P1, P2, q12)) return;
-    T e[NCOMP];
 #pragma unroll
     for (int c = 0; c < NCOMP; ++c) e[c] = 0;
-    unsigned long long nprimq = 0;
+    unsigned nprimq = 0;
     const T ab[3] = {(T)P1.AB[0], (T)P1.AB[1], (T)P1.AB[2]};
     const T cd[3] = {(T)P2.AB[0], (T)P2.AB[1], (T)P2.AB[2]};
-    for (int ip = 0; ip < P1.nprim; ++ip) {
-        const PrimPair pp = a.prims[P1.prim0 + ip];
+    for (int iq = 0; iq < P2.nprim; ++iq) {
+        const PrimPair qq = a.prims[P2.prim0 + iq];
+        const T q = (T)qq.p, iq_ = (T)qq.ip, sq = (T)qq.s, Kq = (T)qq.K;
         // coordinate differences in double, then rounded to the compute type
-        const T PA[3] = {(T)(pp.P[0] - P1.A[0]), (T)(pp.P[1] - P1.A[1]), (T)(pp.P[2] - P1.A[2])};
-        const T p = (T)pp.p, ip_ = (T)pp.ip, sp = (T)pp.s, Kp = (T)pp.K;
-        for (int iq = 0; iq < P2.nprim; ++iq) {
-            const PrimPair qq = a.prims[P2.prim0 + iq];
-            const T q = (T)qq.p, iq_ = (T)qq.ip;
+        const T QC[3] = {(T)(qq.P[0] - P2.A[0]), (T)(qq.P[1] - P2.A[1]), (T)(qq.P[2] - P2.A[2])};
+        for (int ip = 0; ip < P1.nprim; ++ip) {
+            const PrimPair pp = a.prims[P1.prim0 + ip];
+            const T p = (T)pp.p, ip_ = (T)pp.ip;
             const T it = (T)1 / (p + q), rt = sqrt(it);
-            const T bare = (T)kTwoPi25 * sp * (T)qq.s * ip_ * iq_ * rt;
+            const T bare = (T)kTwoPi25 * (T)pp.s * sq * ip_ * iq_ * rt;
             if (bare < (T)a.primcut) continue;
             ++nprimq;
-            const T pref = (T)kTwoPi25 * Kp * (T)qq.K * ip_ * iq_ * rt;
+            const T pref = (T)kTwoPi25 * (T)pp.K * Kq * ip_ * iq_ * rt;
+            const T PA[3] = {(T)(pp.P[0] - P1.A[0]), (T)(pp.P[1] - P1.A[1]), (T)(pp.P[2] - P1.A[2])};
             const T PQ[3] = {(T)(pp.P[0] - qq.P[0]), (T)(pp.P[1] - qq.P[1]), (T)(pp.P[2] - qq.P[2])};
-            const T QC[3] = {(T)(qq.P[0] - P2.A[0]), (T)(qq.P[1] - P2.A[1]), (T)(qq.P[2] - P2.A[2])};
             const T x = p * q * it * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
             T u[NR], w[NR];
             rys_roots<NR, T>(x, u, w);
@@ -184,13 +177,14 @@ __global__ __launch_bounds__(256) void k_eri_lane(const EriArgs a) {
             }
         }
     }
-    if (a.counter) {
-        atomicAdd(a.counter, nprimq);
-        atomicAdd(a.counter + 1, 1ull);
-    }
-    // normalisation (AuxFunctions.hpp:49-64) and permutational factor
-    T f = 1;
-    if (MODE <= kDigestUHF) f = (P1.same ? (T)0.5 : (T)1) * (P2.same ? (T)0.5 : (T)1) * (pab == pcd ? (T)0.5 : (T)1);
+    return nprimq;
+}
+
+// normalisation (AuxFunctions.hpp:49-64) times a factor f
+template <int LA, int LB, int LC, int LD, typename T>
+__device__ __forceinline__ void normalise_block(T* e, T f) {
+    using CI = ClassInfo<LA, LB, LC, LD>;
+    constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND;
 #pragma unroll
     for (int i = 0; i < NA; ++i)
 #pragma unroll
@@ -201,6 +195,31 @@ __global__ __launch_bounds__(256) void k_eri_lane(const EriArgs a) {
                 for (int l = 0; l < ND; ++l)
                     e[((i * NB + j) * NC + k) * ND + l] *=
                         f * (T)(cart_nrm(LA, i) * cart_nrm(LB, j) * cart_nrm(LC, k) * cart_nrm(LD, l));
+}
+
+// ---------------------------------------------------------------------------
+// one lane per shell quartet of an explicit or prefix-space list (block / Schwarz modes)
+// ---------------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD, int MODE, typename T>
+__global__ __launch_bounds__(256) void k_eri_lane(const EriArgs a) {
+    using CI = ClassInfo<LA, LB, LC, LD>;
+    constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND, NCOMP = CI::NCOMP;
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int pab, pcd;
+    double q12;
+    if (!quartet_of(a, g, pab, pcd, q12)) return;
+    const PairHdr P1 = a.pairs[pab];
+    const PairHdr P2 = a.pairs[pcd];
+    if (MODE <= kDigestUHF && q12 >= 0.0 && screened(a, P1, P2, q12)) return;
+    T e[NCOMP];
+    const unsigned nprimq = eri_block<LA, LB, LC, LD, T>(a, P1, P2, e);
+    if (a.counter) {
+        atomicAdd(a.counter, (unsigned long long)nprimq);
+        atomicAdd(a.counter + 1, 1ull);
+    }
+    T f = 1;
+    if (MODE <= kDigestUHF) f = (P1.same ? (T)0.5 : (T)1) * (P2.same ? (T)0.5 : (T)1) * (pab == pcd ? (T)0.5 : (T)1);
+    normalise_block<LA, LB, LC, LD, T>(e, f);
     if constexpr (MODE == kDigestBlock) {
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) a.out[g * NCOMP + c] = (double)e[c];
@@ -215,6 +234,173 @@ __global__ __launch_bounds__(256) void k_eri_lane(const EriArgs a) {
         a.out[pab] = sqrt(m);
     } else {
         digest_regs<LA, LB, LC, LD, MODE, T>(a, P1, P2, e);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fock builds: one wave per (bra pair, chunk of up to 64 kets).  The bra is the same in
+// every lane, so its records are read at one address, the bra search is one per wave,
+// and the Coulomb outputs of the bra are reduced across the wave before one atomic.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+template <int LA, int LB, int LC, int LD, int MODE, typename T>
+__global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
+    using CI = ClassInfo<LA, LB, LC, LD>;
+    constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND, NCOMP = CI::NCOMP;
+    const int lane = threadIdx.x & 63;
+    const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (t >= a.ntask) return;   // whole wave
+    // bra of task t: the largest i with wpre[i] <= t, bracketed by the task block table
+    int lo = a.tblk[t >> 6], hi = a.tblk[(t >> 6) + 1];
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.wpre[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    const int bra = __builtin_amdgcn_readfirstlane(lo);
+    const int chunk = __builtin_amdgcn_readfirstlane((int)(t - a.wpre[bra]));
+    const long long cnt = a.prefix[bra + 1] - a.prefix[bra];
+    const int jk = chunk * 64 + lane;
+    const int pab = a.bra_off + bra;
+    const PairHdr P1 = a.pairs[pab];
+    const int pcd = a.ket_off + (a.kstart ? a.kstart[bra] : 0) + jk;
+    bool active = jk < cnt;
+    PairHdr P2;
+    if (active) {
+        P2 = a.pairs[pcd];
+        active = !screened(a, P1, P2, a.pairQ[pab] * a.pairQ[pcd]);
+    }
+    T e[NCOMP];
+    unsigned nprimq = 0;
+    if (active) {
+        nprimq = eri_block<LA, LB, LC, LD, T>(a, P1, P2, e);
+        const T f = (P1.same ? (T)0.5 : (T)1) * (P2.same ? (T)0.5 : (T)1) * (pab == pcd ? (T)0.5 : (T)1);
+        normalise_block<LA, LB, LC, LD, T>(e, f);
+    } else {
+#pragma unroll
+        for (int c = 0; c < NCOMP; ++c) e[c] = 0;
+    }
+    if (a.counter && active) {
+        atomicAdd(a.counter, (unsigned long long)nprimq);
+        atomicAdd(a.counter + 1, 1ull);
+    }
+    const int n = a.n;
+    const int fa = P1.f1, fb = P1.f2;
+    constexpr T jf = (MODE == kDigestRHF) ? 4 : 2;
+    // Coulomb, bra outputs: W_ab += jf sum_cd e D_cd, summed over the wave
+    {
+        T dcd[NC * ND];
+        if (active) {
+            const int fc = P2.f1, fd = P2.f2;
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+#pragma unroll
+                for (int l = 0; l < ND; ++l) dcd[k * ND + l] = (T)a.D0[(long)(fc + k) * n + fd + l];
+        } else {
+#pragma unroll
+            for (int kl = 0; kl < NC * ND; ++kl) dcd[kl] = 0;
+        }
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                T s = 0;
+#pragma unroll
+                for (int kl = 0; kl < NC * ND; ++kl) s = fma(e[(i * NB + j) * NC * ND + kl], dcd[kl], s);
+                const double tot = wave_sum((double)(jf * s));
+                if (lane == 0) {
+                    atomic_add_f64(a.W0 + (long)(fa + i) * n + fb + j, tot);
+                    if (MODE == kDigestUHF) atomic_add_f64(a.W1 + (long)(fa + i) * n + fb + j, tot);
+                }
+            }
+    }
+    if (!active) return;
+    const int fc = P2.f1, fd = P2.f2;
+    // Coulomb, ket outputs: W_cd += jf sum_ab e D_ab
+    {
+        T dab[NA * NB];
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) dab[i * NB + j] = (T)a.D0[(long)(fa + i) * n + fb + j];
+#pragma unroll
+        for (int kl = 0; kl < NC * ND; ++kl) {
+            T s = 0;
+#pragma unroll
+            for (int ij = 0; ij < NA * NB; ++ij) s = fma(e[ij * NC * ND + kl], dab[ij], s);
+            const int k = kl / ND, l = kl % ND;
+            atomic_add_f64(a.W0 + (long)(fc + k) * n + fd + l, (double)(jf * s));
+            if (MODE == kDigestUHF) atomic_add_f64(a.W1 + (long)(fc + k) * n + fd + l, (double)(jf * s));
+        }
+    }
+    // exchange, once per spin density
+#pragma unroll
+    for (int sp = 0; sp < (MODE == kDigestUHF ? 2 : 1); ++sp) {
+        const double* DK = (MODE == kDigestUHF) ? (sp == 0 ? a.D1 : a.D2) : a.D0;
+        double* W = (sp == 0) ? a.W0 : a.W1;
+        T dbd[NB * ND], dbc[NB * NC], dad[NA * ND], dac[NA * NC];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+#pragma unroll
+            for (int l = 0; l < ND; ++l) dbd[j * ND + l] = (T)DK[(long)(fb + j) * n + fd + l];
+#pragma unroll
+            for (int k = 0; k < NC; ++k) dbc[j * NC + k] = (T)DK[(long)(fb + j) * n + fc + k];
+        }
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+#pragma unroll
+            for (int l = 0; l < ND; ++l) dad[i * ND + l] = (T)DK[(long)(fa + i) * n + fd + l];
+#pragma unroll
+            for (int k = 0; k < NC; ++k) dac[i * NC + k] = (T)DK[(long)(fa + i) * n + fc + k];
+        }
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                T s = 0;
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+#pragma unroll
+                    for (int l = 0; l < ND; ++l) s = fma(e[((i * NB + j) * NC + k) * ND + l], dbd[j * ND + l], s);
+                atomic_add_f64(W + (long)(fa + i) * n + fc + k, (double)(-s));
+            }
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+#pragma unroll
+            for (int l = 0; l < ND; ++l) {
+                T s = 0;
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) s = fma(e[((i * NB + j) * NC + k) * ND + l], dbc[j * NC + k], s);
+                atomic_add_f64(W + (long)(fa + i) * n + fd + l, (double)(-s));
+            }
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                T s = 0;
+#pragma unroll
+                for (int i = 0; i < NA; ++i)
+#pragma unroll
+                    for (int l = 0; l < ND; ++l) s = fma(e[((i * NB + j) * NC + k) * ND + l], dad[i * ND + l], s);
+                atomic_add_f64(W + (long)(fb + j) * n + fc + k, (double)(-s));
+            }
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+            for (int l = 0; l < ND; ++l) {
+                T s = 0;
+#pragma unroll
+                for (int i = 0; i < NA; ++i)
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) s = fma(e[((i * NB + j) * NC + k) * ND + l], dac[i * NC + k], s);
+                atomic_add_f64(W + (long)(fb + j) * n + fd + l, (double)(-s));
+            }
     }
 }
 
@@ -469,7 +655,12 @@ static hipError_t launch_cls(const EriArgs& a, hipStream_t st) {
     EriArgs b = a;
     b.qpb = quartets_per_block(CI::NCOMP);
     const long long nb = (a.nquart + b.qpb - 1) / b.qpb;
-    if constexpr (CI::NCOMP <= 81) {
+    if constexpr (CI::NCOMP <= 81 && MODE <= kDigestUHF) {
+        // Fock builds: wave tasks (bra, chunk of 64 kets), four per block
+        if (a.ntask <= 0 || !a.wpre || !a.tblk) return hipErrorInvalidValue;
+        const long long nt = (a.ntask + 3) / 4;
+        hipLaunchKernelGGL((k_eri_bra<LA, LB, LC, LD, MODE, T>), dim3((unsigned)nt), dim3(256), 0, st, b);
+    } else if constexpr (CI::NCOMP <= 81) {
         hipLaunchKernelGGL((k_eri_lane<LA, LB, LC, LD, MODE, T>), dim3((unsigned)nb), dim3(256), 0, st, b);
     } else {
         hipLaunchKernelGGL((k_eri_wave<LA, LB, LC, LD, MODE, T>), dim3((unsigned)nb), dim3(64), 0, st, b);
@@ -527,6 +718,8 @@ int eri_class_ncomp(int cb, int ck) {
 }
 
 int eri_quartets_per_block(int cb, int ck) { return quartets_per_block(eri_class_ncomp(cb, ck)); }
+
+bool eri_uses_wave_tasks(int cb, int ck) { return eri_class_ncomp(cb, ck) <= 81; }
 
 // ---------------------------------------------------------------------------
 // Rys-table evaluation for the parity tests

@@ -11,6 +11,9 @@ hipError_t eri_launch(int mode, int bra_class, int ket_class, const EriArgs& a, 
 int eri_class_ncomp(int bra_class, int ket_class);
 // quartets one block of the class kernel walks (the block table granularity)
 int eri_quartets_per_block(int bra_class, int ket_class);
+// Fock-build launches of this class run as wave tasks (bra, up to 64 kets): the space needs
+// wpre / tblk / ntask instead of the quartet block table
+bool eri_uses_wave_tasks(int bra_class, int ket_class);
 hipError_t rys_eval_launch(int n, int npts, const double* x, double* u, double* w, hipStream_t st, bool fp32 = false);
 
 // dense helpers (linalg.hip); matrices are row-major n x n

@@ -62,6 +62,23 @@ static void build_prefix(uno_ctx* c, QuartetClass& q, double dmax, double fp32_t
     q.s32.nq = mixed ? q.s32.prefix[nb] : 0;
 }
 
+// wave tasks: bra i owns ceil(count_i / 64) consecutive tasks from wpre[i]; tblk[b] is
+// the bra of task 64 b, bracketing each wave's bra search
+static void build_tasks(QuartetSpace& s) {
+    const int nb = (int)s.prefix.size() - 1;
+    s.wpre.assign(nb + 1, 0);
+    for (int i = 0; i < nb; ++i) s.wpre[i + 1] = s.wpre[i] + (s.prefix[i + 1] - s.prefix[i] + 63) / 64;
+    s.ntask = s.wpre[nb];
+    const long long nblk = (s.ntask + 63) / 64;
+    s.tblk.resize(nblk + 1);
+    int i = 0;
+    for (long long b = 0; b <= nblk; ++b) {
+        const long long t = std::min(b * 64, s.ntask - 1);
+        while (s.wpre[i + 1] <= t) ++i;
+        s.tblk[b] = i;
+    }
+}
+
 // block table: bra holding the first quartet of every block, so that each lane's bra
 // search covers a short range
 static void build_blocks(QuartetSpace& s, int qpb) {
@@ -300,16 +317,26 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
         for (int prec = 0; prec < 2; ++prec) {
             QuartetSpace& sp = prec ? q.s32 : q.s64;
             if (sp.nq == 0) continue;
-            build_blocks(sp, qpb);
+            const bool tasks = eri_uses_wave_tasks(LX.lcls, LY.lcls);
             UNO_HIP(sp.d_prefix.upload(sp.prefix.data(), sp.prefix.size(), st));
-            UNO_HIP(sp.d_blk.upload(sp.blk.data(), sp.blk.size(), st));
+            if (tasks) {
+                build_tasks(sp);
+                UNO_HIP(sp.d_wpre.upload(sp.wpre.data(), sp.wpre.size(), st));
+                UNO_HIP(sp.d_tblk.upload(sp.tblk.data(), sp.tblk.size(), st));
+            } else {
+                build_blocks(sp, qpb);
+                UNO_HIP(sp.d_blk.upload(sp.blk.data(), sp.blk.size(), st));
+            }
             if (prec) UNO_HIP(sp.d_kstart.upload(sp.kstart.data(), sp.kstart.size(), st));
             EriArgs a = base_args(c);
             a.bra_off = LX.off;
             a.ket_off = LY.off;
             a.pairQ = c->d_pairQ.p;
             a.prefix = sp.d_prefix.p;
-            a.blk = sp.d_blk.p;
+            a.blk = tasks ? nullptr : sp.d_blk.p;
+            a.wpre = tasks ? sp.d_wpre.p : nullptr;
+            a.tblk = tasks ? sp.d_tblk.p : nullptr;
+            a.ntask = tasks ? sp.ntask : 0;
             a.kstart = prec ? sp.d_kstart.p : nullptr;
             a.qpb = qpb;
             a.nbra = LX.count;

@@ -66,11 +66,11 @@ struct PairClassList {
 // One quartet space of a launch: per bra, a contiguous range of kets (prefix counts, the
 // first ket of each range when it does not start at 0) and the block table.
 struct QuartetSpace {
-    std::vector<long long> prefix;
-    std::vector<int> kstart, blk;
-    DBuf<long long> d_prefix;
-    DBuf<int> d_kstart, d_blk;
-    long long nq = 0;
+    std::vector<long long> prefix, wpre;
+    std::vector<int> kstart, blk, tblk;
+    DBuf<long long> d_prefix, d_wpre;
+    DBuf<int> d_kstart, d_blk, d_tblk;
+    long long nq = 0, ntask = 0;
 };
 
 struct QuartetClass {
