@@ -1,18 +1,29 @@
 #!/usr/bin/env python3
-"""Benchmark: SCF time-to-converge (|dE| < 1e-8 Eh) and Fock builds/s on one MI355X.
+"""Benchmark: SCF time-to-converge (|dE| < 1e-8 Eh) and Fock builds/s on one MI355X,
+mixed precision + learned guess against the plain FP64 GPU path.
 
 Workload (default): seeded 64-water cluster, 6-31G* (1216 Cartesian functions), RHF,
-core-Hamiltonian guess, Pulay DIIS, FP64 throughout; synthetic geometry from the
-project's own generator (oxygens uniform in a 12.4 A cube at 1 g/cm^3, O-O >= 2.5 A,
-Haar-random orientations, seed 64001).  One step = one full SCF from the geometry:
-context build (shell pairs, Schwarz bounds on the GPU) + SCF iterations to |dE| < 1e-8.
+Pulay DIIS, converged to |dE| < 1e-8 Eh; synthetic geometry from the project's own
+generator (oxygens uniform in a 12.4 A cube at 1 g/cm^3, O-O >= 2.5 A, Haar-random
+orientations, seed 64001).  One step = one full SCF from the geometry: context build
+(shell pairs, Schwarz bounds on the GPU), one-electron matrices, the guess and every
+iteration.
 
-Prints ONE JSON line with the metric, a roofline object for the dominant ERI kernel
-(per-class kernel time from HIP events on the engine's stream, algorithmic FLOPs from
-the per-primitive-quartet operation count of the kernel's algorithm times the primitive
-quartets the kernel evaluated) and a cpu_baseline object (the reference's Rys ERI code,
-compiled from its sources into oracle/_ref, timed on a random sample of the same
-workload's shell quartets on one host core).
+  * timed steps (the metric): mixed precision (FP32 for quartets with Schwarz bound
+    below 1e-6, incremental density-change builds, the last iterations in full FP64)
+    with the learned density guess;
+  * reference run (reported beside it): plain FP64, core-Hamiltonian guess, full builds.
+    The final energies of the two must agree to 1e-8 Eh.
+
+The learned guess loads unomol_amd/models/guess_631gs.bin; without it, a network is
+trained first on small clusters (outside the timed region, reported).
+
+Prints ONE JSON line with the metric, a roofline object for the dominant FP64 ERI kernel
+class (per-class kernel time from HIP events on the engine's stream; algorithmic FLOPs
+= the per-primitive-quartet operation count of the kernel's algorithm times the
+primitive quartets the kernel evaluated) and a cpu_baseline object (the reference's Rys
+ERI code compiled from its sources into oracle/_ref, timed on a random sample of the
+same workload's shell quartets on one host core).
 """
 import argparse
 import ctypes as C
@@ -30,6 +41,12 @@ sys.path.insert(0, ROOT)
 PAIR_L = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]
 PAIR_NAME = ["ss", "ps", "pp", "ds", "dp", "dd"]
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (and matrix) dense peak
+FP32_PEAK_TFLOPS = 157.3
+MODEL = os.path.join(ROOT, "unomol_amd", "models", "guess_631gs.bin")
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
 def ncart(l):
@@ -58,10 +75,6 @@ def flops_per_shell_quartet_digest(la, lb, lc, ld):
     return 6 * 2 * ncomp + ncomp
 
 
-def log(msg):
-    print("[bench] " + msg, file=sys.stderr, flush=True)
-
-
 def cpu_baseline(deck_path, budget_s=15.0):
     """Reference ERI code (oracle/_ref/libref_harness.so) on a random sample of canonical
     shell quartets of the deck, one host core; extrapolated to one full ERI pass."""
@@ -84,6 +97,57 @@ def cpu_baseline(deck_path, budget_s=15.0):
             "shell_quartets_per_s": rate}
 
 
+def profile_classes(N, t, P):
+    """One profiled Fock build on density P: per-class (FP64 and FP32) kernel time and
+    evaluated primitive / shell quartets."""
+    ms = np.zeros(72); launches = np.zeros(72); prim = np.zeros(72); shell = np.zeros(72)
+    N.check(N.lib().uno_profile(t.ctx, 1, None, None, None, None))
+    G = np.zeros_like(P)
+    t.formGmatrix(P, G)
+    N.check(N.lib().uno_profile(t.ctx, -1, N.ptr(ms), N.ptr(launches), N.ptr(prim), N.ptr(shell)))
+    N.check(N.lib().uno_profile(t.ctx, 0, None, None, None, None))
+    classes = []
+    for prec in range(2):
+        for X in range(6):
+            for Y in range(X + 1):
+                ci = 36 * prec + 6 * X + Y
+                if launches[ci] == 0:
+                    continue
+                la, lb = PAIR_L[X]
+                lc, ld = PAIR_L[Y]
+                fl = prim[ci] * flops_per_prim_quartet(la, lb, lc, ld) + shell[ci] * flops_per_shell_quartet_digest(
+                    la, lb, lc, ld)
+                classes.append({"class": "(%s|%s)" % (PAIR_NAME[X], PAIR_NAME[Y]), "precision": "f32" if prec else "f64",
+                                "ms": ms[ci], "prim_quartets": prim[ci], "shell_quartets": shell[ci],
+                                "gflop": fl / 1e9, "tflops": fl / (ms[ci] * 1e-3) / 1e12 if ms[ci] > 0 else 0.0})
+    return classes
+
+
+def roofline_of(classes):
+    f64 = [c for c in classes if c["precision"] == "f64"]
+    dom = max(f64, key=lambda c: c["ms"])
+    tot_fl = sum(c["gflop"] for c in f64) * 1e9
+    tot_ms = sum(c["ms"] for c in f64)
+    return {"bound": "valu-fp64", "kernel": "k_eri %s" % dom["class"],
+            "achieved": round(dom["tflops"], 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(dom["tflops"] / FP64_PEAK_TFLOPS, 5), "traffic": None,
+            "all_eri_kernels": {"ms": round(tot_ms, 3), "tflops": round(tot_fl / (tot_ms * 1e-3) / 1e12, 4),
+                                "frac": round(tot_fl / (tot_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 5)}}
+
+
+def get_model(unomol_amd):
+    if os.path.exists(MODEL):
+        return unomol_amd.MLP(path=MODEL), {"source": "unomol_amd/models/guess_631gs.bin"}
+    from unomol_amd import guess_train as GT
+    t0 = time.perf_counter()
+    X, Y, M = GT.make_training_set(n_clusters=60, min_w=2, max_w=8, seed=1000, log=log)
+    net, loss = GT.train_guess(X, Y, M, steps=2000)
+    dt = time.perf_counter() - t0
+    log("trained a guess network in %.1f s (loss %.2e)" % (dt, loss))
+    return net, {"source": "trained in this run (60 clusters of 2-8 waters)", "train_s": round(dt, 2),
+                 "loss": loss}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,8 +158,9 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true", help="time the plain FP64 path as the steps")
     ap.add_argument("--fock-only", action="store_true",
-                    help="hardware-counter runs: context build, 2 SCF updates, one profiled Fock "
+                    help="hardware-counter runs: context build, 2 FP64 SCF updates, one profiled Fock "
                          "build and a 1 GiB calibration copy; prints the per-class table")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -110,78 +175,74 @@ def main():
     unomol_amd.generate_deck(deck, "water_cluster" if args.waters > 1 else "water", args.waters, args.seed,
                              "6-31G*", 0, args.eps, 200)
     basis = unomol_amd.Basis(deck)
-    nelec = basis.deck["nelec"]
-    opts = dict(maxits=200, eps=args.eps, cflag=3, accel=2, guess=0)
-    if args.fock_only:
-        opts["maxits"] = 2
-        args.warmup, args.steps = 0, 1
+    log("deck: %d functions, %d electrons" % (basis.number_of_orbitals(), basis.deck["nelec"]))
+    fp64_opts = dict(maxits=200, eps=args.eps, cflag=3, accel=2, guess=0)
 
-    def one_scf():
+    if args.fock_only:
+        t = unomol_amd.TwoElectronInts(basis)
+        hf = unomol_amd.RestrictedHartreeFock(basis, t)
+        hf.findEnergy(**dict(fp64_opts, maxits=2))
+        classes = profile_classes(N, t, hf.P)
+        ms = C.c_double()
+        N.check(N.lib().uno_copy_probe(C.c_ulonglong(1 << 30), 1, C.byref(ms)))
+        print(json.dumps({"fock_only": True, "copy_probe_bytes": 1 << 30, "copy_ms": ms.value,
+                          "copy_GBps": 2 * (1 << 30) / (ms.value * 1e-3) / 1e9, "roofline": roofline_of(classes),
+                          "eri_classes": classes}))
+        return
+
+    model, model_info = (None, None) if args.fp64_only else get_model(unomol_amd)
+    fast_opts = dict(fp64_opts, precision=1, incremental=1, guess=2, fp64_final=3)
+
+    def one_scf(opts):
         t0 = time.perf_counter()
         t = unomol_amd.TwoElectronInts(basis)
+        if opts.get("guess") == 2:
+            unomol_amd.set_guess_model(t, model)
         hf = unomol_amd.RestrictedHartreeFock(basis, t)
         e = hf.findEnergy(**opts)
         dt = time.perf_counter() - t0
+        if not hf.result.converged:
+            raise RuntimeError("SCF did not converge")
         return t, hf, e, dt
 
-    log("deck: %d functions, %d electrons" % (basis.number_of_orbitals(), nelec))
+    step_opts = fp64_opts if args.fp64_only else fast_opts
     for i in range(args.warmup):
-        _, _, e, dt = one_scf()
+        _, _, e, dt = one_scf(step_opts)
         log("warmup %d: E=%.10f in %.2f s" % (i, e, dt))
     results = []
     t_all = time.perf_counter()
     for _ in range(args.steps):
-        t, hf, e, dt = one_scf()
-        log("step: E=%.10f in %.2f s" % (e, dt))
+        t, hf, e, dt = one_scf(step_opts)
+        log("step: E=%.10f in %.2f s, %d iterations" % (e, dt, hf.result.iterations))
         results.append((t, hf, e, dt))
     wall = time.perf_counter() - t_all
     t, hf, energy, dt = results[-1]
     r = hf.result
-    fock_per_s = r.nfock / r.fock_s if r.fock_s > 0 else None
 
-    roofline = None
-    classes = []
+    # the plain FP64 reference run (core-H guess, full FP64 builds)
+    if args.fp64_only:
+        t64, hf64, e64, dt64 = t, hf, energy, dt
+    else:
+        t64, hf64, e64, dt64 = one_scf(fp64_opts)
+        log("fp64 reference: E=%.10f in %.2f s, %d iterations" % (e64, dt64, hf64.result.iterations))
+    r64 = hf64.result
+
+    classes, roofline = [], None
     if not args.no_profile:
-        # profiling pass: one Fock build on the converged density with per-class HIP events
-        ms = np.zeros(72); launches = np.zeros(72); prim = np.zeros(72); shell = np.zeros(72)
-        N.check(N.lib().uno_profile(t.ctx, 1, None, None, None, None))
-        n = t.norb
-        G = np.zeros_like(hf.P)
-        t.formGmatrix(hf.P, G)
-        N.check(N.lib().uno_profile(t.ctx, -1, N.ptr(ms), N.ptr(launches), N.ptr(prim), N.ptr(shell)))
-        N.check(N.lib().uno_profile(t.ctx, 0, None, None, None, None))
-        tot_fl, tot_ms = 0.0, 0.0
-        for X in range(6):
-            for Y in range(X + 1):
-                ci = 6 * X + Y
-                if launches[ci] == 0:
-                    continue
-                la, lb = PAIR_L[X]
-                lc, ld = PAIR_L[Y]
-                fl = prim[ci] * flops_per_prim_quartet(la, lb, lc, ld) + shell[ci] * flops_per_shell_quartet_digest(
-                    la, lb, lc, ld)
-                classes.append({"class": "(%s|%s)" % (PAIR_NAME[X], PAIR_NAME[Y]), "ms": ms[ci],
-                                "prim_quartets": prim[ci], "shell_quartets": shell[ci], "gflop": fl / 1e9,
-                                "tflops": fl / (ms[ci] * 1e-3) / 1e12 if ms[ci] > 0 else 0.0})
-                tot_fl += fl
-                tot_ms += ms[ci]
-        dom = max(classes, key=lambda c: c["ms"])
-        roofline = {"bound": "valu-fp64", "kernel": "k_eri %s" % dom["class"],
-                    "achieved": round(dom["tflops"], 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(dom["tflops"] / FP64_PEAK_TFLOPS, 5), "traffic": None,
-                    "all_eri_kernels": {"ms": round(tot_ms, 3), "tflops": round(tot_fl / (tot_ms * 1e-3) / 1e12, 4),
-                                        "frac": round(tot_fl / (tot_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 5)}}
-    if args.fock_only:
-        ms = C.c_double()
-        N.check(N.lib().uno_copy_probe(C.c_ulonglong(1 << 30), 1, C.byref(ms)))
-        print(json.dumps({"fock_only": True, "copy_probe_bytes": 1 << 30, "copy_ms": ms.value,
-                          "copy_GBps": 2 * (1 << 30) / (ms.value * 1e-3) / 1e9, "roofline": roofline,
-                          "eri_classes": classes}))
-        return
+        classes = profile_classes(N, t64, hf64.P)
+        roofline = roofline_of(classes)
     log("cpu baseline")
     cpu = None if args.no_cpu_baseline else cpu_baseline(deck)
+
+    def summary(res, secs):
+        return {"scf_s": round(secs, 4), "iterations": res.iterations, "fock_builds": res.nfock,
+                "full_builds": res.nfull, "fock_s": round(res.fock_s, 4), "diag_s": round(res.diag_s, 4),
+                "guess_s": round(res.guess_s, 4), "shell_quartets": res.quartets, "fp32_quartets": res.quartets32,
+                "fock_builds_per_s": round(res.nfock / res.fock_s, 4) if res.fock_s > 0 else None,
+                "shell_quartets_per_s": round(res.quartets / res.fock_s, 1) if res.fock_s > 0 else None}
+
     out = {
-        "metric": "SCF time-to-converge (|dE|<1e-8 Eh) + Fock builds/s, FP64 GPU",
+        "metric": "SCF time-to-converge (|dE|<1e-8 Eh) + Fock builds/s, mixed+learned vs FP64 GPU",
         "value": round(wall / args.steps, 4),
         "unit": "s",
         "n_gpus": 1,
@@ -191,18 +252,21 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.fp64_only else "f64+f32",
         "data": "synthetic",
-        "config": {"workload": "%d-water 6-31G* RHF (%d functions), seed %d, core-H guess, DIIS, FP64" %
-                               (args.waters, t.norb, args.seed),
+        "config": {"workload": "%d-water 6-31G* RHF (%d functions), seed %d, DIIS, |dE|<%g Eh; steps: %s" %
+                               (args.waters, t.norb, args.seed, args.eps,
+                                "plain FP64, core-H guess" if args.fp64_only else
+                                "mixed FP32/FP64 + incremental builds + learned guess"),
                    "global_batch": 1, "parallelism": "single-gpu"},
         "energy": energy,
-        "iterations": r.iterations,
-        "fock_builds_per_s": round(fock_per_s, 4) if fock_per_s else None,
-        "fock_s": round(r.fock_s, 4),
-        "diag_s": round(r.diag_s, 4),
-        "scf_wall_s": round(r.wall_s, 4),
-        "shell_quartets_per_s": round(r.quartets / r.fock_s, 1) if r.fock_s > 0 else None,
+        "energy_fp64": e64,
+        "energy_diff_vs_fp64": energy - e64,
+        "fp64_s": round(dt64, 4),
+        "speedup_vs_fp64": round(dt64 / (wall / args.steps), 3),
+        "steps_detail": summary(r, wall / args.steps),
+        "fp64_detail": summary(r64, dt64),
+        "guess_model": model_info,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "eri_classes": classes,

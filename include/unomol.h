@@ -154,6 +154,8 @@ int uno_copy_probe(unsigned long long bytes, int reps, double* ms);
 /* One-electron matrices (host, McMurchie-Davidson; OneElectronInts.cpp:127-202):
  * packed S, T, H = T + V. */
 int uno_oneint(const uno_basis* basis, double* S, double* T, double* H);
+/* The same S, T, H computed on the device from a context's basis (the SCF uses these). */
+int uno_oneint_device(uno_ctx* ctx, double* S, double* T, double* H);
 /* Raw dipole/quadrupole integrals about the origin, packed, 9 per element
  * (dx,dy,dz,qxx,qxy,qxz,qyy,qyz,qzz) as written to RMOM.DAT (Moments.cpp:95-186). */
 int uno_moment_ints(const uno_basis* basis, double* m);

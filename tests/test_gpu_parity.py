@@ -390,3 +390,22 @@ def test_learned_guess_fewer_iterations(tmp_path):
     assert hf.result.iterations < ref.result.iterations
     # the guess itself is far closer than the core Hamiltonian's
     assert abs(hf.result.init_energy - e0) < 0.2 * abs(ref.result.init_energy - e0)
+
+
+@pytest.mark.parametrize("tag", ["A_water_631g", "water_631gs", "631.ch4", "w2_631gs"])
+def test_oneint_device_vs_reference(golden, tag):
+    """Device S, T, H against the host restatement (itself pinned to the reference's
+    intsout matrices at 1e-13) and, where the fixture holds them, the reference's own."""
+    say("one-electron matrices on the device " + tag)
+    path = R.deck_path(tag)
+    b, t = _tints(path)
+    S, T, H = t.oneint()
+    hS, hT, hH = unomol_amd.oneint(read_deck(path))
+    for got, ref in ((S, hS), (T, hT), (H, hH)):
+        assert np.abs(got - ref).max() < 1e-13 * max(1.0, np.abs(ref).max())
+    npz = os.path.join(golden, "eri_%s.npz" % tag)
+    if os.path.exists(npz):
+        z = np.load(npz)
+        assert np.abs(S - z["S"]).max() < 1e-13
+        assert np.abs(T - z["T"]).max() < 1e-12
+        assert np.abs(H - z["H"]).max() < 1e-12

@@ -95,6 +95,13 @@ class TwoElectronInts:
         N.check(N.lib().uno_eri_emit(self.ctx, C.c_long(n), N.ptr(val), N.ptr(f), C.byref(cnt)))
         return val, f
 
+    def oneint(self):
+        """S, T, H (packed) computed on the device."""
+        n = self.norb
+        S, T, H = [np.zeros(n * (n + 1) // 2) for _ in range(3)]
+        N.check(N.lib().uno_oneint_device(self.ctx, N.ptr(S), N.ptr(T), N.ptr(H)))
+        return S, T, H
+
     def schwarz(self):
         ns = self.basis.deck["nshell"]
         q = np.zeros(ns * (ns + 1) // 2)

@@ -453,6 +453,26 @@ int uno_profile(uno_ctx* c, int enable, double* ms, double* launches, double* pr
     return UNO_OK;
 }
 
+int uno_oneint_device(uno_ctx* c, double* S, double* T, double* H) {
+    if (!c || !S || !T || !H) { set_error("null argument"); return UNO_EINVAL; }
+    const int n = c->n;
+    const size_t nn = (size_t)n * n;
+    DBuf<double> dS, dT, dH;
+    UNO_HIP(dS.alloc(nn));
+    UNO_HIP(dT.alloc(nn));
+    UNO_HIP(dH.alloc(nn));
+    if (int rc = oneint_device(c, dS.p, dT.p, dH.p)) return rc;
+    double* out[3] = {S, T, H};
+    double* src[3] = {dS.p, dT.p, dH.p};
+    for (int k = 0; k < 3; ++k) {
+        UNO_HIP(k_pack(n, src[k], c->pk.p, c->st));
+        UNO_HIP(hipMemcpyAsync(out[k], c->pk.p, (size_t)n * (n + 1) / 2 * sizeof(double), hipMemcpyDeviceToHost,
+                               c->st));
+        UNO_HIP(hipStreamSynchronize(c->st));
+    }
+    return UNO_OK;
+}
+
 int uno_shell_rotation(int l, const double* Q, double* M) {
     if (l < 0 || l > 2 || !Q || !M) { set_error("bad argument (0 <= l <= 2)"); return UNO_EINVAL; }
     shell_rotation(l, Q, M);

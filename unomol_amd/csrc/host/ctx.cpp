@@ -275,6 +275,32 @@ int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
     }
     UNO_HIP(c->d_shoff.upload(shoff.data(), nsh, st));
     UNO_HIP(c->d_shn.upload(shn.data(), nsh, st));
+    // one-electron data: shells with their normalised primitives, centres, and the shell
+    // pairs of each class with the higher-l shell first
+    {
+        std::vector<OneIntShell> hs(nsh);
+        std::vector<double> al, co;
+        for (int s = 0; s < nsh; ++s) {
+            hs[s] = OneIntShell{B.sh[s].l, B.sh[s].npr, B.sh[s].cen, (int)al.size(), B.off[s]};
+            al.insert(al.end(), B.sh[s].a.begin(), B.sh[s].a.end());
+            co.insert(co.end(), B.sh[s].c.begin(), B.sh[s].c.end());
+        }
+        std::vector<std::vector<int2>> pl(kNumPairClasses);
+        for (int a = 0; a < nsh; ++a)
+            for (int bs = 0; bs <= a; ++bs) {
+                const int s1 = B.sh[a].l >= B.sh[bs].l ? a : bs, s2 = s1 == a ? bs : a;
+                pl[pair_class(B.sh[s1].l, B.sh[s2].l)].push_back(int2{s1, s2});
+            }
+        UNO_HIP(c->oi_shells.upload(hs.data(), hs.size(), st));
+        UNO_HIP(c->oi_alpha.upload(al.data(), al.size(), st));
+        UNO_HIP(c->oi_coef.upload(co.data(), co.size(), st));
+        UNO_HIP(c->oi_xyz.upload(B.xyz.data(), B.xyz.size(), st));
+        UNO_HIP(c->oi_charge.upload(B.q.data(), B.q.size(), st));
+        for (int k = 0; k < kNumPairClasses; ++k) {
+            c->oi_npair[k] = (int)pl[k].size();
+            UNO_HIP(c->oi_pairs[k].upload(pl[k].data(), pl[k].size(), st));
+        }
+    }
     const size_t nn = (size_t)c->n * c->n;
     UNO_HIP(c->D0.alloc(nn));
     UNO_HIP(c->D1.alloc(nn));
@@ -285,6 +311,24 @@ int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
     UNO_HIP(c->pk.alloc((size_t)c->n * (c->n + 1)));
     UNO_HIP(c->scal.alloc(16));
     UNO_HIP(hipStreamSynchronize(st));
+    return UNO_OK;
+}
+
+int oneint_device(uno_ctx* c, double* S, double* T, double* H) {
+    OneIntArgs a{};
+    a.shells = c->oi_shells.p;
+    a.alpha = c->oi_alpha.p;
+    a.coef = c->oi_coef.p;
+    a.xyz = c->oi_xyz.p;
+    a.charge = c->oi_charge.p;
+    a.ncen = c->B.ncen;
+    a.n = c->n;
+    a.S = S;
+    a.T = T;
+    a.H = H;
+    static const int kl[kNumPairClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+    for (int k = 0; k < kNumPairClasses; ++k)
+        UNO_HIP(oneint_launch(kl[k][0], kl[k][1], c->oi_npair[k], c->oi_pairs[k].p, a, c->st));
     return UNO_OK;
 }
 

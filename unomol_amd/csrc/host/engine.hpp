@@ -5,6 +5,7 @@
 #include <vector>
 #include "../../../include/unomol.h"
 #include "../hip/devtypes.hpp"
+#include "../hip/oneint_kernels.hpp"
 #include "basis.hpp"
 
 namespace uno {
@@ -126,12 +127,20 @@ struct uno_ctx {
     double prof_launch[uno::kProfSlots] = {};
     uno::DBuf<unsigned long long> d_counter;
     uno_mlp* guess_model = nullptr;   // not owned: the network of guess = 2
+    // one-electron data on the device: shells, primitives, centres and the shell pairs
+    // of each (la >= lb) class
+    uno::DBuf<uno::OneIntShell> oi_shells;
+    uno::DBuf<double> oi_alpha, oi_coef, oi_xyz, oi_charge;
+    uno::DBuf<int2> oi_pairs[uno::kNumPairClasses];
+    int oi_npair[uno::kNumPairClasses] = {};
     ~uno_ctx();
 };
 
 namespace uno {
 
 int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell);
+// dense S, T, H = T + V (n x n, device) on the context's stream
+int oneint_device(uno_ctx* c, double* S, double* T, double* H);
 // Fock build from dense device densities already in c->D0 (and D1, D2 for UHF).
 // Leaves W0 (and W1) = unsymmetrised accumulators; G = W + W^T.  mixed: quartets whose
 // Schwarz bound Qab Qcd is below c->fp32_thr are evaluated in FP32.

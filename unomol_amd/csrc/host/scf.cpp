@@ -99,9 +99,8 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
         UNO_HIP(p->alloc(nn));
     if (o.incremental)
         for (int s = 0; s < ns; ++s) UNO_HIP(b.Pref[s].alloc(nn));
-    // one-electron matrices on the host, once
-    std::vector<double> hS(n2), hT(n2), hH(n2);
-    one_electron(c->B, hS.data(), hT.data(), hH.data(), nullptr);
+    // one-electron matrices on the device, once
+    if (int rc = oneint_device(c, b.S.p, b.T.p, b.H.p)) return rc;
     double nuc = 0.0;
     for (int i = 0; i < c->B.ncen; ++i)
         for (int j = i + 1; j < c->B.ncen; ++j) {
@@ -110,12 +109,6 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
             const double dx = a[0] - bb[0], dy = a[1] - bb[1], dz = a[2] - bb[2];
             nuc += c->B.q[i] * c->B.q[j] / std::sqrt(dx * dx + dy * dy + dz * dz);
         }
-    UNO_HIP(c->pk.upload(hH.data(), n2, st));
-    UNO_HIP(k_unpack(n, c->pk.p, b.H.p, st));
-    UNO_HIP(c->pk.upload(hT.data(), n2, st));
-    UNO_HIP(k_unpack(n, c->pk.p, b.T.p, st));
-    UNO_HIP(c->pk.upload(hS.data(), n2, st));
-    UNO_HIP(k_unpack(n, c->pk.p, b.S.p, st));
     // X = U s^-1/2 keeping every vector (RHF.hpp:214-233)
     std::vector<double> ev(n), hZ(nn);
     UNO_HIP(hipMemcpyAsync(b.Tm.p, b.S.p, nn * sizeof(double), hipMemcpyDeviceToDevice, st));
