@@ -101,6 +101,11 @@ typedef struct uno_scf_opts {
     int fp64_final;      /* mixed: number of final iterations in full FP64 (default 3) */
     int incremental;     /* 1: build F from the density change (Delta-D) */
     int verbose;         /* 1: per-iteration lines on stderr like the reference */
+    double field[3];     /* uniform electric field (au): H -= field . dipole integrals
+                            (FField.cpp:6-20) */
+    int single_start;    /* 1: one update before the loop and no UHF energy reset, as the
+                            reference's finite-field and DPM SCFs (RHF.hpp:246-255); 2: the
+                            same with eps used as given (no DBL_EPSILON N^2/2 floor) */
 } uno_scf_opts;
 
 typedef struct uno_scf_result {

@@ -17,6 +17,8 @@ Sources, all of them the reference's own data or its own code:
 
 usage: make -C oracle && make -C oracle ref && python tests/golden/make_golden.py
        python tests/golden/make_golden.py --config-b    (the 16-water runs only, a few minutes)
+       python tests/golden/make_golden.py --extra       (finite-field decks and the N atom)
+       python tests/golden/make_golden.py --dpm         (DPM scans of water and its cation)
 """
 import ctypes as C
 import os
@@ -50,17 +52,34 @@ B_DECKS = {
     "B_w16_631gs_cation": ("water_cluster", 16, 16001, "6-31G*", 1),
 }
 B_MAXITS = 3
+# further reference decks: the finite-field analysis ones (int_flag[1] = 1: polarizabilities
+# in finitefield.out) and the nitrogen atom (UHF quartet)
+EXTRA_TAGS = ["n.dhdz", "d6s3p.h2", "dh95.h2", "dh95.c2h2", "dh95.co2", "tzp.ch4", "tz2p.sf6"]
+# DPM polarization-potential scans (int_flag[0] = 1) of the 6-31G water and its cation: a
+# small positron basis on the moving centre (posin.bas) and three grid points
+# (pos.grid.dat); both input files are this project's synthetic choices
+DPM_DECKS = {"dpm_water": ("water", 1, 0, "6-31G", 0), "dpm_water_cation": ("water", 1, 0, "6-31G", 1)}
+POSIN = "4 6 1\n1 0\n2.0 1.0\n1 0\n0.5 1.0\n1 0\n0.125 1.0\n1 1\n0.3 1.0\n"
+GRID = "3\n0.0 0.0 -4.0\n0.0 3.0 3.5\n4.5 0.5 0.5\n"
 
 
-def run_ref(deck_text, outdir, exe="Unomol"):
+def run_ref(deck_text, outdir, exe="Unomol", extra_files=None):
     os.makedirs(outdir, exist_ok=True)
     with tempfile.TemporaryDirectory() as tmp:
         with open(os.path.join(tmp, "patin.dat"), "w") as f:
             f.write(deck_text)
+        for name, text in (extra_files or {}).items():
+            with open(os.path.join(tmp, name), "w") as f:
+                f.write(text)
         subprocess.run([os.path.join(ROOT, "oracle", "_ref", exe)], cwd=tmp, check=True,
                        stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
-        for fn in ("short.gs.out", "scfout.gs.out", "moments.out"):
+        for fn in ("short.gs.out", "scfout.gs.out", "moments.out", "finitefield.out", "vpol.out", "spol.out"):
+            if fn in ("finitefield.out", "vpol.out", "spol.out") and not os.path.exists(os.path.join(tmp, fn)):
+                continue
             shutil.copy(os.path.join(tmp, fn), os.path.join(outdir, fn))
+        for name, text in (extra_files or {}).items():
+            with open(os.path.join(outdir, name), "w") as f:
+                f.write(text)
         with open(os.path.join(outdir, "patin.dat"), "w") as f:
             f.write(deck_text)
 
@@ -87,9 +106,41 @@ def make_config_b():
         print("config B reference run done:", tag, flush=True)
 
 
+def make_extra():
+    dst = os.path.join(HERE, "ref_test")
+    for tag in EXTRA_TAGS:
+        src = os.path.join(REF, "test", "patin.dat." + tag)
+        shutil.copy(src, os.path.join(dst, "patin.dat." + tag))
+        run_ref(open(src).read(), os.path.join(HERE, "ref_run_x2", tag), exe="Unomol_x2")
+        print("extra reference run done:", tag, flush=True)
+
+
+def make_dpm():
+    import unomol_amd
+    gdir = os.path.join(HERE, "decks")
+    for tag, (kind, n, seed, basis, charge) in DPM_DECKS.items():
+        path = os.path.join(gdir, tag + ".dat")
+        unomol_amd.generate_deck(path, kind, n, seed, basis, charge, 1e-10, 300)
+        lines = open(path).read().split("\n")
+        # int_flag line (the fourth line): DPM on, finite field off
+        assert lines[3].split() == ["0", "0"], lines[3]
+        lines[3] = "1 0"
+        text = "\n".join(lines)
+        open(path, "w").write(text)
+        run_ref(text, os.path.join(HERE, "ref_run_x2", tag), exe="Unomol_x2",
+                extra_files={"posin.bas": POSIN, "pos.grid.dat": GRID})
+        print("dpm reference run done:", tag, flush=True)
+
+
 def main():
+    if "--dpm" in sys.argv:
+        make_dpm()
+        return
     if "--config-b" in sys.argv:
         make_config_b()
+        return
+    if "--extra" in sys.argv:
+        make_extra()
         return
     import unomol_amd
     from unomol_amd.deck import read_deck

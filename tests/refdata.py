@@ -87,3 +87,16 @@ def assert_moments(tag, mom, mref, tol=1e-7):
         return
     for k in range(9):
         assert abs(mom[k] - ref[k]) <= tol * max(1.0, abs(ref[k])) + tol
+
+
+def read_finitefield(path):
+    """finitefield.out (RHF.hpp:259-270): field magnitude, per-axis (alpha, pol energy)."""
+    out = {"alpha": [], "polnrg": []}
+    for ln in open(path):
+        t = ln.split()
+        if "magnitude" in ln:
+            out["field"] = float(t[-1])
+        elif len(t) == 3 and t[0] in ("x", "y", "z"):
+            out["alpha"].append(float(t[1]))
+            out["polnrg"].append(float(t[2]))
+    return out

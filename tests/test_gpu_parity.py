@@ -409,3 +409,61 @@ def test_oneint_device_vs_reference(golden, tag):
         assert np.abs(S - z["S"]).max() < 1e-13
         assert np.abs(T - z["T"]).max() < 1e-12
         assert np.abs(H - z["H"]).max() < 1e-12
+
+
+FF_TAGS = [t for t in ("n.dhdz", "d6s3p.h2", "dh95.h2", "dh95.c2h2", "dh95.co2", "tzp.ch4", "tz2p.sf6")
+           if os.path.exists(os.path.join(R.GOLDEN, "ref_run_x2", t, "short.gs.out"))]
+
+
+@pytest.mark.parametrize("tag", FF_TAGS)
+def test_finite_field_decks_vs_reference(tmp_path, tag):
+    """Reference decks with the finite-field analysis (int_flag[1] = 1) and the nitrogen
+    atom: ground state and, where requested, the three field SCFs and polarizabilities
+    against the reference (exact 2-root quadrature)."""
+    say("finite-field deck " + tag)
+    src = os.path.join(R.GOLDEN, "ref_run_x2", tag)
+    (tmp_path / "patin.dat").write_text(open(os.path.join(R.GOLDEN, "ref_test", "patin.dat." + tag)).read())
+    r = unomol_amd.run_deck(tmp_path)
+    assert r.converged
+    ref = R.read_short(os.path.join(src, "short.gs.out"))
+    got = R.read_short(str(tmp_path / "short.gs.out"))
+    assert abs(got["energy"] - ref["energy"]) < 1e-9
+    assert abs(got["init"] - ref["init"]) < 1e-9
+    mref = R.read_scfout(os.path.join(src, "scfout.gs.out"))
+    mine = R.read_scfout(str(tmp_path / "scfout.gs.out"))
+    np.testing.assert_allclose(mine["evals"], mref["evals"], atol=1e-9)
+    ffref = os.path.join(src, "finitefield.out")
+    if os.path.exists(ffref):
+        a = R.read_finitefield(ffref)
+        b = R.read_finitefield(str(tmp_path / "finitefield.out"))
+        assert b["field"] == a["field"]
+        np.testing.assert_allclose(b["polnrg"], a["polnrg"], atol=2e-9)
+        np.testing.assert_allclose(b["alpha"], a["alpha"], atol=2e-9 * 2 / a["field"] ** 2)
+
+
+DPM_TAGS = [t for t in ("dpm_water", "dpm_water_cation")
+            if os.path.exists(os.path.join(R.GOLDEN, "ref_run_x2", t, "vpol.out"))]
+
+
+@pytest.mark.parametrize("tag", DPM_TAGS)
+def test_dpm_scan_vs_reference(tmp_path, tag):
+    """DPM polarization-potential scan (int_flag[0] = 1): positron basis on a moving centre,
+    the six-Gaussian model potential and one SCF per grid point, against the reference."""
+    say("dpm scan " + tag)
+    src = os.path.join(R.GOLDEN, "ref_run_x2", tag)
+    for fn in ("patin.dat", "posin.bas", "pos.grid.dat"):
+        (tmp_path / fn).write_text(open(os.path.join(src, fn)).read())
+    unomol_amd.run_deck(tmp_path)
+    ref = np.loadtxt(os.path.join(src, "vpol.out"), ndmin=2)
+    got = np.loadtxt(str(tmp_path / "vpol.out"), ndmin=2)
+    assert got.shape == ref.shape
+    np.testing.assert_allclose(got[:, :3], ref[:, :3], atol=1e-10)
+    np.testing.assert_allclose(got[:, 3], ref[:, 3], atol=1e-9)                 # vpol
+    r4 = (ref[:, :3] ** 2).sum(axis=1) ** 2
+    np.testing.assert_allclose(got[:, 4], ref[:, 4], atol=1e-9 * 2 * r4.max())  # alfa = -2 vpol r^4
+    if ref.shape[1] > 5:
+        np.testing.assert_allclose(got[:, 5:], ref[:, 5:], atol=1e-9)          # vstat, vstat + vpol
+    sref = np.loadtxt(os.path.join(src, "spol.out"), ndmin=2)
+    sgot = np.loadtxt(str(tmp_path / "spol.out"), ndmin=2)
+    # energies print with 10 digits after the point in e-format: compare to that precision
+    np.testing.assert_allclose(sgot[:, 1:5], sref[:, 1:5], rtol=2e-10, atol=1e-9)

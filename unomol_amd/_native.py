@@ -31,7 +31,8 @@ class Basis(C.Structure):
 class ScfOpts(C.Structure):
     _fields_ = [("maxits", C.c_int), ("eps", C.c_double), ("cflag", C.c_int), ("accel", C.c_int),
                 ("guess", C.c_int), ("precision", C.c_int), ("fp64_final", C.c_int),
-                ("incremental", C.c_int), ("verbose", C.c_int)]
+                ("incremental", C.c_int), ("verbose", C.c_int), ("field", C.c_double * 3),
+                ("single_start", C.c_int)]
 
 
 class ScfResult(C.Structure):

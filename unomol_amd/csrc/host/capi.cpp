@@ -1,5 +1,6 @@
 // C ABI (include/unomol.h) and the drop-in deck runner with the reference's outputs.
 #include <cmath>
+#include <fstream>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -581,16 +582,103 @@ int uno_deck_generate(const char* kind, int n, unsigned long long seed, const ch
     return UNO_OK;
 }
 
+// DPM polarization-potential scan (RHF.hpp:292-388, UHF.hpp:293-383): the basis is
+// augmented by the positron centre (charge +1) carrying the shells of posin.bas; for every
+// point of pos.grid.dat the positron's six-Gaussian model potential (GDPMInts.cpp) is
+// added to H and an SCF (eps 1e-12, one update before the loop) starts from the
+// ground-state density.  vpol = E_final - E_init, alfa = -2 vpol r^4 and the static term
+// vstat = E_init - E_gs (at the first point the reference adds the nuclear repulsion once
+// more, RHF.hpp:336; kept) go to vpol.out and spol.out.
+static int run_dpm(const std::string& dir, const Deck& d, const uno_scf_opts& o, const uno_scf_result& gs,
+                   const std::vector<double>& Pgs) {
+    std::ifstream ab((dir + "/posin.bas").c_str());
+    if (!ab) { set_error("could not open posin.bas"); return UNO_EIO; }
+    int xsh = 0, xno = 0, xmaxl = 0;
+    ab >> xsh >> xno >> xmaxl;
+    Deck a = d;
+    a.ncen = d.ncen + 1;
+    a.charge.push_back(1.0);
+    a.xyz.insert(a.xyz.end(), {0.0, 0.0, 0.0});
+    for (int k = 0; k < xsh; ++k) {
+        int npr = 0, l = 0;
+        if (!(ab >> npr >> l) || npr <= 0 || l < 0) { set_error("malformed posin.bas"); return UNO_EIO; }
+        a.sh_l.push_back(l);
+        a.sh_npr.push_back(npr);
+        a.sh_cen.push_back(d.ncen);
+        a.sh_poff.push_back((int)a.alpha.size());
+        for (int q = 0; q < npr; ++q) {
+            double al = 0.0, co = 0.0;
+            if (!(ab >> al >> co)) { set_error("malformed posin.bas"); return UNO_EIO; }
+            a.alpha.push_back(al);
+            a.coef.push_back(co);
+        }
+        a.norb += ncart(l);
+    }
+    a.nshell = d.nshell + xsh;
+    std::ifstream gr((dir + "/pos.grid.dat").c_str());
+    if (!gr) { set_error("could not open pos.grid.dat"); return UNO_EIO; }
+    int npts = 0;
+    gr >> npts;
+    const bool uhf = d.nelec % 2;
+    const int ns = uhf ? 2 : 1;
+    const long n2g = (long)(a.norb - xno) * (a.norb - xno + 1) / 2;
+    const long n2 = (long)a.norb * (a.norb + 1) / 2;
+    FILE* vout = std::fopen((dir + "/vpol.out").c_str(), "w");
+    FILE* sout = std::fopen((dir + "/spol.out").c_str(), "w");
+    if (!vout || !sout) {
+        if (vout) std::fclose(vout);
+        if (sout) std::fclose(sout);
+        set_error("could not create vpol.out / spol.out");
+        return UNO_EIO;
+    }
+    int rc = UNO_OK;
+    for (int i = 0; i < npts && rc == UNO_OK; ++i) {
+        double R[3];
+        if (!(gr >> R[0] >> R[1] >> R[2])) { set_error("malformed pos.grid.dat"); rc = UNO_EIO; break; }
+        for (int x = 0; x < 3; ++x) a.xyz[3 * d.ncen + x] = R[x];
+        uno_basis b;
+        basis_view(a, b);
+        uno_ctx* c = nullptr;
+        if ((rc = uno_ctx_create(&b, 0, &c))) break;
+        std::vector<double> dh;
+        if ((rc = gdpm_matrix(c, R, dh))) { uno_ctx_destroy(c); break; }
+        // the ground-state density padded with zeros for the positron-centre functions
+        std::vector<double> P(ns * n2, 0.0);
+        for (int s = 0; s < ns; ++s) std::copy(Pgs.begin() + s * n2g, Pgs.begin() + (s + 1) * n2g, P.begin() + s * n2);
+        uno_scf_opts so = o;
+        so.eps = 1e-12;   // set after the deck's floor (RHF.hpp:303): no floor here
+        so.guess = 1;
+        so.single_start = 2;
+        so.field[0] = so.field[1] = so.field[2] = 0.0;
+        std::vector<double> ev(ns * (size_t)a.norb), C(ns * (size_t)a.norb * a.norb);
+        uno_scf_result rs{};
+        const int src = scf_run(c, d.nelec, so, rs, ev.data(), P.data(), C.data(), dh.data());
+        uno_ctx_destroy(c);
+        if (src && src != UNO_ENOCONV) { rc = src; break; }
+        const double vpol = rs.energy - rs.init_energy;
+        const double vstat = rs.init_energy - gs.energy + (i == 0 ? rs.nucrep : 0.0);
+        const double r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+        const double alfa = -2.0 * vpol * r2 * r2;
+        if (uhf)
+            std::fprintf(vout, "%15.10lf %15.10lf %15.10lf %25.15le %25.15le\n", R[0], R[1], R[2], vpol, alfa);
+        else
+            std::fprintf(vout, "%15.10lf %15.10lf %15.10lf %25.15le %25.15le %25.15le %25.15le\n", R[0], R[1], R[2],
+                         vpol, alfa, vstat, vstat + vpol);
+        std::fprintf(sout, "%3d %20.10le %20.10le %20.10le %20.10le %25.15le\n", i, gs.energy, rs.init_energy,
+                     rs.energy, vstat, rs.ediff);
+        std::fflush(vout);
+        std::fflush(sout);
+    }
+    std::fclose(vout);
+    std::fclose(sout);
+    return rc;
+}
+
 int uno_run_deck(const char* dir_c, const uno_scf_opts* opts, uno_scf_result* res) {
     const std::string dir = dir_c ? dir_c : ".";
     Deck d;
     std::string err;
     if (!read_deck(dir + "/patin.dat", d, err)) { set_error(err); return UNO_EIO; }
-    if (d.int_flag[0] || d.int_flag[1])
-        // the ground-state SCF and its outputs are produced; the extra SCFs the reference
-        // runs afterwards (finite field, RHF.hpp:235-271; DPM scan, RHF.hpp:292-388) are not
-        std::fprintf(stderr, "note: int_flag (DPM %d, finite field %d) requested; only the ground-state SCF "
-                             "is run by this engine\n", d.int_flag[0], d.int_flag[1]);
     uno_basis b;
     basis_view(d, b);
     uno_ctx* c = nullptr;
@@ -621,7 +709,44 @@ int uno_run_deck(const char* dir_c, const uno_scf_opts* opts, uno_scf_result* re
     if (rc && rc != UNO_ENOCONV) { uno_ctx_destroy(c); return rc; }
     std::vector<double> S(n2), T(n2), H(n2), M(9 * n2);
     one_electron(c->B, S.data(), T.data(), H.data(), M.data());
-    const int wrc = write_outputs(dir, d, c->B, r, S, T, H, M, P, ev, C, d.maxits);
+    int wrc = write_outputs(dir, d, c->B, r, S, T, H, M, P, ev, C, d.maxits);
+    if (!wrc && d.int_flag[1]) {
+        // finite-field analysis (RHF.hpp:235-271, UHF.hpp:239-272): a field of 5e-3 au along
+        // each axis, SCF from the ground-state density with one update before the loop
+        const double emag = 5.0e-3;
+        double polnrg[3], alf[3];
+        for (int ix = 0; ix < 3 && !wrc; ++ix) {
+            uno_scf_opts of = o;
+            of.field[0] = of.field[1] = of.field[2] = 0.0;
+            of.field[ix] = emag;
+            of.guess = 1;
+            of.single_start = 1;
+            std::vector<double> Pf(P), evf(ev.size()), Cf(C.size());
+            uno_scf_result rf{};
+            const int frc = uno_scf_run(c, d.nelec, &of, &rf, evf.data(), Pf.data(), Cf.data());
+            if (frc && frc != UNO_ENOCONV) { wrc = frc; break; }
+            polnrg[ix] = rf.energy - r.energy;
+            alf[ix] = -polnrg[ix] * 2.0 / emag / emag;
+        }
+        if (!wrc) {
+            FILE* f = create(dir, "finitefield.out");
+            if (!f) { set_error("could not create finitefield.out"); wrc = UNO_EIO; }
+            else {
+                std::fprintf(f, "        Unomol Finite Field Analysis\n");
+                std::fprintf(f, "        Electric field magnitude = %20.12le\n", emag);
+                std::fprintf(f, "     alfa                pol energy\n");
+                std::fprintf(f, " x   %20.12le %20.12le\n", alf[0], polnrg[0]);
+                std::fprintf(f, " y   %20.12le %20.12le\n", alf[1], polnrg[1]);
+                std::fprintf(f, " z   %20.12le %20.12le\n", alf[2], polnrg[2]);
+                if (!uhf) {
+                    std::fprintf(f, " alf isotropic = %20.12le\n", (alf[0] + alf[1] + alf[2]) / 3.0);
+                    std::fprintf(f, " alf aniso     = %20.12le\n", (2.0 * alf[2] - alf[1] - alf[0]) / 3.0);
+                }
+                std::fclose(f);
+            }
+        }
+    }
+    if (!wrc && d.int_flag[0]) wrc = run_dpm(dir, d, o, r, P);
     if (res) *res = r;
     uno_ctx_destroy(c);
     return wrc ? wrc : rc;

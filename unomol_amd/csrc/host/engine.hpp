@@ -152,7 +152,12 @@ int eri_emit(uno_ctx* c, long cap, double* val, int* ijkl, long* count);
 int eigh_device(uno_ctx* c, int n, double* dA, double* dZ, double* h_evals);
 int eigh_standalone(int n, double* dA, double* dZ, double* h_evals, hipStream_t st);
 
-int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, double* evals, double* P, double* C);
+// dh (optional, host packed): added to the core Hamiltonian (model potentials)
+int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, double* evals, double* P, double* C,
+            const double* dh = nullptr);
+// -sum_k c_k (ab|g_k): the positron's six-term Gaussian charge at R against every basis
+// function pair (GDPMInts.cpp:6-146), packed
+int gdpm_matrix(uno_ctx* c, const double* R, std::vector<double>& dh);
 
 // learned density guess (guess.cpp): descriptors of the atom pairs within the cutoff
 // (kGuessIn floats each; pairs as (A, B), A <= B), target blocks and masks from a dense

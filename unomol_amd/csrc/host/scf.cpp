@@ -68,7 +68,7 @@ constexpr int kRebuild = 10;            // incremental builds between full rebui
 constexpr double kMixedSwitch = 1e-5;   // |dE| (Eh) ending the mixed-precision phase
 
 int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, double* evals_out, double* P_io,
-            double* C_out) {
+            double* C_out, const double* dh_extra) {
     const auto t_start = clk::now();
     std::memset(&r, 0, sizeof(r));
     const int n = c->n;
@@ -92,7 +92,7 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
     hipStream_t st = c->st;
     double eps = o.eps;
     const double xeps = DBL_EPSILON * n * n * 0.5;   // Basis.hpp:249-254
-    if (eps < xeps) eps = xeps;
+    if (eps < xeps && o.single_start != 2) eps = xeps;
     ScfBufs b;
     for (DBuf<double>* p : {&b.H, &b.T, &b.S, &b.X, &b.F, &b.Fp, &b.Tm, &b.Cw, &b.G[0], &b.G[1], &b.C[0], &b.C[1],
                             &b.Pold[0], &b.Pold[1], &b.Etmp, &b.P[0], &b.P[1]})
@@ -101,6 +101,21 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
         for (int s = 0; s < ns; ++s) UNO_HIP(b.Pref[s].alloc(nn));
     // one-electron matrices on the device, once
     if (int rc = oneint_device(c, b.S.p, b.T.p, b.H.p)) return rc;
+    if (o.field[0] != 0.0 || o.field[1] != 0.0 || o.field[2] != 0.0) {
+        // finite field: H -= E . d with the raw dipole integrals (FField.cpp:6-20)
+        std::vector<double> M(9 * (size_t)n2), dh(n2);
+        one_electron(c->B, nullptr, nullptr, nullptr, M.data());
+        for (long k = 0; k < n2; ++k)
+            dh[k] = -(o.field[0] * M[9 * k] + o.field[1] * M[9 * k + 1] + o.field[2] * M[9 * k + 2]);
+        UNO_HIP(c->pk.upload(dh.data(), n2, st));
+        UNO_HIP(k_unpack(n, c->pk.p, b.Tm.p, st));
+        UNO_HIP(k_axpby(nn, 1.0, b.H.p, 1.0, b.Tm.p, b.H.p, st));
+    }
+    if (dh_extra) {
+        UNO_HIP(c->pk.upload(dh_extra, n2, st));
+        UNO_HIP(k_unpack(n, c->pk.p, b.Tm.p, st));
+        UNO_HIP(k_axpby(nn, 1.0, b.H.p, 1.0, b.Tm.p, b.H.p, st));
+    }
     double nuc = 0.0;
     for (int i = 0; i < c->B.ncen; ++i)
         for (int j = i + 1; j < c->B.ncen; ++j) {
@@ -319,9 +334,11 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
     eold = 0.0;
     if (int rc = update()) return rc;
     const double init_energy = energy + nuc;
-    iteration = 1;
-    if (uhf) eold = 0.0;   // UHF.hpp:157-158
-    if (int rc = update()) return rc;
+    if (!o.single_start) {
+        iteration = 1;
+        if (uhf) eold = 0.0;   // UHF.hpp:157-158
+        if (int rc = update()) return rc;
+    }
     bool conv = false;
     while (iteration < o.maxits) {
         if (o.accel != 2 && !(ediff < 0.0))
