@@ -1,0 +1,175 @@
+// Dense kernels of the batched SCF: many small molecules (N <= 256 functions each) whose
+// matrices live back to back in one store, row-major N_m x N_m at offset off[m].
+//
+//  * grouped GEMM: C_m = alpha op(A_m) op(B_m) + beta C_m for every molecule, one 32 x 32
+//    output tile per 256-thread block (2 x 2 outputs per thread, K through LDS in steps
+//    of 32); the host lists the (molecule, tile) pairs;
+//  * batched Jacobi eigensolver: one block per molecule, cyclic sweeps in parallel
+//    (round-robin) order, each round applying N/2 disjoint rotations to rows, then
+//    columns, then the eigenvector columns; eigenvalues ascending with their vectors;
+//  * per-molecule reductions (trace products, difference norms) and helpers.
+#include <hip/hip_runtime.h>
+#include "batch_kernels.hpp"
+
+namespace uno {
+
+// ---------------------------------------------------------------------------
+// grouped GEMM
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_grouped_gemm(const GemmTile* __restrict__ tiles, const GemmGroup* __restrict__ groups,
+                                                      double alpha, double beta) {
+    __shared__ double sA[32][33];
+    __shared__ double sB[32][33];
+    const GemmTile t = tiles[blockIdx.x];
+    const GemmGroup g = groups[t.group];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;   // 16 x 16 threads, 2 x 2 outputs each
+    const int i0 = t.ti * 32, j0 = t.tj * 32;
+    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int k0 = 0; k0 < g.k; k0 += 32) {
+        for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+            const int r = e >> 5, c = e & 31;
+            // op(A)[i][k], i = i0 + r, k = k0 + c
+            const int i = i0 + r, k = k0 + c;
+            double av = 0.0;
+            if (i < g.m && k < g.k) av = g.ta ? g.A[(long)k * g.lda + i] : g.A[(long)i * g.lda + k];
+            sA[r][c] = av;
+            // op(B)[k][j], k = k0 + r, j = j0 + c
+            const int kk = k0 + r, j = j0 + c;
+            double bv = 0.0;
+            if (kk < g.k && j < g.n) bv = g.tb ? g.B[(long)j * g.ldb + kk] : g.B[(long)kk * g.ldb + j];
+            sB[r][c] = bv;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int k = 0; k < 32; ++k) {
+            const double a0 = sA[ty * 2][k], a1 = sA[ty * 2 + 1][k];
+            const double b0 = sB[k][tx * 2], b1 = sB[k][tx * 2 + 1];
+            acc[0][0] = fma(a0, b0, acc[0][0]);
+            acc[0][1] = fma(a0, b1, acc[0][1]);
+            acc[1][0] = fma(a1, b0, acc[1][0]);
+            acc[1][1] = fma(a1, b1, acc[1][1]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int i = i0 + ty * 2 + a, j = j0 + tx * 2 + b;
+            if (i < g.m && j < g.n) {
+                double* c = g.C + (long)i * g.ldc + j;
+                *c = alpha * acc[a][b] + (beta != 0.0 ? beta * *c : 0.0);
+            }
+        }
+}
+
+hipError_t grouped_gemm(int ntiles, const GemmTile* tiles, const GemmGroup* groups, double alpha, double beta,
+                        hipStream_t st) {
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_grouped_gemm, dim3(ntiles), dim3(256), 0, st, tiles, groups, alpha, beta);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// batched Jacobi: A (destroyed) -> eigenvalues ascending, eigenvectors in the columns of Z
+// ---------------------------------------------------------------------------
+// Cyclic Jacobi in round-robin order: with m = N rounded up to even (a dummy index N when N
+// is odd), round r pairs (m-1, r) and ((r+k) mod (m-1), (r-k) mod (m-1)), k = 1..m/2-1, so
+// that each round is a perfect matching and m-1 rounds visit every pair once.  A rotation
+// is skipped when |a_pq| <= eps/2 sqrt(|a_pp a_qq|) (it would not change the diagonal);
+// sweeps stop when none is applied.
+__global__ __launch_bounds__(256) void k_batched_jacobi(const BatchMat* __restrict__ mats, int max_sweeps,
+                                                        int* __restrict__ sweeps_out) {
+    __shared__ double cs[128], sn[128];
+    __shared__ int pp[128], qq[128];
+    __shared__ int nrot;
+    const BatchMat M = mats[blockIdx.x];
+    const int n = M.n;
+    double* A = M.A;
+    double* V = M.V;
+    const int tid = threadIdx.x;
+    for (long e = tid; e < (long)n * n; e += blockDim.x) V[e] = (e / n == e % n) ? 1.0 : 0.0;
+    const int m = n + (n & 1);
+    int sweep = 0;
+    while (sweep < max_sweeps && n > 1) {
+        ++sweep;
+        if (tid == 0) nrot = 0;
+        __syncthreads();
+        for (int r = 0; r < m - 1; ++r) {
+            for (int k = tid; k < m / 2; k += blockDim.x) {
+                int a, b;
+                if (k == 0) { a = m - 1; b = r; }
+                else { a = (r + k) % (m - 1); b = (r - k + (m - 1)) % (m - 1); }
+                const int p = a < b ? a : b, q = a < b ? b : a;
+                double c = 1.0, s = 0.0;
+                if (q < n) {
+                    const double apq = A[(long)p * n + q];
+                    const double app = A[(long)p * n + p], aqq = A[(long)q * n + q];
+                    if (apq != 0.0 && fabs(apq) > 1.1102230246251565e-16 * sqrt(fabs(app * aqq))) {
+                        const double theta = (aqq - app) / (2.0 * apq);
+                        const double t = fabs(theta) < 1e150
+                                             ? (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0))
+                                             : 0.5 / theta;
+                        c = 1.0 / sqrt(t * t + 1.0);
+                        s = t * c;
+                        atomicAdd(&nrot, 1);
+                    }
+                }
+                pp[k] = p;
+                qq[k] = q;
+                cs[k] = c;
+                sn[k] = s;
+            }
+            __syncthreads();
+            // rows p, q of A: A <- J^T A
+            for (long e = tid; e < (long)(m / 2) * n; e += blockDim.x) {
+                const int k = (int)(e / n), j = (int)(e % n);
+                const int p = pp[k], q = qq[k];
+                if (q >= n || sn[k] == 0.0) continue;
+                const double c = cs[k], s = sn[k];
+                const double ap = A[(long)p * n + j], aq = A[(long)q * n + j];
+                A[(long)p * n + j] = c * ap - s * aq;
+                A[(long)q * n + j] = s * ap + c * aq;
+            }
+            __syncthreads();
+            // columns p, q of A and of V: A <- A J, V <- V J
+            for (long e = tid; e < (long)(m / 2) * n; e += blockDim.x) {
+                const int k = (int)(e / n), i = (int)(e % n);
+                const int p = pp[k], q = qq[k];
+                if (q >= n || sn[k] == 0.0) continue;
+                const double c = cs[k], s = sn[k];
+                const double ap = A[(long)i * n + p], aq = A[(long)i * n + q];
+                A[(long)i * n + p] = c * ap - s * aq;
+                A[(long)i * n + q] = s * ap + c * aq;
+                const double vp = V[(long)i * n + p], vq = V[(long)i * n + q];
+                V[(long)i * n + p] = c * vp - s * vq;
+                V[(long)i * n + q] = s * vp + c * vq;
+            }
+            __syncthreads();
+        }
+        const int done = nrot == 0;
+        __syncthreads();
+        if (done) break;
+    }
+    __syncthreads();
+    // eigenvalues ascending (ties by index) with their vectors as columns of Z
+    for (int i = tid; i < n; i += blockDim.x) {
+        const double di = A[(long)i * n + i];
+        int rank = 0;
+        for (int j = 0; j < n; ++j) {
+            const double dj = A[(long)j * n + j];
+            if (dj < di || (dj == di && j < i)) ++rank;
+        }
+        M.evals[rank] = di;
+        for (int r = 0; r < n; ++r) M.Z[(long)r * n + rank] = V[(long)r * n + i];
+    }
+    if (tid == 0 && sweeps_out) sweeps_out[blockIdx.x] = sweep;
+}
+
+hipError_t batched_jacobi(int nmat, const BatchMat* mats, int max_sweeps, int* sweeps, hipStream_t st) {
+    if (nmat <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batched_jacobi, dim3(nmat), dim3(256), 0, st, mats, max_sweeps, sweeps);
+    return hipGetLastError();
+}
+
+}  // namespace uno

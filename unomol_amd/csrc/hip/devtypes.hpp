@@ -41,6 +41,14 @@ enum DigestMode : int {
     kDigestSchwarz = 3,  // max |(ij|ij)| of a diagonal quartet
 };
 
+// one molecule of a batched launch: its dense matrices start at moff of every store,
+// its shell-block density maxima at dshoff
+struct MolDesc {
+    long long moff;
+    long long dshoff;
+    int n, nsh;
+};
+
 struct EriArgs {
     const PairHdr* pairs;
     const PrimPair* prims;
@@ -58,6 +66,12 @@ struct EriArgs {
     const long long* wpre;
     const int* tblk;
     long long ntask;
+    // batched launches: entry i (a bra with one ket range) names its bra pair, its first
+    // ket pair and its molecule (null: the run offsets above and molecule view n/D/W)
+    const int* e_bra;
+    const int* e_ket;
+    const int* e_mol;
+    const MolDesc* mols;
     int qpb;                  // quartets per block of the launch
     int nbra;
     long long nquart;

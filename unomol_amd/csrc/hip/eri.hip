@@ -8,117 +8,6 @@
 namespace uno {
 
 // ---------------------------------------------------------------------------
-// shared digestion: the contracted block e[c] (already normalised and scaled by the
-// permutational factor) is contracted with the six density blocks it touches.
-// ---------------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD, int MODE, typename T>
-__device__ __forceinline__ void digest_regs(const EriArgs& a, const PairHdr& P1, const PairHdr& P2, const T* e) {
-    using CI = ClassInfo<LA, LB, LC, LD>;
-    constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND;
-    const int n = a.n;
-    const int fa = P1.f1, fb = P1.f2, fc = P2.f1, fd = P2.f2;
-    constexpr T jf = (MODE == kDigestRHF) ? 4 : 2;
-    const double* DJ = a.D0;
-    // Coulomb: W_ab += jf sum_cd e D_cd ; W_cd += jf sum_ab e D_ab
-    {
-        T dcd[NC * ND], dab[NA * NB];
-#pragma unroll
-        for (int k = 0; k < NC; ++k)
-#pragma unroll
-            for (int l = 0; l < ND; ++l) dcd[k * ND + l] = (T)DJ[(long)(fc + k) * n + fd + l];
-#pragma unroll
-        for (int i = 0; i < NA; ++i)
-#pragma unroll
-            for (int j = 0; j < NB; ++j) dab[i * NB + j] = (T)DJ[(long)(fa + i) * n + fb + j];
-#pragma unroll
-        for (int i = 0; i < NA; ++i)
-#pragma unroll
-            for (int j = 0; j < NB; ++j) {
-                T s = 0;
-#pragma unroll
-                for (int kl = 0; kl < NC * ND; ++kl) s = fma(e[(i * NB + j) * NC * ND + kl], dcd[kl], s);
-                atomic_add_f64(a.W0 + (long)(fa + i) * n + fb + j, (double)(jf * s));
-                if (MODE == kDigestUHF) atomic_add_f64(a.W1 + (long)(fa + i) * n + fb + j, (double)(jf * s));
-            }
-#pragma unroll
-        for (int kl = 0; kl < NC * ND; ++kl) {
-            T s = 0;
-#pragma unroll
-            for (int ij = 0; ij < NA * NB; ++ij) s = fma(e[ij * NC * ND + kl], dab[ij], s);
-            const int k = kl / ND, l = kl % ND;
-            atomic_add_f64(a.W0 + (long)(fc + k) * n + fd + l, (double)(jf * s));
-            if (MODE == kDigestUHF) atomic_add_f64(a.W1 + (long)(fc + k) * n + fd + l, (double)(jf * s));
-        }
-    }
-    // exchange, once per spin density: W_ac -= sum_bd e D_bd, W_ad -= sum_bc e D_bc,
-    // W_bc -= sum_ad e D_ad, W_bd -= sum_ac e D_ac
-#pragma unroll
-    for (int sp = 0; sp < (MODE == kDigestUHF ? 2 : 1); ++sp) {
-        const double* DK = (MODE == kDigestUHF) ? (sp == 0 ? a.D1 : a.D2) : a.D0;
-        double* W = (sp == 0) ? a.W0 : a.W1;
-        T dbd[NB * ND], dbc[NB * NC], dad[NA * ND], dac[NA * NC];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-#pragma unroll
-            for (int l = 0; l < ND; ++l) dbd[j * ND + l] = (T)DK[(long)(fb + j) * n + fd + l];
-#pragma unroll
-            for (int k = 0; k < NC; ++k) dbc[j * NC + k] = (T)DK[(long)(fb + j) * n + fc + k];
-        }
-#pragma unroll
-        for (int i = 0; i < NA; ++i) {
-#pragma unroll
-            for (int l = 0; l < ND; ++l) dad[i * ND + l] = (T)DK[(long)(fa + i) * n + fd + l];
-#pragma unroll
-            for (int k = 0; k < NC; ++k) dac[i * NC + k] = (T)DK[(long)(fa + i) * n + fc + k];
-        }
-#pragma unroll
-        for (int i = 0; i < NA; ++i)
-#pragma unroll
-            for (int k = 0; k < NC; ++k) {
-                T s = 0;
-#pragma unroll
-                for (int j = 0; j < NB; ++j)
-#pragma unroll
-                    for (int l = 0; l < ND; ++l) s = fma(e[((i * NB + j) * NC + k) * ND + l], dbd[j * ND + l], s);
-                atomic_add_f64(W + (long)(fa + i) * n + fc + k, (double)(-s));
-            }
-#pragma unroll
-        for (int i = 0; i < NA; ++i)
-#pragma unroll
-            for (int l = 0; l < ND; ++l) {
-                T s = 0;
-#pragma unroll
-                for (int j = 0; j < NB; ++j)
-#pragma unroll
-                    for (int k = 0; k < NC; ++k) s = fma(e[((i * NB + j) * NC + k) * ND + l], dbc[j * NC + k], s);
-                atomic_add_f64(W + (long)(fa + i) * n + fd + l, (double)(-s));
-            }
-#pragma unroll
-        for (int j = 0; j < NB; ++j)
-#pragma unroll
-            for (int k = 0; k < NC; ++k) {
-                T s = 0;
-#pragma unroll
-                for (int i = 0; i < NA; ++i)
-#pragma unroll
-                    for (int l = 0; l < ND; ++l) s = fma(e[((i * NB + j) * NC + k) * ND + l], dad[i * ND + l], s);
-                atomic_add_f64(W + (long)(fb + j) * n + fc + k, (double)(-s));
-            }
-#pragma unroll
-        for (int j = 0; j < NB; ++j)
-#pragma unroll
-            for (int l = 0; l < ND; ++l) {
-                T s = 0;
-#pragma unroll
-                for (int i = 0; i < NA; ++i)
-#pragma unroll
-                    for (int k = 0; k < NC; ++k) s = fma(e[((i * NB + j) * NC + k) * ND + l], dac[i * NC + k], s);
-                atomic_add_f64(W + (long)(fb + j) * n + fd + l, (double)(-s));
-            }
-    }
-}
-
-// ---------------------------------------------------------------------------
 // contracted (un-normalised) ERI block of one shell quartet in one lane; returns the
 // primitive quartets kept by the cut.  Ket primitives outside, bra primitives inside.
 // ---------------------------------------------------------------------------
@@ -205,21 +94,19 @@ __global__ __launch_bounds__(256) void k_eri_lane(const EriArgs a) {
     using CI = ClassInfo<LA, LB, LC, LD>;
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND, NCOMP = CI::NCOMP;
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    int pab, pcd;
+    static_assert(MODE == kDigestBlock || MODE == kDigestSchwarz, "Fock builds use the wave-task kernels");
+    int pab, pcd, entry;
     double q12;
-    if (!quartet_of(a, g, pab, pcd, q12)) return;
+    if (!quartet_of(a, g, pab, pcd, q12, entry)) return;
     const PairHdr P1 = a.pairs[pab];
     const PairHdr P2 = a.pairs[pcd];
-    if (MODE <= kDigestUHF && q12 >= 0.0 && screened(a, P1, P2, q12)) return;
     T e[NCOMP];
     const unsigned nprimq = eri_block<LA, LB, LC, LD, T>(a, P1, P2, e);
     if (a.counter) {
         atomicAdd(a.counter, (unsigned long long)nprimq);
         atomicAdd(a.counter + 1, 1ull);
     }
-    T f = 1;
-    if (MODE <= kDigestUHF) f = (P1.same ? (T)0.5 : (T)1) * (P2.same ? (T)0.5 : (T)1) * (pab == pcd ? (T)0.5 : (T)1);
-    normalise_block<LA, LB, LC, LD, T>(e, f);
+    normalise_block<LA, LB, LC, LD, T>(e, (T)1);
     if constexpr (MODE == kDigestBlock) {
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) a.out[g * NCOMP + c] = (double)e[c];
@@ -232,8 +119,6 @@ __global__ __launch_bounds__(256) void k_eri_lane(const EriArgs a) {
                 if constexpr (LA == LC && LB == LD) m = fmax(m, fabs((double)e[((i * NB + j) * NC + i) * ND + j]));
             }
         a.out[pab] = sqrt(m);
-    } else {
-        digest_regs<LA, LB, LC, LD, MODE, T>(a, P1, P2, e);
     }
 }
 
@@ -261,18 +146,19 @@ __global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
         const int mid = (lo + hi + 1) >> 1;
         if (a.wpre[mid] <= t) lo = mid; else hi = mid - 1;
     }
-    const int bra = __builtin_amdgcn_readfirstlane(lo);
+    const int bra = __builtin_amdgcn_readfirstlane(lo);   // entry of this task
     const int chunk = __builtin_amdgcn_readfirstlane((int)(t - a.wpre[bra]));
     const long long cnt = a.prefix[bra + 1] - a.prefix[bra];
     const int jk = chunk * 64 + lane;
-    const int pab = a.bra_off + bra;
+    const int pab = __builtin_amdgcn_readfirstlane(entry_bra(a, bra));
     const PairHdr P1 = a.pairs[pab];
-    const int pcd = a.ket_off + (a.kstart ? a.kstart[bra] : 0) + jk;
+    const int pcd = entry_ket(a, bra) + jk;
+    const MolView mv = mol_view(a, bra);
     bool active = jk < cnt;
     PairHdr P2;
     if (active) {
         P2 = a.pairs[pcd];
-        active = !screened(a, P1, P2, a.pairQ[pab] * a.pairQ[pcd]);
+        active = !screened(a, mv, P1, P2, a.pairQ[pab] * a.pairQ[pcd]);
     }
     T e[NCOMP];
     unsigned nprimq = 0;
@@ -288,7 +174,7 @@ __global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
         atomicAdd(a.counter, (unsigned long long)nprimq);
         atomicAdd(a.counter + 1, 1ull);
     }
-    const int n = a.n;
+    const int n = mv.n;
     const int fa = P1.f1, fb = P1.f2;
     constexpr T jf = (MODE == kDigestRHF) ? 4 : 2;
     // Coulomb, bra outputs: W_ab += jf sum_cd e D_cd, summed over the wave
@@ -299,7 +185,7 @@ __global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
 #pragma unroll
             for (int k = 0; k < NC; ++k)
 #pragma unroll
-                for (int l = 0; l < ND; ++l) dcd[k * ND + l] = (T)a.D0[(long)(fc + k) * n + fd + l];
+                for (int l = 0; l < ND; ++l) dcd[k * ND + l] = (T)mv.D0[(long)(fc + k) * n + fd + l];
         } else {
 #pragma unroll
             for (int kl = 0; kl < NC * ND; ++kl) dcd[kl] = 0;
@@ -313,8 +199,8 @@ __global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
                 for (int kl = 0; kl < NC * ND; ++kl) s = fma(e[(i * NB + j) * NC * ND + kl], dcd[kl], s);
                 const double tot = wave_sum((double)(jf * s));
                 if (lane == 0) {
-                    atomic_add_f64(a.W0 + (long)(fa + i) * n + fb + j, tot);
-                    if (MODE == kDigestUHF) atomic_add_f64(a.W1 + (long)(fa + i) * n + fb + j, tot);
+                    atomic_add_f64(mv.W0 + (long)(fa + i) * n + fb + j, tot);
+                    if (MODE == kDigestUHF) atomic_add_f64(mv.W1 + (long)(fa + i) * n + fb + j, tot);
                 }
             }
     }
@@ -326,22 +212,22 @@ __global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
 #pragma unroll
         for (int i = 0; i < NA; ++i)
 #pragma unroll
-            for (int j = 0; j < NB; ++j) dab[i * NB + j] = (T)a.D0[(long)(fa + i) * n + fb + j];
+            for (int j = 0; j < NB; ++j) dab[i * NB + j] = (T)mv.D0[(long)(fa + i) * n + fb + j];
 #pragma unroll
         for (int kl = 0; kl < NC * ND; ++kl) {
             T s = 0;
 #pragma unroll
             for (int ij = 0; ij < NA * NB; ++ij) s = fma(e[ij * NC * ND + kl], dab[ij], s);
             const int k = kl / ND, l = kl % ND;
-            atomic_add_f64(a.W0 + (long)(fc + k) * n + fd + l, (double)(jf * s));
-            if (MODE == kDigestUHF) atomic_add_f64(a.W1 + (long)(fc + k) * n + fd + l, (double)(jf * s));
+            atomic_add_f64(mv.W0 + (long)(fc + k) * n + fd + l, (double)(jf * s));
+            if (MODE == kDigestUHF) atomic_add_f64(mv.W1 + (long)(fc + k) * n + fd + l, (double)(jf * s));
         }
     }
     // exchange, once per spin density
 #pragma unroll
     for (int sp = 0; sp < (MODE == kDigestUHF ? 2 : 1); ++sp) {
-        const double* DK = (MODE == kDigestUHF) ? (sp == 0 ? a.D1 : a.D2) : a.D0;
-        double* W = (sp == 0) ? a.W0 : a.W1;
+        const double* DK = (MODE == kDigestUHF) ? (sp == 0 ? mv.D1 : mv.D2) : mv.D0;
+        double* W = (sp == 0) ? mv.W0 : mv.W1;
         T dbd[NB * ND], dbc[NB * NC], dad[NA * ND], dac[NA * NC];
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
@@ -418,12 +304,13 @@ __device__ __forceinline__ void wave_one(const EriArgs& a, const long long g) {
     __shared__ T sI[NR * 3][TBL];
     __shared__ T sE[NCOMP];
     const int lane = threadIdx.x;
-    int pab, pcd;
+    int pab, pcd, entry;
     double q12;
-    if (!quartet_of(a, g, pab, pcd, q12)) return;
+    if (!quartet_of(a, g, pab, pcd, q12, entry)) return;
     const PairHdr P1 = a.pairs[pab];
     const PairHdr P2 = a.pairs[pcd];
-    if (MODE <= kDigestUHF && q12 >= 0.0 && screened(a, P1, P2, q12)) return;   // uniform per block
+    const MolView mv = mol_view(a, entry);
+    if (MODE <= kDigestUHF && q12 >= 0.0 && screened(a, mv, P1, P2, q12)) return;   // uniform per block
     // each lane owns components lane + 64 m; precompute their table indices
     int ix[PER], iy[PER], iz[PER];
     T acc[PER];
@@ -574,7 +461,7 @@ __device__ __forceinline__ void wave_one(const EriArgs& a, const long long g) {
         }
         return;
     } else {
-        const int n = a.n;
+        const int n = mv.n;
         const int fa = P1.f1, fb = P1.f2, fc = P2.f1, fd = P2.f2;
         constexpr T jf = (MODE == kDigestRHF) ? 4 : 2;
         // Coulomb outputs: NA*NB (bra) + NC*ND (ket)
@@ -584,22 +471,22 @@ __device__ __forceinline__ void wave_one(const EriArgs& a, const long long g) {
                 const int i = o / NB, j = o % NB;
                 for (int k = 0; k < NC; ++k)
                     for (int l = 0; l < ND; ++l)
-                        s = fma(sE[((i * NB + j) * NC + k) * ND + l], (T)a.D0[(long)(fc + k) * n + fd + l], s);
-                atomic_add_f64(a.W0 + (long)(fa + i) * n + fb + j, (double)(jf * s));
-                if (MODE == kDigestUHF) atomic_add_f64(a.W1 + (long)(fa + i) * n + fb + j, (double)(jf * s));
+                        s = fma(sE[((i * NB + j) * NC + k) * ND + l], (T)mv.D0[(long)(fc + k) * n + fd + l], s);
+                atomic_add_f64(mv.W0 + (long)(fa + i) * n + fb + j, (double)(jf * s));
+                if (MODE == kDigestUHF) atomic_add_f64(mv.W1 + (long)(fa + i) * n + fb + j, (double)(jf * s));
             } else {
                 const int kl = o - NA * NB, k = kl / ND, l = kl % ND;
                 for (int i = 0; i < NA; ++i)
                     for (int j = 0; j < NB; ++j)
-                        s = fma(sE[((i * NB + j) * NC + k) * ND + l], (T)a.D0[(long)(fa + i) * n + fb + j], s);
-                atomic_add_f64(a.W0 + (long)(fc + k) * n + fd + l, (double)(jf * s));
-                if (MODE == kDigestUHF) atomic_add_f64(a.W1 + (long)(fc + k) * n + fd + l, (double)(jf * s));
+                        s = fma(sE[((i * NB + j) * NC + k) * ND + l], (T)mv.D0[(long)(fa + i) * n + fb + j], s);
+                atomic_add_f64(mv.W0 + (long)(fc + k) * n + fd + l, (double)(jf * s));
+                if (MODE == kDigestUHF) atomic_add_f64(mv.W1 + (long)(fc + k) * n + fd + l, (double)(jf * s));
             }
         }
         constexpr int NK = NA * NC + NA * ND + NB * NC + NB * ND;
         for (int sp = 0; sp < (MODE == kDigestUHF ? 2 : 1); ++sp) {
-            const double* DK = (MODE == kDigestUHF) ? (sp == 0 ? a.D1 : a.D2) : a.D0;
-            double* W = sp == 0 ? a.W0 : a.W1;
+            const double* DK = (MODE == kDigestUHF) ? (sp == 0 ? mv.D1 : mv.D2) : mv.D0;
+            double* W = sp == 0 ? mv.W0 : mv.W1;
             for (int o = lane; o < NK; o += 64) {
                 T s = 0;
                 if (o < NA * NC) {                       // (a,c) <- D_bd

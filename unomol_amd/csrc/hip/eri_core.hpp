@@ -161,15 +161,51 @@ struct ClassInfo {
     }
 };
 
-// Resolve the quartet handled by linear index g: (bra pair, ket pair)
-// and the product of their Schwarz bounds (negative for explicit lists: no screening)
-__device__ __forceinline__ bool quartet_of(const EriArgs& a, long long g, int& pab, int& pcd, double& q12) {
+// The matrices one entry's quartets read and write: the launch's own, or for batched
+// launches the entry's molecule's slice of every store
+struct MolView {
+    const double* D0;
+    const double* D1;
+    const double* D2;
+    double* W0;
+    double* W1;
+    const double* dsh;
+    int n, nsh;
+};
+
+__device__ __forceinline__ MolView mol_view(const EriArgs& a, int entry) {
+    MolView v{a.D0, a.D1, a.D2, a.W0, a.W1, a.dsh, a.n, a.nsh};
+    if (a.mols) {
+        const MolDesc m = a.mols[a.e_mol[entry]];
+        v.D0 = a.D0 + m.moff;
+        v.D1 = a.D1 ? a.D1 + m.moff : nullptr;
+        v.D2 = a.D2 ? a.D2 + m.moff : nullptr;
+        v.W0 = a.W0 + m.moff;
+        v.W1 = a.W1 ? a.W1 + m.moff : nullptr;
+        v.dsh = a.dsh ? a.dsh + m.dshoff : nullptr;
+        v.n = m.n;
+        v.nsh = m.nsh;
+    }
+    return v;
+}
+
+// bra pair and first ket pair of entry i
+__device__ __forceinline__ int entry_bra(const EriArgs& a, int i) { return a.e_bra ? a.e_bra[i] : a.bra_off + i; }
+__device__ __forceinline__ int entry_ket(const EriArgs& a, int i) {
+    return a.e_ket ? a.e_ket[i] : a.ket_off + (a.kstart ? a.kstart[i] : 0);
+}
+
+// Resolve the quartet handled by linear index g: (bra pair, ket pair), the entry, and the
+// product of their Schwarz bounds (negative for explicit lists: no screening)
+__device__ __forceinline__ bool quartet_of(const EriArgs& a, long long g, int& pab, int& pcd, double& q12,
+                                           int& entry) {
     if (g >= a.nquart) return false;
     if (a.explicit_q) {
         const int2 e = a.explicit_q[g];
         pab = e.x;
         pcd = e.y;
         q12 = -1.0;
+        entry = 0;
         return true;
     }
     // the block table brackets the bra of every quartet of this block
@@ -180,8 +216,9 @@ __device__ __forceinline__ bool quartet_of(const EriArgs& a, long long g, int& p
         if (a.prefix[mid] <= g) lo = mid; else hi = mid - 1;
     }
     const int jk = (int)(g - a.prefix[lo]);
-    pab = a.bra_off + lo;
-    pcd = a.ket_off + (a.kstart ? a.kstart[lo] : 0) + jk;
+    entry = lo;
+    pab = entry_bra(a, lo);
+    pcd = entry_ket(a, lo) + jk;
     q12 = a.pairQ[pab] * a.pairQ[pcd];
     return true;
 }
@@ -189,10 +226,11 @@ __device__ __forceinline__ bool quartet_of(const EriArgs& a, long long g, int& p
 __device__ __forceinline__ void atomic_add_f64(double* p, double v) { unsafeAtomicAdd(p, v); }
 
 // density screen: Qab Qcd max(4|D_ab|, 4|D_cd|, |D_ac|, |D_ad|, |D_bc|, |D_bd|) < tau
-__device__ __forceinline__ bool screened(const EriArgs& a, const PairHdr& P1, const PairHdr& P2, double q12) {
-    if (!a.dsh) return q12 * a.dmax_global < a.tau;
-    const int ns = a.nsh;
-    const double* m = a.dsh;
+__device__ __forceinline__ bool screened(const EriArgs& a, const MolView& v, const PairHdr& P1, const PairHdr& P2,
+                                         double q12) {
+    if (!v.dsh) return q12 * a.dmax_global < a.tau;
+    const int ns = v.nsh;
+    const double* m = v.dsh;
     const int s1 = P1.sh1, s2 = P1.sh2, s3 = P2.sh1, s4 = P2.sh2;
     double d = 4.0 * fmax(m[s1 * ns + s2], m[s3 * ns + s4]);
     d = fmax(d, fmax(fmax(m[s1 * ns + s3], m[s1 * ns + s4]), fmax(m[s2 * ns + s3], m[s2 * ns + s4])));
