@@ -223,6 +223,20 @@ int uno_ctx_set_guess(uno_ctx* ctx, uno_mlp* model);
  * rotation Q of the molecule (row-major 3x3; l <= 2): phi'_mu = sum_nu M_mu,nu phi_nu. */
 int uno_shell_rotation(int l, const double* Q, double* M);
 
+/* ---- Batched SCF (no reference counterpart: the reference runs one molecule per
+ * process).  Many molecules of uneven size (each at most 256 functions), closed-shell
+ * and open-shell mixed (odd electron counts run UHF), iterated in lockstep on the
+ * device: one ERI launch per class pair across all molecules, grouped GEMMs and a
+ * batched Jacobi eigensolver for the dense steps, Pulay DIIS per molecule.  FP64 full
+ * builds from the core-Hamiltonian guess; opts->cflag / eps / maxits as for uno_scf_run. */
+typedef struct uno_batch uno_batch;
+int uno_batch_create(int nmol, const uno_basis* bases, const int* nelec, uno_batch** out);
+int uno_batch_destroy(uno_batch* batch);
+int uno_batch_nmol(const uno_batch* batch);
+/* res[nmol] per molecule (wall_s, fock_s and quartets are the batch totals);
+ * energies (optional) [nmol] total energies. */
+int uno_batch_run(uno_batch* batch, const uno_scf_opts* opts, uno_scf_result* res, double* energies);
+
 #ifdef __cplusplus
 }
 #endif

@@ -467,3 +467,39 @@ def test_dpm_scan_vs_reference(tmp_path, tag):
     sgot = np.loadtxt(str(tmp_path / "spol.out"), ndmin=2)
     # energies print with 10 digits after the point in e-format: compare to that precision
     np.testing.assert_allclose(sgot[:, 1:5], sref[:, 1:5], rtol=2e-10, atol=1e-9)
+
+
+def _batch_decks(tmp_path, count, seed):
+    """A mixed pool like the batched workload: distorted small molecules and 1-3-water
+    clusters in 6-31G / 6-31G*, every other one a +1 cation (UHF)."""
+    rng = np.random.default_rng(seed)
+    paths = []
+    for i in range(count):
+        basis = "6-31G*" if rng.uniform() < 0.5 else "6-31G"
+        charge = i % 2
+        p = tmp_path / ("m%d.dat" % i)
+        if rng.uniform() < 0.6:
+            unomol_amd.generate_deck(p, "molecule", int(rng.integers(0, 9)), int(seed * 1000 + i), basis, charge,
+                                     1e-10, 200)
+        else:
+            unomol_amd.generate_deck(p, "water_cluster", int(rng.integers(1, 4)), int(seed * 1000 + i), basis,
+                                     charge, 1e-10, 200)
+        paths.append(str(p))
+    return paths
+
+
+def test_batched_scf_matches_single(tmp_path):
+    """Batched lockstep SCF (one launch per class pair over all molecules, grouped GEMMs,
+    batched Jacobi) against the single-molecule SCF of each molecule."""
+    say("batched scf: 12 mixed molecules")
+    paths = _batch_decks(tmp_path, 12, 7)
+    batch = unomol_amd.Batch(paths)
+    e, res = batch.run(maxits=100, eps=1e-10, cflag=3, accel=2)
+    for i, p in enumerate(paths):
+        b = unomol_amd.Basis(p)
+        t = unomol_amd.TwoElectronInts(b)
+        cls = unomol_amd.UnRestrictedHartreeFock if b.deck["nelec"] % 2 else unomol_amd.RestrictedHartreeFock
+        hf = cls(b, t)
+        e1 = hf.findEnergy(maxits=100, eps=1e-10, cflag=3, accel=2)
+        assert res[i].converged and hf.result.converged
+        assert abs(e[i] - e1) < 1e-8, (i, p, e[i], e1)

@@ -8,4 +8,4 @@ package only binds it.  See ``api`` for the mirror of the reference's class surf
 from .api import (Basis, TwoElectronInts, SymmPack, RestrictedHartreeFock,  # noqa: F401
                   UnRestrictedHartreeFock, run_deck, generate_deck, rys_eval, oneint, moment_ints,
                   scf_options, shell_rotation, guess_features, guess_samples, guess_assemble, MLP,
-                  set_guess_model)
+                  set_guess_model, Batch)

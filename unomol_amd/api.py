@@ -334,3 +334,28 @@ def set_guess_model(tints, mlp):
     """Network used by findEnergy(guess=2) on this context (kept alive by the caller)."""
     N.check(N.lib().uno_ctx_set_guess(tints.ctx, mlp.h if mlp is not None else None))
     tints._guess_model = mlp
+
+
+class Batch:
+    """Batched SCF of many molecules (decks: list of deck dicts or paths) on the device."""
+
+    def __init__(self, decks):
+        self.decks = [read_deck(d) if isinstance(d, (str, os.PathLike)) else d for d in decks]
+        self._structs = [N.basis_struct(d) for d in self.decks]
+        arr = (N.Basis * len(self._structs))(*self._structs)
+        self._arr = arr
+        nelec = (C.c_int * len(self.decks))(*[d["nelec"] for d in self.decks])
+        self.h = C.c_void_p()
+        N.check(N.lib().uno_batch_create(len(self.decks), arr, nelec, C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            N.lib().uno_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def run(self, **opts):
+        o = scf_options(**opts)
+        res = (N.ScfResult * len(self.decks))()
+        e = np.zeros(len(self.decks))
+        N.check(N.lib().uno_batch_run(self.h, C.byref(o), res, N.ptr(e)), allow=(N.UNO_ENOCONV,))
+        return e, list(res)

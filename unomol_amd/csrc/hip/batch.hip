@@ -172,4 +172,122 @@ hipError_t batched_jacobi(int nmat, const BatchMat* mats, int max_sweeps, int* s
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// per-molecule helpers: one 256-thread block per slot
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double block_reduce_sum(double v, double* red) {
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += red[k];
+    __syncthreads();
+    return s;
+}
+
+// out[k] (+)= sum_ij A_k,ij B_k,ij over slot k's n x n matrices
+__global__ __launch_bounds__(256) void k_batch_dot(const MatSlot* __restrict__ slots, const double* __restrict__ A,
+                                                   const double* __restrict__ B, double* __restrict__ out, int acc) {
+    __shared__ double red[4];
+    const MatSlot m = slots[blockIdx.x];
+    double s = 0.0;
+    for (long e = threadIdx.x; e < (long)m.n * m.n; e += blockDim.x) s += A[m.off + e] * B[m.off + e];
+    s = block_reduce_sum(s, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = acc ? out[blockIdx.x] + s : s;
+}
+
+// out[k] = sqrt(sum (A - B)^2) / n over slot k
+__global__ __launch_bounds__(256) void k_batch_diffnorm(const MatSlot* __restrict__ slots, const double* __restrict__ A,
+                                                        const double* __restrict__ B, double* __restrict__ out) {
+    __shared__ double red[4];
+    const MatSlot m = slots[blockIdx.x];
+    double s = 0.0;
+    for (long e = threadIdx.x; e < (long)m.n * m.n; e += blockDim.x) {
+        const double d = A[m.off + e] - B[m.off + e];
+        s += d * d;
+    }
+    s = block_reduce_sum(s, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = sqrt(s) / m.n;
+}
+
+// G = W + W^T (or G += ...) per slot
+__global__ __launch_bounds__(256) void k_batch_sym(const MatSlot* __restrict__ slots, const double* __restrict__ W,
+                                                   double* __restrict__ G, int acc) {
+    const MatSlot m = slots[blockIdx.x];
+    for (long e = threadIdx.x; e < (long)m.n * m.n; e += blockDim.x) {
+        const int i = (int)(e / m.n), j = (int)(e % m.n);
+        const double v = W[m.off + e] + W[m.off + (long)j * m.n + i];
+        G[m.off + e] = acc ? G[m.off + e] + v : v;
+    }
+}
+
+// Dsh_k[s][t] = max |D| over the function block of shells s, t of slot k's molecule (D1, D2
+// optional further densities of the same layout)
+__global__ __launch_bounds__(256) void k_batch_shell_max(const MatSlot* __restrict__ slots, const int* __restrict__ sh_off,
+                                                         const int* __restrict__ sh_n, const double* __restrict__ D,
+                                                         const double* __restrict__ D1, const double* __restrict__ D2,
+                                                         double* __restrict__ Dsh) {
+    const MatSlot m = slots[blockIdx.x];
+    const int ns = m.nsh;
+    for (int st = threadIdx.x; st < ns * ns; st += blockDim.x) {
+        const int s = st / ns, t = st % ns;
+        const int o1 = sh_off[m.shoff + s], n1 = sh_n[m.shoff + s];
+        const int o2 = sh_off[m.shoff + t], n2 = sh_n[m.shoff + t];
+        double v = 0.0;
+        for (int a = 0; a < n1; ++a)
+            for (int b = 0; b < n2; ++b) {
+                const long ij = m.off + (long)(o1 + a) * m.n + o2 + b;
+                v = fmax(v, fabs(D[ij]));
+                if (D1) v = fmax(v, fabs(D1[ij]));
+                if (D2) v = fmax(v, fabs(D2[ij]));
+            }
+        Dsh[m.dshoff + st] = v;
+    }
+}
+
+// F_k = sum_j c[k * ncoef + j] H_j,k: a linear combination of ncoef stores per slot
+__global__ __launch_bounds__(256) void k_batch_lincomb(const MatSlot* __restrict__ slots, int ncoef,
+                                                       const double* __restrict__ coef, const double* const* __restrict__ src,
+                                                       double* __restrict__ F) {
+    const MatSlot m = slots[blockIdx.x];
+    for (long e = threadIdx.x; e < (long)m.n * m.n; e += blockDim.x) {
+        double v = 0.0;
+        for (int j = 0; j < ncoef; ++j) {
+            const double cj = coef[(long)blockIdx.x * ncoef + j];
+            if (cj != 0.0) v = fma(cj, src[j][m.off + e], v);
+        }
+        F[m.off + e] = v;
+    }
+}
+
+hipError_t batch_dot(int nslot, const MatSlot* slots, const double* A, const double* B, double* out, bool acc,
+                     hipStream_t st) {
+    if (nslot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_dot, dim3(nslot), dim3(256), 0, st, slots, A, B, out, acc ? 1 : 0);
+    return hipGetLastError();
+}
+hipError_t batch_diffnorm(int nslot, const MatSlot* slots, const double* A, const double* B, double* out,
+                          hipStream_t st) {
+    if (nslot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_diffnorm, dim3(nslot), dim3(256), 0, st, slots, A, B, out);
+    return hipGetLastError();
+}
+hipError_t batch_sym(int nslot, const MatSlot* slots, const double* W, double* G, bool acc, hipStream_t st) {
+    if (nslot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_sym, dim3(nslot), dim3(256), 0, st, slots, W, G, acc ? 1 : 0);
+    return hipGetLastError();
+}
+hipError_t batch_shell_max(int nslot, const MatSlot* slots, const int* sh_off, const int* sh_n, const double* D,
+                           const double* D1, const double* D2, double* Dsh, hipStream_t st) {
+    if (nslot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_shell_max, dim3(nslot), dim3(256), 0, st, slots, sh_off, sh_n, D, D1, D2, Dsh);
+    return hipGetLastError();
+}
+hipError_t batch_lincomb(int nslot, const MatSlot* slots, int ncoef, const double* coef, const double* const* src,
+                         double* F, hipStream_t st) {
+    if (nslot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_lincomb, dim3(nslot), dim3(256), 0, st, slots, ncoef, coef, src, F);
+    return hipGetLastError();
+}
+
 }  // namespace uno

@@ -31,4 +31,23 @@ struct BatchMat {
 // n <= 256 per matrix; sweeps[b] (optional) receives the sweeps taken
 hipError_t batched_jacobi(int nmat, const BatchMat* mats, int max_sweeps, int* sweeps, hipStream_t st);
 
+// one molecule's n x n matrix in a store (and, for the shell maxima, its shells)
+struct MatSlot {
+    long long off;      // first element in every store
+    int n;
+    int nsh;            // shells of the molecule
+    int shoff;          // first shell in the shell tables
+    long long dshoff;   // first element of its nsh x nsh block maxima
+};
+hipError_t batch_dot(int nslot, const MatSlot* slots, const double* A, const double* B, double* out, bool acc,
+                     hipStream_t st);
+hipError_t batch_diffnorm(int nslot, const MatSlot* slots, const double* A, const double* B, double* out,
+                          hipStream_t st);
+hipError_t batch_sym(int nslot, const MatSlot* slots, const double* W, double* G, bool acc, hipStream_t st);
+hipError_t batch_shell_max(int nslot, const MatSlot* slots, const int* sh_off, const int* sh_n, const double* D,
+                           const double* D1, const double* D2, double* Dsh, hipStream_t st);
+// F_k = sum_j coef[k ncoef + j] src[j]_k (src: device array of ncoef device store pointers)
+hipError_t batch_lincomb(int nslot, const MatSlot* slots, int ncoef, const double* coef, const double* const* src,
+                         double* F, hipStream_t st);
+
 }  // namespace uno

@@ -1,0 +1,609 @@
+// Batched SCF: many molecules of uneven size (each N <= 256 functions), RHF and UHF mixed,
+// iterated in lockstep on one device (no reference counterpart: the reference runs one
+// molecule per process).
+//
+// Every molecule keeps its own context for the pair data (shell pairs, Schwarz bounds,
+// runs sorted by bound); their pair and primitive arrays are concatenated into one device
+// array.  A Fock build is one launch per (bra class, ket class, RHF/UHF) over all
+// molecules: the quartet space is a list of entries (bra pair, first ket pair, ket count,
+// molecule) built from the same screened prefixes as a single-molecule build, and the
+// kernels read and accumulate each molecule's slice of the density / accumulator stores.
+// Dense steps run as grouped GEMMs over every (molecule, spin) slot and a batched Jacobi
+// eigensolver (one block per matrix).  The iteration is the single-molecule one with Pulay
+// DIIS (8 vectors, error X^T(FPS - SPF)X); molecules stop updating once converged.
+#include <algorithm>
+#include <chrono>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+#include "engine.hpp"
+#include "../hip/batch_kernels.hpp"
+#include "../hip/kernels.hpp"
+
+using namespace uno;
+
+namespace {
+
+struct BatchMol {
+    uno_ctx* ctx = nullptr;
+    int nelec = 0, n = 0, nsh = 0, nocc[2] = {0, 0};
+    bool uhf = false;
+    long long moff = 0, dshoff = 0;
+    int shoff = 0, pair_off = 0;
+    // state
+    bool active = true, converged = false;
+    int iterations = 0;
+    double energy = 0.0, eold = 0.0, ediff = 0.0, pdiff = 0.0, nuc = 0.0, init_energy = 0.0;
+};
+
+constexpr int kBatchDiis = 8;
+
+using clk = std::chrono::steady_clock;
+double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+bool solve_small(int m, std::vector<double> A, std::vector<double>& b) {
+    for (int k = 0; k < m; ++k) {
+        int piv = k;
+        for (int i = k + 1; i < m; ++i)
+            if (std::fabs(A[i * m + k]) > std::fabs(A[piv * m + k])) piv = i;
+        if (A[piv * m + k] == 0.0) return false;
+        if (piv != k) {
+            for (int j = 0; j < m; ++j) std::swap(A[k * m + j], A[piv * m + j]);
+            std::swap(b[k], b[piv]);
+        }
+        for (int i = k + 1; i < m; ++i) {
+            const double f = A[i * m + k] / A[k * m + k];
+            for (int j = k; j < m; ++j) A[i * m + j] -= f * A[k * m + j];
+            b[i] -= f * b[k];
+        }
+    }
+    for (int k = m - 1; k >= 0; --k) {
+        double s = b[k];
+        for (int j = k + 1; j < m; ++j) s -= A[k * m + j] * b[j];
+        b[k] = s / A[k * m + k];
+    }
+    return true;
+}
+
+}  // namespace
+
+struct uno_batch {
+    std::vector<BatchMol> mol;
+    hipStream_t st = nullptr;
+    long long store = 0, dsh_total = 0;
+    DBuf<PairHdr> pairs;
+    DBuf<PrimPair> prims;
+    DBuf<double> pairQ;
+    DBuf<int> sh_off, sh_n;
+    DBuf<MolDesc> mdesc;
+    // stores (each sum_m N_m^2 doubles); index 0 alpha / closed shell, 1 beta
+    DBuf<double> H, S, X, P[2], Pold[2], F[2], G[2], C[2], Fp, Tm, Tm2, V, Z, D0, D1, D2, W0, W1, Dsh, ev[2];
+    DBuf<double> dF[kBatchDiis][2], dE[kBatchDiis][2];
+    DBuf<double> red;
+    ~uno_batch() {
+        for (BatchMol& m : mol)
+            if (m.ctx) uno_ctx_destroy(m.ctx);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+namespace {
+
+// slots: per (molecule, spin) matrix; spin 1 only for UHF molecules
+void make_slots(const uno_batch* b, bool only_active, int spin, std::vector<MatSlot>& out, std::vector<int>& who) {
+    out.clear();
+    who.clear();
+    for (int m = 0; m < (int)b->mol.size(); ++m) {
+        const BatchMol& x = b->mol[m];
+        if (only_active && !x.active) continue;
+        if (spin == 1 && !x.uhf) continue;
+        out.push_back(MatSlot{x.moff, x.n, x.nsh, x.shoff, x.dshoff});
+        who.push_back(m);
+    }
+}
+
+struct GemmBatch {
+    std::vector<GemmGroup> g;
+    std::vector<GemmTile> t;
+    DBuf<GemmGroup> dg;
+    DBuf<GemmTile> dt;
+    void add(int m, int n, int k, bool ta, bool tb, const double* A, const double* B, double* C, int lda, int ldb,
+             int ldc) {
+        if (m <= 0 || n <= 0 || k <= 0) return;
+        const int gi = (int)g.size();
+        g.push_back(GemmGroup{m, n, k, ta ? 1 : 0, tb ? 1 : 0, A, B, C, lda, ldb, ldc});
+        for (int i = 0; i < (m + 31) / 32; ++i)
+            for (int j = 0; j < (n + 31) / 32; ++j) t.push_back(GemmTile{gi, i, j});
+    }
+    int run(double alpha, double beta, hipStream_t st) {
+        if (t.empty()) return UNO_OK;
+        UNO_HIP(dg.upload(g.data(), g.size(), st));
+        UNO_HIP(dt.upload(t.data(), t.size(), st));
+        UNO_HIP(grouped_gemm((int)t.size(), dt.p, dg.p, alpha, beta, st));
+        // the group / tile arrays are pageable host memory: keep them alive until done
+        UNO_HIP(hipStreamSynchronize(st));
+        return UNO_OK;
+    }
+};
+
+// Z, evals of every slot's symmetric matrix A (destroyed), A given per slot
+int batch_eigh(uno_batch* b, const std::vector<MatSlot>& sl, double* A, double* evals_base,
+               const std::vector<long long>& evoff) {
+    std::vector<BatchMat> bm(sl.size());
+    for (size_t k = 0; k < sl.size(); ++k)
+        bm[k] = BatchMat{sl[k].n, A + sl[k].off, b->V.p + sl[k].off, evals_base + evoff[k], b->Z.p + sl[k].off};
+    DBuf<BatchMat> d;
+    UNO_HIP(d.upload(bm.data(), bm.size(), b->st));
+    UNO_HIP(batched_jacobi((int)bm.size(), d.p, 50, nullptr, b->st));
+    UNO_HIP(hipStreamSynchronize(b->st));
+    return UNO_OK;
+}
+
+}  // namespace
+
+namespace uno {
+
+// batched Fock build on the densities in D0 (D1, D2): W0 (W1) accumulators
+static int batch_fock(uno_batch* b, const std::vector<double>& dmax, double& quartets) {
+    hipStream_t st = b->st;
+    UNO_HIP(k_fill(b->store, 0.0, b->W0.p, st));
+    UNO_HIP(k_fill(b->store, 0.0, b->W1.p, st));
+    // entries grouped by (bra class, ket class, open shell)
+    struct Ent { std::vector<int> bra, ket, mol; std::vector<long long> cnt; };
+    std::map<std::tuple<int, int, int>, Ent> groups;
+    for (int m = 0; m < (int)b->mol.size(); ++m) {
+        BatchMol& x = b->mol[m];
+        if (!x.active) continue;
+        uno_ctx* c = x.ctx;
+        for (QuartetClass& q : c->qcls) {
+            build_prefix(c, q, 4.0 * dmax[m], 0.0);
+            if (q.s64.nq == 0) continue;
+            const PairClassList& LX = c->cls[q.X];
+            const PairClassList& LY = c->cls[q.Y];
+            Ent& e = groups[std::make_tuple(LX.lcls, LY.lcls, x.uhf ? 1 : 0)];
+            for (int i = 0; i < LX.count; ++i) {
+                const long long cnt = q.s64.prefix[i + 1] - q.s64.prefix[i];
+                if (cnt == 0) continue;
+                e.bra.push_back(x.pair_off + LX.off + i);
+                e.ket.push_back(x.pair_off + LY.off);
+                e.mol.push_back(m);
+                e.cnt.push_back(cnt);
+            }
+        }
+    }
+    for (auto& kv : groups) {
+        const int cx = std::get<0>(kv.first), cy = std::get<1>(kv.first), uhf = std::get<2>(kv.first);
+        Ent& e = kv.second;
+        const int ne = (int)e.bra.size();
+        QuartetSpace sp;
+        sp.prefix.assign(ne + 1, 0);
+        for (int i = 0; i < ne; ++i) sp.prefix[i + 1] = sp.prefix[i] + e.cnt[i];
+        sp.nq = sp.prefix[ne];
+        const bool tasks = eri_uses_wave_tasks(cx, cy);
+        const int qpb = eri_quartets_per_block(cx, cy);
+        DBuf<int> dbra, dket, dmol;
+        UNO_HIP(dbra.upload(e.bra.data(), ne, st));
+        UNO_HIP(dket.upload(e.ket.data(), ne, st));
+        UNO_HIP(dmol.upload(e.mol.data(), ne, st));
+        UNO_HIP(sp.d_prefix.upload(sp.prefix.data(), sp.prefix.size(), st));
+        if (tasks) {
+            build_tasks(sp);
+            UNO_HIP(sp.d_wpre.upload(sp.wpre.data(), sp.wpre.size(), st));
+            UNO_HIP(sp.d_tblk.upload(sp.tblk.data(), sp.tblk.size(), st));
+        } else {
+            build_blocks(sp, qpb);
+            UNO_HIP(sp.d_blk.upload(sp.blk.data(), sp.blk.size(), st));
+        }
+        EriArgs a{};
+        a.pairs = b->pairs.p;
+        a.prims = b->prims.p;
+        a.pairQ = b->pairQ.p;
+        a.tau = b->mol[0].ctx->tau;
+        a.primcut = b->mol[0].ctx->primcut;
+        a.prefix = sp.d_prefix.p;
+        a.blk = tasks ? nullptr : sp.d_blk.p;
+        a.wpre = tasks ? sp.d_wpre.p : nullptr;
+        a.tblk = tasks ? sp.d_tblk.p : nullptr;
+        a.ntask = tasks ? sp.ntask : 0;
+        a.qpb = qpb;
+        a.nquart = sp.nq;
+        a.e_bra = dbra.p;
+        a.e_ket = dket.p;
+        a.e_mol = dmol.p;
+        a.mols = b->mdesc.p;
+        a.dsh = b->Dsh.p;
+        a.dmax_global = 0.0;
+        a.D0 = b->D0.p;
+        a.W0 = b->W0.p;
+        if (uhf) {
+            a.D1 = b->D1.p;
+            a.D2 = b->D2.p;
+            a.W1 = b->W1.p;
+        }
+        UNO_HIP(eri_launch(uhf ? kDigestUHF : kDigestRHF, cx, cy, a, st));
+        quartets += (double)sp.nq;
+        // the entry arrays are freed at scope end: finish the launch first
+        UNO_HIP(hipStreamSynchronize(st));
+    }
+    return UNO_OK;
+}
+
+}  // namespace uno
+
+extern "C" {
+
+int uno_batch_create(int nmol, const uno_basis* bases, const int* nelec, uno_batch** out) {
+    if (nmol <= 0 || !bases || !nelec || !out) { set_error("bad argument"); return UNO_EINVAL; }
+    auto* b = new uno_batch;
+    b->mol.resize(nmol);
+    std::vector<PairHdr> pairs;
+    std::vector<PrimPair> prims;
+    std::vector<double> q;
+    std::vector<int> shoff, shn;
+    std::vector<MolDesc> md;
+    for (int m = 0; m < nmol; ++m) {
+        BatchMol& x = b->mol[m];
+        if (int rc = uno_ctx_create(&bases[m], 0, &x.ctx)) { delete b; return rc; }
+        uno_ctx* c = x.ctx;
+        x.n = c->n;
+        x.nsh = c->nsh;
+        x.nelec = nelec[m];
+        x.uhf = nelec[m] % 2 != 0;
+        x.nocc[0] = x.uhf ? nelec[m] - nelec[m] / 2 : nelec[m] / 2;
+        x.nocc[1] = nelec[m] / 2;
+        if (x.n > 256 || nelec[m] <= 0 || x.nocc[0] > x.n) {
+            set_error("batched SCF: molecule " + std::to_string(m) + " has more than 256 functions or a bad "
+                      "electron count");
+            delete b;
+            return UNO_EINVAL;
+        }
+        x.moff = b->store;
+        b->store += (long long)x.n * x.n;
+        x.dshoff = b->dsh_total;
+        b->dsh_total += (long long)x.nsh * x.nsh;
+        x.shoff = (int)shoff.size();
+        for (int s = 0; s < x.nsh; ++s) {
+            shoff.push_back(c->B.off[s]);
+            shn.push_back(ncart(c->B.sh[s].l));
+        }
+        x.pair_off = (int)pairs.size();
+        const int p0 = (int)prims.size();
+        for (PairHdr h : c->pairs) {
+            h.prim0 += p0;
+            pairs.push_back(h);
+        }
+        prims.insert(prims.end(), c->prims.begin(), c->prims.end());
+        q.insert(q.end(), c->pairQ.begin(), c->pairQ.end());
+        md.push_back(MolDesc{x.moff, x.dshoff, x.n, x.nsh});
+        for (int i = 0; i < c->B.ncen; ++i)
+            for (int j = i + 1; j < c->B.ncen; ++j) {
+                const double* r1 = &c->B.xyz[3 * i];
+                const double* r2 = &c->B.xyz[3 * j];
+                const double dx = r1[0] - r2[0], dy = r1[1] - r2[1], dz = r1[2] - r2[2];
+                x.nuc += c->B.q[i] * c->B.q[j] / std::sqrt(dx * dx + dy * dy + dz * dz);
+            }
+    }
+    int ndev = 0;
+    (void)hipGetDeviceCount(&ndev);
+    hipError_t e = hipStreamCreate(&b->st);
+    if (e != hipSuccess) { set_error("HIP stream creation failed"); delete b; return UNO_EDEVICE; }
+    hipStream_t st = b->st;
+    auto up = [&](auto& buf, const auto& v) { return buf.upload(v.data(), v.size(), st); };
+    if (up(b->pairs, pairs) != hipSuccess || up(b->prims, prims) != hipSuccess || up(b->pairQ, q) != hipSuccess ||
+        up(b->sh_off, shoff) != hipSuccess || up(b->sh_n, shn) != hipSuccess || up(b->mdesc, md) != hipSuccess) {
+        set_error("batched SCF: device upload failed");
+        delete b;
+        return UNO_ENOMEM;
+    }
+    const size_t S = (size_t)b->store;
+    DBuf<double>* stores[] = {&b->H, &b->S, &b->X, &b->P[0], &b->P[1], &b->Pold[0], &b->Pold[1], &b->F[0], &b->F[1],
+                              &b->G[0], &b->G[1], &b->C[0], &b->C[1], &b->Fp, &b->Tm, &b->Tm2, &b->V, &b->Z,
+                              &b->D0, &b->D1, &b->D2, &b->W0, &b->W1};
+    for (DBuf<double>* p : stores)
+        if (p->alloc(S) != hipSuccess) { set_error("batched SCF: device allocation failed"); delete b; return UNO_ENOMEM; }
+    for (int k = 0; k < kBatchDiis; ++k)
+        for (int s = 0; s < 2; ++s)
+            if (b->dF[k][s].alloc(S) != hipSuccess || b->dE[k][s].alloc(S) != hipSuccess) {
+                set_error("batched SCF: device allocation failed");
+                delete b;
+                return UNO_ENOMEM;
+            }
+    if (b->Dsh.alloc(std::max<long long>(1, b->dsh_total)) != hipSuccess || b->ev[0].alloc(S) != hipSuccess ||
+        b->ev[1].alloc(S) != hipSuccess || b->red.alloc(8 * (size_t)nmol * (kBatchDiis + 4)) != hipSuccess) {
+        set_error("batched SCF: device allocation failed");
+        delete b;
+        return UNO_ENOMEM;
+    }
+    *out = b;
+    return UNO_OK;
+}
+
+int uno_batch_destroy(uno_batch* b) {
+    delete b;
+    return UNO_OK;
+}
+
+int uno_batch_nmol(const uno_batch* b) { return b ? (int)b->mol.size() : 0; }
+
+int uno_batch_run(uno_batch* b, const uno_scf_opts* opts, uno_scf_result* res, double* energies) {
+    if (!b || !res) { set_error("null argument"); return UNO_EINVAL; }
+    uno_scf_opts o;
+    if (opts) o = *opts; else uno_scf_defaults(&o);
+    if (o.precision || o.incremental || o.guess != 0) {
+        set_error("batched SCF: FP64 full builds from the core-Hamiltonian guess only");
+        return UNO_EINVAL;
+    }
+    const auto t0 = clk::now();
+    hipStream_t st = b->st;
+    const int nm = (int)b->mol.size();
+    const long long S = b->store;
+    // per-molecule eigenvalue offsets into ev stores: reuse the matrix offsets (n <= n^2)
+    // one-electron matrices of every molecule
+    for (BatchMol& x : b->mol) {
+        x.active = true;
+        x.converged = false;
+        x.iterations = 0;
+        x.eold = x.energy = x.ediff = x.pdiff = 0.0;
+        if (int rc = oneint_device(x.ctx, b->S.p + x.moff, b->Tm.p + x.moff, b->H.p + x.moff)) return rc;
+        UNO_HIP(hipStreamSynchronize(x.ctx->st));
+    }
+    for (int s = 0; s < 2; ++s) {
+        UNO_HIP(k_fill(S, 0.0, b->P[s].p, st));
+        UNO_HIP(k_fill(S, 0.0, b->G[s].p, st));
+    }
+    std::vector<MatSlot> sl[2];
+    std::vector<int> who[2];
+    std::vector<DBuf<MatSlot>> dsl(2);
+    auto refresh_slots = [&]() -> int {
+        for (int s = 0; s < 2; ++s) {
+            make_slots(b, true, s, sl[s], who[s]);
+            if (!sl[s].empty()) UNO_HIP(dsl[s].upload(sl[s].data(), sl[s].size(), st));
+        }
+        return UNO_OK;
+    };
+    if (int rc = refresh_slots()) return rc;
+    auto evoff = [&](int s) {
+        std::vector<long long> o;
+        for (const MatSlot& m : sl[s]) o.push_back(m.off);
+        return o;
+    };
+    // X = U s^-1/2 (RHF.hpp:214-233): Jacobi of S, column scaling on the host read-back
+    {
+        UNO_HIP(hipMemcpyAsync(b->Tm.p, b->S.p, S * sizeof(double), hipMemcpyDeviceToDevice, st));
+        if (int rc = batch_eigh(b, sl[0], b->Tm.p, b->ev[0].p, evoff(0))) return rc;
+        std::vector<double> ev(S), Z(S);
+        UNO_HIP(hipMemcpyAsync(ev.data(), b->ev[0].p, S * sizeof(double), hipMemcpyDeviceToHost, st));
+        UNO_HIP(hipMemcpyAsync(Z.data(), b->Z.p, S * sizeof(double), hipMemcpyDeviceToHost, st));
+        UNO_HIP(hipStreamSynchronize(st));
+        for (const BatchMol& x : b->mol) {
+            for (int j = 0; j < x.n; ++j) {
+                double f = ev[x.moff + j];
+                if (std::fabs(f) < 1e-7 && f < 0.0) f = -f;
+                if (!(f > 0.0)) { set_error("Zero or negative eigenvalue in an overlap matrix"); return UNO_ESINGULAR; }
+                const double sc = 1.0 / std::sqrt(f);
+                for (int i = 0; i < x.n; ++i) Z[x.moff + (long long)i * x.n + j] *= sc;
+            }
+        }
+        UNO_HIP(b->X.upload(Z.data(), S, st));
+    }
+    // F (store, per spin) -> P per spin slot: X^T F X, Jacobi, C = X W, P = C_occ C_occ^T
+    auto diag_to_P = [&](int s, const double* Fs) -> int {
+        if (sl[s].empty()) return UNO_OK;
+        GemmBatch g1, g2;
+        for (const MatSlot& m : sl[s]) g1.add(m.n, m.n, m.n, false, false, Fs + m.off, b->X.p + m.off, b->Tm.p + m.off, m.n, m.n, m.n);
+        if (int rc = g1.run(1.0, 0.0, st)) return rc;
+        for (const MatSlot& m : sl[s]) g2.add(m.n, m.n, m.n, true, false, b->X.p + m.off, b->Tm.p + m.off, b->Fp.p + m.off, m.n, m.n, m.n);
+        if (int rc = g2.run(1.0, 0.0, st)) return rc;
+        if (int rc = batch_eigh(b, sl[s], b->Fp.p, b->ev[s].p, evoff(s))) return rc;
+        GemmBatch g3, g4;
+        for (const MatSlot& m : sl[s]) g3.add(m.n, m.n, m.n, false, false, b->X.p + m.off, b->Z.p + m.off, b->C[s].p + m.off, m.n, m.n, m.n);
+        if (int rc = g3.run(1.0, 0.0, st)) return rc;
+        for (size_t k = 0; k < sl[s].size(); ++k) {
+            const MatSlot& m = sl[s][k];
+            g4.add(m.n, m.n, b->mol[who[s][k]].nocc[s], false, true, b->C[s].p + m.off, b->C[s].p + m.off,
+                   b->P[s].p + m.off, m.n, m.n, m.n);
+        }
+        return g4.run(1.0, 0.0, st);
+    };
+    // core-Hamiltonian guess (UHF beta reuses the alpha orbitals, UHF.hpp:206-215)
+    if (int rc = diag_to_P(0, b->H.p)) return rc;
+    if (!sl[1].empty()) {
+        GemmBatch g;
+        for (size_t k = 0; k < sl[1].size(); ++k) {
+            const MatSlot& m = sl[1][k];
+            g.add(m.n, m.n, b->mol[who[1][k]].nocc[1], false, true, b->C[0].p + m.off, b->C[0].p + m.off,
+                  b->P[1].p + m.off, m.n, m.n, m.n);
+        }
+        if (int rc = g.run(1.0, 0.0, st)) return rc;
+    }
+    double eps = o.eps;
+    std::vector<double> hs(4 * (size_t)nm + 8);
+    int ndiis = 0, slot_next = 0;
+    std::vector<double> Bm((size_t)nm * kBatchDiis * kBatchDiis, 0.0);
+    int iteration = 0;
+    double fock_s = 0.0;
+    double quartets = 0.0;
+    while (true) {
+        // densities of the build: D1 = Pa, D2 = Pb (zero for closed shells), D0 = D1 + D2
+        const auto tf = clk::now();
+        UNO_HIP(hipMemcpyAsync(b->D1.p, b->P[0].p, S * sizeof(double), hipMemcpyDeviceToDevice, st));
+        UNO_HIP(hipMemcpyAsync(b->D2.p, b->P[1].p, S * sizeof(double), hipMemcpyDeviceToDevice, st));
+        UNO_HIP(k_axpby(S, 1.0, b->D1.p, 1.0, b->D2.p, b->D0.p, st));
+        UNO_HIP(batch_shell_max((int)sl[0].size(), dsl[0].p, b->sh_off.p, b->sh_n.p, b->D0.p, b->D1.p, b->D2.p,
+                                b->Dsh.p, st));
+        std::vector<double> dsh(b->dsh_total);
+        UNO_HIP(hipMemcpyAsync(dsh.data(), b->Dsh.p, dsh.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        UNO_HIP(hipStreamSynchronize(st));
+        std::vector<double> dmax(nm, 0.0);
+        for (int m = 0; m < nm; ++m)
+            if (b->mol[m].active)
+                for (long long k = 0; k < (long long)b->mol[m].nsh * b->mol[m].nsh; ++k)
+                    dmax[m] = std::max(dmax[m], dsh[b->mol[m].dshoff + k]);
+        if (int rc = batch_fock(b, dmax, quartets)) return rc;
+        UNO_HIP(batch_sym((int)sl[0].size(), dsl[0].p, b->W0.p, b->G[0].p, false, st));
+        if (!sl[1].empty()) UNO_HIP(batch_sym((int)sl[1].size(), dsl[1].p, b->W1.p, b->G[1].p, false, st));
+        fock_s += secs(tf, clk::now());
+        // energies: e1 = tr(Pa H) + tr(Pb H), e2 = tr(Pa Ga) + tr(Pb Gb)
+        double* r = b->red.p;
+        const int n0 = (int)sl[0].size();
+        UNO_HIP(batch_dot(n0, dsl[0].p, b->P[0].p, b->H.p, r, false, st));
+        UNO_HIP(batch_dot(n0, dsl[0].p, b->P[1].p, b->H.p, r, true, st));
+        UNO_HIP(batch_dot(n0, dsl[0].p, b->P[0].p, b->G[0].p, r + nm, false, st));
+        UNO_HIP(batch_dot(n0, dsl[0].p, b->P[1].p, b->G[1].p, r + nm, true, st));
+        UNO_HIP(hipMemcpyAsync(hs.data(), r, 2 * (size_t)nm * sizeof(double), hipMemcpyDeviceToHost, st));
+        UNO_HIP(hipStreamSynchronize(st));
+        for (int k = 0; k < n0; ++k) {
+            BatchMol& x = b->mol[who[0][k]];
+            const double e1 = hs[k], e2 = hs[nm + k];
+            x.energy = x.uhf ? e1 + 0.5 * e2 : 2.0 * e1 + e2;
+            x.ediff = x.energy - x.eold;
+            x.eold = x.energy;
+            if (iteration == 0) x.init_energy = x.energy + x.nuc;
+        }
+        // F = H + G; DIIS on every active slot (one shared history position per iteration)
+        for (int s = 0; s < 2; ++s) UNO_HIP(k_axpby(S, 1.0, b->H.p, 1.0, b->G[s].p, b->F[s].p, st));
+        const int slot = slot_next;
+        slot_next = (slot_next + 1) % kBatchDiis;
+        if (ndiis < kBatchDiis) ++ndiis;
+        for (int s = 0; s < 2; ++s) {
+            if (sl[s].empty()) continue;
+            UNO_HIP(hipMemcpyAsync(b->dF[slot][s].p, b->F[s].p, S * sizeof(double), hipMemcpyDeviceToDevice, st));
+            GemmBatch g1, g2, g3, g4;
+            for (const MatSlot& m : sl[s]) g1.add(m.n, m.n, m.n, false, false, b->F[s].p + m.off, b->P[s].p + m.off, b->Tm.p + m.off, m.n, m.n, m.n);
+            if (int rc = g1.run(1.0, 0.0, st)) return rc;
+            for (const MatSlot& m : sl[s]) g2.add(m.n, m.n, m.n, false, false, b->Tm.p + m.off, b->S.p + m.off, b->Fp.p + m.off, m.n, m.n, m.n);
+            if (int rc = g2.run(1.0, 0.0, st)) return rc;
+            // FPS - SPF = FPS - (FPS)^T
+            UNO_HIP(batch_sym((int)sl[s].size(), dsl[s].p, b->Fp.p, b->Tm2.p, false, st));
+            UNO_HIP(k_axpby(S, 2.0, b->Fp.p, -1.0, b->Tm2.p, b->Tm2.p, st));
+            for (const MatSlot& m : sl[s]) g3.add(m.n, m.n, m.n, false, false, b->Tm2.p + m.off, b->X.p + m.off, b->Tm.p + m.off, m.n, m.n, m.n);
+            if (int rc = g3.run(1.0, 0.0, st)) return rc;
+            for (const MatSlot& m : sl[s]) g4.add(m.n, m.n, m.n, true, false, b->X.p + m.off, b->Tm.p + m.off, b->dE[slot][s].p + m.off, m.n, m.n, m.n);
+            if (int rc = g4.run(1.0, 0.0, st)) return rc;
+        }
+        // B-matrix row of the new vector: sum over spins of <E_new, E_j>
+        std::vector<double> row((size_t)kBatchDiis * nm, 0.0);
+        for (int j = 0; j < ndiis; ++j) {
+            UNO_HIP(batch_dot(n0, dsl[0].p, b->dE[slot][0].p, b->dE[j][0].p, r, false, st));
+            std::vector<double> t0v(n0);
+            UNO_HIP(hipMemcpyAsync(t0v.data(), r, n0 * sizeof(double), hipMemcpyDeviceToHost, st));
+            std::vector<double> t1v(sl[1].size());
+            if (!sl[1].empty()) {
+                UNO_HIP(batch_dot((int)sl[1].size(), dsl[1].p, b->dE[slot][1].p, b->dE[j][1].p, r + nm, false, st));
+                UNO_HIP(hipMemcpyAsync(t1v.data(), r + nm, t1v.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+            }
+            UNO_HIP(hipStreamSynchronize(st));
+            for (int k = 0; k < n0; ++k) row[(size_t)who[0][k] * kBatchDiis + j] = t0v[k];
+            for (size_t k = 0; k < t1v.size(); ++k) row[(size_t)who[1][k] * kBatchDiis + j] += t1v[k];
+        }
+        std::vector<double> coef((size_t)n0 * kBatchDiis, 0.0);
+        bool any = false;
+        for (int k = 0; k < n0; ++k) {
+            const int m = who[0][k];
+            double* Bmm = &Bm[(size_t)m * kBatchDiis * kBatchDiis];
+            for (int j = 0; j < ndiis; ++j) Bmm[slot * kBatchDiis + j] = Bmm[j * kBatchDiis + slot] = row[(size_t)m * kBatchDiis + j];
+            coef[(size_t)k * kBatchDiis + slot] = 1.0;
+            if (ndiis >= 2) {
+                const int q = ndiis + 1;
+                std::vector<double> A(q * q, 0.0), rhs(q, 0.0);
+                for (int i = 0; i < ndiis; ++i) {
+                    for (int j = 0; j < ndiis; ++j) A[i * q + j] = Bmm[i * kBatchDiis + j];
+                    A[i * q + ndiis] = A[ndiis * q + i] = -1.0;
+                }
+                rhs[ndiis] = -1.0;
+                if (solve_small(q, A, rhs)) {
+                    for (int j = 0; j < kBatchDiis; ++j) coef[(size_t)k * kBatchDiis + j] = j < ndiis ? rhs[j] : 0.0;
+                    any = true;
+                }
+            }
+        }
+        if (any) {
+            DBuf<double> dcoef;
+            UNO_HIP(dcoef.upload(coef.data(), coef.size(), st));
+            for (int s = 0; s < 2; ++s) {
+                if (sl[s].empty()) continue;
+                std::vector<const double*> src(kBatchDiis);
+                for (int j = 0; j < kBatchDiis; ++j) src[j] = b->dF[j][s].p;
+                DBuf<const double*> dsrc;
+                UNO_HIP(dsrc.upload(src.data(), src.size(), st));
+                // coefficients of slot k of spin s: the molecule's row (spin 1 slots are a subset)
+                std::vector<double> cs((size_t)sl[s].size() * kBatchDiis);
+                for (size_t k = 0; k < sl[s].size(); ++k) {
+                    const int m = who[s][k];
+                    const size_t k0 = std::find(who[0].begin(), who[0].end(), m) - who[0].begin();
+                    std::copy(&coef[k0 * kBatchDiis], &coef[k0 * kBatchDiis] + kBatchDiis, &cs[k * kBatchDiis]);
+                }
+                DBuf<double> dcs;
+                UNO_HIP(dcs.upload(cs.data(), cs.size(), st));
+                UNO_HIP(batch_lincomb((int)sl[s].size(), dsl[s].p, kBatchDiis, dcs.p, dsrc.p, b->F[s].p, st));
+                UNO_HIP(hipStreamSynchronize(st));
+            }
+        }
+        // new densities
+        for (int s = 0; s < 2; ++s)
+            UNO_HIP(hipMemcpyAsync(b->Pold[s].p, b->P[s].p, S * sizeof(double), hipMemcpyDeviceToDevice, st));
+        for (int s = 0; s < 2; ++s)
+            if (int rc = diag_to_P(s, b->F[s].p)) return rc;
+        UNO_HIP(batch_diffnorm(n0, dsl[0].p, b->P[0].p, b->Pold[0].p, r, st));
+        if (!sl[1].empty()) UNO_HIP(batch_diffnorm((int)sl[1].size(), dsl[1].p, b->P[1].p, b->Pold[1].p, r + nm, st));
+        std::vector<double> pd(2 * (size_t)nm, 0.0);
+        UNO_HIP(hipMemcpyAsync(pd.data(), r, 2 * (size_t)nm * sizeof(double), hipMemcpyDeviceToHost, st));
+        UNO_HIP(hipStreamSynchronize(st));
+        ++iteration;
+        bool changed = false;
+        for (int k = 0; k < n0; ++k) {
+            BatchMol& x = b->mol[who[0][k]];
+            x.pdiff = pd[k];
+            x.iterations = iteration;
+        }
+        for (size_t k = 0; k < sl[1].size(); ++k) b->mol[who[1][k]].pdiff += pd[nm + k];
+        for (int k = 0; k < n0; ++k) {
+            BatchMol& x = b->mol[who[0][k]];
+            bool conv = false;
+            if (iteration >= 2) {
+                if (o.cflag == 0) conv = x.ediff < eps;
+                else if (o.cflag == 1) conv = x.pdiff < eps;
+                else if (o.cflag == 3) conv = std::fabs(x.ediff) < eps;
+                else conv = x.pdiff < eps && x.ediff < eps;
+            }
+            if (conv || iteration >= o.maxits) {
+                x.converged = conv;
+                x.active = false;
+                changed = true;
+            }
+        }
+        bool any_active = false;
+        for (const BatchMol& x : b->mol) any_active = any_active || x.active;
+        if (!any_active) break;
+        if (changed) {
+            if (int rc = refresh_slots()) return rc;
+        }
+    }
+    const double wall = secs(t0, clk::now());
+    int rc = UNO_OK;
+    for (int m = 0; m < nm; ++m) {
+        const BatchMol& x = b->mol[m];
+        uno_scf_result& r = res[m];
+        std::memset(&r, 0, sizeof(r));
+        r.energy = x.energy + x.nuc;
+        r.init_energy = x.init_energy;
+        r.ediff = x.ediff;
+        r.pdiff = x.pdiff;
+        r.nucrep = x.nuc;
+        r.electronic = x.energy;
+        r.iterations = x.iterations;
+        r.converged = x.converged ? 1 : 0;
+        r.uhf = x.uhf ? 1 : 0;
+        r.nfock = x.iterations;
+        r.wall_s = wall;
+        r.fock_s = fock_s;
+        r.quartets = quartets;
+        if (energies) energies[m] = r.energy;
+        if (!x.converged) rc = UNO_ENOCONV;
+    }
+    return rc;
+}
+
+}  // extern "C"

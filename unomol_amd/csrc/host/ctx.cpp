@@ -30,7 +30,7 @@ static int ensure_device() {
 // Quartet spaces of one launch.  Both runs are sorted by Schwarz bound (descending), so
 // for bra i the kets passing Qi Qj dmax >= tau form a prefix of the ket run, and within
 // it the kets with Qi Qj >= fp32_thr (FP64 in a mixed build) form a shorter prefix.
-static void build_prefix(uno_ctx* c, QuartetClass& q, double dmax, double fp32_thr) {
+void build_prefix(uno_ctx* c, QuartetClass& q, double dmax, double fp32_thr) {
     const PairClassList& Lb = c->cls[q.X];
     const PairClassList& Lk = c->cls[q.Y];
     const int nb = (int)Lb.Q.size();
@@ -64,7 +64,7 @@ static void build_prefix(uno_ctx* c, QuartetClass& q, double dmax, double fp32_t
 
 // wave tasks: bra i owns ceil(count_i / 64) consecutive tasks from wpre[i]; tblk[b] is
 // the bra of task 64 b, bracketing each wave's bra search
-static void build_tasks(QuartetSpace& s) {
+void build_tasks(QuartetSpace& s) {
     const int nb = (int)s.prefix.size() - 1;
     s.wpre.assign(nb + 1, 0);
     for (int i = 0; i < nb; ++i) s.wpre[i + 1] = s.wpre[i] + (s.prefix[i + 1] - s.prefix[i] + 63) / 64;
@@ -81,7 +81,7 @@ static void build_tasks(QuartetSpace& s) {
 
 // block table: bra holding the first quartet of every block, so that each lane's bra
 // search covers a short range
-static void build_blocks(QuartetSpace& s, int qpb) {
+void build_blocks(QuartetSpace& s, int qpb) {
     const long long nblk = (s.nq + qpb - 1) / qpb;
     s.blk.resize(nblk + 1);
     int i = 0;
@@ -92,7 +92,7 @@ static void build_blocks(QuartetSpace& s, int qpb) {
     }
 }
 
-static EriArgs base_args(uno_ctx* c) {
+EriArgs base_args(uno_ctx* c) {
     EriArgs a{};
     a.pairs = c->d_pairs.p;
     a.prims = c->d_prims.p;

@@ -139,6 +139,11 @@ struct uno_ctx {
 namespace uno {
 
 int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell);
+// quartet-space helpers of the Fock build (ctx.cpp)
+void build_prefix(uno_ctx* c, QuartetClass& q, double dmax, double fp32_thr);
+void build_tasks(QuartetSpace& s);
+void build_blocks(QuartetSpace& s, int qpb);
+EriArgs base_args(uno_ctx* c);
 // dense S, T, H = T + V (n x n, device) on the context's stream
 int oneint_device(uno_ctx* c, double* S, double* T, double* H);
 // Fock build from dense device densities already in c->D0 (and D1, D2 for UHF).
