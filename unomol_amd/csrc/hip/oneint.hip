@@ -8,6 +8,7 @@
 // nuclear-attraction Boys function keeps the reference's semantics
 // (MD_Rfunction.hpp:2185-2206): the pure large-t limit above t = 20; below it the m-series
 // summed until a term is < 1e-12, then downward recursion.
+// The "skip" centre (the DPM positron, OneElectronInts.cpp:43) is left out of V.
 // Outputs are dense row-major N x N; each lane writes its block and its transpose.
 #include <hip/hip_runtime.h>
 #include "oneint_kernels.hpp"
@@ -116,6 +117,7 @@ __global__ __launch_bounds__(128) void k_oneint(int npair, const int2* __restric
                     for (int z = 0; z <= L; ++z) V[x][y][z] = 0.0;
             const double pref = 2.0 * M_PI * ip * K;
             for (int c = 0; c < a.ncen; ++c) {
+                if (c == a.skip) continue;   // OneElectronInts.cpp:43
                 const double pc[3] = {P[0] - a.xyz[3 * c], P[1] - a.xyz[3 * c + 1], P[2] - a.xyz[3 * c + 2]};
                 double f[L + 1];
                 boys_ref(f, p * (pc[0] * pc[0] + pc[1] * pc[1] + pc[2] * pc[2]), L);
@@ -154,8 +156,6 @@ __global__ __launch_bounds__(128) void k_oneint(int npair, const int2* __restric
             for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
                 for (int ib = 0; ib < NB; ++ib) {
-                    constexpr int dummy = 0;
-                    (void)dummy;
                     const int l1 = cpow(LA, ia, 0), m1 = cpow(LA, ia, 1), n1 = cpow(LA, ia, 2);
                     const int l2 = cpow(LB, ib, 0), m2 = cpow(LB, ib, 1), n2 = cpow(LB, ib, 2);
                     const double nf = cnorm(LA, ia) * cnorm(LB, ib) * cc;

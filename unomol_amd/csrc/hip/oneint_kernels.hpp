@@ -15,6 +15,7 @@ struct OneIntArgs {
     const double* xyz;      // centres (bohr), 3 per centre
     const double* charge;   // nuclear charges
     int ncen, n;
+    int skip;               // centre left out of the nuclear attraction (-1: none)
     double* S;              // dense n x n outputs
     double* T;
     double* H;

@@ -281,6 +281,7 @@ int uno_ctx_set(uno_ctx* c, const char* key, double v) {
     std::string k(key);
     if (k == "schwarz") c->tau = v;
     else if (k == "prim_cut") c->primcut = v;
+    else if (k == "skip_center") c->skip_center = (int)v;
     else if (k == "fp32_threshold") c->fp32_thr = v;
     else if (k == "precision") c->precision = (int)v;
     else { set_error("unknown option " + k); return UNO_EINVAL; }
@@ -292,6 +293,7 @@ int uno_ctx_get(uno_ctx* c, const char* key, double* v) {
     std::string k(key);
     if (k == "schwarz") *v = c->tau;
     else if (k == "prim_cut") *v = c->primcut;
+    else if (k == "skip_center") *v = c->skip_center;
     else if (k == "fp32_threshold") *v = c->fp32_thr;
     else if (k == "precision") *v = c->precision;
     else if (k == "last_quartets") *v = c->last_quartets;
@@ -640,6 +642,7 @@ static int run_dpm(const std::string& dir, const Deck& d, const uno_scf_opts& o,
         basis_view(a, b);
         uno_ctx* c = nullptr;
         if ((rc = uno_ctx_create(&b, 0, &c))) break;
+        c->skip_center = d.ncen;   // the positron's charge is not in V (OneElectronInts.cpp:43)
         std::vector<double> dh;
         if ((rc = gdpm_matrix(c, R, dh))) { uno_ctx_destroy(c); break; }
         // the ground-state density padded with zeros for the positron-centre functions

@@ -323,6 +323,7 @@ int oneint_device(uno_ctx* c, double* S, double* T, double* H) {
     a.charge = c->oi_charge.p;
     a.ncen = c->B.ncen;
     a.n = c->n;
+    a.skip = c->skip_center;
     a.S = S;
     a.T = T;
     a.H = H;

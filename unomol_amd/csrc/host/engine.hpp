@@ -127,6 +127,7 @@ struct uno_ctx {
     double prof_launch[uno::kProfSlots] = {};
     uno::DBuf<unsigned long long> d_counter;
     uno_mlp* guess_model = nullptr;   // not owned: the network of guess = 2
+    int skip_center = -1;             // centre left out of the nuclear attraction (DPM positron)
     // one-electron data on the device: shells, primitives, centres and the shell pairs
     // of each (la >= lb) class
     uno::DBuf<uno::OneIntShell> oi_shells;
