@@ -436,9 +436,15 @@ def test_finite_field_decks_vs_reference(tmp_path, tag):
     if tag not in FF_DEGENERATE_CORE_H:
         assert abs(got["init"] - ref["init"]) < 1e-9
     if not mref["converged"]:
-        # the reference's damped SCF does not converge on this deck within its 299
-        # iterations; neither does the same iteration here
-        assert not r.converged and mine["iterations"] == mref["iterations"]
+        # the reference's damped SCF does not meet its test on this deck within 299
+        # iterations: either it sits on the rounding floor of the solution (|dE| ~ 1e-13,
+        # the density change just above the eps floor), where this engine may or may not
+        # pass the same test but holds the same energy, or it still oscillates, and then
+        # so does the same iteration here
+        if abs(ref["ediff"]) < 1e-9:
+            assert abs(got["energy"] - ref["energy"]) < 1e-9
+        else:
+            assert not r.converged and mine["iterations"] == mref["iterations"]
         return
     assert r.converged
     assert abs(got["energy"] - ref["energy"]) < 1e-9
