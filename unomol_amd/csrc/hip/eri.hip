@@ -533,7 +533,10 @@ __global__ __launch_bounds__(64) void k_eri_wave(const EriArgs a) {
 // launcher: dispatch on the class and digest mode
 // ---------------------------------------------------------------------------
 // lane kernels: one quartet per lane, 256 per block; wave kernels: 8 per block
-static constexpr int quartets_per_block(int ncomp) { return ncomp <= 81 ? 256 : 8; }
+// classes up to kLaneMaxComp components run one quartet per lane (the component block in
+// registers); larger ones one quartet per wave
+static constexpr int kLaneMaxComp = 162;
+static constexpr int quartets_per_block(int ncomp) { return ncomp <= kLaneMaxComp ? 256 : 8; }
 
 template <int LA, int LB, int LC, int LD, int MODE, typename T = double>
 static hipError_t launch_cls(const EriArgs& a, hipStream_t st) {
@@ -542,12 +545,12 @@ static hipError_t launch_cls(const EriArgs& a, hipStream_t st) {
     EriArgs b = a;
     b.qpb = quartets_per_block(CI::NCOMP);
     const long long nb = (a.nquart + b.qpb - 1) / b.qpb;
-    if constexpr (CI::NCOMP <= 81 && MODE <= kDigestUHF) {
+    if constexpr (CI::NCOMP <= kLaneMaxComp && MODE <= kDigestUHF) {
         // Fock builds: wave tasks (bra, chunk of 64 kets), four per block
         if (a.ntask <= 0 || !a.wpre || !a.tblk) return hipErrorInvalidValue;
         const long long nt = (a.ntask + 3) / 4;
         hipLaunchKernelGGL((k_eri_bra<LA, LB, LC, LD, MODE, T>), dim3((unsigned)nt), dim3(256), 0, st, b);
-    } else if constexpr (CI::NCOMP <= 81) {
+    } else if constexpr (CI::NCOMP <= kLaneMaxComp) {
         hipLaunchKernelGGL((k_eri_lane<LA, LB, LC, LD, MODE, T>), dim3((unsigned)nb), dim3(256), 0, st, b);
     } else {
         hipLaunchKernelGGL((k_eri_wave<LA, LB, LC, LD, MODE, T>), dim3((unsigned)nb), dim3(64), 0, st, b);
@@ -606,7 +609,7 @@ int eri_class_ncomp(int cb, int ck) {
 
 int eri_quartets_per_block(int cb, int ck) { return quartets_per_block(eri_class_ncomp(cb, ck)); }
 
-bool eri_uses_wave_tasks(int cb, int ck) { return eri_class_ncomp(cb, ck) <= 81; }
+bool eri_uses_wave_tasks(int cb, int ck) { return eri_class_ncomp(cb, ck) <= kLaneMaxComp; }
 
 // ---------------------------------------------------------------------------
 // Rys-table evaluation for the parity tests

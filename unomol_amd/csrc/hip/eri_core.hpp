@@ -9,7 +9,7 @@
 // from this project's own Chebyshev tables (rys_table.inc, gen_rys_table.py).
 //
 // Two execution shapes:
-//   * one lane per shell quartet (classes with <= 81 components): VRR, transfer and the
+//   * one lane per shell quartet (classes with <= 162 components): VRR, transfer and the
 //     contracted component block live in registers, fully unrolled per class;
 //   * one 64-lane wave per shell quartet (larger classes): lanes split the roots, the
 //     (root, axis) recurrences, the transfer entries and the component assembly, with the
