@@ -18,12 +18,15 @@ iteration.
 The learned guess loads unomol_amd/models/guess_631gs.bin; without it, a network is
 trained first on small clusters (outside the timed region, reported).
 
-Prints ONE JSON line with the metric, a roofline object for the dominant FP64 ERI kernel
-class (per-class kernel time from HIP events on the engine's stream; algorithmic FLOPs
-= the per-primitive-quartet operation count of the kernel's algorithm times the
-primitive quartets the kernel evaluated) and a cpu_baseline object (the reference's Rys
-ERI code compiled from its sources into oracle/_ref, timed on a random sample of the
-same workload's shell quartets on one host core).
+Prints ONE JSON line with the metric, a roofline object for the dominant ERI kernel of
+the timed steps (the class/precision kernel with the largest summed time; per-launch
+time from HIP events recorded on the engine's stream around each launch inside the timed
+region; algorithmic FLOPs = the per-primitive-quartet operation count of the kernel's
+algorithm times the primitive quartets the kernel evaluated, counted by the kernel; HBM
+traffic per launch from the committed FETCH_SIZE/WRITE_SIZE passes when they cover the
+same kernel) and a cpu_baseline object (the reference's Rys ERI code compiled from its
+sources into oracle/_ref, timed on a random sample of the same workload's shell quartets
+on one host core).
 """
 import argparse
 import ctypes as C
@@ -42,6 +45,7 @@ PAIR_L = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]
 PAIR_NAME = ["ss", "ps", "pp", "ds", "dp", "dd"]
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (and matrix) dense peak
 FP32_PEAK_TFLOPS = 157.3
+LANE_MAX_COMP = 162       # classes up to this many components run one quartet per lane
 MODEL = os.path.join(ROOT, "unomol_amd", "models", "guess_631gs.bin")
 
 
@@ -97,15 +101,24 @@ def cpu_baseline(deck_path, budget_s=15.0):
             "shell_quartets_per_s": rate}
 
 
-def profile_classes(N, t, P):
-    """One profiled Fock build on density P: per-class (FP64 and FP32) kernel time and
-    evaluated primitive / shell quartets."""
+def read_profile(N, t):
+    """Per-class kernel time (HIP events on the engine's stream), launches and evaluated
+    primitive / shell quartets accumulated since profiling was switched on."""
     ms = np.zeros(72); launches = np.zeros(72); prim = np.zeros(72); shell = np.zeros(72)
+    N.check(N.lib().uno_profile(t.ctx, -1, N.ptr(ms), N.ptr(launches), N.ptr(prim), N.ptr(shell)))
+    N.check(N.lib().uno_profile(t.ctx, 0, None, None, None, None))
+    return ms, launches, prim, shell
+
+
+def profile_classes(N, t, P):
+    """One profiled Fock build on density P."""
     N.check(N.lib().uno_profile(t.ctx, 1, None, None, None, None))
     G = np.zeros_like(P)
     t.formGmatrix(P, G)
-    N.check(N.lib().uno_profile(t.ctx, -1, N.ptr(ms), N.ptr(launches), N.ptr(prim), N.ptr(shell)))
-    N.check(N.lib().uno_profile(t.ctx, 0, None, None, None, None))
+    return class_table(*read_profile(N, t))
+
+
+def class_table(ms, launches, prim, shell):
     classes = []
     for prec in range(2):
         for X in range(6):
@@ -118,21 +131,52 @@ def profile_classes(N, t, P):
                 fl = prim[ci] * flops_per_prim_quartet(la, lb, lc, ld) + shell[ci] * flops_per_shell_quartet_digest(
                     la, lb, lc, ld)
                 classes.append({"class": "(%s|%s)" % (PAIR_NAME[X], PAIR_NAME[Y]), "precision": "f32" if prec else "f64",
-                                "ms": ms[ci], "prim_quartets": prim[ci], "shell_quartets": shell[ci],
+                                "ms": ms[ci], "launches": launches[ci], "prim_quartets": prim[ci],
+                                "shell_quartets": shell[ci],
                                 "gflop": fl / 1e9, "tflops": fl / (ms[ci] * 1e-3) / 1e12 if ms[ci] > 0 else 0.0})
     return classes
 
 
+def kernel_name(cls):
+    """Demangled name of the Fock-build kernel of a class entry (as rocprofv3 prints it)."""
+    X, Y = PAIR_NAME.index(cls["class"][1:3]), PAIR_NAME.index(cls["class"][4:6])
+    la, lb = PAIR_L[X]
+    lc, ld = PAIR_L[Y]
+    lane = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) <= LANE_MAX_COMP
+    return "void uno::%s<%d, %d, %d, %d, 0, %s>(uno::EriArgs)" % (
+        "k_eri_bra" if lane else "k_eri_wave", la, lb, lc, ld, "float" if cls["precision"] == "f32" else "double")
+
+
+def traffic_of(kernel):
+    """HBM bytes per launch of the kernel from the committed FETCH_SIZE / WRITE_SIZE passes
+    (profiles/eri_traffic_w64_631gs.md, last line = JSON), or None if they cover another kernel."""
+    path = os.path.join(ROOT, "profiles", "eri_traffic_w64_631gs.md")
+    if not os.path.exists(path):
+        return None
+    rec = None
+    for line in open(path):
+        line = line.strip()
+        if line.startswith("{"):
+            rec = json.loads(line)
+    if not rec or rec.get("kernel") != kernel:
+        return None
+    return rec.get("bytes_per_launch")
+
+
 def roofline_of(classes):
-    f64 = [c for c in classes if c["precision"] == "f64"]
-    dom = max(f64, key=lambda c: c["ms"])
-    tot_fl = sum(c["gflop"] for c in f64) * 1e9
-    tot_ms = sum(c["ms"] for c in f64)
-    return {"bound": "valu-fp64", "kernel": "k_eri %s" % dom["class"],
-            "achieved": round(dom["tflops"], 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(dom["tflops"] / FP64_PEAK_TFLOPS, 5), "traffic": None,
-            "all_eri_kernels": {"ms": round(tot_ms, 3), "tflops": round(tot_fl / (tot_ms * 1e-3) / 1e12, 4),
-                                "frac": round(tot_fl / (tot_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 5)}}
+    """Roofline entry for the dominant ERI kernel (largest summed time) of the classes."""
+    dom = max(classes, key=lambda c: c["ms"])
+    peak = FP32_PEAK_TFLOPS if dom["precision"] == "f32" else FP64_PEAK_TFLOPS
+    tot_fl = sum(c["gflop"] for c in classes) * 1e9
+    tot_ms = sum(c["ms"] for c in classes)
+    name = kernel_name(dom)
+    return {"bound": "mfma", "pipe": "%s VALU: no dense contraction; MI355X's published dense %s peak, "
+                                     "%.1f TFLOP/s, is the bound" % (dom["precision"].upper(), dom["precision"].upper(), peak),
+            "kernel": name, "class": dom["class"], "precision": dom["precision"],
+            "launches": int(dom["launches"]), "avg_launch_ms": round(dom["ms"] / max(dom["launches"], 1), 4),
+            "achieved": round(dom["tflops"], 4), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(dom["tflops"] / peak, 5), "traffic": traffic_of(name),
+            "all_eri_kernels": {"ms": round(tot_ms, 3), "tflops": round(tot_fl / (tot_ms * 1e-3) / 1e12, 4)}}
 
 
 def get_model(unomol_amd):
@@ -193,11 +237,13 @@ def main():
     model, model_info = (None, None) if args.fp64_only else get_model(unomol_amd)
     fast_opts = dict(fp64_opts, precision=1, incremental=1, guess=2, fp64_final=3)
 
-    def one_scf(opts):
+    def one_scf(opts, profile=False):
         t0 = time.perf_counter()
         t = unomol_amd.TwoElectronInts(basis)
         if opts.get("guess") == 2:
             unomol_amd.set_guess_model(t, model)
+        if profile:
+            N.check(N.lib().uno_profile(t.ctx, 1, None, None, None, None))
         hf = unomol_amd.RestrictedHartreeFock(basis, t)
         e = hf.findEnergy(**opts)
         dt = time.perf_counter() - t0
@@ -209,15 +255,29 @@ def main():
     for i in range(args.warmup):
         _, _, e, dt = one_scf(step_opts)
         log("warmup %d: E=%.10f in %.2f s" % (i, e, dt))
+    # timed region: every step's Fock-build kernels carry per-class HIP events on the
+    # engine's stream (and slot-spread evaluated-quartet counters)
     results = []
+    acc = [np.zeros(72) for _ in range(4)]
     t_all = time.perf_counter()
     for _ in range(args.steps):
-        t, hf, e, dt = one_scf(step_opts)
+        t, hf, e, dt = one_scf(step_opts, profile=not args.no_profile)
+        if not args.no_profile:
+            for k, v in enumerate(read_profile(N, t)):
+                acc[k] += v
         log("step: E=%.10f in %.2f s, %d iterations" % (e, dt, hf.result.iterations))
         results.append((t, hf, e, dt))
     wall = time.perf_counter() - t_all
     t, hf, energy, dt = results[-1]
     r = hf.result
+    classes, roofline = [], None
+    if not args.no_profile:
+        for v in acc:   # per step
+            v /= args.steps
+        classes = class_table(*acc)
+        roofline = roofline_of(classes)
+        log("dominant kernel %s: %d launches per step, %.3f ms per launch, %.3f TFLOP/s" %
+            (roofline["kernel"], roofline["launches"], roofline["avg_launch_ms"], roofline["achieved"]))
 
     # the plain FP64 reference run (core-H guess, full FP64 builds)
     if args.fp64_only:
@@ -226,11 +286,11 @@ def main():
         t64, hf64, e64, dt64 = one_scf(fp64_opts)
         log("fp64 reference: E=%.10f in %.2f s, %d iterations" % (e64, dt64, hf64.result.iterations))
     r64 = hf64.result
-
-    classes, roofline = [], None
-    if not args.no_profile:
-        classes = profile_classes(N, t64, hf64.P)
-        roofline = roofline_of(classes)
+    # streaming copy of 1 GiB: the HBM reference point for the traffic counters
+    cms = C.c_double()
+    N.check(N.lib().uno_copy_probe(C.c_ulonglong(1 << 30), 1, C.byref(cms)))
+    copy = {"bytes_each_way": 1 << 30, "ms": round(cms.value, 4),
+            "GBps": round(2 * (1 << 30) / (cms.value * 1e-3) / 1e9, 1)}
     log("cpu baseline")
     cpu = None if args.no_cpu_baseline else cpu_baseline(deck)
 
@@ -269,6 +329,7 @@ def main():
         "guess_model": model_info,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "hbm_copy_probe": copy,
         "eri_classes": classes,
     }
     print(json.dumps(out))

@@ -28,6 +28,9 @@ struct alignas(16) PrimPair {
     double pad;
 };
 
+// slot pairs of the evaluated-quartet counters of one launch (spread to avoid contention)
+constexpr int kCounterSlots = 64;
+
 // pair class index for (l1 >= l2), l <= 2: ss=0 ps=1 pp=2 ds=3 dp=4 dd=5
 constexpr int pair_class(int l1, int l2) { return l1 * (l1 + 1) / 2 + l2; }
 constexpr int kMaxL = 2;
@@ -89,7 +92,8 @@ struct EriArgs {
     double* W0;               // RHF: W; UHF: Wa
     double* W1;               // UHF: Wb
     double* out;              // block / Schwarz output
-    unsigned long long* counter;  // evaluated-quartet counter (may be null)
+    unsigned long long* counter;  // evaluated-quartet counters (may be null): kCounterSlots
+                                  // (primitive, shell) pairs, summed by the host
 };
 
 }  // namespace uno

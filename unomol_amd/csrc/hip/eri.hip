@@ -9,10 +9,13 @@ namespace uno {
 
 // ---------------------------------------------------------------------------
 // contracted (un-normalised) ERI block of one shell quartet in one lane; returns the
-// primitive quartets kept by the cut.  Ket primitives outside, bra primitives inside.
+// primitive quartets kept by the cut.  Ket primitives outside, bra primitives inside; the
+// next bra primitive record is fetched while the current one is evaluated.  tab is the
+// Rys table of this class's root count (global or an LDS copy).
 // ---------------------------------------------------------------------------
 template <int LA, int LB, int LC, int LD, typename T>
-__device__ __forceinline__ unsigned eri_block(const EriArgs& a, const PairHdr& P1, const PairHdr& P2, T* e) {
+__device__ __forceinline__ unsigned eri_block(const EriArgs& a, const PairHdr& P1, const PairHdr& P2, T* e,
+                                              const T* tab) {
     using CI = ClassInfo<LA, LB, LC, LD>;
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND, NCOMP = CI::NCOMP, NR = CI::NR;
 #pragma unroll
@@ -20,15 +23,20 @@ __device__ __forceinline__ unsigned eri_block(const EriArgs& a, const PairHdr& P
     unsigned nprimq = 0;
     const T ab[3] = {(T)P1.AB[0], (T)P1.AB[1], (T)P1.AB[2]};
     const T cd[3] = {(T)P2.AB[0], (T)P2.AB[1], (T)P2.AB[2]};
+    const PrimPair* bp = a.prims + P1.prim0;
     for (int iq = 0; iq < P2.nprim; ++iq) {
         const PrimPair qq = a.prims[P2.prim0 + iq];
         const T q = (T)qq.p, iq_ = (T)qq.ip, sq = (T)qq.s, Kq = (T)qq.K;
         // coordinate differences in double, then rounded to the compute type
         const T QC[3] = {(T)(qq.P[0] - P2.A[0]), (T)(qq.P[1] - P2.A[1]), (T)(qq.P[2] - P2.A[2])};
+        PrimPair nx;
+        if (P1.nprim > 0) nx = bp[0];
         for (int ip = 0; ip < P1.nprim; ++ip) {
-            const PrimPair pp = a.prims[P1.prim0 + ip];
+            const PrimPair pp = nx;
+            if (ip + 1 < P1.nprim) nx = bp[ip + 1];
             const T p = (T)pp.p, ip_ = (T)pp.ip;
-            const T it = (T)1 / (p + q), rt = sqrt(it);
+            // rt = (p+q)^(-1/2), it = 1/(p+q)
+            const T rt = rsqrt_nr(p + q), it = rt * rt;
             const T bare = (T)kTwoPi25 * (T)pp.s * sq * ip_ * iq_ * rt;
             if (bare < (T)a.primcut) continue;
             ++nprimq;
@@ -37,7 +45,7 @@ __device__ __forceinline__ unsigned eri_block(const EriArgs& a, const PairHdr& P
             const T PQ[3] = {(T)(pp.P[0] - qq.P[0]), (T)(pp.P[1] - qq.P[1]), (T)(pp.P[2] - qq.P[2])};
             const T x = p * q * it * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
             T u[NR], w[NR];
-            rys_roots<NR, T>(x, u, w);
+            rys_roots<NR, T>(x, u, w, tab);
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 const T fff = u[r] * it;
@@ -101,10 +109,11 @@ __global__ __launch_bounds__(256) void k_eri_lane(const EriArgs a) {
     const PairHdr P1 = a.pairs[pab];
     const PairHdr P2 = a.pairs[pcd];
     T e[NCOMP];
-    const unsigned nprimq = eri_block<LA, LB, LC, LD, T>(a, P1, P2, e);
+    const unsigned nprimq = eri_block<LA, LB, LC, LD, T>(a, P1, P2, e, rys_table_global<CI::NR, T>());
     if (a.counter) {
-        atomicAdd(a.counter, (unsigned long long)nprimq);
-        atomicAdd(a.counter + 1, 1ull);
+        unsigned long long* slot = a.counter + 2 * (blockIdx.x & (kCounterSlots - 1));
+        atomicAdd(slot, (unsigned long long)nprimq);
+        atomicAdd(slot + 1, 1ull);
     }
     normalise_block<LA, LB, LC, LD, T>(e, (T)1);
     if constexpr (MODE == kDigestBlock) {
@@ -121,6 +130,9 @@ __global__ __launch_bounds__(256) void k_eri_lane(const EriArgs a) {
         a.out[pab] = sqrt(m);
     }
 }
+
+// Rys tables of up to this many roots are staged in LDS by the Fock-build kernels
+constexpr int kLdsTabMaxRoots = 3;
 
 // ---------------------------------------------------------------------------
 // Fock builds: one wave per (bra pair, chunk of up to 64 kets).  The bra is the same in
@@ -139,6 +151,16 @@ __global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND, NCOMP = CI::NCOMP;
     const int lane = threadIdx.x & 63;
     const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    // the Rys table of this root count, copied to LDS by the block (up to 35 KB)
+    constexpr int NR = CI::NR;
+    constexpr bool kLdsTab = NR <= kLdsTabMaxRoots;
+    __shared__ T tab_lds[kLdsTab ? rys_table_size<NR, T>() : 1];
+    const T* tab = rys_table_global<NR, T>();
+    if constexpr (kLdsTab) {
+        for (int i = threadIdx.x; i < rys_table_size<NR, T>(); i += blockDim.x) tab_lds[i] = tab[i];
+        __syncthreads();
+        tab = tab_lds;
+    }
     if (t >= a.ntask) return;   // whole wave
     // bra of task t: the largest i with wpre[i] <= t, bracketed by the task block table
     int lo = a.tblk[t >> 6], hi = a.tblk[(t >> 6) + 1];
@@ -163,16 +185,24 @@ __global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
     T e[NCOMP];
     unsigned nprimq = 0;
     if (active) {
-        nprimq = eri_block<LA, LB, LC, LD, T>(a, P1, P2, e);
+        nprimq = eri_block<LA, LB, LC, LD, T>(a, P1, P2, e, tab);
         const T f = (P1.same ? (T)0.5 : (T)1) * (P2.same ? (T)0.5 : (T)1) * (pab == pcd ? (T)0.5 : (T)1);
         normalise_block<LA, LB, LC, LD, T>(e, f);
     } else {
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) e[c] = 0;
     }
-    if (a.counter && active) {
-        atomicAdd(a.counter, (unsigned long long)nprimq);
-        atomicAdd(a.counter + 1, 1ull);
+    if (a.counter) {
+        // evaluated primitive and shell quartets of the wave, into one of 64 slot pairs
+        int np = (int)nprimq;
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) np += __shfl_xor(np, m, 64);
+        const unsigned long long na = __popcll(__ballot(active));
+        if (lane == 0) {
+            unsigned long long* slot = a.counter + 2 * (blockIdx.x & (kCounterSlots - 1));
+            atomicAdd(slot, (unsigned long long)np);
+            atomicAdd(slot + 1, na);
+        }
     }
     const int n = mv.n;
     const int fa = P1.f1, fb = P1.f2;
@@ -423,8 +453,9 @@ __device__ __forceinline__ void wave_one(const EriArgs& a, const long long g) {
         }
     }
     if (a.counter && lane == 0) {
-        atomicAdd(a.counter, nprimq);
-        atomicAdd(a.counter + 1, 1ull);
+        unsigned long long* slot = a.counter + 2 * (blockIdx.x & (kCounterSlots - 1));
+        atomicAdd(slot, nprimq);
+        atomicAdd(slot + 1, 1ull);
     }
     T f = 1;
     if (MODE <= kDigestUHF) f = (P1.same ? (T)0.5 : (T)1) * (P2.same ? (T)0.5 : (T)1) * (pab == pcd ? (T)0.5 : (T)1);

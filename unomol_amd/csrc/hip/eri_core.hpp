@@ -51,14 +51,45 @@ __host__ __device__ constexpr double binom_c(int n, int k) {
 
 constexpr double kTwoPi25 = 34.986836655249725;   // 2 pi^(5/2)
 
-// Rys roots u = t^2 and weights for argument x: Clenshaw on the per-interval Chebyshev
-// series (degree 12 in double, degree 6 in float), Gauss-Hermite limit for large x
+// 1/sqrt(s) for s in the normal range: the hardware estimate refined by two Newton steps
+// in double (the estimate is good to about 2^-23 relative, each step squares the error);
+// single precision uses the estimate as it is (1 ulp)
+template <typename T>
+__device__ __forceinline__ T rsqrt_nr(T s) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_amdgcn_rsqf(s);
+    } else {
+        const T h = (T)0.5 * s;
+        T y = __builtin_amdgcn_rsq(s);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const T r = fma(-(h * y), y, (T)0.5);
+            y = fma(y, r, y);
+        }
+        return y;
+    }
+}
+
+constexpr int kRysAsym[6] = {0, 40, 48, 56, 62, 68};
+// entries of the Chebyshev table of NR roots: intervals x 2 NR functions x coefficients
 template <int NR, typename T>
-__device__ __forceinline__ void rys_roots(T x, T* u, T* w) {
-    constexpr int kAsym[6] = {0, 40, 48, 56, 62, 68};
+__host__ __device__ constexpr int rys_table_size() {
+    return kRysAsym[NR] * 2 * NR * (sizeof(T) == 4 ? UNO_RYS_NCOEF_F : UNO_RYS_NCOEF);
+}
+template <int NR, typename T>
+__device__ __forceinline__ const T* rys_table_global() {
+    if constexpr (sizeof(T) == 4) return uno_rys_coef_f + uno_rys_off_f[NR];
+    else return uno_rys_coef + uno_rys_off[NR];
+}
+
+// Rys roots u = t^2 and weights for argument x: Clenshaw on the per-interval Chebyshev
+// series (degree 12 in double, degree 6 in float), Gauss-Hermite limit for large x.
+// tab is the table of NR roots (global memory or a copy in LDS).
+template <int NR, typename T>
+__device__ __forceinline__ void rys_roots(T x, T* u, T* w, const T* tab) {
     constexpr bool f32 = sizeof(T) == 4;
-    if (x >= (T)kAsym[NR]) {
-        const T ix = (T)1 / x, isx = sqrt(ix);
+    if (x >= (T)kRysAsym[NR]) {
+        const T isx = rsqrt_nr(x), ix = isx * isx;
         constexpr int o = NR * (NR - 1) / 2;
 #pragma unroll
         for (int i = 0; i < NR; ++i) {
@@ -75,9 +106,7 @@ __device__ __forceinline__ void rys_roots(T x, T* u, T* w) {
     constexpr int NC = f32 ? UNO_RYS_NCOEF_F : UNO_RYS_NCOEF;
     const int k = (int)x;
     const T s = (T)2 * (x - (T)k) - (T)1, s2 = (T)2 * s;
-    const T* c;
-    if constexpr (f32) c = uno_rys_coef_f + uno_rys_off_f[NR] + k * (2 * NR * NC);
-    else c = uno_rys_coef + uno_rys_off[NR] + k * (2 * NR * NC);
+    const T* c = tab + k * (2 * NR * NC);
 #pragma unroll
     for (int f = 0; f < 2 * NR; ++f) {
         const T* cf = c + f * NC;
@@ -91,6 +120,10 @@ __device__ __forceinline__ void rys_roots(T x, T* u, T* w) {
         const T v = fma(s, b1, cf[0] - b2);
         if (f < NR) u[f] = v; else w[f - NR] = v;
     }
+}
+template <int NR, typename T>
+__device__ __forceinline__ void rys_roots(T x, T* u, T* w) {
+    rys_roots<NR, T>(x, u, w, rys_table_global<NR, T>());
 }
 
 // One axis of one root: Rys 2-D recurrence G[e][f] (e <= LA+LB, f <= LC+LD) then the

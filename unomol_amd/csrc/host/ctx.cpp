@@ -351,8 +351,8 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
     std::vector<hipEvent_t> ev0, ev1;
     std::vector<int> evcls;
     if (c->profile) {
-        UNO_HIP(c->d_counter.alloc(2 * kProfSlots));
-        UNO_HIP(hipMemsetAsync(c->d_counter.p, 0, 2 * kProfSlots * sizeof(unsigned long long), st));
+        UNO_HIP(c->d_counter.alloc(2 * kCounterSlots * kProfSlots));
+        UNO_HIP(hipMemsetAsync(c->d_counter.p, 0, 2 * kCounterSlots * kProfSlots * sizeof(unsigned long long), st));
     }
     for (QuartetClass& q : c->qcls) {
         build_prefix(c, q, 4.0 * dmax, mixed ? c->fp32_thr : 0.0);
@@ -398,7 +398,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
             }
             const int ci = prec * kNumPairClasses * kNumPairClasses + LX.lcls * kNumPairClasses + LY.lcls;
             if (c->profile) {
-                a.counter = c->d_counter.p + 2 * ci;
+                a.counter = c->d_counter.p + 2 * kCounterSlots * ci;
                 hipEvent_t e0, e1;
                 UNO_HIP(hipEventCreate(&e0));
                 UNO_HIP(hipEventCreate(&e1));
@@ -416,7 +416,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
     c->last_quartets32 = total32;
     if (c->profile) {
         UNO_HIP(hipStreamSynchronize(st));
-        std::vector<unsigned long long> cnt(2 * kProfSlots);
+        std::vector<unsigned long long> cnt(2 * kCounterSlots * kProfSlots);
         UNO_HIP(hipMemcpy(cnt.data(), c->d_counter.p, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
         for (size_t k = 0; k < ev0.size(); ++k) {
             float ms = 0.f;
@@ -426,10 +426,11 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
             (void)hipEventDestroy(ev0[k]);
             (void)hipEventDestroy(ev1[k]);
         }
-        for (int ci = 0; ci < kProfSlots; ++ci) {
-            c->prof_prim[ci] += (double)cnt[2 * ci];
-            c->prof_shell[ci] += (double)cnt[2 * ci + 1];
-        }
+        for (int ci = 0; ci < kProfSlots; ++ci)
+            for (int sl = 0; sl < kCounterSlots; ++sl) {
+                c->prof_prim[ci] += (double)cnt[2 * (kCounterSlots * ci + sl)];
+                c->prof_shell[ci] += (double)cnt[2 * (kCounterSlots * ci + sl) + 1];
+            }
     }
     return UNO_OK;
 }

@@ -1,0 +1,93 @@
+"""Short text summaries of rocprofv3 CSV output (``--output-format csv``) for the ERI kernels.
+
+    python -m unomol_amd.profsum trace <dir>/run_kernel_stats.csv <dir>/run_kernel_trace.csv \
+        --kernel "<demangled name>" --first L
+        per-kernel table of the whole run, then the named kernel's average duration over
+        its first L dispatches (with ``bench.py --warmup 0 --steps 1`` those are the timed
+        step's launches) and over all of its dispatches;
+
+    python -m unomol_amd.profsum pmc <dir>/run_counter_collection.csv --counter FETCH_SIZE \
+        --kernel "<demangled name>" --first L --copy-bytes 1073741824
+        the counter (KiB per dispatch) averaged over the kernel's first L dispatches, the
+        same counter read on the streaming-copy kernel ``k_copy_probe`` (which moves
+        ``--copy-bytes`` each way), and the kernel's bytes per dispatch scaled by
+        (copy bytes / copy reading).
+
+Both print Markdown; ``pmc`` ends with one JSON line holding the numbers.
+"""
+import argparse
+import csv
+import json
+import sys
+
+COPY_KERNEL = "k_copy_probe"
+
+
+def _dispatches(rows, name_key, kernel):
+    out = [r for r in rows if r[name_key] == kernel]
+    out.sort(key=lambda r: int(r.get("Dispatch_Id") or r.get("Correlation_Id") or 0))
+    return out
+
+
+def trace(stats_csv, trace_csv, kernel, first, top=25):
+    stats = list(csv.DictReader(open(stats_csv)))
+    stats.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    print("| kernel | calls | total ms | avg us | % |")
+    print("|---|---|---|---|---|")
+    for r in stats[:top]:
+        print("| `%s` | %s | %.2f | %.1f | %.2f |" % (r["Name"][:100], r["Calls"], float(r["TotalDurationNs"]) / 1e6,
+                                                   float(r["AverageNs"]) / 1e3, float(r["Percentage"])))
+    if not kernel:
+        return
+    rows = _dispatches(list(csv.DictReader(open(trace_csv))), "Kernel_Name", kernel)
+    dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6 for r in rows]
+    print()
+    print("kernel `%s`: %d dispatches in the run" % (kernel, len(dur)))
+    if dur:
+        n = min(first or len(dur), len(dur))
+        print("- first %d dispatches: average %.4f ms, total %.3f ms" % (n, sum(dur[:n]) / n, sum(dur[:n])))
+        print("- all dispatches: average %.4f ms, total %.3f ms" % (sum(dur) / len(dur), sum(dur)))
+
+
+def pmc(counter_csv, counter, kernel, first, copy_bytes):
+    rows = [r for r in csv.DictReader(open(counter_csv)) if r["Counter_Name"] == counter]
+    kr = _dispatches(rows, "Kernel_Name", kernel)
+    cr = [r for r in rows if COPY_KERNEL in r["Kernel_Name"]]
+    if not kr or not cr:
+        sys.exit("no %s dispatches of %s or of the copy kernel in %s" % (counter, kernel, counter_csv))
+    n = min(first or len(kr), len(kr))
+    kib = sum(float(r["Counter_Value"]) for r in kr[:n]) / n
+    kib_all = sum(float(r["Counter_Value"]) for r in kr) / len(kr)
+    copy_kib = sum(float(r["Counter_Value"]) for r in cr) / len(cr)
+    scale = copy_bytes / (copy_kib * 1024.0)
+    print("counter %s, kernel `%s`" % (counter, kernel))
+    print("- first %d dispatches: %.1f KiB per dispatch (all %d: %.1f)" % (n, kib, len(kr), kib_all))
+    print("- copy kernel (%d bytes each way): %.1f KiB per dispatch -> scale %.4f" % (copy_bytes, copy_kib, scale))
+    print("- kernel bytes per dispatch, scaled: %.4e" % (kib * 1024.0 * scale))
+    print(json.dumps({"counter": counter, "kernel": kernel, "dispatches": n, "kib_per_dispatch": kib,
+                      "copy_kib": copy_kib, "scale": scale, "bytes_per_dispatch": kib * 1024.0 * scale}))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("trace")
+    t.add_argument("stats_csv")
+    t.add_argument("trace_csv")
+    t.add_argument("--kernel", default="")
+    t.add_argument("--first", type=int, default=0)
+    p = sub.add_parser("pmc")
+    p.add_argument("counter_csv")
+    p.add_argument("--counter", required=True)
+    p.add_argument("--kernel", required=True)
+    p.add_argument("--first", type=int, default=0)
+    p.add_argument("--copy-bytes", type=float, default=float(1 << 30))
+    a = ap.parse_args(argv)
+    if a.cmd == "trace":
+        trace(a.stats_csv, a.trace_csv, a.kernel, a.first)
+    else:
+        pmc(a.counter_csv, a.counter, a.kernel, a.first, a.copy_bytes)
+
+
+if __name__ == "__main__":
+    main()
