@@ -21,8 +21,9 @@ trained first on small clusters (outside the timed region, reported).
 Prints ONE JSON line with the metric, a roofline object for the dominant ERI kernel of
 the timed steps (the class/precision kernel with the largest summed time; per-launch
 time from HIP events recorded on the engine's stream around each launch inside the timed
-region; algorithmic FLOPs = the per-primitive-quartet operation count of the kernel's
-algorithm times the primitive quartets the kernel evaluated, counted by the kernel; HBM
+region; algorithmic FLOPs = unomol_amd.flopmodel's per-primitive-quartet and
+per-shell-quartet operation counts of the kernel's own arithmetic times the quartets the
+kernel evaluated, counted by the kernel; HBM
 traffic per launch from the committed FETCH_SIZE/WRITE_SIZE passes when they cover the
 same kernel) and a cpu_baseline object (the reference's Rys ERI code compiled from its
 sources into oracle/_ref, timed on a random sample of the same workload's shell quartets
@@ -55,28 +56,6 @@ def log(msg):
 
 def ncart(l):
     return (l + 1) * (l + 2) // 2
-
-
-def flops_per_prim_quartet(la, lb, lc, ld):
-    """Operation count of the kernel's algorithm for one primitive quartet (an FMA counts
-    as 2): prefactors/geometry, Chebyshev-Clenshaw roots and weights, per root and axis the
-    Rys recurrence and both binomial transfers, and the component assembly."""
-    E, F = la + lb, lc + ld
-    nr = (E + F) // 2 + 1
-    ncomp = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
-    setup = 32
-    roots = 2 * nr * (12 * 3 + 3) + 6
-    vrr = 3 * max(E - 1, 0) + F * (E + 1) * 5
-    ket = sum(2 * (d + 1) for e in range(E + 1) for c in range(lc + 1) for d in range(ld + 1))
-    bra = sum(2 * (b + 1) for a in range(la + 1) for b in range(lb + 1) for c in range(lc + 1) for d in range(ld + 1))
-    per_axis = 6 + vrr + ket + bra
-    per_root = 8 + 3 * per_axis + 4 * ncomp
-    return setup + roots + nr * per_root
-
-
-def flops_per_shell_quartet_digest(la, lb, lc, ld):
-    ncomp = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
-    return 6 * 2 * ncomp + ncomp
 
 
 def cpu_baseline(deck_path, budget_s=15.0):
@@ -119,6 +98,10 @@ def profile_classes(N, t, P):
 
 
 def class_table(ms, launches, prim, shell):
+    """Per-class entries with the algorithmic FLOPs of unomol_amd.flopmodel (the kernel's
+    own arithmetic per primitive quartet and per shell quartet, constant-folded, dead code
+    removed) times the quartets the kernel counted."""
+    from unomol_amd import flopmodel
     classes = []
     for prec in range(2):
         for X in range(6):
@@ -128,12 +111,12 @@ def class_table(ms, launches, prim, shell):
                     continue
                 la, lb = PAIR_L[X]
                 lc, ld = PAIR_L[Y]
-                fl = prim[ci] * flops_per_prim_quartet(la, lb, lc, ld) + shell[ci] * flops_per_shell_quartet_digest(
-                    la, lb, lc, ld)
-                classes.append({"class": "(%s|%s)" % (PAIR_NAME[X], PAIR_NAME[Y]), "precision": "f32" if prec else "f64",
-                                "ms": ms[ci], "launches": launches[ci], "prim_quartets": prim[ci],
-                                "shell_quartets": shell[ci],
-                                "gflop": fl / 1e9, "tflops": fl / (ms[ci] * 1e-3) / 1e12 if ms[ci] > 0 else 0.0})
+                fl = flopmodel.class_flops(la, lb, lc, ld, "f32" if prec else "f64", prim[ci], shell[ci])
+                cls = {"class": "(%s|%s)" % (PAIR_NAME[X], PAIR_NAME[Y]), "precision": "f32" if prec else "f64"}
+                cls.update({"kernel": kernel_name(cls), "ms": ms[ci], "launches": launches[ci],
+                            "prim_quartets": prim[ci], "shell_quartets": shell[ci], "gflop": fl / 1e9,
+                            "tflops": fl / (ms[ci] * 1e-3) / 1e12 if ms[ci] > 0 else 0.0})
+                classes.append(cls)
     return classes
 
 

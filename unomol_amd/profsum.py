@@ -13,7 +13,13 @@
         ``--copy-bytes`` each way), and the kernel's bytes per dispatch scaled by
         (copy bytes / copy reading).
 
-Both print Markdown; ``pmc`` ends with one JSON line holding the numbers.
+    python -m unomol_amd.profsum flops <dir>/run_counter_collection.csv <bench.json>
+        per ERI kernel of the bench's timed step: executed VALU FLOPs from
+        SQ_INSTS_VALU_FLOPS_FP64 / _FP32 (plus the _TRANS counters) over the kernel's first
+        L dispatches (L = the step's launches of that class) against the bench's
+        algorithmic FLOPs for the same launches.
+
+All print Markdown; ``pmc`` ends with one JSON line holding the numbers.
 """
 import argparse
 import csv
@@ -68,6 +74,33 @@ def pmc(counter_csv, counter, kernel, first, copy_bytes):
                       "copy_kib": copy_kib, "scale": scale, "bytes_per_dispatch": kib * 1024.0 * scale}))
 
 
+def flops(counter_csv, bench_json):
+    bench = json.loads(open(bench_json).read().strip().splitlines()[-1])
+    by_kernel = {}
+    for r in csv.DictReader(open(counter_csv)):
+        by_kernel.setdefault(r["Kernel_Name"], []).append(r)
+    print("| kernel | class | launches | model GFLOP | counter GFLOP | counter/model |")
+    print("|---|---|---|---|---|---|")
+    tot_m = tot_c = 0.0
+    for c in sorted(bench["eri_classes"], key=lambda c: -c["gflop"]):
+        rows = by_kernel.get(c["kernel"], [])
+        f32 = c["precision"] == "f32"
+        names = ("SQ_INSTS_VALU_FLOPS_FP32", "SQ_INSTS_VALU_FLOPS_FP32_TRANS") if f32 else \
+                ("SQ_INSTS_VALU_FLOPS_FP64", "SQ_INSTS_VALU_FLOPS_FP64_TRANS")
+        disp = sorted({int(r["Dispatch_Id"]) for r in rows})[:int(round(c["launches"]))]
+        keep = set(disp)
+        cnt = sum(float(r["Counter_Value"]) for r in rows
+                  if int(r["Dispatch_Id"]) in keep and r["Counter_Name"] in names)
+        if not disp:
+            continue
+        tot_m += c["gflop"]
+        tot_c += cnt / 1e9
+        print("| `%s` | %s %s | %d | %.3f | %.3f | %.3f |" % (c["kernel"], c["class"], c["precision"], len(disp),
+                                                          c["gflop"], cnt / 1e9, cnt / 1e9 / max(c["gflop"], 1e-30)))
+    print()
+    print("all ERI kernels: model %.2f GFLOP, counters %.2f GFLOP, ratio %.3f" % (tot_m, tot_c, tot_c / max(tot_m, 1e-30)))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -82,9 +115,14 @@ def main(argv=None):
     p.add_argument("--kernel", required=True)
     p.add_argument("--first", type=int, default=0)
     p.add_argument("--copy-bytes", type=float, default=float(1 << 30))
+    f = sub.add_parser("flops")
+    f.add_argument("counter_csv")
+    f.add_argument("bench_json")
     a = ap.parse_args(argv)
     if a.cmd == "trace":
         trace(a.stats_csv, a.trace_csv, a.kernel, a.first)
+    elif a.cmd == "flops":
+        flops(a.counter_csv, a.bench_json)
     else:
         pmc(a.counter_csv, a.counter, a.kernel, a.first, a.copy_bytes)
 
