@@ -47,6 +47,8 @@ PAIR_NAME = ["ss", "ps", "pp", "ds", "dp", "dd"]
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (and matrix) dense peak
 FP32_PEAK_TFLOPS = 157.3
 LANE_MAX_COMP = 162       # classes up to this many components run one quartet per lane
+PROF_SLOTS = 288          # profiling slots: 72 kind + 36 precision + 6 X + Y
+ROW_KINDS = {1: "true, true", 2: "false, true", 3: "false, false"}   # k_eri_rows <TL, FB>
 MODEL = os.path.join(ROOT, "unomol_amd", "models", "guess_631gs.bin")
 
 
@@ -83,7 +85,7 @@ def cpu_baseline(deck_path, budget_s=15.0):
 def read_profile(N, t):
     """Per-class kernel time (HIP events on the engine's stream), launches and evaluated
     primitive / shell quartets accumulated since profiling was switched on."""
-    ms = np.zeros(72); launches = np.zeros(72); prim = np.zeros(72); shell = np.zeros(72)
+    ms, launches, prim, shell = (np.zeros(PROF_SLOTS) for _ in range(4))
     N.check(N.lib().uno_profile(t.ctx, -1, N.ptr(ms), N.ptr(launches), N.ptr(prim), N.ptr(shell)))
     N.check(N.lib().uno_profile(t.ctx, 0, None, None, None, None))
     return ms, launches, prim, shell
@@ -103,31 +105,39 @@ def class_table(ms, launches, prim, shell):
     removed) times the quartets the kernel counted."""
     from unomol_amd import flopmodel
     classes = []
-    for prec in range(2):
-        for X in range(6):
-            for Y in range(X + 1):
-                ci = 36 * prec + 6 * X + Y
-                if launches[ci] == 0:
-                    continue
-                la, lb = PAIR_L[X]
-                lc, ld = PAIR_L[Y]
-                fl = flopmodel.class_flops(la, lb, lc, ld, "f32" if prec else "f64", prim[ci], shell[ci])
-                cls = {"class": "(%s|%s)" % (PAIR_NAME[X], PAIR_NAME[Y]), "precision": "f32" if prec else "f64"}
-                cls.update({"kernel": kernel_name(cls), "ms": ms[ci], "launches": launches[ci],
-                            "prim_quartets": prim[ci], "shell_quartets": shell[ci], "gflop": fl / 1e9,
-                            "tflops": fl / (ms[ci] * 1e-3) / 1e12 if ms[ci] > 0 else 0.0})
-                classes.append(cls)
+    for kind in range(4):
+        for prec in range(2):
+            for X in range(6):
+                for Y in range(X + 1):
+                    ci = 72 * kind + 36 * prec + 6 * X + Y
+                    if launches[ci] == 0:
+                        continue
+                    la, lb = PAIR_L[X]
+                    lc, ld = PAIR_L[Y]
+                    fl = flopmodel.class_flops(la, lb, lc, ld, "f32" if prec else "f64", prim[ci], shell[ci])
+                    cls = {"class": "(%s|%s)" % (PAIR_NAME[X], PAIR_NAME[Y]), "precision": "f32" if prec else "f64",
+                           "kind": kind}
+                    cls.update({"kernel": kernel_name(cls), "ms": ms[ci], "launches": launches[ci],
+                                "prim_quartets": prim[ci], "shell_quartets": shell[ci], "gflop": fl / 1e9,
+                                "tflops": fl / (ms[ci] * 1e-3) / 1e12 if ms[ci] > 0 else 0.0})
+                    classes.append(cls)
     return classes
 
 
 def kernel_name(cls):
-    """Demangled name of the Fock-build kernel of a class entry (as rocprofv3 prints it)."""
+    """Demangled name of the Fock-build kernel of a class entry (as rocprofv3 prints it):
+    kind 0 the wave-task (lane classes) or per-quartet wave kernel, kinds 1-3 the
+    row-merged kernel variants."""
     X, Y = PAIR_NAME.index(cls["class"][1:3]), PAIR_NAME.index(cls["class"][4:6])
     la, lb = PAIR_L[X]
     lc, ld = PAIR_L[Y]
+    t = "float" if cls["precision"] == "f32" else "double"
+    if cls.get("kind", 0):
+        return "void uno::k_eri_rows<%d, %d, %d, %d, 0, %s, %s>(uno::EriArgs)" % (la, lb, lc, ld, t,
+                                                                                  ROW_KINDS[cls["kind"]])
     lane = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) <= LANE_MAX_COMP
-    return "void uno::%s<%d, %d, %d, %d, 0, %s>(uno::EriArgs)" % (
-        "k_eri_bra" if lane else "k_eri_wave", la, lb, lc, ld, "float" if cls["precision"] == "f32" else "double")
+    return "void uno::%s<%d, %d, %d, %d, 0, %s>(uno::EriArgs)" % ("k_eri_bra" if lane else "k_eri_wave", la, lb, lc,
+                                                                  ld, t)
 
 
 def traffic_of(kernel):
@@ -241,7 +251,7 @@ def main():
     # timed region: every step's Fock-build kernels carry per-class HIP events on the
     # engine's stream (and slot-spread evaluated-quartet counters)
     results = []
-    acc = [np.zeros(72) for _ in range(4)]
+    acc = [np.zeros(PROF_SLOTS) for _ in range(4)]
     t_all = time.perf_counter()
     for _ in range(args.steps):
         t, hf, e, dt = one_scf(step_opts, profile=not args.no_profile)

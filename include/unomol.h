@@ -67,7 +67,13 @@ int uno_ctx_destroy(uno_ctx* ctx);
  * builds of uno_fock_rhf/uno_fock_uhf; uno_scf_run takes its own option),
  * "prim_cut" (bare primitive prefactor cut, default 1e-12 as TwoElectronInts.cpp:428),
  * "skip_center" (a centre left out of the nuclear attraction, as the reference leaves out
- * the DPM positron, OneElectronInts.cpp:43; default -1, none). */
+ * the DPM positron, OneElectronInts.cpp:43; default -1, none), "row_merge" (1: bras with
+ * many kets run as blocks that merge their exchange contributions in LDS; 0: every bra
+ * as per-wave tasks; default 1), "row_min_kets" (the ket count from which a bra is
+ * row-merged; default 192), "row_variant" (LDS layout of the row-merged blocks: 1 every
+ * bra row and the Rys table, 2 every bra row, 3 the first bra shell's rows; 0, the
+ * default, takes the first that fits the LDS budget; a forced one that does not fit
+ * falls back to that choice). */
 int uno_ctx_set(uno_ctx* ctx, const char* key, double value);
 /* Reads the knobs above and, read-only: "last_quartets" (shell quartets evaluated by the
  * last Fock build), "last_quartets32" (of which in FP32), "pairs" (significant shell pairs), "pair_runs" (runs of the pair
@@ -148,8 +154,11 @@ int uno_schwarz(uno_ctx* ctx, double* q);
  * profiling mode in which every Fock build records, per quartet class (bra pair class
  * X, ket pair class Y, index 6*X+Y; pair classes ss ps pp ds dp dd), the kernel time
  * (HIP events, ms), launches, and the primitive and shell quartets evaluated.
- * enable=0 turns it off; stats are read with enable=-1.  Arrays hold 72 entries: the
- * 36 FP64 class pairs, then the same 36 for the FP32 launches of mixed builds. */
+ * enable=0 turns it off; stats are read with enable=-1.  Arrays hold 288 entries,
+ * index 72*kind + 36*prec + 6*X + Y: kind 0 the per-wave-task (or per-quartet) kernels,
+ * kinds 1-3 the row-merged kernels (1: every bra row and the Rys table in LDS, 2: every
+ * bra row, 3: the first bra shell's rows); prec 0 FP64, 1 the FP32 launches of mixed
+ * builds. */
 int uno_profile(uno_ctx* ctx, int enable, double* ms, double* launches, double* prim_quartets,
                 double* shell_quartets);
 

@@ -263,6 +263,84 @@ def test_fock_fp32_quartets_vs_fp64():
     assert np.abs(Gmix - G64).max() < 1e-9
 
 
+@pytest.mark.parametrize("tag", ["631.h2o", "w2_631gs"])
+def test_fock_rows_rhf_vs_oracle(tag):
+    """Every bra through the row-merged kernel (exchange merged in LDS rows, then added
+    row-contiguously) against the oracle's brute-force G."""
+    say("fock rhf, all bras row-merged, vs oracle " + tag)
+    path = R.deck_path(tag)
+    deck = read_deck(path)
+    n = deck["norb"]
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((n, n)) * 0.1
+    P = _pack(A + A.T)
+    b, t = _tints(path, schwarz=1e-16)
+    t.set("row_min_kets", 1)
+    G = np.zeros_like(P)
+    t.formGmatrix(P, G)
+    Gref = _oracle_g(deck, P)
+    assert np.abs(G - Gref).max() < 1e-11
+    t.formGmatrix(P, G)
+    assert np.abs(G - 2 * Gref).max() < 2e-11
+
+
+def test_fock_rows_uhf_vs_oracle():
+    say("fock uhf, all bras row-merged, vs oracle water_631gs_cation")
+    path = R.deck_path("water_631gs_cation")
+    deck = read_deck(path)
+    n = deck["norb"]
+    rng = np.random.default_rng(12)
+    A = rng.standard_normal((n, n)) * 0.1
+    B = rng.standard_normal((n, n)) * 0.1
+    PA, PB = _pack(A + A.T), _pack(B + B.T)
+    b, t = _tints(path, schwarz=1e-16)
+    t.set("row_min_kets", 1)
+    GA, GB = np.zeros_like(PA), np.zeros_like(PB)
+    t.formGmatrix(PA, PB, GA, GB)
+    RA, RB = _oracle_g(deck, PA, PB)
+    assert np.abs(GA - RA).max() < 1e-11
+    assert np.abs(GB - RB).max() < 1e-11
+
+
+@pytest.mark.parametrize("uhf", [False, True])
+def test_fock_rows_vs_waves_config_b(uhf):
+    """16-water 6-31G* (304 functions): the default split (bras with >= 192 kets
+    row-merged) against every bra as wave tasks, FP64 and all-FP32 quartets."""
+    say("fock rows vs waves, config B deck, uhf=%s" % uhf)
+    path = os.path.join(R.GOLDEN, "decks", "B_w16_631gs.dat")
+    deck = read_deck(path)
+    n = deck["norb"]
+    rng = np.random.default_rng(13)
+    A = rng.standard_normal((n, n)) * 0.02
+    Bm = rng.standard_normal((n, n)) * 0.02
+    PA, PB = _pack(A + A.T + np.eye(n)), _pack(Bm + Bm.T + np.eye(n))
+    b, t = _tints(path)
+
+    def build():
+        if uhf:
+            GA, GB = np.zeros_like(PA), np.zeros_like(PB)
+            t.formGmatrix(PA, PB, GA, GB)
+            return np.concatenate([GA, GB])
+        G = np.zeros_like(PA)
+        t.formGmatrix(PA, G)
+        return G
+
+    for prec, thr, tol in ((0, 1e-6, 1e-11), (1, 1e30, 1e-5)):
+        t.set("precision", prec)
+        t.set("fp32_threshold", thr)
+        t.set("row_merge", 0)
+        Gwave = build()
+        scale = np.abs(Gwave).max()
+        t.set("row_merge", 1)
+        # LDS variants: rows + Rys table, every bra row, the first bra shell's rows
+        for variant in (1, 2, 3):
+            t.set("row_variant", variant)
+            Grow = build()
+            assert np.abs(Grow - Gwave).max() < tol * max(scale, 1.0), (prec, variant, np.abs(Grow - Gwave).max(),
+                                                                         scale)
+        t.set("row_variant", 0)
+
+
 def test_scf_incremental_same_energy():
     say("scf incremental builds w2_631gs")
     path = R.deck_path("w2_631gs")

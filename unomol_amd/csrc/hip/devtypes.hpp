@@ -69,6 +69,14 @@ struct EriArgs {
     const long long* wpre;
     const int* tblk;
     long long ntask;
+    // row-merged launches (bras with many kets): block task b belongs to the r-th listed
+    // bra, rpre[r] <= b < rpre[r+1]; rbra[r] is its entry; a block covers rseg kets of it
+    // and merges its exchange contributions to the bra's rows in LDS before adding them
+    const int* rbra;
+    const long long* rpre;
+    int nrbra;
+    long long nrtask;
+    int rseg;
     // batched launches: entry i (a bra with one ket range) names its bra pair, its first
     // ket pair and its molecule (null: the run offsets above and molecule view n/D/W)
     const int* e_bra;

@@ -133,6 +133,9 @@ __global__ __launch_bounds__(256) void k_eri_lane(const EriArgs a) {
 
 // Rys tables of up to this many roots are staged in LDS by the Fock-build kernels
 constexpr int kLdsTabMaxRoots = 3;
+// LDS a row-merged block may take (rows, then the Rys table if it still fits): two blocks
+// per compute unit
+constexpr size_t kRowLdsBudget = 80 * 1024;
 
 // ---------------------------------------------------------------------------
 // Fock builds: one wave per (bra pair, chunk of up to 64 kets).  The bra is the same in
@@ -317,6 +320,217 @@ __global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
                     for (int k = 0; k < NC; ++k) s = fma(e[((i * NB + j) * NC + k) * ND + l], dac[i * NC + k], s);
                 atomic_add_f64(W + (long)(fb + j) * n + fd + l, (double)(-s));
             }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fock builds, bras with many kets: one block per (bra, segment of rseg kets), 256 kets
+// per pass.  The exchange contributions all land in the bra's rows of W (rows fa.., fb..),
+// so the block merges them in LDS (one row of n doubles per bra function and spin) and
+// adds the non-zero entries at the end with row-contiguous atomics; the bra's Coulomb
+// outputs are reduced over the block.  Only the ket's Coulomb outputs are added per
+// quartet.  Dynamic LDS: (NA + NB) x spins x n doubles, or with FB false only the NA rows
+// of the first bra shell (the second shell's rows then take their atomics directly); TL:
+// the Rys table is copied to LDS as well (otherwise read from global memory).
+// ---------------------------------------------------------------------------
+template <int LA, int LB, int MODE, bool FB>
+__host__ __device__ constexpr int row_count() {
+    return (ncart_c(LA) + (FB ? ncart_c(LB) : 0)) * (MODE == kDigestUHF ? 2 : 1);
+}
+
+template <int LA, int LB, int LC, int LD, int MODE, typename T, bool TL, bool FB>
+__global__ __launch_bounds__(256) void k_eri_rows(const EriArgs a) {
+    using CI = ClassInfo<LA, LB, LC, LD>;
+    constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND, NCOMP = CI::NCOMP, NR = CI::NR;
+    constexpr int NAB = NA + (FB ? NB : 0), NROW = row_count<LA, LB, MODE, FB>();
+    constexpr int NSP = MODE == kDigestUHF ? 2 : 1;
+    extern __shared__ double rows[];
+    __shared__ T tab_lds[TL ? rys_table_size<NR, T>() : 1];
+    __shared__ double jred[4][NA * NB];
+    __shared__ unsigned long long cred[4][2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the block's bra (uniform): largest r with rpre[r] <= blockIdx.x
+    const long long b = blockIdx.x;
+    int lo = 0, hi = a.nrbra - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.rpre[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    const int entry = __builtin_amdgcn_readfirstlane(a.rbra[lo]);
+    const long long seg = b - a.rpre[lo];
+    const long long cnt = a.prefix[entry + 1] - a.prefix[entry];
+    const long long k0 = seg * a.rseg, k1 = min(cnt, k0 + (long long)a.rseg);
+    const MolView mv = mol_view(a, entry);
+    const int n = mv.n;
+    for (int i = threadIdx.x; i < NROW * n; i += blockDim.x) rows[i] = 0.0;
+    const T* tab = rys_table_global<NR, T>();
+    if constexpr (TL) {
+        for (int i = threadIdx.x; i < rys_table_size<NR, T>(); i += blockDim.x) tab_lds[i] = tab[i];
+        tab = tab_lds;
+    }
+    __syncthreads();
+    const int pab = __builtin_amdgcn_readfirstlane(entry_bra(a, entry));
+    const PairHdr P1 = a.pairs[pab];
+    const int ket0 = entry_ket(a, entry);
+    const int fa = P1.f1, fb = P1.f2;
+    constexpr T jf = (MODE == kDigestRHF) ? 4 : 2;
+    double jacc[NA * NB];
+#pragma unroll
+    for (int ij = 0; ij < NA * NB; ++ij) jacc[ij] = 0.0;
+    T dab[NA * NB];
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) dab[i * NB + j] = (T)mv.D0[(long)(fa + i) * n + fb + j];
+    unsigned long long nprim = 0, nshell = 0;
+    for (long long base = k0; base < k1; base += blockDim.x) {
+        const long long jk = base + threadIdx.x;
+        bool active = jk < k1;
+        PairHdr P2;
+        const int pcd = ket0 + (int)jk;
+        if (active) {
+            P2 = a.pairs[pcd];
+            active = !screened(a, mv, P1, P2, a.pairQ[pab] * a.pairQ[pcd]);
+        }
+        if (!active) continue;
+        T e[NCOMP];
+        nprim += eri_block<LA, LB, LC, LD, T>(a, P1, P2, e, tab);
+        ++nshell;
+        const T f = (P1.same ? (T)0.5 : (T)1) * (P2.same ? (T)0.5 : (T)1) * (pab == pcd ? (T)0.5 : (T)1);
+        normalise_block<LA, LB, LC, LD, T>(e, f);
+        const int fc = P2.f1, fd = P2.f2;
+        // Coulomb, bra outputs: kept per thread
+        {
+            T dcd[NC * ND];
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+#pragma unroll
+                for (int l = 0; l < ND; ++l) dcd[k * ND + l] = (T)mv.D0[(long)(fc + k) * n + fd + l];
+#pragma unroll
+            for (int ij = 0; ij < NA * NB; ++ij) {
+                T s = 0;
+#pragma unroll
+                for (int kl = 0; kl < NC * ND; ++kl) s = fma(e[ij * NC * ND + kl], dcd[kl], s);
+                jacc[ij] += (double)(jf * s);
+            }
+        }
+        // Coulomb, ket outputs: one atomic each
+#pragma unroll
+        for (int kl = 0; kl < NC * ND; ++kl) {
+            T s = 0;
+#pragma unroll
+            for (int ij = 0; ij < NA * NB; ++ij) s = fma(e[ij * NC * ND + kl], dab[ij], s);
+            const int k = kl / ND, l = kl % ND;
+            atomic_add_f64(mv.W0 + (long)(fc + k) * n + fd + l, (double)(jf * s));
+            if (MODE == kDigestUHF) atomic_add_f64(mv.W1 + (long)(fc + k) * n + fd + l, (double)(jf * s));
+        }
+        // exchange into the LDS rows: row i (a function i), row NA + j (b function j, FB)
+#pragma unroll
+        for (int sp = 0; sp < NSP; ++sp) {
+            const double* DK = (MODE == kDigestUHF) ? (sp == 0 ? mv.D1 : mv.D2) : mv.D0;
+            double* R = rows + (long)sp * NAB * n;
+            double* WB = (MODE == kDigestUHF && sp) ? mv.W1 : mv.W0;
+            T dbd[NB * ND], dbc[NB * NC], dad[NA * ND], dac[NA * NC];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+#pragma unroll
+                for (int l = 0; l < ND; ++l) dbd[j * ND + l] = (T)DK[(long)(fb + j) * n + fd + l];
+#pragma unroll
+                for (int k = 0; k < NC; ++k) dbc[j * NC + k] = (T)DK[(long)(fb + j) * n + fc + k];
+            }
+#pragma unroll
+            for (int i = 0; i < NA; ++i) {
+#pragma unroll
+                for (int l = 0; l < ND; ++l) dad[i * ND + l] = (T)DK[(long)(fa + i) * n + fd + l];
+#pragma unroll
+                for (int k = 0; k < NC; ++k) dac[i * NC + k] = (T)DK[(long)(fa + i) * n + fc + k];
+            }
+#pragma unroll
+            for (int i = 0; i < NA; ++i) {
+#pragma unroll
+                for (int k = 0; k < NC; ++k) {
+                    T s = 0;
+#pragma unroll
+                    for (int j = 0; j < NB; ++j)
+#pragma unroll
+                        for (int l = 0; l < ND; ++l) s = fma(e[((i * NB + j) * NC + k) * ND + l], dbd[j * ND + l], s);
+                    atomicAdd(R + (long)i * n + fc + k, (double)(-s));
+                }
+#pragma unroll
+                for (int l = 0; l < ND; ++l) {
+                    T s = 0;
+#pragma unroll
+                    for (int j = 0; j < NB; ++j)
+#pragma unroll
+                        for (int k = 0; k < NC; ++k) s = fma(e[((i * NB + j) * NC + k) * ND + l], dbc[j * NC + k], s);
+                    atomicAdd(R + (long)i * n + fd + l, (double)(-s));
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+#pragma unroll
+                for (int k = 0; k < NC; ++k) {
+                    T s = 0;
+#pragma unroll
+                    for (int i = 0; i < NA; ++i)
+#pragma unroll
+                        for (int l = 0; l < ND; ++l) s = fma(e[((i * NB + j) * NC + k) * ND + l], dad[i * ND + l], s);
+                    if constexpr (FB) atomicAdd(R + (long)(NA + j) * n + fc + k, (double)(-s));
+                    else atomic_add_f64(WB + (long)(fb + j) * n + fc + k, (double)(-s));
+                }
+#pragma unroll
+                for (int l = 0; l < ND; ++l) {
+                    T s = 0;
+#pragma unroll
+                    for (int i = 0; i < NA; ++i)
+#pragma unroll
+                        for (int k = 0; k < NC; ++k) s = fma(e[((i * NB + j) * NC + k) * ND + l], dac[i * NC + k], s);
+                    if constexpr (FB) atomicAdd(R + (long)(NA + j) * n + fd + l, (double)(-s));
+                    else atomic_add_f64(WB + (long)(fb + j) * n + fd + l, (double)(-s));
+                }
+            }
+        }
+    }
+    // bra Coulomb outputs: wave sums, then the four waves
+#pragma unroll
+    for (int ij = 0; ij < NA * NB; ++ij) {
+        const double v = wave_sum(jacc[ij]);
+        if (lane == 0) jred[wave][ij] = v;
+    }
+    if (a.counter) {
+        int np = (int)nprim, ns = (int)nshell;
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            np += __shfl_xor(np, m, 64);
+            ns += __shfl_xor(ns, m, 64);
+        }
+        if (lane == 0) {
+            cred[wave][0] = (unsigned long long)np;
+            cred[wave][1] = (unsigned long long)ns;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < NA * NB) {
+        const int i = threadIdx.x / NB, j = threadIdx.x % NB;
+        const double v = jred[0][threadIdx.x] + jred[1][threadIdx.x] + jred[2][threadIdx.x] + jred[3][threadIdx.x];
+        atomic_add_f64(mv.W0 + (long)(fa + i) * n + fb + j, v);
+        if (MODE == kDigestUHF) atomic_add_f64(mv.W1 + (long)(fa + i) * n + fb + j, v);
+    }
+    if (a.counter && threadIdx.x == 0) {
+        unsigned long long* slot = a.counter + 2 * (blockIdx.x & (kCounterSlots - 1));
+        atomicAdd(slot, cred[0][0] + cred[1][0] + cred[2][0] + cred[3][0]);
+        atomicAdd(slot + 1, cred[0][1] + cred[1][1] + cred[2][1] + cred[3][1]);
+    }
+    // merged exchange rows: consecutive threads take consecutive columns of a row
+#pragma unroll
+    for (int r = 0; r < NROW; ++r) {
+        const int sp = r / NAB, rr = r % NAB;
+        const int grow = rr < NA ? fa + rr : fb + rr - NA;
+        double* W = (MODE == kDigestUHF && sp) ? mv.W1 : mv.W0;
+        for (int col = threadIdx.x; col < n; col += blockDim.x) {
+            const double v = rows[(long)r * n + col];
+            if (v != 0.0) atomic_add_f64(W + (long)grow * n + col, v);
+        }
     }
 }
 
@@ -587,6 +801,79 @@ static hipError_t launch_cls(const EriArgs& a, hipStream_t st) {
         hipLaunchKernelGGL((k_eri_wave<LA, LB, LC, LD, MODE, T>), dim3((unsigned)nb), dim3(64), 0, st, b);
     }
     return hipGetLastError();
+}
+
+// row-merged Fock-build launch of a lane class: one block per row task
+template <int LA, int LB, int LC, int LD, int MODE, typename T, bool TL, bool FB>
+static hipError_t launch_rows_v(const EriArgs& a, hipStream_t st) {
+    const size_t dyn = (size_t)row_count<LA, LB, MODE, FB>() * a.n * sizeof(double);
+    static size_t attr = 0;   // dynamic LDS limit set for this instantiation
+    if (dyn > attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eri_rows<LA, LB, LC, LD, MODE, T, TL, FB>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+        if (e != hipSuccess) return e;
+        attr = dyn;
+    }
+    hipLaunchKernelGGL((k_eri_rows<LA, LB, LC, LD, MODE, T, TL, FB>), dim3((unsigned)a.nrtask), dim3(256), dyn, st, a);
+    return hipGetLastError();
+}
+
+// variant chosen by the host (eri_rows_variant): 1 all bra rows + table in LDS, 2 all bra
+// rows, 3 the first shell's rows only
+template <int LA, int LB, int LC, int LD, int MODE, typename T>
+static hipError_t launch_rows(const EriArgs& a, int variant, hipStream_t st) {
+    using CI = ClassInfo<LA, LB, LC, LD>;
+    if constexpr (CI::NCOMP <= kLaneMaxComp && MODE <= kDigestUHF) {
+        if (a.nrtask <= 0 || !a.rbra || !a.rpre || a.rseg <= 0) return hipErrorInvalidValue;
+        switch (variant) {
+            case 1: return launch_rows_v<LA, LB, LC, LD, MODE, T, true, true>(a, st);
+            case 2: return launch_rows_v<LA, LB, LC, LD, MODE, T, false, true>(a, st);
+            case 3: return launch_rows_v<LA, LB, LC, LD, MODE, T, false, false>(a, st);
+            default: return hipErrorInvalidValue;
+        }
+    } else {
+        return hipErrorInvalidValue;
+    }
+}
+
+template <int MODE, typename T>
+static hipError_t launch_mode_rows(int cb, int ck, const EriArgs& a, int variant, hipStream_t st) {
+#define UNO_Q(X, Y, LA, LB, LC, LD) \
+    if (cb == X && ck == Y) return launch_rows<LA, LB, LC, LD, MODE, T>(a, variant, st);
+    UNO_Q(0, 0, 0, 0, 0, 0)
+    UNO_Q(1, 0, 1, 0, 0, 0) UNO_Q(1, 1, 1, 0, 1, 0)
+    UNO_Q(2, 0, 1, 1, 0, 0) UNO_Q(2, 1, 1, 1, 1, 0) UNO_Q(2, 2, 1, 1, 1, 1)
+    UNO_Q(3, 0, 2, 0, 0, 0) UNO_Q(3, 1, 2, 0, 1, 0) UNO_Q(3, 2, 2, 0, 1, 1) UNO_Q(3, 3, 2, 0, 2, 0)
+    UNO_Q(4, 0, 2, 1, 0, 0) UNO_Q(4, 1, 2, 1, 1, 0) UNO_Q(4, 2, 2, 1, 1, 1) UNO_Q(4, 3, 2, 1, 2, 0)
+    UNO_Q(5, 0, 2, 2, 0, 0) UNO_Q(5, 1, 2, 2, 1, 0)
+#undef UNO_Q
+    return hipErrorInvalidValue;
+}
+
+hipError_t eri_launch_rows(int mode, int cb, int ck, const EriArgs& a, int variant, hipStream_t st, bool fp32) {
+    if (mode == kDigestRHF) return fp32 ? launch_mode_rows<kDigestRHF, float>(cb, ck, a, variant, st)
+                                        : launch_mode_rows<kDigestRHF, double>(cb, ck, a, variant, st);
+    if (mode == kDigestUHF) return fp32 ? launch_mode_rows<kDigestUHF, float>(cb, ck, a, variant, st)
+                                        : launch_mode_rows<kDigestUHF, double>(cb, ck, a, variant, st);
+    return hipErrorInvalidValue;
+}
+
+int eri_rows_variant(int cb, int ck, int mode, int n, bool fp32, int force) {
+    if (!eri_uses_wave_tasks(cb, ck)) return 0;
+    static const int na[6] = {1, 3, 3, 6, 6, 6};          // functions of the first bra shell
+    static const int nb[6] = {1, 1, 3, 1, 3, 6};          // and of the second
+    static const int lsum[6] = {0, 1, 2, 2, 3, 4};
+    const int nr = (lsum[cb] + lsum[ck]) / 2 + 1;
+    const size_t tab = fp32 ? (size_t)kRysAsym[nr] * 2 * nr * UNO_RYS_NCOEF_F * sizeof(float)
+                            : (size_t)kRysAsym[nr] * 2 * nr * UNO_RYS_NCOEF * sizeof(double);
+    const size_t sp = mode == kDigestUHF ? 2 : 1, reserve = 1024;
+    const size_t all = (size_t)(na[cb] + nb[cb]) * sp * n * sizeof(double), first = (size_t)na[cb] * sp * n * sizeof(double);
+    const bool fits[4] = {true, all + tab + reserve <= kRowLdsBudget, all + reserve <= kRowLdsBudget,
+                          first + reserve <= kRowLdsBudget};
+    if (force >= 1 && force <= 3 && fits[force]) return force;
+    for (int v = 1; v <= 3; ++v)
+        if (fits[v]) return v;
+    return 0;
 }
 
 template <int MODE, typename T>

@@ -14,6 +14,14 @@ int eri_quartets_per_block(int bra_class, int ket_class);
 // Fock-build launches of this class run as wave tasks (bra, up to 64 kets): the space needs
 // wpre / tblk / ntask instead of the quartet block table
 bool eri_uses_wave_tasks(int bra_class, int ket_class);
+// row-merged Fock-build launch (bras with many kets) of a wave-task class: a.rbra / a.rpre /
+// a.nrtask / a.rseg describe the block tasks; variant from eri_rows_variant: 1 every bra
+// row and the Rys table in LDS, 2 every bra row, 3 the first bra shell's rows only, 0 none
+// fits the LDS budget at n functions (the class then runs as wave tasks only); force 1-3
+// picks that variant when it fits
+hipError_t eri_launch_rows(int mode, int bra_class, int ket_class, const EriArgs& a, int variant, hipStream_t st,
+                           bool fp32);
+int eri_rows_variant(int bra_class, int ket_class, int mode, int n, bool fp32, int force);
 hipError_t rys_eval_launch(int n, int npts, const double* x, double* u, double* w, hipStream_t st, bool fp32 = false);
 
 // dense helpers (linalg.hip); matrices are row-major n x n

@@ -284,6 +284,9 @@ int uno_ctx_set(uno_ctx* c, const char* key, double v) {
     else if (k == "skip_center") c->skip_center = (int)v;
     else if (k == "fp32_threshold") c->fp32_thr = v;
     else if (k == "precision") c->precision = (int)v;
+    else if (k == "row_merge") c->row_merge = (int)v;
+    else if (k == "row_min_kets") c->row_min_kets = (long long)v;
+    else if (k == "row_variant") c->row_variant = (int)v;
     else { set_error("unknown option " + k); return UNO_EINVAL; }
     return UNO_OK;
 }
@@ -296,6 +299,9 @@ int uno_ctx_get(uno_ctx* c, const char* key, double* v) {
     else if (k == "skip_center") *v = c->skip_center;
     else if (k == "fp32_threshold") *v = c->fp32_thr;
     else if (k == "precision") *v = c->precision;
+    else if (k == "row_merge") *v = c->row_merge;
+    else if (k == "row_min_kets") *v = (double)c->row_min_kets;
+    else if (k == "row_variant") *v = c->row_variant;
     else if (k == "last_quartets") *v = c->last_quartets;
     else if (k == "last_quartets32") *v = c->last_quartets32;
     else if (k == "pair_runs") *v = (double)c->cls.size();

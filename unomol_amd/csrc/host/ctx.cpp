@@ -64,10 +64,29 @@ void build_prefix(uno_ctx* c, QuartetClass& q, double dmax, double fp32_thr) {
 
 // wave tasks: bra i owns ceil(count_i / 64) consecutive tasks from wpre[i]; tblk[b] is
 // the bra of task 64 b, bracketing each wave's bra search
-void build_tasks(QuartetSpace& s) {
+// row-merged block tasks for the bras with at least min_kets kets (none if min_kets <= 0)
+void build_rows(QuartetSpace& s, long long min_kets, int seg) {
+    const int nb = (int)s.prefix.size() - 1;
+    s.rbra.clear();
+    s.rpre.assign(1, 0);
+    if (min_kets > 0)
+        for (int i = 0; i < nb; ++i) {
+            const long long cnt = s.prefix[i + 1] - s.prefix[i];
+            if (cnt < min_kets) continue;
+            s.rbra.push_back(i);
+            s.rpre.push_back(s.rpre.back() + (cnt + seg - 1) / seg);
+        }
+    s.nrtask = s.rpre.back();
+}
+
+// wave tasks (bra, chunk of 64 kets) for the bras not run as row tasks
+void build_tasks(QuartetSpace& s, long long row_min_kets) {
     const int nb = (int)s.prefix.size() - 1;
     s.wpre.assign(nb + 1, 0);
-    for (int i = 0; i < nb; ++i) s.wpre[i + 1] = s.wpre[i] + (s.prefix[i + 1] - s.prefix[i] + 63) / 64;
+    for (int i = 0; i < nb; ++i) {
+        const long long cnt = s.prefix[i + 1] - s.prefix[i];
+        s.wpre[i + 1] = s.wpre[i] + (row_min_kets > 0 && cnt >= row_min_kets ? 0 : (cnt + 63) / 64);
+    }
     s.ntask = s.wpre[nb];
     const long long nblk = (s.ntask + 63) / 64;
     s.tblk.resize(nblk + 1);
@@ -363,11 +382,20 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
             QuartetSpace& sp = prec ? q.s32 : q.s64;
             if (sp.nq == 0) continue;
             const bool tasks = eri_uses_wave_tasks(LX.lcls, LY.lcls);
+            const int mode = uhf ? kDigestUHF : kDigestRHF;
+            const int variant =
+                tasks && c->row_merge ? eri_rows_variant(LX.lcls, LY.lcls, mode, c->n, prec == 1, c->row_variant) : 0;
+            const bool rows = variant > 0;
             UNO_HIP(sp.d_prefix.upload(sp.prefix.data(), sp.prefix.size(), st));
             if (tasks) {
-                build_tasks(sp);
+                build_rows(sp, rows ? std::max(1LL, c->row_min_kets) : 0, kRowSeg);
+                build_tasks(sp, rows ? std::max(1LL, c->row_min_kets) : 0);
                 UNO_HIP(sp.d_wpre.upload(sp.wpre.data(), sp.wpre.size(), st));
                 UNO_HIP(sp.d_tblk.upload(sp.tblk.data(), sp.tblk.size(), st));
+                if (sp.nrtask > 0) {
+                    UNO_HIP(sp.d_rbra.upload(sp.rbra.data(), sp.rbra.size(), st));
+                    UNO_HIP(sp.d_rpre.upload(sp.rpre.data(), sp.rpre.size(), st));
+                }
             } else {
                 build_blocks(sp, qpb);
                 UNO_HIP(sp.d_blk.upload(sp.blk.data(), sp.blk.size(), st));
@@ -382,6 +410,11 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
             a.wpre = tasks ? sp.d_wpre.p : nullptr;
             a.tblk = tasks ? sp.d_tblk.p : nullptr;
             a.ntask = tasks ? sp.ntask : 0;
+            a.rbra = sp.nrtask > 0 ? sp.d_rbra.p : nullptr;
+            a.rpre = sp.nrtask > 0 ? sp.d_rpre.p : nullptr;
+            a.nrbra = (int)sp.rbra.size();
+            a.nrtask = tasks ? sp.nrtask : 0;
+            a.rseg = kRowSeg;
             a.kstart = prec ? sp.d_kstart.p : nullptr;
             a.qpb = qpb;
             a.nbra = LX.count;
@@ -397,18 +430,28 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
                 a.W1 = c->W1.p;
             }
             const int ci = prec * kNumPairClasses * kNumPairClasses + LX.lcls * kNumPairClasses + LY.lcls;
-            if (c->profile) {
-                a.counter = c->d_counter.p + 2 * kCounterSlots * ci;
-                hipEvent_t e0, e1;
-                UNO_HIP(hipEventCreate(&e0));
-                UNO_HIP(hipEventCreate(&e1));
-                ev0.push_back(e0);
-                ev1.push_back(e1);
-                evcls.push_back(ci);
-                UNO_HIP(hipEventRecord(e0, st));
-            }
-            UNO_HIP(eri_launch(uhf ? kDigestUHF : kDigestRHF, LX.lcls, LY.lcls, a, st, prec == 1));
-            if (c->profile) UNO_HIP(hipEventRecord(ev1.back(), st));
+            // one launch per kernel kind, each with its own profiling slot
+            auto launch = [&](int kind) -> int {
+                const int slot = kind * 2 * kNumPairClasses * kNumPairClasses + ci;
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                if (c->profile) {
+                    a.counter = c->d_counter.p + 2 * kCounterSlots * slot;
+                    UNO_HIP(hipEventCreate(&e0));
+                    UNO_HIP(hipEventCreate(&e1));
+                    ev0.push_back(e0);
+                    ev1.push_back(e1);
+                    evcls.push_back(slot);
+                    UNO_HIP(hipEventRecord(e0, st));
+                }
+                if (kind) UNO_HIP(eri_launch_rows(mode, LX.lcls, LY.lcls, a, kind, st, prec == 1));
+                else UNO_HIP(eri_launch(mode, LX.lcls, LY.lcls, a, st, prec == 1));
+                if (c->profile) UNO_HIP(hipEventRecord(e1, st));
+                return UNO_OK;
+            };
+            if (a.nrtask > 0)
+                if (int rc = launch(variant)) return rc;
+            if (!tasks || a.ntask > 0)
+                if (int rc = launch(0)) return rc;
             (prec ? total32 : total) += (double)sp.nq;
         }
     }

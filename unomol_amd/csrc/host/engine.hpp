@@ -67,12 +67,18 @@ struct PairClassList {
 // One quartet space of a launch: per bra, a contiguous range of kets (prefix counts, the
 // first ket of each range when it does not start at 0) and the block table.
 struct QuartetSpace {
-    std::vector<long long> prefix, wpre;
-    std::vector<int> kstart, blk, tblk;
-    DBuf<long long> d_prefix, d_wpre;
-    DBuf<int> d_kstart, d_blk, d_tblk;
-    long long nq = 0, ntask = 0;
+    std::vector<long long> prefix, wpre, rpre;
+    std::vector<int> kstart, blk, tblk, rbra;
+    DBuf<long long> d_prefix, d_wpre, d_rpre;
+    DBuf<int> d_kstart, d_blk, d_tblk, d_rbra;
+    long long nq = 0, ntask = 0, nrtask = 0;
 };
+
+// row-merged Fock-build tasks: bras with at least row_min_kets (default kRowMinKets) kets
+// in a space run as blocks of kRowSeg kets whose exchange contributions are merged in LDS
+// (the LDS variant per class and n from eri_rows_variant)
+constexpr long long kRowMinKets = 192;
+constexpr int kRowSeg = 1024;
 
 struct QuartetClass {
     int X, Y;                   // indices into the runs, X >= Y (bra run, ket run)
@@ -80,9 +86,11 @@ struct QuartetClass {
     QuartetSpace s32;           // mixed builds: quartets whose Schwarz bound is below fp32_thr
 };
 
-// profiling slots: 36 class pairs (bra pair class X, ket pair class Y, 6 X + Y) for the
-// FP64 launches, then 36 for the FP32 launches of mixed builds
-constexpr int kProfSlots = 2 * kNumPairClasses * kNumPairClasses;
+// profiling slots: kernel kind (0 the wave-task / per-quartet kernels, 1-3 the row-merged
+// variants), then precision (FP64, FP32), then the class pair (6 X + Y):
+// slot = 72 kind + 36 prec + 6 X + Y
+constexpr int kProfKinds = 4;
+constexpr int kProfSlots = kProfKinds * 2 * kNumPairClasses * kNumPairClasses;
 
 constexpr int kNumBuckets = 6;
 inline int prim_bucket(int nprim) {
@@ -121,6 +129,10 @@ struct uno_ctx {
     // profiling pass: per quartet class (bra X, ket Y) kernel milliseconds and the
     // primitive / shell quartets the kernels evaluated
     int profile = 0;
+    // Fock builds run bras with many kets as row-merged blocks (1) or all as wave tasks (0)
+    int row_merge = 1;
+    long long row_min_kets = uno::kRowMinKets;
+    int row_variant = 0;     // row-merged LDS variant: 0 the best that fits, 1-3 forced
     double prof_ms[uno::kProfSlots] = {};
     double prof_prim[uno::kProfSlots] = {};
     double prof_shell[uno::kProfSlots] = {};
@@ -142,7 +154,8 @@ namespace uno {
 int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell);
 // quartet-space helpers of the Fock build (ctx.cpp)
 void build_prefix(uno_ctx* c, QuartetClass& q, double dmax, double fp32_thr);
-void build_tasks(QuartetSpace& s);
+void build_tasks(QuartetSpace& s, long long row_min_kets = 0);
+void build_rows(QuartetSpace& s, long long min_kets, int seg);
 void build_blocks(QuartetSpace& s, int qpb);
 EriArgs base_args(uno_ctx* c);
 // dense S, T, H = T + V (n x n, device) on the context's stream
