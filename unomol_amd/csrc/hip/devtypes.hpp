@@ -17,7 +17,8 @@ struct alignas(16) PairHdr {
     int nprim;      // number of primitive pairs
     int same;       // both shells are the same shell (a == b)
     int sh1, sh2;   // shell indices (first, second)
-    int pad;
+    int jo;         // offset of the pair's function block in the pair-major Coulomb
+                    // accumulator (runs of pairs are contiguous there)
 };
 
 struct alignas(16) PrimPair {
@@ -100,6 +101,8 @@ struct EriArgs {
     double* W0;               // RHF: W; UHF: Wa
     double* W1;               // UHF: Wb
     double* out;              // block / Schwarz output
+    double* jp;               // pair-major accumulator of the ket Coulomb outputs (pair.jo);
+                              // null: they go straight to W
     unsigned long long* counter;  // evaluated-quartet counters (may be null): kCounterSlots
                                   // (primitive, shell) pairs, summed by the host
 };

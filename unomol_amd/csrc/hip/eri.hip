@@ -148,6 +148,21 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// ket Coulomb output (k, l) of ket pair P2: into the pair-major accumulator when the launch
+// has one (consecutive kets of a run are consecutive there, so a wave's atomics share
+// cache lines; the host adds it to W after the build), else straight into W (both spins)
+template <int MODE>
+__device__ __forceinline__ void jket_add(const EriArgs& a, const MolView& mv, const PairHdr& P2, int k, int l,
+                                         int nd, double v) {
+    if (a.jp) {
+        atomic_add_f64(a.jp + P2.jo + k * nd + l, v);
+    } else {
+        const long o = (long)(P2.f1 + k) * mv.n + P2.f2 + l;
+        atomic_add_f64(mv.W0 + o, v);
+        if (MODE == kDigestUHF) atomic_add_f64(mv.W1 + o, v);
+    }
+}
+
 template <int LA, int LB, int LC, int LD, int MODE, typename T>
 __global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
     using CI = ClassInfo<LA, LB, LC, LD>;
@@ -251,9 +266,7 @@ __global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
             T s = 0;
 #pragma unroll
             for (int ij = 0; ij < NA * NB; ++ij) s = fma(e[ij * NC * ND + kl], dab[ij], s);
-            const int k = kl / ND, l = kl % ND;
-            atomic_add_f64(mv.W0 + (long)(fc + k) * n + fd + l, (double)(jf * s));
-            if (MODE == kDigestUHF) atomic_add_f64(mv.W1 + (long)(fc + k) * n + fd + l, (double)(jf * s));
+            jket_add<MODE>(a, mv, P2, kl / ND, kl % ND, ND, (double)(jf * s));
         }
     }
     // exchange, once per spin density
@@ -420,9 +433,7 @@ __global__ __launch_bounds__(256) void k_eri_rows(const EriArgs a) {
             T s = 0;
 #pragma unroll
             for (int ij = 0; ij < NA * NB; ++ij) s = fma(e[ij * NC * ND + kl], dab[ij], s);
-            const int k = kl / ND, l = kl % ND;
-            atomic_add_f64(mv.W0 + (long)(fc + k) * n + fd + l, (double)(jf * s));
-            if (MODE == kDigestUHF) atomic_add_f64(mv.W1 + (long)(fc + k) * n + fd + l, (double)(jf * s));
+            jket_add<MODE>(a, mv, P2, kl / ND, kl % ND, ND, (double)(jf * s));
         }
         // exchange into the LDS rows: row i (a function i), row NA + j (b function j, FB)
 #pragma unroll
@@ -724,8 +735,7 @@ __device__ __forceinline__ void wave_one(const EriArgs& a, const long long g) {
                 for (int i = 0; i < NA; ++i)
                     for (int j = 0; j < NB; ++j)
                         s = fma(sE[((i * NB + j) * NC + k) * ND + l], (T)mv.D0[(long)(fa + i) * n + fb + j], s);
-                atomic_add_f64(mv.W0 + (long)(fc + k) * n + fd + l, (double)(jf * s));
-                if (MODE == kDigestUHF) atomic_add_f64(mv.W1 + (long)(fc + k) * n + fd + l, (double)(jf * s));
+                jket_add<MODE>(a, mv, P2, k, l, ND, (double)(jf * s));
             }
         }
         constexpr int NK = NA * NC + NA * ND + NB * NC + NB * ND;
@@ -928,6 +938,34 @@ int eri_class_ncomp(int cb, int ck) {
 int eri_quartets_per_block(int cb, int ck) { return quartets_per_block(eri_class_ncomp(cb, ck)); }
 
 bool eri_uses_wave_tasks(int cb, int ck) { return eri_class_ncomp(cb, ck) <= kLaneMaxComp; }
+
+// ---------------------------------------------------------------------------
+// pair-major Coulomb accumulator into W
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_jp_scatter(int np, const PairHdr* __restrict__ pairs, long long njp,
+                                                    const double* __restrict__ jp, double* W0, double* W1, int n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const PairHdr h = pairs[p];
+    const long long c1 = p + 1 < np ? (long long)pairs[p + 1].jo : njp;
+    const int nc = (int)(c1 - h.jo);
+    // block shape from its size: 1 ss, 3 ps, 9 pp, 6 ds, 18 dp, 36 dd
+    const int nd = nc == 9 || nc == 18 ? 3 : nc == 36 ? 6 : 1;
+    for (int kl = 0; kl < nc; ++kl) {
+        const double v = jp[h.jo + kl];
+        if (v == 0.0) continue;
+        const long o = (long)(h.f1 + kl / nd) * n + h.f2 + kl % nd;
+        W0[o] += v;
+        if (W1) W1[o] += v;
+    }
+}
+
+hipError_t jp_scatter(int npairs, const PairHdr* pairs, long long njp, const double* jp, double* W0, double* W1, int n,
+                      hipStream_t st) {
+    if (npairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_jp_scatter, dim3((npairs + 255) / 256), dim3(256), 0, st, npairs, pairs, njp, jp, W0, W1, n);
+    return hipGetLastError();
+}
 
 // ---------------------------------------------------------------------------
 // Rys-table evaluation for the parity tests

@@ -254,6 +254,17 @@ int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
     c->pairs.swap(pairs2);
     c->prims.swap(prims2);
     c->pairQ.swap(q2);
+    // pair-major Coulomb accumulator: each pair's function block, in pair order
+    {
+        static const int kPairComp[kNumPairClasses] = {1, 3, 9, 6, 18, 36};
+        long long jo = 0;
+        for (int k = 0; k < np; ++k) {
+            c->pairs[k].jo = (int)jo;
+            jo += kPairComp[runkey[order[k]] / kNumBuckets];
+        }
+        c->njp = jo;
+        UNO_HIP(c->JP.alloc((size_t)std::max(jo, 1LL)));
+    }
     UNO_HIP(c->d_pairs.upload(c->pairs.data(), np, st));
     UNO_HIP(c->d_prims.upload(c->prims.data(), c->prims.size(), st));
     UNO_HIP(c->d_pairQ.upload(c->pairQ.data(), np, st));
@@ -357,6 +368,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
     const long nn = (long)c->n * c->n;
     UNO_HIP(k_fill(nn, 0.0, c->W0.p, st));
     if (uhf) UNO_HIP(k_fill(nn, 0.0, c->W1.p, st));
+    UNO_HIP(k_fill(c->njp, 0.0, c->JP.p, st));
     // shell-block density maxima for the per-quartet screen, and the global bound
     UNO_HIP(k_shell_max(c->n, c->nsh, c->d_shoff.p, c->d_shn.p, c->D0.p, uhf ? c->D1.p : nullptr,
                         uhf ? c->D2.p : nullptr, c->Dsh.p, st));
@@ -421,6 +433,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
             a.nquart = sp.nq;
             a.same_class = q.X == q.Y;
             a.dsh = c->Dsh.p;
+            a.jp = c->JP.p;
             a.dmax_global = 4.0 * dmax;
             a.D0 = c->D0.p;
             a.W0 = c->W0.p;
@@ -455,6 +468,8 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
             (prec ? total32 : total) += (double)sp.nq;
         }
     }
+    // the kets' Coulomb outputs, accumulated per pair, into W
+    UNO_HIP(jp_scatter((int)c->pairs.size(), c->d_pairs.p, c->njp, c->JP.p, c->W0.p, uhf ? c->W1.p : nullptr, c->n, st));
     c->last_quartets = total + total32;
     c->last_quartets32 = total32;
     if (c->profile) {

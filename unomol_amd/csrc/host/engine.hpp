@@ -124,6 +124,8 @@ struct uno_ctx {
     uno::DBuf<int> d_shoff, d_shn;
     // device work matrices (n x n)
     uno::DBuf<double> D0, D1, D2, W0, W1, Dsh, pk, scal;
+    uno::DBuf<double> JP;        // pair-major accumulator of the kets' Coulomb outputs
+    long long njp = 0;
     uno::EigenWS* eig = nullptr;
     double last_quartets = 0.0, last_quartets32 = 0.0;
     // profiling pass: per quartet class (bra X, ket Y) kernel milliseconds and the
