@@ -101,8 +101,8 @@ struct EriArgs {
     double* W0;               // RHF: W; UHF: Wa
     double* W1;               // UHF: Wb
     double* out;              // block / Schwarz output
-    double* jp;               // pair-major accumulator of the ket Coulomb outputs (pair.jo);
-                              // null: they go straight to W
+    double* jp;               // pair-major accumulator of the ket Coulomb outputs (pair.jo)
+                              // of the per-lane kernels; null: they go straight to W
     unsigned long long* counter;  // evaluated-quartet counters (may be null): kCounterSlots
                                   // (primitive, shell) pairs, summed by the host
 };

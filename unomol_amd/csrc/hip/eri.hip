@@ -148,19 +148,16 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// ket Coulomb output (k, l) of ket pair P2: into the pair-major accumulator when the launch
-// has one (consecutive kets of a run are consecutive there, so a wave's atomics share
-// cache lines; the host adds it to W after the build), else straight into W (both spins)
+// ket Coulomb output (k, l) of ket pair P2 in the one-quartet-per-lane kernels: into the
+// pair-major accumulator when the launch has one (consecutive kets of a run are
+// consecutive there, so a wave's atomics share cache lines; the host adds it to W after the
+// build), else straight into W (both spins)
 template <int MODE>
 __device__ __forceinline__ void jket_add(const EriArgs& a, const MolView& mv, const PairHdr& P2, int k, int l,
                                          int nd, double v) {
-    if (a.jp) {
-        atomic_add_f64(a.jp + P2.jo + k * nd + l, v);
-    } else {
-        const long o = (long)(P2.f1 + k) * mv.n + P2.f2 + l;
-        atomic_add_f64(mv.W0 + o, v);
-        if (MODE == kDigestUHF) atomic_add_f64(mv.W1 + o, v);
-    }
+    const long o = (long)(P2.f1 + k) * mv.n + P2.f2 + l;
+    atomic_add_f64(a.jp ? a.jp + P2.jo + k * nd + l : mv.W0 + o, v);
+    if (MODE == kDigestUHF && !a.jp) atomic_add_f64(mv.W1 + o, v);
 }
 
 template <int LA, int LB, int LC, int LD, int MODE, typename T>
@@ -735,7 +732,9 @@ __device__ __forceinline__ void wave_one(const EriArgs& a, const long long g) {
                 for (int i = 0; i < NA; ++i)
                     for (int j = 0; j < NB; ++j)
                         s = fma(sE[((i * NB + j) * NC + k) * ND + l], (T)mv.D0[(long)(fa + i) * n + fb + j], s);
-                jket_add<MODE>(a, mv, P2, k, l, ND, (double)(jf * s));
+                // straight into W (a wave's lanes already cover the ket block's contiguous rows)
+                atomic_add_f64(mv.W0 + (long)(fc + k) * n + fd + l, (double)(jf * s));
+                if (MODE == kDigestUHF) atomic_add_f64(mv.W1 + (long)(fc + k) * n + fd + l, (double)(jf * s));
             }
         }
         constexpr int NK = NA * NC + NA * ND + NB * NC + NB * ND;

@@ -36,10 +36,12 @@ hipError_t k_pack_sym_add(int n, const double* W, double* packed, hipStream_t st
 hipError_t k_sym_w(int n, const double* W, double* G, hipStream_t st, bool accumulate = false);
 hipError_t k_axpby(long len, double a, const double* x, double b, const double* y, double* z, hipStream_t st);
 hipError_t k_fill(long len, double v, double* x, hipStream_t st);
-// out[0] += sum_ij A_ij B_ij (full matrices, deterministic single-block reduction)
-hipError_t k_dot(long len, const double* A, const double* B, double* out, hipStream_t st);
+// out[0] += sum_ij A_ij B_ij (full matrices); deterministic two-pass reduction whose
+// workspace ws holds kRedBlocks partial sums (one workspace per stream)
+constexpr int kRedBlocks = 240;
+hipError_t k_dot(long len, const double* A, const double* B, double* out, double* ws, hipStream_t st);
 // out[0] = sqrt(sum (A-B)^2) / n
-hipError_t k_diffnorm(long len, int n, const double* A, const double* B, double* out, hipStream_t st);
+hipError_t k_diffnorm(long len, int n, const double* A, const double* B, double* out, double* ws, hipStream_t st);
 // Dsh[s][t] = max |D_ij| over the function block of shells s, t
 hipError_t k_shell_max(int n, int nsh, const int* sh_off, const int* sh_n, const double* D, const double* D1,
                        const double* D2, double* Dsh, hipStream_t st);

@@ -82,7 +82,7 @@ hipError_t k_fill(long len, double v, double* x, hipStream_t st) {
     return hipGetLastError();
 }
 
-// deterministic single-block reductions (1024 threads, fixed order)
+// deterministic block reduction (fixed order; blockDim.x a power of two)
 __device__ double block_sum(double v, double* sh) {
     const int t = threadIdx.x;
     sh[t] = v;
@@ -94,30 +94,42 @@ __device__ double block_sum(double v, double* sh) {
     return sh[0];
 }
 
-__global__ __launch_bounds__(1024) void k_dot_(long len, const double* A, const double* B, double* out) {
-    __shared__ double sh[1024];
+// two-pass reductions over long vectors, deterministic: kRedBlocks blocks each sum a
+// contiguous chunk in a fixed order into ws[block], then one block adds ws in order
+__global__ __launch_bounds__(256) void k_red_part(long len, const double* A, const double* B, int diff, double* ws) {
+    __shared__ double sh[256];
+    const long chunk = (len + gridDim.x - 1) / gridDim.x;
+    const long lo = (long)blockIdx.x * chunk, hi = lo + chunk < len ? lo + chunk : len;
     double s = 0.0;
-    for (long i = threadIdx.x; i < len; i += 1024) s = fma(A[i], B[i], s);
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) out[0] += s;
-}
-hipError_t k_dot(long len, const double* A, const double* B, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_dot_, dim3(1), dim3(1024), 0, st, len, A, B, out);
-    return hipGetLastError();
-}
-
-__global__ __launch_bounds__(1024) void k_diffnorm_(long len, int n, const double* A, const double* B, double* out) {
-    __shared__ double sh[1024];
-    double s = 0.0;
-    for (long i = threadIdx.x; i < len; i += 1024) {
-        const double d = A[i] - B[i];
-        s = fma(d, d, s);
+    for (long i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        if (diff) {
+            const double d = A[i] - B[i];
+            s = fma(d, d, s);
+        } else {
+            s = fma(A[i], B[i], s);
+        }
     }
     s = block_sum(s, sh);
-    if (threadIdx.x == 0) out[0] = sqrt(s) / n;
+    if (threadIdx.x == 0) ws[blockIdx.x] = s;
 }
-hipError_t k_diffnorm(long len, int n, const double* A, const double* B, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_diffnorm_, dim3(1), dim3(1024), 0, st, len, n, A, B, out);
+// mode 0: out += sum; mode 1: out = sqrt(sum) / n
+__global__ __launch_bounds__(256) void k_red_final(int nb, const double* ws, int mode, int n, double* out) {
+    __shared__ double sh[256];
+    double s = (int)threadIdx.x < nb ? ws[threadIdx.x] : 0.0;
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) {
+        if (mode == 0) out[0] += s;
+        else out[0] = sqrt(s) / n;
+    }
+}
+hipError_t k_dot(long len, const double* A, const double* B, double* out, double* ws, hipStream_t st) {
+    hipLaunchKernelGGL(k_red_part, dim3(kRedBlocks), dim3(256), 0, st, len, A, B, 0, ws);
+    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(256), 0, st, kRedBlocks, (const double*)ws, 0, 0, out);
+    return hipGetLastError();
+}
+hipError_t k_diffnorm(long len, int n, const double* A, const double* B, double* out, double* ws, hipStream_t st) {
+    hipLaunchKernelGGL(k_red_part, dim3(kRedBlocks), dim3(256), 0, st, len, A, B, 1, ws);
+    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(256), 0, st, kRedBlocks, (const double*)ws, 1, n, out);
     return hipGetLastError();
 }
 

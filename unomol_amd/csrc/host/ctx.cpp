@@ -340,6 +340,7 @@ int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
     UNO_HIP(c->Dsh.alloc((size_t)nsh * nsh));
     UNO_HIP(c->pk.alloc((size_t)c->n * (c->n + 1)));
     UNO_HIP(c->scal.alloc(16));
+    UNO_HIP(c->red.alloc(kRedBlocks));
     UNO_HIP(hipStreamSynchronize(st));
     return UNO_OK;
 }

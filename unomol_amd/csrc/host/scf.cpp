@@ -236,8 +236,8 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
         r.quartets32 += c->last_quartets32;
         UNO_HIP(k_fill(4, 0.0, c->scal.p, st));
         for (int s = 0; s < ns; ++s) {
-            UNO_HIP(k_dot(nn, P[s], b.H.p, c->scal.p, st));
-            UNO_HIP(k_dot(nn, P[s], b.G[s].p, c->scal.p + 1, st));
+            UNO_HIP(k_dot(nn, P[s], b.H.p, c->scal.p, c->red.p, st));
+            UNO_HIP(k_dot(nn, P[s], b.G[s].p, c->scal.p + 1, c->red.p, st));
         }
         UNO_HIP(hipMemcpyAsync(hs, c->scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
         UNO_HIP(hipStreamSynchronize(st));
@@ -274,7 +274,7 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
             for (int j = 0; j < ndiis; ++j)
                 for (int s = 0; s < ns; ++s)
                     UNO_HIP(k_dot(nn, b.diisE.p + ((size_t)slot * ns + s) * nn, b.diisE.p + ((size_t)j * ns + s) * nn,
-                                  dr.p + j, st));
+                                  dr.p + j, c->red.p, st));
             UNO_HIP(hipMemcpyAsync(row.data(), dr.p, kDiis * sizeof(double), hipMemcpyDeviceToHost, st));
             UNO_HIP(hipStreamSynchronize(st));
             for (int j = 0; j < ndiis; ++j) Bm[slot * kDiis + j] = Bm[j * kDiis + slot] = row[j];
@@ -300,7 +300,7 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
             UNO_HIP(hipMemcpyAsync(b.Pold[s].p, P[s], nn * sizeof(double), hipMemcpyDeviceToDevice, st));
             // spin-beta Fock lives in Etmp; copy to F's slot is unnecessary: diag reads Fs[s]
             if (int rc = diag_to_P(Fs[s], s, nocc[s])) return rc;
-            UNO_HIP(k_diffnorm(nn, n, P[s], b.Pold[s].p, c->scal.p + 2 + s, st));
+            UNO_HIP(k_diffnorm(nn, n, P[s], b.Pold[s].p, c->scal.p + 2 + s, c->red.p, st));
         }
         UNO_HIP(hipMemcpyAsync(hs + 2, c->scal.p + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
         UNO_HIP(hipStreamSynchronize(st));
@@ -348,7 +348,7 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
     }
     // kinetic energy and outputs
     UNO_HIP(k_fill(1, 0.0, c->scal.p, st));
-    for (int s = 0; s < ns; ++s) UNO_HIP(k_dot(nn, P[s], b.T.p, c->scal.p, st));
+    for (int s = 0; s < ns; ++s) UNO_HIP(k_dot(nn, P[s], b.T.p, c->scal.p, c->red.p, st));
     double kin = 0.0;
     UNO_HIP(hipMemcpyAsync(&kin, c->scal.p, sizeof(double), hipMemcpyDeviceToHost, st));
     UNO_HIP(hipStreamSynchronize(st));
