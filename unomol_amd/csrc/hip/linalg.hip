@@ -255,70 +255,74 @@ hipError_t gemm_f64(bool ta, bool tb, int m, int n, int k, double alpha, const d
 //   p = tau A22 v, K = tau/2 p.v, w = p - K v, A22 -= v w^T + w v^T
 // d[k] = A[k][k], e[k] = alpha.  Reflector k is stored in row k of V.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_hh_vec(int n, int k, double* A, double* V, double* tau, double* d,
-                                                 double* e) {
-    __shared__ double sh[1024];
+// Step k, launch 1 (one wave per row of the trailing matrix, four rows per block): every
+// block forms the reflector of column k from row k of A (the trailing matrix stays
+// symmetric), block 0 stores it (V row k, tau, d, e); each wave computes
+// p_row = tau A_row . v and the block adds its rows' p_row v_row to part[block].
+__global__ __launch_bounds__(256) void k_hh_vsymv(int n, int k, const double* A, double* V, double* tau, double* d,
+                                                  double* e, double* p, double* part) {
+    __shared__ double sh[256];
+    __shared__ double pv[4];
     const int m = n - k - 1;
+    const double* rk = A + (long)k * n + k + 1;   // row k, columns k+1.. = the column below the diagonal
     double s = 0.0;
-    for (int i = threadIdx.x; i < m; i += 1024) {
-        const double x = A[(long)(k + 1 + i) * n + k];
-        s = fma(x, x, s);
-    }
+    for (int i = threadIdx.x; i < m; i += 256) s = fma(rk[i], rk[i], s);
     const double nx2 = block_sum(s, sh);
-    const double x0 = A[(long)(k + 1) * n + k];
-    const double nx = sqrt(nx2);
-    const double alpha = x0 > 0.0 ? -nx : nx;
-    const double v0 = x0 - alpha;
+    const double x0 = rk[0];
+    const double nx = sqrt(nx2), alpha = x0 > 0.0 ? -nx : nx, v0 = x0 - alpha;
     const double vn2 = nx2 - x0 * x0 + v0 * v0;
-    for (int i = threadIdx.x; i < m; i += 1024) {
-        const double x = A[(long)(k + 1 + i) * n + k];
-        V[(long)k * n + k + 1 + i] = i == 0 ? v0 : x;
-    }
-    if (threadIdx.x == 0) {
-        d[k] = A[(long)k * n + k];
-        if (nx2 == 0.0 || vn2 == 0.0) {
-            tau[k] = 0.0;
-            e[k] = x0;
-        } else {
-            tau[k] = 2.0 / vn2;
-            e[k] = alpha;
+    const double t = (nx2 == 0.0 || vn2 == 0.0) ? 0.0 : 2.0 / vn2;
+    if (blockIdx.x == 0) {
+        for (int i = threadIdx.x; i < m; i += 256) V[(long)k * n + k + 1 + i] = i == 0 ? v0 : rk[i];
+        if (threadIdx.x == 0) {
+            tau[k] = t;
+            d[k] = A[(long)k * n + k];
+            e[k] = t == 0.0 ? x0 : alpha;
         }
     }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + wave;
+    double pr = 0.0;
+    if (row < m) {
+        const double* a = A + (long)(k + 1 + row) * n + k + 1;
+        double acc = 0.0;
+        for (int j = lane; j < m; j += 64) acc = fma(a[j], j == 0 ? v0 : rk[j], acc);
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+        pr = t * acc;
+        if (lane == 0) p[row] = pr;
+    }
+    if (lane == 0) pv[wave] = row < m ? pr * (row == 0 ? v0 : rk[row]) : 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (pv[0] + pv[1]) + (pv[2] + pv[3]);
 }
 
-// p_i = tau sum_j A22[i][j] v_j, one wave per row
-__global__ __launch_bounds__(256) void k_hh_symv(int n, int k, const double* A, const double* V, const double* tau,
-                                                 double* p) {
+// Step k, launch 2 (eight rows per block): K = tau/2 p.v from the partial sums (fixed
+// order, every block), w = p - K v, and the rank-2 update A -= v w^T + w v^T of the
+// block's rows.
+__global__ __launch_bounds__(256) void k_hh_wrank2(int n, int k, double* A, const double* V, const double* tau,
+                                                   const double* p, const double* part, int nparts) {
+    extern __shared__ double lds[];
+    double* sv = lds;          // v, m entries
+    double* sw = lds + n;      // w
+    __shared__ double sh[256];
     const int m = n - k - 1;
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (row >= m) return;
-    const double* a = A + (long)(k + 1 + row) * n + k + 1;
-    const double* v = V + (long)k * n + k + 1;
     double s = 0.0;
-    for (int j = lane; j < m; j += 64) s = fma(a[j], v[j], s);
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-    if (lane == 0) p[row] = tau[k] * s;
-}
-
-// scal[0] = K = tau/2 p.v ; p <- w = p - K v
-__global__ __launch_bounds__(1024) void k_hh_w(int n, int k, const double* V, const double* tau, double* p,
-                                               double* scal) {
-    __shared__ double sh[1024];
-    const int m = n - k - 1;
-    const double* v = V + (long)k * n + k + 1;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < m; i += 1024) s = fma(p[i], v[i], s);
+    for (int b = threadIdx.x; b < nparts; b += 256) s += part[b];
     const double K = 0.5 * tau[k] * block_sum(s, sh);
-    for (int i = threadIdx.x; i < m; i += 1024) p[i] -= K * v[i];
-}
-
-__global__ __launch_bounds__(256) void k_hh_rank2(int n, int k, double* A, const double* V, const double* w) {
-    const int m = n - k - 1;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)m * m) return;
-    const int i = (int)(t / m), j = (int)(t % m);
     const double* v = V + (long)k * n + k + 1;
-    A[(long)(k + 1 + i) * n + k + 1 + j] -= v[i] * w[j] + w[i] * v[j];
+    for (int j = threadIdx.x; j < m; j += 256) {
+        const double vj = v[j];
+        sv[j] = vj;
+        sw[j] = p[j] - K * vj;
+    }
+    __syncthreads();
+    for (int rr = 0; rr < 8; ++rr) {
+        const int r = blockIdx.x * 8 + rr;
+        if (r >= m) break;
+        const double vi = sv[r], wi = sw[r];
+        double* a = A + (long)(k + 1 + r) * n + k + 1;
+        for (int j = threadIdx.x; j < m; j += 256) a[j] -= vi * sw[j] + wi * sv[j];
+    }
 }
 
 __global__ void k_hh_last(int n, double* A, double* d, double* e) {
@@ -333,13 +337,17 @@ __global__ void k_hh_last(int n, double* A, double* d, double* e) {
 hipError_t householder_tridiag(int n, EigWork& w, hipStream_t st) {
     // reflector rows must be zero outside their support for the blocked back-transform
     UNO_LAUNCH_FILL((long)n * n, w.V, st);
+    // partial dots of launch 1 live in the inverse-iteration work space (unused until the
+    // tridiagonal is done)
+    double* part = w.wk;
+    const size_t lds = 2 * (size_t)n * sizeof(double);
     for (int k = 0; k + 2 < n; ++k) {
         const int m = n - k - 1;
-        hipLaunchKernelGGL(k_hh_vec, dim3(1), dim3(1024), 0, st, n, k, w.A, w.V, w.tau, w.d, w.e);
-        hipLaunchKernelGGL(k_hh_symv, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, st, n, k, w.A, w.V, w.tau, w.p);
-        hipLaunchKernelGGL(k_hh_w, dim3(1), dim3(1024), 0, st, n, k, w.V, w.tau, w.p, w.scal);
-        const long mm = (long)m * m;
-        hipLaunchKernelGGL(k_hh_rank2, dim3((unsigned)((mm + 255) / 256)), dim3(256), 0, st, n, k, w.A, w.V, w.p);
+        const int nb1 = (m + 3) / 4, nb2 = (m + 7) / 8;
+        hipLaunchKernelGGL(k_hh_vsymv, dim3((unsigned)nb1), dim3(256), 0, st, n, k, w.A, w.V, w.tau, w.d, w.e, w.p,
+                           part);
+        hipLaunchKernelGGL(k_hh_wrank2, dim3((unsigned)nb2), dim3(256), lds, st, n, k, w.A, w.V, w.tau, w.p, part,
+                           nb1);
     }
     hipLaunchKernelGGL(k_hh_last, dim3(1), dim3(1), 0, st, n, w.A, w.d, w.e);
     return hipGetLastError();
