@@ -127,6 +127,8 @@ typedef struct uno_scf_result {
     double quartets32;    /* of which evaluated in FP32 (mixed precision) */
     int nfull;            /* full (not incremental) Fock builds */
     double guess_s;       /* seconds spent on the learned guess (guess = 2) */
+    double wall99_s;      /* batched runs: seconds until 99 % of the molecules had stopped
+                             (converged or at maxits); single runs: wall_s */
 } uno_scf_result;
 
 void uno_scf_defaults(uno_scf_opts* opts);

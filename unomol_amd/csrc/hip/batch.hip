@@ -76,19 +76,33 @@ hipError_t grouped_gemm(int ntiles, const GemmTile* tiles, const GemmGroup* grou
 // Cyclic Jacobi in round-robin order: with m = N rounded up to even (a dummy index N when N
 // is odd), round r pairs (m-1, r) and ((r+k) mod (m-1), (r-k) mod (m-1)), k = 1..m/2-1, so
 // that each round is a perfect matching and m-1 rounds visit every pair once.  A rotation
-// is skipped when |a_pq| <= eps/2 sqrt(|a_pp a_qq|) (it would not change the diagonal);
-// sweeps stop when none is applied.
-__global__ __launch_bounds__(256) void k_batched_jacobi(const BatchMat* __restrict__ mats, int max_sweeps,
-                                                        int* __restrict__ sweeps_out) {
+// is skipped when |a_pq| <= max(eps/2 sqrt(|a_pp a_qq|), 1e-15 ||A||_F) (it would not change
+// the diagonal, or the pair is converged to working accuracy of the whole matrix: the
+// relative test alone never settles next to zero diagonal entries, where rounding keeps
+// a_pq above it); sweeps stop when none is applied.
+constexpr int kJacobiThreads = 1024;
+__global__ __launch_bounds__(kJacobiThreads) void k_batched_jacobi(const BatchMat* __restrict__ mats, int max_sweeps,
+                                                                   int* __restrict__ sweeps_out) {
     __shared__ double cs[128], sn[128];
     __shared__ int pp[128], qq[128];
     __shared__ int nrot;
+    __shared__ double red[kJacobiThreads / 64];
     const BatchMat M = mats[blockIdx.x];
     const int n = M.n;
     double* A = M.A;
     double* V = M.V;
     const int tid = threadIdx.x;
-    for (long e = tid; e < (long)n * n; e += blockDim.x) V[e] = (e / n == e % n) ? 1.0 : 0.0;
+    double f2 = 0.0;
+    for (long e = tid; e < (long)n * n; e += blockDim.x) {
+        V[e] = (e / n == e % n) ? 1.0 : 0.0;
+        f2 = fma(A[e], A[e], f2);
+    }
+    for (int o = 32; o >= 1; o >>= 1) f2 += __shfl_xor(f2, o, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = f2;
+    __syncthreads();
+    double fro2 = 0.0;
+    for (int w = 0; w < kJacobiThreads / 64; ++w) fro2 += red[w];
+    const double abs_tol = 1e-15 * sqrt(fro2);
     const int m = n + (n & 1);
     int sweep = 0;
     while (sweep < max_sweeps && n > 1) {
@@ -105,7 +119,7 @@ __global__ __launch_bounds__(256) void k_batched_jacobi(const BatchMat* __restri
                 if (q < n) {
                     const double apq = A[(long)p * n + q];
                     const double app = A[(long)p * n + p], aqq = A[(long)q * n + q];
-                    if (apq != 0.0 && fabs(apq) > 1.1102230246251565e-16 * sqrt(fabs(app * aqq))) {
+                    if (fabs(apq) > fmax(1.1102230246251565e-16 * sqrt(fabs(app * aqq)), abs_tol)) {
                         const double theta = (aqq - app) / (2.0 * apq);
                         const double t = fabs(theta) < 1e150
                                              ? (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0))
@@ -168,7 +182,7 @@ __global__ __launch_bounds__(256) void k_batched_jacobi(const BatchMat* __restri
 
 hipError_t batched_jacobi(int nmat, const BatchMat* mats, int max_sweeps, int* sweeps, hipStream_t st) {
     if (nmat <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_batched_jacobi, dim3(nmat), dim3(256), 0, st, mats, max_sweeps, sweeps);
+    hipLaunchKernelGGL(k_batched_jacobi, dim3(nmat), dim3(kJacobiThreads), 0, st, mats, max_sweeps, sweeps);
     return hipGetLastError();
 }
 

@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -67,6 +68,25 @@ bool solve_small(int m, std::vector<double> A, std::vector<double>& b) {
     return true;
 }
 
+// Device copies of the small per-iteration arrays (GEMM groups and tiles, matrix lists,
+// quartet entries, DIIS coefficients): the k-th upload of an iteration reuses slot k,
+// grown when needed, so the steady state allocates nothing.  Every iteration ends with a
+// stream synchronisation before reset() hands the slots out again.
+struct Scratch {
+    std::vector<DBuf<char>> slot;
+    size_t next = 0;
+    void reset() { next = 0; }
+    template <class T>
+    hipError_t upload(const T* h, size_t n, hipStream_t st, T** out) {
+        if (next == slot.size()) slot.emplace_back();
+        DBuf<char>& s = slot[next++];
+        hipError_t e = s.alloc(std::max<size_t>(n, 1) * sizeof(T));
+        if (e != hipSuccess) return e;
+        *out = reinterpret_cast<T*>(s.p);
+        return n ? hipMemcpyAsync(s.p, h, n * sizeof(T), hipMemcpyHostToDevice, st) : hipSuccess;
+    }
+};
+
 }  // namespace
 
 struct uno_batch {
@@ -82,6 +102,7 @@ struct uno_batch {
     DBuf<double> H, S, X, P[2], Pold[2], F[2], G[2], C[2], Fp, Tm, Tm2, V, Z, D0, D1, D2, W0, W1, Dsh, ev[2];
     DBuf<double> dF[kBatchDiis][2], dE[kBatchDiis][2];
     DBuf<double> red;
+    Scratch scratch;
     ~uno_batch() {
         for (BatchMol& m : mol)
             if (m.ctx) uno_ctx_destroy(m.ctx);
@@ -107,8 +128,8 @@ void make_slots(const uno_batch* b, bool only_active, int spin, std::vector<MatS
 struct GemmBatch {
     std::vector<GemmGroup> g;
     std::vector<GemmTile> t;
-    DBuf<GemmGroup> dg;
-    DBuf<GemmTile> dt;
+    Scratch* sc;
+    explicit GemmBatch(Scratch& s) : sc(&s) {}
     void add(int m, int n, int k, bool ta, bool tb, const double* A, const double* B, double* C, int lda, int ldb,
              int ldc) {
         if (m <= 0 || n <= 0 || k <= 0) return;
@@ -119,9 +140,11 @@ struct GemmBatch {
     }
     int run(double alpha, double beta, hipStream_t st) {
         if (t.empty()) return UNO_OK;
-        UNO_HIP(dg.upload(g.data(), g.size(), st));
-        UNO_HIP(dt.upload(t.data(), t.size(), st));
-        UNO_HIP(grouped_gemm((int)t.size(), dt.p, dg.p, alpha, beta, st));
+        GemmGroup* dg = nullptr;
+        GemmTile* dt = nullptr;
+        UNO_HIP(sc->upload(g.data(), g.size(), st, &dg));
+        UNO_HIP(sc->upload(t.data(), t.size(), st, &dt));
+        UNO_HIP(grouped_gemm((int)t.size(), dt, dg, alpha, beta, st));
         // the group / tile arrays are pageable host memory: keep them alive until done
         UNO_HIP(hipStreamSynchronize(st));
         return UNO_OK;
@@ -134,9 +157,9 @@ int batch_eigh(uno_batch* b, const std::vector<MatSlot>& sl, double* A, double* 
     std::vector<BatchMat> bm(sl.size());
     for (size_t k = 0; k < sl.size(); ++k)
         bm[k] = BatchMat{sl[k].n, A + sl[k].off, b->V.p + sl[k].off, evals_base + evoff[k], b->Z.p + sl[k].off};
-    DBuf<BatchMat> d;
-    UNO_HIP(d.upload(bm.data(), bm.size(), b->st));
-    UNO_HIP(batched_jacobi((int)bm.size(), d.p, 50, nullptr, b->st));
+    BatchMat* d = nullptr;
+    UNO_HIP(b->scratch.upload(bm.data(), bm.size(), b->st, &d));
+    UNO_HIP(batched_jacobi((int)bm.size(), d, 50, nullptr, b->st));
     UNO_HIP(hipStreamSynchronize(b->st));
     return UNO_OK;
 }
@@ -173,28 +196,33 @@ static int batch_fock(uno_batch* b, const std::vector<double>& dmax, double& qua
             }
         }
     }
+    // host arrays stay alive until the final synchronisation (their uploads are async)
+    std::vector<QuartetSpace> spaces(groups.size());
+    size_t gi = 0;
     for (auto& kv : groups) {
         const int cx = std::get<0>(kv.first), cy = std::get<1>(kv.first), uhf = std::get<2>(kv.first);
         Ent& e = kv.second;
         const int ne = (int)e.bra.size();
-        QuartetSpace sp;
+        QuartetSpace& sp = spaces[gi++];
         sp.prefix.assign(ne + 1, 0);
         for (int i = 0; i < ne; ++i) sp.prefix[i + 1] = sp.prefix[i] + e.cnt[i];
         sp.nq = sp.prefix[ne];
         const bool tasks = eri_uses_wave_tasks(cx, cy);
         const int qpb = eri_quartets_per_block(cx, cy);
-        DBuf<int> dbra, dket, dmol;
-        UNO_HIP(dbra.upload(e.bra.data(), ne, st));
-        UNO_HIP(dket.upload(e.ket.data(), ne, st));
-        UNO_HIP(dmol.upload(e.mol.data(), ne, st));
-        UNO_HIP(sp.d_prefix.upload(sp.prefix.data(), sp.prefix.size(), st));
+        int *dbra = nullptr, *dket = nullptr, *dmol = nullptr, *dtblk = nullptr, *dblk = nullptr;
+        long long *dprefix = nullptr, *dwpre = nullptr;
+        Scratch& sc = b->scratch;
+        UNO_HIP(sc.upload(e.bra.data(), ne, st, &dbra));
+        UNO_HIP(sc.upload(e.ket.data(), ne, st, &dket));
+        UNO_HIP(sc.upload(e.mol.data(), ne, st, &dmol));
+        UNO_HIP(sc.upload(sp.prefix.data(), sp.prefix.size(), st, &dprefix));
         if (tasks) {
             build_tasks(sp);
-            UNO_HIP(sp.d_wpre.upload(sp.wpre.data(), sp.wpre.size(), st));
-            UNO_HIP(sp.d_tblk.upload(sp.tblk.data(), sp.tblk.size(), st));
+            UNO_HIP(sc.upload(sp.wpre.data(), sp.wpre.size(), st, &dwpre));
+            UNO_HIP(sc.upload(sp.tblk.data(), sp.tblk.size(), st, &dtblk));
         } else {
             build_blocks(sp, qpb);
-            UNO_HIP(sp.d_blk.upload(sp.blk.data(), sp.blk.size(), st));
+            UNO_HIP(sc.upload(sp.blk.data(), sp.blk.size(), st, &dblk));
         }
         EriArgs a{};
         a.pairs = b->pairs.p;
@@ -202,16 +230,16 @@ static int batch_fock(uno_batch* b, const std::vector<double>& dmax, double& qua
         a.pairQ = b->pairQ.p;
         a.tau = b->mol[0].ctx->tau;
         a.primcut = b->mol[0].ctx->primcut;
-        a.prefix = sp.d_prefix.p;
-        a.blk = tasks ? nullptr : sp.d_blk.p;
-        a.wpre = tasks ? sp.d_wpre.p : nullptr;
-        a.tblk = tasks ? sp.d_tblk.p : nullptr;
+        a.prefix = dprefix;
+        a.blk = tasks ? nullptr : dblk;
+        a.wpre = tasks ? dwpre : nullptr;
+        a.tblk = tasks ? dtblk : nullptr;
         a.ntask = tasks ? sp.ntask : 0;
         a.qpb = qpb;
         a.nquart = sp.nq;
-        a.e_bra = dbra.p;
-        a.e_ket = dket.p;
-        a.e_mol = dmol.p;
+        a.e_bra = dbra;
+        a.e_ket = dket;
+        a.e_mol = dmol;
         a.mols = b->mdesc.p;
         a.dsh = b->Dsh.p;
         a.dmax_global = 0.0;
@@ -224,9 +252,8 @@ static int batch_fock(uno_batch* b, const std::vector<double>& dmax, double& qua
         }
         UNO_HIP(eri_launch(uhf ? kDigestUHF : kDigestRHF, cx, cy, a, st));
         quartets += (double)sp.nq;
-        // the entry arrays are freed at scope end: finish the launch first
-        UNO_HIP(hipStreamSynchronize(st));
     }
+    UNO_HIP(hipStreamSynchronize(st));
     return UNO_OK;
 }
 
@@ -337,6 +364,7 @@ int uno_batch_run(uno_batch* b, const uno_scf_opts* opts, uno_scf_result* res, d
     }
     const auto t0 = clk::now();
     hipStream_t st = b->st;
+    b->scratch.reset();
     const int nm = (int)b->mol.size();
     const long long S = b->store;
     // per-molecule eigenvalue offsets into ev stores: reuse the matrix offsets (n <= n^2)
@@ -389,15 +417,18 @@ int uno_batch_run(uno_batch* b, const uno_scf_opts* opts, uno_scf_result* res, d
         UNO_HIP(b->X.upload(Z.data(), S, st));
     }
     // F (store, per spin) -> P per spin slot: X^T F X, Jacobi, C = X W, P = C_occ C_occ^T
+    double diag_s = 0.0;
     auto diag_to_P = [&](int s, const double* Fs) -> int {
         if (sl[s].empty()) return UNO_OK;
-        GemmBatch g1, g2;
+        const auto td = clk::now();
+        struct Acc { double& t; clk::time_point t0; ~Acc() { t += secs(t0, clk::now()); } } acc{diag_s, td};
+        GemmBatch g1(b->scratch), g2(b->scratch);
         for (const MatSlot& m : sl[s]) g1.add(m.n, m.n, m.n, false, false, Fs + m.off, b->X.p + m.off, b->Tm.p + m.off, m.n, m.n, m.n);
         if (int rc = g1.run(1.0, 0.0, st)) return rc;
         for (const MatSlot& m : sl[s]) g2.add(m.n, m.n, m.n, true, false, b->X.p + m.off, b->Tm.p + m.off, b->Fp.p + m.off, m.n, m.n, m.n);
         if (int rc = g2.run(1.0, 0.0, st)) return rc;
         if (int rc = batch_eigh(b, sl[s], b->Fp.p, b->ev[s].p, evoff(s))) return rc;
-        GemmBatch g3, g4;
+        GemmBatch g3(b->scratch), g4(b->scratch);
         for (const MatSlot& m : sl[s]) g3.add(m.n, m.n, m.n, false, false, b->X.p + m.off, b->Z.p + m.off, b->C[s].p + m.off, m.n, m.n, m.n);
         if (int rc = g3.run(1.0, 0.0, st)) return rc;
         for (size_t k = 0; k < sl[s].size(); ++k) {
@@ -410,7 +441,7 @@ int uno_batch_run(uno_batch* b, const uno_scf_opts* opts, uno_scf_result* res, d
     // core-Hamiltonian guess (UHF beta reuses the alpha orbitals, UHF.hpp:206-215)
     if (int rc = diag_to_P(0, b->H.p)) return rc;
     if (!sl[1].empty()) {
-        GemmBatch g;
+        GemmBatch g(b->scratch);
         for (size_t k = 0; k < sl[1].size(); ++k) {
             const MatSlot& m = sl[1][k];
             g.add(m.n, m.n, b->mol[who[1][k]].nocc[1], false, true, b->C[0].p + m.off, b->C[0].p + m.off,
@@ -425,7 +456,10 @@ int uno_batch_run(uno_batch* b, const uno_scf_opts* opts, uno_scf_result* res, d
     int iteration = 0;
     double fock_s = 0.0;
     double quartets = 0.0;
+    int nstopped = 0;
+    double wall99 = -1.0;
     while (true) {
+        b->scratch.reset();   // the previous iteration ended synchronised
         // densities of the build: D1 = Pa, D2 = Pb (zero for closed shells), D0 = D1 + D2
         const auto tf = clk::now();
         UNO_HIP(hipMemcpyAsync(b->D1.p, b->P[0].p, S * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -470,7 +504,7 @@ int uno_batch_run(uno_batch* b, const uno_scf_opts* opts, uno_scf_result* res, d
         for (int s = 0; s < 2; ++s) {
             if (sl[s].empty()) continue;
             UNO_HIP(hipMemcpyAsync(b->dF[slot][s].p, b->F[s].p, S * sizeof(double), hipMemcpyDeviceToDevice, st));
-            GemmBatch g1, g2, g3, g4;
+            GemmBatch g1(b->scratch), g2(b->scratch), g3(b->scratch), g4(b->scratch);
             for (const MatSlot& m : sl[s]) g1.add(m.n, m.n, m.n, false, false, b->F[s].p + m.off, b->P[s].p + m.off, b->Tm.p + m.off, m.n, m.n, m.n);
             if (int rc = g1.run(1.0, 0.0, st)) return rc;
             for (const MatSlot& m : sl[s]) g2.add(m.n, m.n, m.n, false, false, b->Tm.p + m.off, b->S.p + m.off, b->Fp.p + m.off, m.n, m.n, m.n);
@@ -520,14 +554,12 @@ int uno_batch_run(uno_batch* b, const uno_scf_opts* opts, uno_scf_result* res, d
             }
         }
         if (any) {
-            DBuf<double> dcoef;
-            UNO_HIP(dcoef.upload(coef.data(), coef.size(), st));
             for (int s = 0; s < 2; ++s) {
                 if (sl[s].empty()) continue;
                 std::vector<const double*> src(kBatchDiis);
                 for (int j = 0; j < kBatchDiis; ++j) src[j] = b->dF[j][s].p;
-                DBuf<const double*> dsrc;
-                UNO_HIP(dsrc.upload(src.data(), src.size(), st));
+                const double** dsrc = nullptr;
+                UNO_HIP(b->scratch.upload(src.data(), src.size(), st, &dsrc));
                 // coefficients of slot k of spin s: the molecule's row (spin 1 slots are a subset)
                 std::vector<double> cs((size_t)sl[s].size() * kBatchDiis);
                 for (size_t k = 0; k < sl[s].size(); ++k) {
@@ -535,9 +567,9 @@ int uno_batch_run(uno_batch* b, const uno_scf_opts* opts, uno_scf_result* res, d
                     const size_t k0 = std::find(who[0].begin(), who[0].end(), m) - who[0].begin();
                     std::copy(&coef[k0 * kBatchDiis], &coef[k0 * kBatchDiis] + kBatchDiis, &cs[k * kBatchDiis]);
                 }
-                DBuf<double> dcs;
-                UNO_HIP(dcs.upload(cs.data(), cs.size(), st));
-                UNO_HIP(batch_lincomb((int)sl[s].size(), dsl[s].p, kBatchDiis, dcs.p, dsrc.p, b->F[s].p, st));
+                double* dcs = nullptr;
+                UNO_HIP(b->scratch.upload(cs.data(), cs.size(), st, &dcs));
+                UNO_HIP(batch_lincomb((int)sl[s].size(), dsl[s].p, kBatchDiis, dcs, dsrc, b->F[s].p, st));
                 UNO_HIP(hipStreamSynchronize(st));
             }
         }
@@ -572,8 +604,14 @@ int uno_batch_run(uno_batch* b, const uno_scf_opts* opts, uno_scf_result* res, d
                 x.converged = conv;
                 x.active = false;
                 changed = true;
+                ++nstopped;
             }
         }
+        if (wall99 < 0.0 && 100 * nstopped >= 99 * nm) wall99 = secs(t0, clk::now());
+        if (o.verbose)
+            fprintf(stderr, "[batch] iteration %d: %d molecules were active, %d stopped; Fock %.3f s, "
+                            "eigen/GEMM %.3f s, wall %.3f s\n", iteration, n0, nstopped, fock_s, diag_s,
+                    secs(t0, clk::now()));
         bool any_active = false;
         for (const BatchMol& x : b->mol) any_active = any_active || x.active;
         if (!any_active) break;
@@ -598,7 +636,9 @@ int uno_batch_run(uno_batch* b, const uno_scf_opts* opts, uno_scf_result* res, d
         r.uhf = x.uhf ? 1 : 0;
         r.nfock = x.iterations;
         r.wall_s = wall;
+        r.wall99_s = wall99 < 0.0 ? wall : wall99;
         r.fock_s = fock_s;
+        r.diag_s = diag_s;
         r.quartets = quartets;
         if (energies) energies[m] = r.energy;
         if (!x.converged) rc = UNO_ENOCONV;

@@ -376,6 +376,7 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
     r.fock_s = fock_s;
     r.diag_s = diag_s;
     r.wall_s = secs(t_start, clk::now());
+    r.wall99_s = r.wall_s;
     return conv ? UNO_OK : UNO_ENOCONV;
 }
 
