@@ -151,6 +151,15 @@ int uno_rys_eval_f32(int n, int npts, const double* x, double* u, double* w);
 /* Debug/parity: Schwarz bounds sqrt(max|(ab|ab)|) per shell pair a>=b, packed order. */
 int uno_schwarz(uno_ctx* ctx, double* q);
 
+/* Autotuner (no reference counterpart): times every row-merge configuration (LDS variant,
+ * the ket count from which a bra is row-merged, or wave tasks only) of every quartet space
+ * of one Fock build on the given density (PB null: RHF P; else UHF Pa, Pb; packed, as
+ * uno_fock_rhf/uhf) and keeps the fastest per space, for this context and for every later
+ * context of the same basis layout in this process.  mixed = 1 tunes the mixed-precision
+ * build's spaces.  best_ms (optional) receives the tuned build's summed kernel time.
+ * G is not produced; the next Fock build starts from scratch. */
+int uno_autotune(uno_ctx* ctx, const double* P, const double* PB, int mixed, double* best_ms);
+
 /* Profiling of the ERI/Fock kernels (no reference counterpart; the reference only
  * prints stage timers, TwoElectronInts.cpp:690-695).  enable=1 resets and turns on a
  * profiling mode in which every Fock build records, per quartet class (bra pair class

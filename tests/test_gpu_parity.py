@@ -341,6 +341,46 @@ def test_fock_rows_vs_waves_config_b(uhf):
         t.set("row_variant", 0)
 
 
+def test_autotune_keeps_fock_matrix():
+    """The autotuner times the row-merge configurations of every quartet space and keeps the
+    fastest: the Fock matrix built afterwards (RHF and UHF, FP64 and mixed) equals the
+    default configuration's, and a new context of the same deck picks the choices up."""
+    say("autotune, config B deck")
+    path = os.path.join(R.GOLDEN, "decks", "B_w16_631gs.dat")
+    deck = read_deck(path)
+    n = deck["norb"]
+    rng = np.random.default_rng(14)
+    A = rng.standard_normal((n, n)) * 0.02
+    Bm = rng.standard_normal((n, n)) * 0.02
+    PA, PB = _pack(A + A.T + np.eye(n)), _pack(Bm + Bm.T + np.eye(n))
+    b, t = _tints(path)
+    G0 = np.zeros_like(PA)
+    t.formGmatrix(PA, G0)
+    GA0, GB0 = np.zeros_like(PA), np.zeros_like(PB)
+    t.formGmatrix(PA, PB, GA0, GB0)
+    assert t.autotune(PA) > 0.0
+    assert t.autotune(PA, PB) > 0.0
+    G1 = np.zeros_like(PA)
+    t.formGmatrix(PA, G1)
+    GA1, GB1 = np.zeros_like(PA), np.zeros_like(PB)
+    t.formGmatrix(PA, PB, GA1, GB1)
+    assert np.abs(G1 - G0).max() < 1e-11 * max(1.0, np.abs(G0).max())
+    assert np.abs(GA1 - GA0).max() < 1e-11 * max(1.0, np.abs(GA0).max())
+    assert np.abs(GB1 - GB0).max() < 1e-11 * max(1.0, np.abs(GB0).max())
+    t.set("precision", 1)
+    Gm0 = np.zeros_like(PA)
+    t.formGmatrix(PA, Gm0)
+    assert t.autotune(PA, mixed=True) > 0.0
+    Gm1 = np.zeros_like(PA)
+    t.formGmatrix(PA, Gm1)
+    assert np.abs(Gm1 - Gm0).max() < 1e-11 * max(1.0, np.abs(Gm0).max())
+    # a second context of the same deck builds the same G with the remembered choices
+    b2, t2 = _tints(path)
+    G2 = np.zeros_like(PA)
+    t2.formGmatrix(PA, G2)
+    assert np.abs(G2 - G0).max() < 1e-11 * max(1.0, np.abs(G0).max())
+
+
 def test_scf_incremental_same_energy():
     say("scf incremental builds w2_631gs")
     path = R.deck_path("w2_631gs")

@@ -72,6 +72,16 @@ class TwoElectronInts:
         N.check(N.lib().uno_ctx_get(self.ctx, key.encode(), C.byref(v)))
         return v.value
 
+    def autotune(self, P, PB=None, mixed=False):
+        """Times the row-merge configurations of every quartet space on density P (packed;
+        PB for UHF) and keeps the fastest; returns the tuned build's kernel milliseconds."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        pb = None if PB is None else np.ascontiguousarray(PB, dtype=np.float64)
+        ms = C.c_double()
+        N.check(N.lib().uno_autotune(self.ctx, N.ptr(P), None if pb is None else N.ptr(pb), 1 if mixed else 0,
+                                     C.byref(ms)))
+        return ms.value
+
     def formGmatrix(self, *args):
         """formGmatrix(P, G) (RHF) or formGmatrix(PA, PB, GA, GB) (UHF); accumulates."""
         arrs = [a for a in args]

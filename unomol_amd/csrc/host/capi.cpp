@@ -348,6 +348,30 @@ int uno_fock_uhf(uno_ctx* c, const double* PA, const double* PB, double* GA, dou
     return UNO_OK;
 }
 
+int uno_autotune(uno_ctx* c, const double* PA, const double* PB, int mixed, double* best_ms) {
+    if (!c || !PA) { set_error("null argument"); return UNO_EINVAL; }
+    const int n = c->n;
+    const long n2 = (long)n * (n + 1) / 2, nn = (long)n * n;
+    hipStream_t st = c->st;
+    const bool uhf = PB != nullptr;
+    UNO_HIP(c->pk.upload(PA, n2, st));
+    UNO_HIP(k_unpack(n, c->pk.p, uhf ? c->D1.p : c->D0.p, st));
+    if (uhf) {
+        UNO_HIP(c->pk.upload(PB, n2, st));
+        UNO_HIP(k_unpack(n, c->pk.p, c->D2.p, st));
+        UNO_HIP(k_axpby(nn, 1.0, c->D1.p, 1.0, c->D2.p, c->D0.p, st));
+    }
+    c->tuning = 1;
+    c->tune_ms = 0.0;
+    const int rc = fock_build(c, uhf, -1.0, mixed != 0);
+    c->tuning = 0;
+    if (rc) return rc;
+    UNO_HIP(hipStreamSynchronize(st));
+    remember_tuning(c);
+    if (best_ms) *best_ms = c->tune_ms;
+    return UNO_OK;
+}
+
 int uno_eigh_packed(int n, double* a, double* z, double* evals) {
     if (n <= 0 || !a || !z || !evals) { set_error("bad argument"); return UNO_EINVAL; }
     hipStream_t st;

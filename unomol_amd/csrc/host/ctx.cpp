@@ -111,6 +111,25 @@ void build_blocks(QuartetSpace& s, int qpb) {
     }
 }
 
+// tuned row-merge configurations, shared by every context of the same layout (basis size,
+// shells, pairs and quartet spaces) in this process
+static std::map<std::tuple<int, int, int, int>, std::map<std::tuple<int, int, int, int>, TuneChoice>> g_tuning;
+
+static std::tuple<int, int, int, int> tuning_key(const uno_ctx* c) {
+    return std::make_tuple(c->n, c->nsh, (int)c->pairs.size(), (int)c->qcls.size());
+}
+
+void remember_tuning(uno_ctx* c) {
+    auto& m = g_tuning[tuning_key(c)];
+    for (const auto& kv : c->tune) m[kv.first] = kv.second;
+}
+
+static void recall_tuning(uno_ctx* c) {
+    auto it = g_tuning.find(tuning_key(c));
+    c->tune.clear();
+    if (it != g_tuning.end()) c->tune = it->second;
+}
+
 EriArgs base_args(uno_ctx* c) {
     EriArgs a{};
     a.pairs = c->d_pairs.p;
@@ -298,6 +317,7 @@ int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
             q.Y = Y;
             c->qcls.push_back(std::move(q));
         }
+    recall_tuning(c);
     std::vector<int> shoff(nsh), shn(nsh);
     for (int s = 0; s < nsh; ++s) {
         shoff[s] = B.off[s];
@@ -396,20 +416,24 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
             if (sp.nq == 0) continue;
             const bool tasks = eri_uses_wave_tasks(LX.lcls, LY.lcls);
             const int mode = uhf ? kDigestUHF : kDigestRHF;
-            const int variant =
+            const int qi = (int)(&q - c->qcls.data());
+            // row-merge configuration: the tuned one for this space if any, else the defaults
+            int variant =
                 tasks && c->row_merge ? eri_rows_variant(LX.lcls, LY.lcls, mode, c->n, prec == 1, c->row_variant) : 0;
-            const bool rows = variant > 0;
-            UNO_HIP(sp.d_prefix.upload(sp.prefix.data(), sp.prefix.size(), st));
-            if (tasks) {
-                build_rows(sp, rows ? std::max(1LL, c->row_min_kets) : 0, kRowSeg);
-                build_tasks(sp, rows ? std::max(1LL, c->row_min_kets) : 0);
-                UNO_HIP(sp.d_wpre.upload(sp.wpre.data(), sp.wpre.size(), st));
-                UNO_HIP(sp.d_tblk.upload(sp.tblk.data(), sp.tblk.size(), st));
-                if (sp.nrtask > 0) {
-                    UNO_HIP(sp.d_rbra.upload(sp.rbra.data(), sp.rbra.size(), st));
-                    UNO_HIP(sp.d_rpre.upload(sp.rpre.data(), sp.rpre.size(), st));
+            long long min_kets = std::max(1LL, c->row_min_kets);
+            const auto tk = std::make_tuple(qi, prec, uhf ? 1 : 0, mixed ? 1 : 0);
+            if (tasks && c->row_merge && !c->tuning) {
+                auto it = c->tune.find(tk);
+                if (it != c->tune.end()) {
+                    const int v = it->second.variant;
+                    if (v == 0 || eri_rows_variant(LX.lcls, LY.lcls, mode, c->n, prec == 1, v) == v) {
+                        variant = v;
+                        min_kets = it->second.min_kets;
+                    }
                 }
-            } else {
+            }
+            UNO_HIP(sp.d_prefix.upload(sp.prefix.data(), sp.prefix.size(), st));
+            if (!tasks) {
                 build_blocks(sp, qpb);
                 UNO_HIP(sp.d_blk.upload(sp.blk.data(), sp.blk.size(), st));
             }
@@ -420,13 +444,6 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
             a.pairQ = c->d_pairQ.p;
             a.prefix = sp.d_prefix.p;
             a.blk = tasks ? nullptr : sp.d_blk.p;
-            a.wpre = tasks ? sp.d_wpre.p : nullptr;
-            a.tblk = tasks ? sp.d_tblk.p : nullptr;
-            a.ntask = tasks ? sp.ntask : 0;
-            a.rbra = sp.nrtask > 0 ? sp.d_rbra.p : nullptr;
-            a.rpre = sp.nrtask > 0 ? sp.d_rpre.p : nullptr;
-            a.nrbra = (int)sp.rbra.size();
-            a.nrtask = tasks ? sp.nrtask : 0;
             a.rseg = kRowSeg;
             a.kstart = prec ? sp.d_kstart.p : nullptr;
             a.qpb = qpb;
@@ -448,7 +465,8 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
             auto launch = [&](int kind) -> int {
                 const int slot = kind * 2 * kNumPairClasses * kNumPairClasses + ci;
                 hipEvent_t e0 = nullptr, e1 = nullptr;
-                if (c->profile) {
+                const bool prof = c->profile && !c->tuning;
+                if (prof) {
                     a.counter = c->d_counter.p + 2 * kCounterSlots * slot;
                     UNO_HIP(hipEventCreate(&e0));
                     UNO_HIP(hipEventCreate(&e1));
@@ -459,13 +477,69 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
                 }
                 if (kind) UNO_HIP(eri_launch_rows(mode, LX.lcls, LY.lcls, a, kind, st, prec == 1));
                 else UNO_HIP(eri_launch(mode, LX.lcls, LY.lcls, a, st, prec == 1));
-                if (c->profile) UNO_HIP(hipEventRecord(e1, st));
+                if (prof) UNO_HIP(hipEventRecord(e1, st));
                 return UNO_OK;
             };
-            if (a.nrtask > 0)
-                if (int rc = launch(variant)) return rc;
-            if (!tasks || a.ntask > 0)
-                if (int rc = launch(0)) return rc;
+            // the space's launches under one configuration: bras with >= min kets row-merged
+            // with LDS variant v (v = 0: every bra as wave tasks)
+            auto run_config = [&](int v, long long mk) -> int {
+                if (tasks) {
+                    build_rows(sp, v ? mk : 0, kRowSeg);
+                    build_tasks(sp, v ? mk : 0);
+                    UNO_HIP(sp.d_wpre.upload(sp.wpre.data(), sp.wpre.size(), st));
+                    UNO_HIP(sp.d_tblk.upload(sp.tblk.data(), sp.tblk.size(), st));
+                    if (sp.nrtask > 0) {
+                        UNO_HIP(sp.d_rbra.upload(sp.rbra.data(), sp.rbra.size(), st));
+                        UNO_HIP(sp.d_rpre.upload(sp.rpre.data(), sp.rpre.size(), st));
+                    }
+                }
+                a.wpre = tasks ? sp.d_wpre.p : nullptr;
+                a.tblk = tasks ? sp.d_tblk.p : nullptr;
+                a.ntask = tasks ? sp.ntask : 0;
+                a.rbra = sp.nrtask > 0 ? sp.d_rbra.p : nullptr;
+                a.rpre = sp.nrtask > 0 ? sp.d_rpre.p : nullptr;
+                a.nrbra = (int)sp.rbra.size();
+                a.nrtask = tasks ? sp.nrtask : 0;
+                if (a.nrtask > 0)
+                    if (int rc = launch(v)) return rc;
+                if (!tasks || a.ntask > 0)
+                    if (int rc = launch(0)) return rc;
+                return UNO_OK;
+            };
+            if (c->tuning && tasks && c->row_merge) {
+                // time every candidate on this space (each one run twice, the faster counts)
+                std::vector<TuneChoice> cand{{0, 0}};
+                for (int v = 1; v <= 3; ++v)
+                    if (eri_rows_variant(LX.lcls, LY.lcls, mode, c->n, prec == 1, v) == v)
+                        for (long long mk : {64LL, 192LL, 512LL}) cand.push_back({v, mk});
+                TuneChoice best{variant, min_kets};
+                float best_ms = 1e30f;
+                hipEvent_t t0, t1;
+                UNO_HIP(hipEventCreate(&t0));
+                UNO_HIP(hipEventCreate(&t1));
+                for (const TuneChoice& tc : cand) {
+                    float ms_min = 1e30f;
+                    for (int rep = 0; rep < 2; ++rep) {
+                        UNO_HIP(hipEventRecord(t0, st));
+                        if (int rc = run_config(tc.variant, tc.min_kets)) return rc;
+                        UNO_HIP(hipEventRecord(t1, st));
+                        UNO_HIP(hipEventSynchronize(t1));
+                        float ms = 0.f;
+                        UNO_HIP(hipEventElapsedTime(&ms, t0, t1));
+                        ms_min = std::min(ms_min, ms);
+                    }
+                    if (ms_min < best_ms) {
+                        best_ms = ms_min;
+                        best = tc;
+                    }
+                }
+                (void)hipEventDestroy(t0);
+                (void)hipEventDestroy(t1);
+                c->tune[tk] = best;
+                c->tune_ms += best_ms;
+            } else {
+                if (int rc = run_config(variant, min_kets)) return rc;
+            }
             (prec ? total32 : total) += (double)sp.nq;
         }
     }

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <string>
+#include <map>
+#include <tuple>
 #include <vector>
 #include "../../../include/unomol.h"
 #include "../hip/devtypes.hpp"
@@ -80,6 +82,11 @@ struct QuartetSpace {
 constexpr long long kRowMinKets = 192;
 constexpr int kRowSeg = 1024;
 
+struct TuneChoice {
+    int variant;
+    long long min_kets;
+};
+
 struct QuartetClass {
     int X, Y;                   // indices into the runs, X >= Y (bra run, ket run)
     QuartetSpace s64;           // FP64 quartets (all of them unless the build is mixed)
@@ -136,6 +143,12 @@ struct uno_ctx {
     int row_merge = 1;
     long long row_min_kets = uno::kRowMinKets;
     int row_variant = 0;     // row-merged LDS variant: 0 the best that fits, 1-3 forced
+    // autotuned row-merge configuration per quartet space (run-pair index, precision, UHF,
+    // mixed build): LDS variant (0 = wave tasks only) and the ket count from which a bra is
+    // row-merged.  tuning = 1 while uno_autotune times the candidates.
+    std::map<std::tuple<int, int, int, int>, uno::TuneChoice> tune;
+    int tuning = 0;
+    double tune_ms = 0.0;
     double prof_ms[uno::kProfSlots] = {};
     double prof_prim[uno::kProfSlots] = {};
     double prof_shell[uno::kProfSlots] = {};
@@ -161,6 +174,8 @@ void build_tasks(QuartetSpace& s, long long row_min_kets = 0);
 void build_rows(QuartetSpace& s, long long min_kets, int seg);
 void build_blocks(QuartetSpace& s, int qpb);
 EriArgs base_args(uno_ctx* c);
+// keeps c's tuned configurations for later contexts of the same layout
+void remember_tuning(uno_ctx* c);
 // dense S, T, H = T + V (n x n, device) on the context's stream
 int oneint_device(uno_ctx* c, double* S, double* T, double* H);
 // Fock build from dense device densities already in c->D0 (and D1, D2 for UHF).

@@ -1,21 +1,21 @@
 """Short text summaries of rocprofv3 CSV output (``--output-format csv``) for the ERI kernels.
 
     python -m unomol_amd.profsum trace <dir>/run_kernel_stats.csv <dir>/run_kernel_trace.csv \
-        --kernel "<demangled name>" --first L
+        --kernel "<demangled name>" --last L
         per-kernel table of the whole run, then the named kernel's average duration over
-        its first L dispatches (with ``bench.py --warmup 0 --steps 1`` those are the timed
-        step's launches) and over all of its dispatches;
+        its last L dispatches (bench.py runs its timed steps last, so with ``--steps 1``
+        those are the timed step's launches) and over all of its dispatches;
 
     python -m unomol_amd.profsum pmc <dir>/run_counter_collection.csv --counter FETCH_SIZE \
-        --kernel "<demangled name>" --first L --copy-bytes 1073741824
-        the counter (KiB per dispatch) averaged over the kernel's first L dispatches, the
+        --kernel "<demangled name>" --last L --copy-bytes 1073741824
+        the counter (KiB per dispatch) averaged over the kernel's last L dispatches, the
         same counter read on the streaming-copy kernel ``k_copy_probe`` (which moves
         ``--copy-bytes`` each way), and the kernel's bytes per dispatch scaled by
         (copy bytes / copy reading).
 
     python -m unomol_amd.profsum flops <dir>/run_counter_collection.csv <bench.json>
         per ERI kernel of the bench's timed step: executed VALU FLOPs from
-        SQ_INSTS_VALU_FLOPS_FP64 / _FP32 (plus the _TRANS counters) over the kernel's first
+        SQ_INSTS_VALU_FLOPS_FP64 / _FP32 (plus the _TRANS counters) over the kernel's last
         L dispatches (L = the step's launches of that class) against the bench's
         algorithmic FLOPs for the same launches.
 
@@ -35,7 +35,7 @@ def _dispatches(rows, name_key, kernel):
     return out
 
 
-def trace(stats_csv, trace_csv, kernel, first, top=25):
+def trace(stats_csv, trace_csv, kernel, last, top=25):
     stats = list(csv.DictReader(open(stats_csv)))
     stats.sort(key=lambda r: -float(r["TotalDurationNs"]))
     print("| kernel | calls | total ms | avg us | % |")
@@ -50,24 +50,24 @@ def trace(stats_csv, trace_csv, kernel, first, top=25):
     print()
     print("kernel `%s`: %d dispatches in the run" % (kernel, len(dur)))
     if dur:
-        n = min(first or len(dur), len(dur))
-        print("- first %d dispatches: average %.4f ms, total %.3f ms" % (n, sum(dur[:n]) / n, sum(dur[:n])))
+        n = min(last or len(dur), len(dur))
+        print("- last %d dispatches: average %.4f ms, total %.3f ms" % (n, sum(dur[-n:]) / n, sum(dur[-n:])))
         print("- all dispatches: average %.4f ms, total %.3f ms" % (sum(dur) / len(dur), sum(dur)))
 
 
-def pmc(counter_csv, counter, kernel, first, copy_bytes):
+def pmc(counter_csv, counter, kernel, last, copy_bytes):
     rows = [r for r in csv.DictReader(open(counter_csv)) if r["Counter_Name"] == counter]
     kr = _dispatches(rows, "Kernel_Name", kernel)
     cr = [r for r in rows if COPY_KERNEL in r["Kernel_Name"]]
     if not kr or not cr:
         sys.exit("no %s dispatches of %s or of the copy kernel in %s" % (counter, kernel, counter_csv))
-    n = min(first or len(kr), len(kr))
-    kib = sum(float(r["Counter_Value"]) for r in kr[:n]) / n
+    n = min(last or len(kr), len(kr))
+    kib = sum(float(r["Counter_Value"]) for r in kr[-n:]) / n
     kib_all = sum(float(r["Counter_Value"]) for r in kr) / len(kr)
     copy_kib = sum(float(r["Counter_Value"]) for r in cr) / len(cr)
     scale = copy_bytes / (copy_kib * 1024.0)
     print("counter %s, kernel `%s`" % (counter, kernel))
-    print("- first %d dispatches: %.1f KiB per dispatch (all %d: %.1f)" % (n, kib, len(kr), kib_all))
+    print("- last %d dispatches: %.1f KiB per dispatch (all %d: %.1f)" % (n, kib, len(kr), kib_all))
     print("- copy kernel (%d bytes each way): %.1f KiB per dispatch -> scale %.4f" % (copy_bytes, copy_kib, scale))
     print("- kernel bytes per dispatch, scaled: %.4e" % (kib * 1024.0 * scale))
     print(json.dumps({"counter": counter, "kernel": kernel, "dispatches": n, "kib_per_dispatch": kib,
@@ -87,7 +87,7 @@ def flops(counter_csv, bench_json):
         f32 = c["precision"] == "f32"
         names = ("SQ_INSTS_VALU_FLOPS_FP32", "SQ_INSTS_VALU_FLOPS_FP32_TRANS") if f32 else \
                 ("SQ_INSTS_VALU_FLOPS_FP64", "SQ_INSTS_VALU_FLOPS_FP64_TRANS")
-        disp = sorted({int(r["Dispatch_Id"]) for r in rows})[:int(round(c["launches"]))]
+        disp = sorted({int(r["Dispatch_Id"]) for r in rows})[-int(round(c["launches"])):]
         keep = set(disp)
         cnt = sum(float(r["Counter_Value"]) for r in rows
                   if int(r["Dispatch_Id"]) in keep and r["Counter_Name"] in names)
@@ -108,23 +108,23 @@ def main(argv=None):
     t.add_argument("stats_csv")
     t.add_argument("trace_csv")
     t.add_argument("--kernel", default="")
-    t.add_argument("--first", type=int, default=0)
+    t.add_argument("--last", type=int, default=0)
     p = sub.add_parser("pmc")
     p.add_argument("counter_csv")
     p.add_argument("--counter", required=True)
     p.add_argument("--kernel", required=True)
-    p.add_argument("--first", type=int, default=0)
+    p.add_argument("--last", type=int, default=0)
     p.add_argument("--copy-bytes", type=float, default=float(1 << 30))
     f = sub.add_parser("flops")
     f.add_argument("counter_csv")
     f.add_argument("bench_json")
     a = ap.parse_args(argv)
     if a.cmd == "trace":
-        trace(a.stats_csv, a.trace_csv, a.kernel, a.first)
+        trace(a.stats_csv, a.trace_csv, a.kernel, a.last)
     elif a.cmd == "flops":
         flops(a.counter_csv, a.bench_json)
     else:
-        pmc(a.counter_csv, a.counter, a.kernel, a.first, a.copy_bytes)
+        pmc(a.counter_csv, a.counter, a.kernel, a.last, a.copy_bytes)
 
 
 if __name__ == "__main__":
