@@ -178,25 +178,28 @@ hipError_t k_absmax(long len, const double* x, double* out, hipStream_t st) {
 // A(16x4): lane l holds A[l%16][l/16]; B(4x16): lane l holds B[l/16][l%16];
 // D(16x16): lane l holds D[4*r + l/16][l%16] in register r (pinned by an on-device probe).
 // ---------------------------------------------------------------------------
+// Split K (gridDim.z > 1): block z covers k in [z kchunk, (z+1) kchunk) and adds
+// alpha * its partial product into C, which the launcher has already scaled by beta.
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, double alpha, const double* __restrict__ A,
                                               int lda, const double* __restrict__ B, int ldb, double beta,
-                                              double* __restrict__ C, int ldc) {
+                                              double* __restrict__ C, int ldc, int kchunk) {
     __shared__ double sA[16][64 + 2];
     __shared__ double sB[16][64 + 2];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int bi = blockIdx.y * 64, bj = blockIdx.x * 64;
     const int wi = (wave >> 1) * 32, wj = (wave & 1) * 32;
+    const int kb = blockIdx.z * kchunk, ke = kb + kchunk < K ? kb + kchunk : K;
     double4_t acc[2][2];
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) acc[a][b] = (double4_t){0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    for (int k0 = kb; k0 < ke; k0 += 16) {
         for (int e = tid; e < 1024; e += 256) {
             int i, kk;
             if (TA) { kk = e / 64; i = e % 64; } else { i = e / 16; kk = e % 16; }
             const int gi = bi + i, gk = k0 + kk;
             double v = 0.0;
-            if (gi < M && gk < K) v = TA ? A[(long)gk * lda + gi] : A[(long)gi * lda + gk];
+            if (gi < M && gk < ke) v = TA ? A[(long)gk * lda + gi] : A[(long)gi * lda + gk];
             sA[kk][i] = v;
         }
         for (int e = tid; e < 1024; e += 256) {
@@ -204,7 +207,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, double alpha,
             if (TB) { j = e / 16; kk = e % 16; } else { kk = e / 64; j = e % 64; }
             const int gj = bj + j, gk = k0 + kk;
             double v = 0.0;
-            if (gj < N && gk < K) v = TB ? B[(long)gj * ldb + gk] : B[(long)gk * ldb + gj];
+            if (gj < N && gk < ke) v = TB ? B[(long)gj * ldb + gk] : B[(long)gk * ldb + gj];
             sB[kk][j] = v;
         }
         __syncthreads();
@@ -234,18 +237,44 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, double alpha,
                 const int gj = bj + wj + tb * 16 + (lane & 15);
                 if (gi < M && gj < N) {
                     double* c = C + (long)gi * ldc + gj;
-                    *c = (beta == 0.0 ? 0.0 : beta * *c) + alpha * acc[ta][tb][r];
+                    if (gridDim.z > 1) unsafeAtomicAdd(c, alpha * acc[ta][tb][r]);
+                    else *c = (beta == 0.0 ? 0.0 : beta * *c) + alpha * acc[ta][tb][r];
                 }
             }
 }
 
+// C <- beta C on an M x N region (beta = 0 writes zeros)
+__global__ void k_scale_c(int M, int N, double beta, double* C, int ldc) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)M * N) return;
+    double* c = C + (t / N) * ldc + t % N;
+    *c = beta == 0.0 ? 0.0 : beta * *c;
+}
+
 hipError_t gemm_f64(bool ta, bool tb, int m, int n, int k, double alpha, const double* A, int lda, const double* B,
                     int ldb, double beta, double* C, int ldc, hipStream_t st) {
-    const dim3 grid((unsigned)((n + 63) / 64), (unsigned)((m + 63) / 64)), blk(256);
-    if (!ta && !tb) hipLaunchKernelGGL((k_gemm<false, false>), grid, blk, 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-    else if (ta && !tb) hipLaunchKernelGGL((k_gemm<true, false>), grid, blk, 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-    else if (!ta && tb) hipLaunchKernelGGL((k_gemm<false, true>), grid, blk, 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-    else hipLaunchKernelGGL((k_gemm<true, true>), grid, blk, 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    // few output tiles and a long K (the WY back-transform's 32-row products): split K so
+    // that the launch has about 512 blocks, each chunk at least 64 long
+    const long tiles = (long)((n + 63) / 64) * ((m + 63) / 64);
+    int ksplit = 1, kchunk = k;
+    if (tiles < 128 && k >= 256) {
+        ksplit = (int)std::min<long>(512 / tiles, k / 64);
+        if (ksplit > 1) {
+            kchunk = ((k + ksplit - 1) / ksplit + 15) / 16 * 16;
+            ksplit = (k + kchunk - 1) / kchunk;
+        } else {
+            ksplit = 1;
+        }
+    }
+    if (ksplit > 1) {
+        const long mn = (long)m * n;
+        hipLaunchKernelGGL(k_scale_c, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st, m, n, beta, C, ldc);
+    }
+    const dim3 grid((unsigned)((n + 63) / 64), (unsigned)((m + 63) / 64), (unsigned)ksplit), blk(256);
+    if (!ta && !tb) hipLaunchKernelGGL((k_gemm<false, false>), grid, blk, 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kchunk);
+    else if (ta && !tb) hipLaunchKernelGGL((k_gemm<true, false>), grid, blk, 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kchunk);
+    else if (!ta && tb) hipLaunchKernelGGL((k_gemm<false, true>), grid, blk, 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kchunk);
+    else hipLaunchKernelGGL((k_gemm<true, true>), grid, blk, 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kchunk);
     return hipGetLastError();
 }
 
