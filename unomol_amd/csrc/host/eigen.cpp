@@ -23,27 +23,12 @@ struct EigenWS {
     DBuf<int> clusters;
     std::vector<double> hev;
     std::vector<int> hcl;
-    // the tridiagonalisation's 2(n-2) launches, captured once per size as a HIP graph on
-    // this work space's buffers and replayed for every later matrix of that size
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int graph_n = 0;
-    bool use_graph = true;   // one-shot work spaces launch directly
-    ~EigenWS() { drop_graph(); }
-    void drop_graph() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
-        graph_n = 0;
-    }
     hipError_t ensure(int nn) {
         if (nn <= n && Q.p) return hipSuccess;
-        drop_graph();   // its kernels point at the buffers being replaced
         n = nn;
         const size_t sq = (size_t)nn * nn;
         hipError_t r;
-        if ((r = A.alloc(sq)) || (r = Q.alloc(sq)) || (r = V.alloc(sq)) || (r = tau.alloc(nn)) || (r = p.alloc(nn)) ||
+        if ((r = Q.alloc(sq)) || (r = V.alloc(sq)) || (r = tau.alloc(nn)) || (r = p.alloc(nn)) ||
             (r = d.alloc(nn)) ||
             (r = e.alloc(nn)) || (r = scal.alloc(4)) || (r = evals.alloc(nn)) || (r = Y.alloc(sq)) ||
             (r = wk.alloc(5 * sq)) || (r = wyS.alloc(32 * 32)) || (r = wyT.alloc(32 * 32)) ||
@@ -81,29 +66,9 @@ static int eigh_impl(EigenWS& w, int n, double* dA, double* dZ, double* h_evals,
         UNO_HIP(hipStreamSynchronize(st));
         return UNO_OK;
     }
-    // the matrix goes to the work space's own buffer so the captured launches stay valid
-    UNO_HIP(hipMemcpyAsync(w.A.p, dA, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, st));
-    EigWork ew{w.A.p, w.Q.p, w.V.p, w.tau.p, w.p.p, w.d.p, w.e.p, w.scal.p, w.evals.p, w.Y.p, w.wk.p,
+    EigWork ew{dA, w.Q.p, w.V.p, w.tau.p, w.p.p, w.d.p, w.e.p, w.scal.p, w.evals.p, w.Y.p, w.wk.p,
                w.wyS.p, w.wyT.p, w.wy1.p, w.wy2.p};
-    if (!w.use_graph) {
-        UNO_HIP(householder_tridiag(n, ew, st));
-    } else {
-        if (w.graph_n != n) {
-            w.drop_graph();
-            UNO_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            const hipError_t ce = householder_tridiag(n, ew, st);
-            hipGraph_t g = nullptr;
-            const hipError_t ee = hipStreamEndCapture(st, &g);
-            if (ce != hipSuccess || ee != hipSuccess) {
-                if (g) (void)hipGraphDestroy(g);
-                UNO_HIP(ce != hipSuccess ? ce : ee);
-            }
-            w.graph = g;
-            UNO_HIP(hipGraphInstantiate(&w.exec, w.graph, nullptr, nullptr, 0));
-            w.graph_n = n;
-        }
-        UNO_HIP(hipGraphLaunch(w.exec, st));
-    }
+    UNO_HIP(householder_tridiag(n, ew, st));
     UNO_HIP(tridiag_eigen(n, ew, -1, nullptr, st));
     w.hev.resize(n);
     UNO_HIP(hipMemcpyAsync(w.hev.data(), w.evals.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -138,7 +103,6 @@ int eigh_device(uno_ctx* c, int n, double* dA, double* dZ, double* h_evals) {
 
 int eigh_standalone(int n, double* dA, double* dZ, double* h_evals, hipStream_t st) {
     EigenWS w;
-    w.use_graph = false;
     const int rc = eigh_impl(w, n, dA, dZ, h_evals, st);
     (void)hipStreamSynchronize(st);
     return rc;
