@@ -5,6 +5,7 @@
 // build: every iteration recomputes the significant shell quartets on the GPU and
 // contracts them with the density on the fly.
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <numeric>
 #include "engine.hpp"
@@ -38,19 +39,27 @@ void build_prefix(uno_ctx* c, QuartetClass& q, double dmax, double fp32_thr) {
     q.s64.prefix.assign(nb + 1, 0);
     q.s32.prefix.assign(mixed ? nb + 1 : 1, 0);
     q.s32.kstart.assign(mixed ? nb : 0, 0);
+    // both runs are sorted by bound, descending: the thresholds grow with the bra index, so
+    // the counts of kets at or above them only shrink, and two pointers walk the ket run
+    // once (the same counts as a binary search per bra)
+    const long long nk = (long long)Lk.Q.size();
+    long long p_all = nk, p32 = nk;
     for (int i = 0; i < nb; ++i) {
         long long cnt = 0, c64 = 0;
         const double qd = Lb.Q[i] * dmax;
         if (qd > 0.0) {
             const double thr = c->tau / qd;
-            cnt = std::partition_point(Lk.Q.begin(), Lk.Q.end(), [&](double v) { return v >= thr; }) - Lk.Q.begin();
+            while (p_all > 0 && !(Lk.Q[p_all - 1] >= thr)) --p_all;
+            cnt = p_all;
             if (q.X == q.Y) cnt = std::min<long long>(cnt, i + 1);
+        } else {
+            p_all = 0;
         }
         c64 = cnt;
         if (mixed && cnt > 0) {
             const double t32 = fp32_thr / Lb.Q[i];
-            c64 = std::partition_point(Lk.Q.begin(), Lk.Q.begin() + cnt, [&](double v) { return v >= t32; }) -
-                  Lk.Q.begin();
+            while (p32 > 0 && !(Lk.Q[p32 - 1] >= t32)) --p32;
+            c64 = std::min(p32, cnt);
         }
         q.s64.prefix[i + 1] = q.s64.prefix[i] + c64;
         if (mixed) {
@@ -406,24 +415,42 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
         UNO_HIP(c->d_counter.alloc(2 * kCounterSlots * kProfSlots));
         UNO_HIP(hipMemsetAsync(c->d_counter.p, 0, 2 * kCounterSlots * kProfSlots * sizeof(unsigned long long), st));
     }
+    const int mode = uhf ? kDigestUHF : kDigestRHF;
+
+    // Pass 1 (host): every non-empty space's configuration and index arrays, appended to one
+    // staging buffer (uploaded once per build; the tuner uploads per candidate instead)
+    struct SpaceRec {
+        QuartetClass* q;
+        int prec, variant;
+        bool tasks;
+        size_t o_prefix, o_wpre, o_tblk, o_rbra, o_rpre, o_blk, o_kstart;
+    };
+    std::vector<SpaceRec> recs;
+    std::vector<char>& hs = c->stage_h;
+    hs.clear();
+    auto put = [&](const auto& v) -> size_t {
+        const size_t off = (hs.size() + 15) / 16 * 16;
+        const size_t bytes = v.size() * sizeof(v[0]);
+        hs.resize(off + bytes);
+        if (bytes) std::memcpy(hs.data() + off, v.data(), bytes);
+        return off;
+    };
     for (QuartetClass& q : c->qcls) {
         build_prefix(c, q, 4.0 * dmax, mixed ? c->fp32_thr : 0.0);
         const PairClassList& LX = c->cls[q.X];
         const PairClassList& LY = c->cls[q.Y];
-        const int qpb = eri_quartets_per_block(LX.lcls, LY.lcls);
+        const int qi = (int)(&q - c->qcls.data());
         for (int prec = 0; prec < 2; ++prec) {
             QuartetSpace& sp = prec ? q.s32 : q.s64;
             if (sp.nq == 0) continue;
-            const bool tasks = eri_uses_wave_tasks(LX.lcls, LY.lcls);
-            const int mode = uhf ? kDigestUHF : kDigestRHF;
-            const int qi = (int)(&q - c->qcls.data());
+            SpaceRec r{&q, prec, 0, eri_uses_wave_tasks(LX.lcls, LY.lcls), 0, 0, 0, 0, 0, 0, 0};
             // row-merge configuration: the tuned one for this space if any, else the defaults
-            int variant =
-                tasks && c->row_merge ? eri_rows_variant(LX.lcls, LY.lcls, mode, c->n, prec == 1, c->row_variant) : 0;
+            int variant = r.tasks && c->row_merge
+                              ? eri_rows_variant(LX.lcls, LY.lcls, mode, c->n, prec == 1, c->row_variant)
+                              : 0;
             long long min_kets = std::max(1LL, c->row_min_kets);
-            const auto tk = std::make_tuple(qi, prec, uhf ? 1 : 0, mixed ? 1 : 0);
-            if (tasks && c->row_merge && !c->tuning) {
-                auto it = c->tune.find(tk);
+            if (r.tasks && c->row_merge && !c->tuning) {
+                auto it = c->tune.find(std::make_tuple(qi, prec, uhf ? 1 : 0, mixed ? 1 : 0));
                 if (it != c->tune.end()) {
                     const int v = it->second.variant;
                     if (v == 0 || eri_rows_variant(LX.lcls, LY.lcls, mode, c->n, prec == 1, v) == v) {
@@ -432,116 +459,166 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
                     }
                 }
             }
-            UNO_HIP(sp.d_prefix.upload(sp.prefix.data(), sp.prefix.size(), st));
-            if (!tasks) {
-                build_blocks(sp, qpb);
-                UNO_HIP(sp.d_blk.upload(sp.blk.data(), sp.blk.size(), st));
+            r.variant = variant;
+            if (r.tasks) {
+                build_rows(sp, variant ? min_kets : 0, kRowSeg);
+                build_tasks(sp, variant ? min_kets : 0);
+            } else {
+                build_blocks(sp, eri_quartets_per_block(LX.lcls, LY.lcls));
             }
-            if (prec) UNO_HIP(sp.d_kstart.upload(sp.kstart.data(), sp.kstart.size(), st));
-            EriArgs a = base_args(c);
-            a.bra_off = LX.off;
-            a.ket_off = LY.off;
-            a.pairQ = c->d_pairQ.p;
-            a.prefix = sp.d_prefix.p;
-            a.blk = tasks ? nullptr : sp.d_blk.p;
-            a.rseg = kRowSeg;
-            a.kstart = prec ? sp.d_kstart.p : nullptr;
-            a.qpb = qpb;
-            a.nbra = LX.count;
-            a.nquart = sp.nq;
-            a.same_class = q.X == q.Y;
-            a.dsh = c->Dsh.p;
-            a.jp = c->JP.p;
-            a.dmax_global = 4.0 * dmax;
-            a.D0 = c->D0.p;
-            a.W0 = c->W0.p;
-            if (uhf) {
-                a.D1 = c->D1.p;
-                a.D2 = c->D2.p;
-                a.W1 = c->W1.p;
-            }
-            const int ci = prec * kNumPairClasses * kNumPairClasses + LX.lcls * kNumPairClasses + LY.lcls;
-            // one launch per kernel kind, each with its own profiling slot
-            auto launch = [&](int kind) -> int {
-                const int slot = kind * 2 * kNumPairClasses * kNumPairClasses + ci;
-                hipEvent_t e0 = nullptr, e1 = nullptr;
-                const bool prof = c->profile && !c->tuning;
-                if (prof) {
-                    a.counter = c->d_counter.p + 2 * kCounterSlots * slot;
-                    UNO_HIP(hipEventCreate(&e0));
-                    UNO_HIP(hipEventCreate(&e1));
-                    ev0.push_back(e0);
-                    ev1.push_back(e1);
-                    evcls.push_back(slot);
-                    UNO_HIP(hipEventRecord(e0, st));
+            if (!c->tuning) {
+                r.o_prefix = put(sp.prefix);
+                if (r.tasks) {
+                    r.o_wpre = put(sp.wpre);
+                    r.o_tblk = put(sp.tblk);
+                    r.o_rbra = put(sp.rbra);
+                    r.o_rpre = put(sp.rpre);
+                } else {
+                    r.o_blk = put(sp.blk);
                 }
-                if (kind) UNO_HIP(eri_launch_rows(mode, LX.lcls, LY.lcls, a, kind, st, prec == 1));
-                else UNO_HIP(eri_launch(mode, LX.lcls, LY.lcls, a, st, prec == 1));
-                if (prof) UNO_HIP(hipEventRecord(e1, st));
-                return UNO_OK;
-            };
-            // the space's launches under one configuration: bras with >= min kets row-merged
-            // with LDS variant v (v = 0: every bra as wave tasks)
-            auto run_config = [&](int v, long long mk) -> int {
+                if (prec) r.o_kstart = put(sp.kstart);
+            }
+            recs.push_back(r);
+        }
+    }
+    char* ds = nullptr;
+    if (!c->tuning && !hs.empty()) {
+        UNO_HIP(c->stage_d.alloc(hs.size()));
+        UNO_HIP(hipMemcpyAsync(c->stage_d.p, hs.data(), hs.size(), hipMemcpyHostToDevice, st));
+        ds = c->stage_d.p;
+    }
+
+    // Pass 2: the launches
+    for (const SpaceRec& r : recs) {
+        QuartetClass& q = *r.q;
+        const int prec = r.prec;
+        QuartetSpace& sp = prec ? q.s32 : q.s64;
+        const PairClassList& LX = c->cls[q.X];
+        const PairClassList& LY = c->cls[q.Y];
+        const int qi = (int)(&q - c->qcls.data());
+        const bool tasks = r.tasks;
+        EriArgs a = base_args(c);
+        a.bra_off = LX.off;
+        a.ket_off = LY.off;
+        a.pairQ = c->d_pairQ.p;
+        a.rseg = kRowSeg;
+        a.qpb = eri_quartets_per_block(LX.lcls, LY.lcls);
+        a.nbra = LX.count;
+        a.nquart = sp.nq;
+        a.same_class = q.X == q.Y;
+        a.dsh = c->Dsh.p;
+        a.jp = c->JP.p;
+        a.dmax_global = 4.0 * dmax;
+        a.D0 = c->D0.p;
+        a.W0 = c->W0.p;
+        if (uhf) {
+            a.D1 = c->D1.p;
+            a.D2 = c->D2.p;
+            a.W1 = c->W1.p;
+        }
+        // device index arrays: the staging buffer's slices, or (tuning) the space's own
+        auto set_index = [&]() -> int {
+            if (ds) {
+                a.prefix = reinterpret_cast<const long long*>(ds + r.o_prefix);
+                a.blk = tasks ? nullptr : reinterpret_cast<const int*>(ds + r.o_blk);
+                a.wpre = tasks ? reinterpret_cast<const long long*>(ds + r.o_wpre) : nullptr;
+                a.tblk = tasks ? reinterpret_cast<const int*>(ds + r.o_tblk) : nullptr;
+                a.rbra = tasks && sp.nrtask > 0 ? reinterpret_cast<const int*>(ds + r.o_rbra) : nullptr;
+                a.rpre = tasks && sp.nrtask > 0 ? reinterpret_cast<const long long*>(ds + r.o_rpre) : nullptr;
+                a.kstart = prec ? reinterpret_cast<const int*>(ds + r.o_kstart) : nullptr;
+            } else {
+                UNO_HIP(sp.d_prefix.upload(sp.prefix.data(), sp.prefix.size(), st));
                 if (tasks) {
-                    build_rows(sp, v ? mk : 0, kRowSeg);
-                    build_tasks(sp, v ? mk : 0);
                     UNO_HIP(sp.d_wpre.upload(sp.wpre.data(), sp.wpre.size(), st));
                     UNO_HIP(sp.d_tblk.upload(sp.tblk.data(), sp.tblk.size(), st));
                     if (sp.nrtask > 0) {
                         UNO_HIP(sp.d_rbra.upload(sp.rbra.data(), sp.rbra.size(), st));
                         UNO_HIP(sp.d_rpre.upload(sp.rpre.data(), sp.rpre.size(), st));
                     }
+                } else {
+                    UNO_HIP(sp.d_blk.upload(sp.blk.data(), sp.blk.size(), st));
                 }
+                if (prec) UNO_HIP(sp.d_kstart.upload(sp.kstart.data(), sp.kstart.size(), st));
+                a.prefix = sp.d_prefix.p;
+                a.blk = tasks ? nullptr : sp.d_blk.p;
                 a.wpre = tasks ? sp.d_wpre.p : nullptr;
                 a.tblk = tasks ? sp.d_tblk.p : nullptr;
-                a.ntask = tasks ? sp.ntask : 0;
-                a.rbra = sp.nrtask > 0 ? sp.d_rbra.p : nullptr;
-                a.rpre = sp.nrtask > 0 ? sp.d_rpre.p : nullptr;
-                a.nrbra = (int)sp.rbra.size();
-                a.nrtask = tasks ? sp.nrtask : 0;
-                if (a.nrtask > 0)
-                    if (int rc = launch(v)) return rc;
-                if (!tasks || a.ntask > 0)
-                    if (int rc = launch(0)) return rc;
-                return UNO_OK;
-            };
-            if (c->tuning && tasks && c->row_merge) {
-                // time every candidate on this space (each one run twice, the faster counts)
-                std::vector<TuneChoice> cand{{0, 0}};
-                for (int v = 1; v <= 3; ++v)
-                    if (eri_rows_variant(LX.lcls, LY.lcls, mode, c->n, prec == 1, v) == v)
-                        for (long long mk : {64LL, 192LL, 512LL}) cand.push_back({v, mk});
-                TuneChoice best{variant, min_kets};
-                float best_ms = 1e30f;
-                hipEvent_t t0, t1;
-                UNO_HIP(hipEventCreate(&t0));
-                UNO_HIP(hipEventCreate(&t1));
-                for (const TuneChoice& tc : cand) {
-                    float ms_min = 1e30f;
-                    for (int rep = 0; rep < 2; ++rep) {
-                        UNO_HIP(hipEventRecord(t0, st));
-                        if (int rc = run_config(tc.variant, tc.min_kets)) return rc;
-                        UNO_HIP(hipEventRecord(t1, st));
-                        UNO_HIP(hipEventSynchronize(t1));
-                        float ms = 0.f;
-                        UNO_HIP(hipEventElapsedTime(&ms, t0, t1));
-                        ms_min = std::min(ms_min, ms);
-                    }
-                    if (ms_min < best_ms) {
-                        best_ms = ms_min;
-                        best = tc;
-                    }
-                }
-                (void)hipEventDestroy(t0);
-                (void)hipEventDestroy(t1);
-                c->tune[tk] = best;
-                c->tune_ms += best_ms;
-            } else {
-                if (int rc = run_config(variant, min_kets)) return rc;
+                a.rbra = tasks && sp.nrtask > 0 ? sp.d_rbra.p : nullptr;
+                a.rpre = tasks && sp.nrtask > 0 ? sp.d_rpre.p : nullptr;
+                a.kstart = prec ? sp.d_kstart.p : nullptr;
             }
-            (prec ? total32 : total) += (double)sp.nq;
+            a.ntask = tasks ? sp.ntask : 0;
+            a.nrbra = (int)sp.rbra.size();
+            a.nrtask = tasks ? sp.nrtask : 0;
+            return UNO_OK;
+        };
+        const int ci = prec * kNumPairClasses * kNumPairClasses + LX.lcls * kNumPairClasses + LY.lcls;
+        // one launch per kernel kind, each with its own profiling slot
+        auto launch = [&](int kind) -> int {
+            const int slot = kind * 2 * kNumPairClasses * kNumPairClasses + ci;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            const bool prof = c->profile && !c->tuning;
+            if (prof) {
+                a.counter = c->d_counter.p + 2 * kCounterSlots * slot;
+                UNO_HIP(hipEventCreate(&e0));
+                UNO_HIP(hipEventCreate(&e1));
+                ev0.push_back(e0);
+                ev1.push_back(e1);
+                evcls.push_back(slot);
+                UNO_HIP(hipEventRecord(e0, st));
+            }
+            if (kind) UNO_HIP(eri_launch_rows(mode, LX.lcls, LY.lcls, a, kind, st, prec == 1));
+            else UNO_HIP(eri_launch(mode, LX.lcls, LY.lcls, a, st, prec == 1));
+            if (prof) UNO_HIP(hipEventRecord(e1, st));
+            return UNO_OK;
+        };
+        // the space's launches: row-merged bras with LDS variant v, then the wave tasks
+        auto run = [&](int v) -> int {
+            if (a.nrtask > 0)
+                if (int rc = launch(v)) return rc;
+            if (!tasks || a.ntask > 0)
+                if (int rc = launch(0)) return rc;
+            return UNO_OK;
+        };
+        if (c->tuning && tasks && c->row_merge) {
+            // time every candidate on this space (each one run twice, the faster counts)
+            std::vector<TuneChoice> cand{{0, 0}};
+            for (int v = 1; v <= 3; ++v)
+                if (eri_rows_variant(LX.lcls, LY.lcls, mode, c->n, prec == 1, v) == v)
+                    for (long long mk : {64LL, 192LL, 512LL}) cand.push_back({v, mk});
+            TuneChoice best{r.variant, std::max(1LL, c->row_min_kets)};
+            float best_ms = 1e30f;
+            hipEvent_t t0, t1;
+            UNO_HIP(hipEventCreate(&t0));
+            UNO_HIP(hipEventCreate(&t1));
+            for (const TuneChoice& tc : cand) {
+                build_rows(sp, tc.variant ? tc.min_kets : 0, kRowSeg);
+                build_tasks(sp, tc.variant ? tc.min_kets : 0);
+                if (int rc = set_index()) return rc;
+                float ms_min = 1e30f;
+                for (int rep = 0; rep < 2; ++rep) {
+                    UNO_HIP(hipEventRecord(t0, st));
+                    if (int rc = run(tc.variant)) return rc;
+                    UNO_HIP(hipEventRecord(t1, st));
+                    UNO_HIP(hipEventSynchronize(t1));
+                    float ms = 0.f;
+                    UNO_HIP(hipEventElapsedTime(&ms, t0, t1));
+                    ms_min = std::min(ms_min, ms);
+                }
+                if (ms_min < best_ms) {
+                    best_ms = ms_min;
+                    best = tc;
+                }
+            }
+            (void)hipEventDestroy(t0);
+            (void)hipEventDestroy(t1);
+            c->tune[std::make_tuple(qi, prec, uhf ? 1 : 0, mixed ? 1 : 0)] = best;
+            c->tune_ms += best_ms;
+        } else {
+            if (int rc = set_index()) return rc;
+            if (int rc = run(r.variant)) return rc;
         }
+        (prec ? total32 : total) += (double)sp.nq;
     }
     // the kets' Coulomb outputs, accumulated per pair, into W
     UNO_HIP(jp_scatter((int)c->pairs.size(), c->d_pairs.p, c->njp, c->JP.p, c->W0.p, uhf ? c->W1.p : nullptr, c->n, st));
@@ -565,6 +642,8 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
                 c->prof_shell[ci] += (double)cnt[2 * (kCounterSlots * ci + sl) + 1];
             }
     }
+    // the staging buffer is pageable host memory: its copy must be done before it is reused
+    if (ds) UNO_HIP(hipStreamSynchronize(st));
     return UNO_OK;
 }
 

@@ -133,6 +133,8 @@ struct uno_ctx {
     uno::DBuf<double> D0, D1, D2, W0, W1, Dsh, pk, scal;
     uno::DBuf<double> JP;        // pair-major accumulator of the kets' Coulomb outputs
     uno::DBuf<double> red;       // partial sums of the two-pass reductions
+    std::vector<char> stage_h;   // a Fock build's index arrays, uploaded once per build
+    uno::DBuf<char> stage_d;
     long long njp = 0;
     uno::EigenWS* eig = nullptr;
     double last_quartets = 0.0, last_quartets32 = 0.0;
