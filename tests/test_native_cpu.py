@@ -91,43 +91,81 @@ def _read_inc_array(text, name, dtype):
     return np.array(vals, dtype=dtype)
 
 
-def test_fp32_rys_table_vs_mpmath(golden):
-    """The single-precision Rys table of the FP32 quartets, evaluated in float32 as the
-    kernels do (Clenshaw on the degree-6 series of each unit interval, Gauss-Hermite
-    limit beyond), against 40-digit mpmath roots and weights."""
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_rys_power_tables_vs_mpmath(golden, prec):
+    """The Rys tables the kernels evaluate (rys_mono.inc: the Chebyshev series of each unit
+    interval re-expanded in powers of s = 2(x-k)-1), evaluated with Horner in the kernel's
+    precision (Gauss-Hermite limit beyond X_n), against 40-digit mpmath roots and weights.
+    Double: within 3e-15 relative (numpy rounds the product and the sum apart; the kernels
+    fuse them).  Float: a few ulp on top of the 1.7e-8 truncation of the degree-6 series."""
     src = os.path.join(ROOT, "unomol_amd", "csrc")
     t64 = open(os.path.join(src, "rys_table.inc")).read()
     t32 = open(os.path.join(src, "rys_table_f32.inc")).read()
+    mono = open(os.path.join(src, "rys_mono.inc")).read()
     xasym = [int(v) for v in re.search(r"uno_rys_xasym\[\d+\] = \{(.*?)\}", t64).group(1).split(",")]
-    off = [int(v) for v in re.search(r"uno_rys_off_f\[\d+\] = \{(.*?)\}", t32).group(1).split(",")]
-    nc = int(re.search(r"#define UNO_RYS_NCOEF_F (\d+)", t32).group(1))
-    coef = _read_inc_array(t32, "uno_rys_coef_f", np.float32)
-    au = _read_inc_array(t32, "uno_rys_asym_u_f", np.float32)
-    aw = _read_inc_array(t32, "uno_rys_asym_w_f", np.float32)
+    if prec == "f64":
+        ft = np.float64
+        off = [int(v) for v in re.search(r"uno_rys_off\[\d+\] = \{(.*?)\}", t64).group(1).split(",")]
+        nc = int(re.search(r"#define UNO_RYS_NCOEF (\d+)", t64).group(1))
+        coef = _read_inc_array(mono, "uno_rys_mono", ft)
+        au = _read_inc_array(t64, "uno_rys_asym_u", ft)
+        aw = _read_inc_array(t64, "uno_rys_asym_w", ft)
+    else:
+        ft = np.float32
+        off = [int(v) for v in re.search(r"uno_rys_off_f\[\d+\] = \{(.*?)\}", t32).group(1).split(",")]
+        nc = int(re.search(r"#define UNO_RYS_NCOEF_F (\d+)", t32).group(1))
+        coef = _read_inc_array(mono, "uno_rys_mono_f", ft)
+        au = _read_inc_array(t32, "uno_rys_asym_u_f", ft)
+        aw = _read_inc_array(t32, "uno_rys_asym_w_f", ft)
     z = np.load(os.path.join(golden, "rys_mp.npz"))
-    f32 = np.float32
     for n in range(1, 6):
         for i, x64 in enumerate(z["x"]):
-            x = f32(x64)
+            x = ft(x64)
             if x >= xasym[n]:
                 o = n * (n - 1) // 2
                 u = au[o:o + n] / x
                 w = aw[o:o + n] / np.sqrt(x)
             else:
                 k = int(x)
-                s = f32(2) * (x - f32(k)) - f32(1)
+                s = ft(2) * (x - ft(k)) - ft(1)
                 vals = []
                 for fi in range(2 * n):
                     c = coef[off[n] + (k * 2 * n + fi) * nc: off[n] + (k * 2 * n + fi + 1) * nc]
-                    b1, b2 = f32(0), f32(0)
-                    for m in range(nc - 1, 0, -1):
-                        b1, b2 = f32(2) * s * b1 + (c[m] - b2), b1
-                    vals.append(s * b1 + (c[0] - b2))
+                    v = c[nc - 1]
+                    for m in range(nc - 2, -1, -1):
+                        v = v * s + c[m]
+                    vals.append(v)
                 u, w = np.array(vals[:n]), np.array(vals[n:])
-            # float32 evaluation: a few ulp on top of the 1.7e-8 truncation bound; the
-            # mpmath reference is taken at x rounded to float like the argument
-            np.testing.assert_allclose(u, z["u"][n, i, :n], rtol=2e-6 if x64 != float(x) else 1e-6)
-            np.testing.assert_allclose(w, z["w"][n, i, :n], rtol=2e-6 if x64 != float(x) else 1e-6)
+            if prec == "f64":
+                rtol = 3e-15
+            else:
+                # the mpmath reference is taken at x rounded to float like the argument
+                rtol = 2e-6 if x64 != float(x) else 1e-6
+            np.testing.assert_allclose(u, z["u"][n, i, :n], rtol=rtol)
+            np.testing.assert_allclose(w, z["w"][n, i, :n], rtol=rtol)
+
+
+def test_rys_power_tables_are_chebyshev_re_expansion():
+    """rys_mono.inc holds the Chebyshev series of rys_table.inc / rys_table_f32.inc in the
+    power basis: at s = -1, 0, 1 the power sums are sum a_k (-1)^k, a_0 and sum a_k, and the
+    Chebyshev sums sum c_m (-1)^m, sum c_m T_m(0) and sum c_m; they agree to rounding for
+    every interval and function (a dropped or misplaced coefficient fails this)."""
+    src = os.path.join(ROOT, "unomol_amd", "csrc")
+    mono = open(os.path.join(src, "rys_mono.inc")).read()
+    for tab, name, mname, ncn in [("rys_table.inc", "uno_rys_coef", "uno_rys_mono", "UNO_RYS_NCOEF"),
+                                  ("rys_table_f32.inc", "uno_rys_coef_f", "uno_rys_mono_f", "UNO_RYS_NCOEF_F")]:
+        t = open(os.path.join(src, tab)).read()
+        nc = int(re.search(r"#define %s (\d+)" % ncn, t).group(1))
+        c = _read_inc_array(t, name, np.float64).reshape(-1, nc)
+        a = _read_inc_array(mono, mname, np.float64).reshape(-1, nc)
+        assert c.shape == a.shape
+        m = np.arange(nc)
+        t0 = np.cos(m * np.pi / 2)   # T_m(0)
+        t0[np.abs(t0) < 0.5] = 0.0
+        scale = np.abs(c).sum(axis=1)
+        for cheb, power in [(c @ ((-1.0) ** m), a @ ((-1.0) ** m)), (c @ t0, a[:, 0]), (c.sum(axis=1), a.sum(axis=1))]:
+            tol = 4e-15 if name == "uno_rys_coef" else 4e-7
+            assert np.all(np.abs(cheb - power) <= tol * scale)
 
 
 def _rot(seed):

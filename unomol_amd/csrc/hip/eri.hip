@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Fock builds, bras with many kets: one block per (bra, segment of rseg kets), 256 kets
+// Fock builds, bras with many kets: one block per (bra, segment of rseg kets), kRowThreads kets
 // per pass.  The exchange contributions all land in the bra's rows of W (rows fa.., fb..),
 // so the block merges them in LDS (one row of n doubles per bra function and spin) and
 // adds the non-zero entries at the end with row-contiguous atomics; the bra's Coulomb
@@ -348,16 +348,21 @@ __host__ __device__ constexpr int row_count() {
     return (ncart_c(LA) + (FB ? ncart_c(LB) : 0)) * (MODE == kDigestUHF ? 2 : 1);
 }
 
+// threads of a row-merged block (512 was measured slower: fewer blocks for the bras with
+// few kets, and the same per-wave time)
+constexpr int kRowThreads = 256;
+constexpr int kRowWaves = kRowThreads / 64;
+
 template <int LA, int LB, int LC, int LD, int MODE, typename T, bool TL, bool FB>
-__global__ __launch_bounds__(256) void k_eri_rows(const EriArgs a) {
+__global__ __launch_bounds__(kRowThreads) void k_eri_rows(const EriArgs a) {
     using CI = ClassInfo<LA, LB, LC, LD>;
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND, NCOMP = CI::NCOMP, NR = CI::NR;
     constexpr int NAB = NA + (FB ? NB : 0), NROW = row_count<LA, LB, MODE, FB>();
     constexpr int NSP = MODE == kDigestUHF ? 2 : 1;
     extern __shared__ double rows[];
     __shared__ T tab_lds[TL ? rys_table_size<NR, T>() : 1];
-    __shared__ double jred[4][NA * NB];
-    __shared__ unsigned long long cred[4][2];
+    __shared__ double jred[kRowWaves][NA * NB];
+    __shared__ unsigned long long cred[kRowWaves][2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // the block's bra (uniform): largest r with rpre[r] <= blockIdx.x
     const long long b = blockIdx.x;
@@ -520,14 +525,21 @@ __global__ __launch_bounds__(256) void k_eri_rows(const EriArgs a) {
     __syncthreads();
     if (threadIdx.x < NA * NB) {
         const int i = threadIdx.x / NB, j = threadIdx.x % NB;
-        const double v = jred[0][threadIdx.x] + jred[1][threadIdx.x] + jred[2][threadIdx.x] + jred[3][threadIdx.x];
+        double v = 0.0;
+#pragma unroll
+        for (int w = 0; w < kRowWaves; ++w) v += jred[w][threadIdx.x];
         atomic_add_f64(mv.W0 + (long)(fa + i) * n + fb + j, v);
         if (MODE == kDigestUHF) atomic_add_f64(mv.W1 + (long)(fa + i) * n + fb + j, v);
     }
     if (a.counter && threadIdx.x == 0) {
         unsigned long long* slot = a.counter + 2 * (blockIdx.x & (kCounterSlots - 1));
-        atomicAdd(slot, cred[0][0] + cred[1][0] + cred[2][0] + cred[3][0]);
-        atomicAdd(slot + 1, cred[0][1] + cred[1][1] + cred[2][1] + cred[3][1]);
+        unsigned long long np = 0, ns = 0;
+        for (int w = 0; w < kRowWaves; ++w) {
+            np += cred[w][0];
+            ns += cred[w][1];
+        }
+        atomicAdd(slot, np);
+        atomicAdd(slot + 1, ns);
     }
     // merged exchange rows: consecutive threads take consecutive columns of a row
 #pragma unroll
@@ -823,7 +835,8 @@ static hipError_t launch_rows_v(const EriArgs& a, hipStream_t st) {
         if (e != hipSuccess) return e;
         attr = dyn;
     }
-    hipLaunchKernelGGL((k_eri_rows<LA, LB, LC, LD, MODE, T, TL, FB>), dim3((unsigned)a.nrtask), dim3(256), dyn, st, a);
+    hipLaunchKernelGGL((k_eri_rows<LA, LB, LC, LD, MODE, T, TL, FB>), dim3((unsigned)a.nrtask), dim3(kRowThreads), dyn, st,
+                       a);
     return hipGetLastError();
 }
 

@@ -26,6 +26,7 @@ namespace uno {
 #define UNO_RYS_QUAL static __device__ const
 #include "../rys_table.inc"
 #include "../rys_table_f32.inc"
+#include "../rys_mono.inc"
 #undef UNO_RYS_QUAL
 
 __host__ __device__ constexpr int ncart_c(int l) { return (l + 1) * (l + 2) / 2; }
@@ -76,15 +77,18 @@ template <int NR, typename T>
 __host__ __device__ constexpr int rys_table_size() {
     return kRysAsym[NR] * 2 * NR * (sizeof(T) == 4 ? UNO_RYS_NCOEF_F : UNO_RYS_NCOEF);
 }
+// the power-basis table (rys_mono.inc) of NR roots: per unit interval, 2 NR series of
+// NCOEF coefficients each, roots first
 template <int NR, typename T>
 __device__ __forceinline__ const T* rys_table_global() {
-    if constexpr (sizeof(T) == 4) return uno_rys_coef_f + uno_rys_off_f[NR];
-    else return uno_rys_coef + uno_rys_off[NR];
+    if constexpr (sizeof(T) == 4) return uno_rys_mono_f + uno_rys_off_f[NR];
+    else return uno_rys_mono + uno_rys_off[NR];
 }
 
-// Rys roots u = t^2 and weights for argument x: Clenshaw on the per-interval Chebyshev
-// series (degree 12 in double, degree 6 in float), Gauss-Hermite limit for large x.
-// tab is the table of NR roots (global memory or a copy in LDS).
+// Rys roots u = t^2 and weights for argument x: Horner on the per-interval series in
+// s = 2(x-k)-1 (degree 12 in double, degree 6 in float; the Chebyshev tables re-expanded
+// in powers of s), Gauss-Hermite limit for large x.  tab is the table of NR roots (global
+// memory or a copy in LDS).
 template <int NR, typename T>
 __device__ __forceinline__ void rys_roots(T x, T* u, T* w, const T* tab) {
     constexpr bool f32 = sizeof(T) == 4;
@@ -105,19 +109,14 @@ __device__ __forceinline__ void rys_roots(T x, T* u, T* w, const T* tab) {
     }
     constexpr int NC = f32 ? UNO_RYS_NCOEF_F : UNO_RYS_NCOEF;
     const int k = (int)x;
-    const T s = (T)2 * (x - (T)k) - (T)1, s2 = (T)2 * s;
+    const T s = (T)2 * (x - (T)k) - (T)1;
     const T* c = tab + k * (2 * NR * NC);
 #pragma unroll
     for (int f = 0; f < 2 * NR; ++f) {
         const T* cf = c + f * NC;
-        T b1 = 0, b2 = 0;
+        T v = cf[NC - 1];
 #pragma unroll
-        for (int m = NC - 1; m >= 1; --m) {
-            const T b0 = fma(s2, b1, cf[m] - b2);
-            b2 = b1;
-            b1 = b0;
-        }
-        const T v = fma(s, b1, cf[0] - b2);
+        for (int m = NC - 2; m >= 0; --m) v = fma(v, s, cf[m]);
         if (f < NR) u[f] = v; else w[f - NR] = v;
     }
 }

@@ -218,19 +218,17 @@ def prim_flops(la, lb, lc, ld, prec="f64"):
     n64 = 6
     dot = g.fma(PQ[2], PQ[2], g.fma(PQ[1], PQ[1], g.mul(PQ[0], PQ[0])))
     x = g.mul(g.mul(g.mul(p, q), it), dot)
-    # table path: k = (int)x, s = 2 (x - k) - 1, Clenshaw over NC coefficients per function
+    # table path: k = (int)x, s = 2 (x - k) - 1, Horner over NC power-basis coefficients
+    # per function
     xk = g.free("cvt", x)   # integer conversion and back
     ss = g.fma(2.0, g.sub(x, xk), -1.0)
-    s2 = g.mul(2.0, ss)
     nc = NCOEF[prec]
     u, w = [], []
     for fidx in range(2 * nr):
         cf = [v("cf%d_%d" % (fidx, m)) for m in range(nc)]
-        b1 = b2 = 0.0
-        for m in range(nc - 1, 0, -1):
-            b0 = g.fma(s2, b1, g.sub(cf[m], b2))
-            b2, b1 = b1, b0
-        val = g.fma(ss, b1, g.sub(cf[0], b2))
+        val = cf[nc - 1]
+        for m in range(nc - 2, -1, -1):
+            val = g.fma(val, ss, cf[m])
         (u if fidx < nr else w).append(val)
     abp = _powers(g, "ab", lb)
     cdp = _powers(g, "cd", ld)
