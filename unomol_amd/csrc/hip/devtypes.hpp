@@ -78,6 +78,12 @@ struct EriArgs {
     int nrbra;
     long long nrtask;
     int rseg;
+    // columns of those LDS rows: the functions the launch's ket run touches, in function
+    // order (whole shells, so a shell's functions stay consecutive); colmap[f] = column of
+    // function f (-1: untouched), colinv[c] = function of column c, ncols columns
+    const int* colmap;
+    const int* colinv;
+    int ncols;
     // batched launches: entry i (a bra with one ket range) names its bra pair, its first
     // ket pair and its molecule (null: the run offsets above and molecule view n/D/W)
     const int* e_bra;

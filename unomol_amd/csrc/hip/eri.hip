@@ -339,9 +339,10 @@ __global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
 // so the block merges them in LDS (one row of n doubles per bra function and spin) and
 // adds the non-zero entries at the end with row-contiguous atomics; the bra's Coulomb
 // outputs are reduced over the block.  Only the ket's Coulomb outputs are added per
-// quartet.  Dynamic LDS: (NA + NB) x spins x n doubles, or with FB false only the NA rows
-// of the first bra shell (the second shell's rows then take their atomics directly); TL:
-// the Rys table is copied to LDS as well (otherwise read from global memory).
+// quartet.  Dynamic LDS: (NA + NB) x spins x ncols doubles (ncols: the functions the ket
+// run touches), or with FB false only the NA rows of the first bra shell (the second
+// shell's rows then take their atomics directly); TL: the Rys table is copied to LDS as
+// well (otherwise read from global memory).
 // ---------------------------------------------------------------------------
 template <int LA, int LB, int MODE, bool FB>
 __host__ __device__ constexpr int row_count() {
@@ -376,8 +377,8 @@ __global__ __launch_bounds__(kRowThreads) void k_eri_rows(const EriArgs a) {
     const long long cnt = a.prefix[entry + 1] - a.prefix[entry];
     const long long k0 = seg * a.rseg, k1 = min(cnt, k0 + (long long)a.rseg);
     const MolView mv = mol_view(a, entry);
-    const int n = mv.n;
-    for (int i = threadIdx.x; i < NROW * n; i += blockDim.x) rows[i] = 0.0;
+    const int n = mv.n, nc = a.ncols;
+    for (int i = threadIdx.x; i < NROW * nc; i += blockDim.x) rows[i] = 0.0;
     const T* tab = rys_table_global<NR, T>();
     if constexpr (TL) {
         for (int i = threadIdx.x; i < rys_table_size<NR, T>(); i += blockDim.x) tab_lds[i] = tab[i];
@@ -414,6 +415,7 @@ __global__ __launch_bounds__(kRowThreads) void k_eri_rows(const EriArgs a) {
         const T f = (P1.same ? (T)0.5 : (T)1) * (P2.same ? (T)0.5 : (T)1) * (pab == pcd ? (T)0.5 : (T)1);
         normalise_block<LA, LB, LC, LD, T>(e, f);
         const int fc = P2.f1, fd = P2.f2;
+        const int lc = a.colmap[fc], ld = a.colmap[fd];   // their LDS row columns
         // Coulomb, bra outputs: kept per thread
         {
             T dcd[NC * ND];
@@ -441,7 +443,7 @@ __global__ __launch_bounds__(kRowThreads) void k_eri_rows(const EriArgs a) {
 #pragma unroll
         for (int sp = 0; sp < NSP; ++sp) {
             const double* DK = (MODE == kDigestUHF) ? (sp == 0 ? mv.D1 : mv.D2) : mv.D0;
-            double* R = rows + (long)sp * NAB * n;
+            double* R = rows + sp * NAB * nc;
             double* WB = (MODE == kDigestUHF && sp) ? mv.W1 : mv.W0;
             T dbd[NB * ND], dbc[NB * NC], dad[NA * ND], dac[NA * NC];
 #pragma unroll
@@ -467,7 +469,7 @@ __global__ __launch_bounds__(kRowThreads) void k_eri_rows(const EriArgs a) {
                     for (int j = 0; j < NB; ++j)
 #pragma unroll
                         for (int l = 0; l < ND; ++l) s = fma(e[((i * NB + j) * NC + k) * ND + l], dbd[j * ND + l], s);
-                    atomicAdd(R + (long)i * n + fc + k, (double)(-s));
+                    atomicAdd(R + i * nc + lc + k, (double)(-s));
                 }
 #pragma unroll
                 for (int l = 0; l < ND; ++l) {
@@ -476,7 +478,7 @@ __global__ __launch_bounds__(kRowThreads) void k_eri_rows(const EriArgs a) {
                     for (int j = 0; j < NB; ++j)
 #pragma unroll
                         for (int k = 0; k < NC; ++k) s = fma(e[((i * NB + j) * NC + k) * ND + l], dbc[j * NC + k], s);
-                    atomicAdd(R + (long)i * n + fd + l, (double)(-s));
+                    atomicAdd(R + i * nc + ld + l, (double)(-s));
                 }
             }
 #pragma unroll
@@ -488,7 +490,7 @@ __global__ __launch_bounds__(kRowThreads) void k_eri_rows(const EriArgs a) {
                     for (int i = 0; i < NA; ++i)
 #pragma unroll
                         for (int l = 0; l < ND; ++l) s = fma(e[((i * NB + j) * NC + k) * ND + l], dad[i * ND + l], s);
-                    if constexpr (FB) atomicAdd(R + (long)(NA + j) * n + fc + k, (double)(-s));
+                    if constexpr (FB) atomicAdd(R + (NA + j) * nc + lc + k, (double)(-s));
                     else atomic_add_f64(WB + (long)(fb + j) * n + fc + k, (double)(-s));
                 }
 #pragma unroll
@@ -498,7 +500,7 @@ __global__ __launch_bounds__(kRowThreads) void k_eri_rows(const EriArgs a) {
                     for (int i = 0; i < NA; ++i)
 #pragma unroll
                         for (int k = 0; k < NC; ++k) s = fma(e[((i * NB + j) * NC + k) * ND + l], dac[i * NC + k], s);
-                    if constexpr (FB) atomicAdd(R + (long)(NA + j) * n + fd + l, (double)(-s));
+                    if constexpr (FB) atomicAdd(R + (NA + j) * nc + ld + l, (double)(-s));
                     else atomic_add_f64(WB + (long)(fb + j) * n + fd + l, (double)(-s));
                 }
             }
@@ -547,9 +549,9 @@ __global__ __launch_bounds__(kRowThreads) void k_eri_rows(const EriArgs a) {
         const int sp = r / NAB, rr = r % NAB;
         const int grow = rr < NA ? fa + rr : fb + rr - NA;
         double* W = (MODE == kDigestUHF && sp) ? mv.W1 : mv.W0;
-        for (int col = threadIdx.x; col < n; col += blockDim.x) {
-            const double v = rows[(long)r * n + col];
-            if (v != 0.0) atomic_add_f64(W + (long)grow * n + col, v);
+        for (int col = threadIdx.x; col < nc; col += blockDim.x) {
+            const double v = rows[r * nc + col];
+            if (v != 0.0) atomic_add_f64(W + (long)grow * n + a.colinv[col], v);
         }
     }
 }
@@ -827,7 +829,7 @@ static hipError_t launch_cls(const EriArgs& a, hipStream_t st) {
 // row-merged Fock-build launch of a lane class: one block per row task
 template <int LA, int LB, int LC, int LD, int MODE, typename T, bool TL, bool FB>
 static hipError_t launch_rows_v(const EriArgs& a, hipStream_t st) {
-    const size_t dyn = (size_t)row_count<LA, LB, MODE, FB>() * a.n * sizeof(double);
+    const size_t dyn = (size_t)row_count<LA, LB, MODE, FB>() * a.ncols * sizeof(double);
     static size_t attr = 0;   // dynamic LDS limit set for this instantiation
     if (dyn > attr) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eri_rows<LA, LB, LC, LD, MODE, T, TL, FB>),
@@ -846,7 +848,9 @@ template <int LA, int LB, int LC, int LD, int MODE, typename T>
 static hipError_t launch_rows(const EriArgs& a, int variant, hipStream_t st) {
     using CI = ClassInfo<LA, LB, LC, LD>;
     if constexpr (CI::NCOMP <= kLaneMaxComp && MODE <= kDigestUHF) {
-        if (a.nrtask <= 0 || !a.rbra || !a.rpre || a.rseg <= 0) return hipErrorInvalidValue;
+        if (a.nrtask <= 0 || !a.rbra || !a.rpre || a.rseg <= 0 || !a.colmap || !a.colinv || a.ncols <= 0 ||
+            a.ncols > a.n)
+            return hipErrorInvalidValue;
         switch (variant) {
             case 1: return launch_rows_v<LA, LB, LC, LD, MODE, T, true, true>(a, st);
             case 2: return launch_rows_v<LA, LB, LC, LD, MODE, T, false, true>(a, st);
@@ -880,7 +884,7 @@ hipError_t eri_launch_rows(int mode, int cb, int ck, const EriArgs& a, int varia
     return hipErrorInvalidValue;
 }
 
-int eri_rows_variant(int cb, int ck, int mode, int n, bool fp32, int force) {
+int eri_rows_variant(int cb, int ck, int mode, int ncols, bool fp32, int force) {
     if (!eri_uses_wave_tasks(cb, ck)) return 0;
     static const int na[6] = {1, 3, 3, 6, 6, 6};          // functions of the first bra shell
     static const int nb[6] = {1, 1, 3, 1, 3, 6};          // and of the second
@@ -889,7 +893,8 @@ int eri_rows_variant(int cb, int ck, int mode, int n, bool fp32, int force) {
     const size_t tab = fp32 ? (size_t)kRysAsym[nr] * 2 * nr * UNO_RYS_NCOEF_F * sizeof(float)
                             : (size_t)kRysAsym[nr] * 2 * nr * UNO_RYS_NCOEF * sizeof(double);
     const size_t sp = mode == kDigestUHF ? 2 : 1, reserve = 1024;
-    const size_t all = (size_t)(na[cb] + nb[cb]) * sp * n * sizeof(double), first = (size_t)na[cb] * sp * n * sizeof(double);
+    const size_t all = (size_t)(na[cb] + nb[cb]) * sp * ncols * sizeof(double),
+                 first = (size_t)na[cb] * sp * ncols * sizeof(double);
     const bool fits[4] = {true, all + tab + reserve <= kRowLdsBudget, all + reserve <= kRowLdsBudget,
                           first + reserve <= kRowLdsBudget};
     if (force >= 1 && force <= 3 && fits[force]) return force;

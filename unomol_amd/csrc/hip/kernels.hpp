@@ -15,13 +15,13 @@ int eri_quartets_per_block(int bra_class, int ket_class);
 // wpre / tblk / ntask instead of the quartet block table
 bool eri_uses_wave_tasks(int bra_class, int ket_class);
 // row-merged Fock-build launch (bras with many kets) of a wave-task class: a.rbra / a.rpre /
-// a.nrtask / a.rseg describe the block tasks; variant from eri_rows_variant: 1 every bra
-// row and the Rys table in LDS, 2 every bra row, 3 the first bra shell's rows only, 0 none
-// fits the LDS budget at n functions (the class then runs as wave tasks only); force 1-3
-// picks that variant when it fits
+// a.nrtask / a.rseg describe the block tasks, a.colmap / a.colinv / a.ncols the LDS row
+// columns; variant from eri_rows_variant: 1 every bra row and the Rys table in LDS, 2 every
+// bra row, 3 the first bra shell's rows only, 0 none fits the LDS budget at ncols columns
+// (the class then runs as wave tasks only); force 1-3 picks that variant when it fits
 hipError_t eri_launch_rows(int mode, int bra_class, int ket_class, const EriArgs& a, int variant, hipStream_t st,
                            bool fp32);
-int eri_rows_variant(int bra_class, int ket_class, int mode, int n, bool fp32, int force);
+int eri_rows_variant(int bra_class, int ket_class, int mode, int ncols, bool fp32, int force);
 // W[f1 + k][f2 + l] += jp[jo + k nd + l] (and W1) for every pair of the array (pair blocks
 // are disjoint, no atomics); the block size of a pair follows from the next pair's jo
 hipError_t jp_scatter(int npairs, const PairHdr* pairs, long long njp, const double* jp, double* W0, double* W1, int n,

@@ -318,6 +318,34 @@ int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
                 set_error("internal error: pair run is not sorted by Schwarz bound");
                 return UNO_EINVAL;
             }
+    // columns of the row-merged launches' LDS rows: per ket run, the functions its pairs
+    // touch (whole shells), in function order
+    {
+        static const int kPairL[kNumPairClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+        const int nr = (int)c->cls.size(), n = c->n;
+        std::vector<int> cmap((size_t)nr * n, -1), cinv((size_t)nr * n, 0);
+        c->run_ncols.assign(nr, 0);
+        for (int r = 0; r < nr; ++r) {
+            const PairClassList& L = c->cls[r];
+            const int n1 = (kPairL[L.lcls][0] + 1) * (kPairL[L.lcls][0] + 2) / 2;
+            const int n2 = (kPairL[L.lcls][1] + 1) * (kPairL[L.lcls][1] + 2) / 2;
+            std::vector<char> used(n, 0);
+            for (int k = L.off; k < L.off + L.count; ++k) {
+                for (int f = 0; f < n1; ++f) used[c->pairs[k].f1 + f] = 1;
+                for (int f = 0; f < n2; ++f) used[c->pairs[k].f2 + f] = 1;
+            }
+            int m = 0;
+            for (int f = 0; f < n; ++f)
+                if (used[f]) {
+                    cmap[(size_t)r * n + f] = m;
+                    cinv[(size_t)r * n + m] = f;
+                    ++m;
+                }
+            c->run_ncols[r] = m;
+        }
+        UNO_HIP(c->d_colmap.upload(cmap.data(), cmap.size(), st));
+        UNO_HIP(c->d_colinv.upload(cinv.data(), cinv.size(), st));
+    }
     c->qcls.clear();
     for (int X = 0; X < (int)c->cls.size(); ++X)
         for (int Y = 0; Y <= X; ++Y) {
@@ -446,14 +474,14 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
             SpaceRec r{&q, prec, 0, eri_uses_wave_tasks(LX.lcls, LY.lcls), 0, 0, 0, 0, 0, 0, 0};
             // row-merge configuration: the tuned one for this space if any, else the defaults
             int variant = r.tasks && c->row_merge
-                              ? eri_rows_variant(LX.lcls, LY.lcls, mode, c->n, prec == 1, c->row_variant)
+                              ? eri_rows_variant(LX.lcls, LY.lcls, mode, c->run_ncols[q.Y], prec == 1, c->row_variant)
                               : 0;
             long long min_kets = std::max(1LL, c->row_min_kets);
             if (r.tasks && c->row_merge && !c->tuning) {
                 auto it = c->tune.find(std::make_tuple(qi, prec, uhf ? 1 : 0, mixed ? 1 : 0));
                 if (it != c->tune.end()) {
                     const int v = it->second.variant;
-                    if (v == 0 || eri_rows_variant(LX.lcls, LY.lcls, mode, c->n, prec == 1, v) == v) {
+                    if (v == 0 || eri_rows_variant(LX.lcls, LY.lcls, mode, c->run_ncols[q.Y], prec == 1, v) == v) {
                         variant = v;
                         min_kets = it->second.min_kets;
                     }
@@ -502,6 +530,9 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
         a.ket_off = LY.off;
         a.pairQ = c->d_pairQ.p;
         a.rseg = kRowSeg;
+        a.colmap = c->d_colmap.p + (size_t)q.Y * c->n;
+        a.colinv = c->d_colinv.p + (size_t)q.Y * c->n;
+        a.ncols = c->run_ncols[q.Y];
         a.qpb = eri_quartets_per_block(LX.lcls, LY.lcls);
         a.nbra = LX.count;
         a.nquart = sp.nq;
@@ -584,7 +615,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
             // time every candidate on this space (each one run twice, the faster counts)
             std::vector<TuneChoice> cand{{0, 0}};
             for (int v = 1; v <= 3; ++v)
-                if (eri_rows_variant(LX.lcls, LY.lcls, mode, c->n, prec == 1, v) == v)
+                if (eri_rows_variant(LX.lcls, LY.lcls, mode, c->run_ncols[q.Y], prec == 1, v) == v)
                     for (long long mk : {64LL, 192LL, 512LL}) cand.push_back({v, mk});
             TuneChoice best{r.variant, std::max(1LL, c->row_min_kets)};
             float best_ms = 1e30f;

@@ -133,6 +133,10 @@ struct uno_ctx {
     uno::DBuf<double> D0, D1, D2, W0, W1, Dsh, pk, scal;
     uno::DBuf<double> JP;        // pair-major accumulator of the kets' Coulomb outputs
     uno::DBuf<double> red;       // partial sums of the two-pass reductions
+    // LDS row columns of the row-merged launches, per pair run (ket side): run_ncols[r]
+    // columns, d_colmap[r n + f] = column of function f or -1, d_colinv[r n + c] = function
+    std::vector<int> run_ncols;
+    uno::DBuf<int> d_colmap, d_colinv;
     std::vector<char> stage_h;   // a Fock build's index arrays, uploaded once per build
     uno::DBuf<char> stage_d;
     long long njp = 0;
