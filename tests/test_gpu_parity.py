@@ -146,7 +146,7 @@ def test_pair_runs_grouped(tmp_path):
     assert int(t.get("pairs")) > 0
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 13, 64, 200])
+@pytest.mark.parametrize("n", [1, 2, 3, 13, 64, 200, 777])
 def test_eigh_vs_numpy(n):
     say("eigh n=%d" % n)
     rng = np.random.default_rng(n)

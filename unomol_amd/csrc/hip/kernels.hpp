@@ -62,7 +62,7 @@ struct EigWork {
     double* e;      // n  off-diagonal of T
     double* scal;   // 4 scalars
     double* evals;  // n  eigenvalues of T (ascending)
-    double* Y;      // n x n eigenvectors of T, one per row
+    double* Y;      // n x n eigenvectors of T, one per column (Yt)
     double* wk;     // 5 n^2 inverse-iteration work
     double* wyS;    // 32 x 32
     double* wyT;    // 32 x 32
@@ -71,8 +71,8 @@ struct EigWork {
 };
 hipError_t householder_tridiag(int n, EigWork& w, hipStream_t st);
 hipError_t accumulate_q(int n, EigWork& w, hipStream_t st);
-// nclusters < 0: bisection for w.evals; otherwise inverse iteration into w.Y followed by
-// Gram-Schmidt inside the given clusters ({first, end} pairs, device memory)
+// nclusters < 0: multisection for w.evals; otherwise inverse iteration into the columns of
+// w.Y followed by Gram-Schmidt inside the given clusters ({first, end} pairs, device memory)
 hipError_t tridiag_eigen(int n, EigWork& w, int nclusters, const int* clusters, hipStream_t st);
 
 // streaming copy of `bytes` (read + write), reps times; total milliseconds in *ms

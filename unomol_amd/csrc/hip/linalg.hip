@@ -429,8 +429,11 @@ hipError_t accumulate_q(int n, EigWork& w, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
-// Tridiagonal eigenvalues by bisection on Sturm counts: thread i finds the i-th
-// smallest eigenvalue inside the Gershgorin interval.
+// Tridiagonal eigenvalues by multisection on Sturm counts: one wave per eigenvalue.  The
+// 64 lanes count at 64 interior points of the current interval at once, so each round
+// narrows it 65-fold (about 9 rounds to full precision instead of 55 bisection steps);
+// the search starts from the Gershgorin interval and stops at the same width as plain
+// bisection would (2 eps max|end| + 4 pivmin).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int sturm_count(int n, const double* d, const double* e2, double x, double pivmin) {
     int c = 0;
@@ -445,6 +448,16 @@ __device__ __forceinline__ int sturm_count(int n, const double* d, const double*
     return c;
 }
 
+__device__ __forceinline__ double wave_min(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// four eigenvalues per block (one per wave); d and e^2 staged in LDS
 __global__ __launch_bounds__(256) void k_bisect(int n, const double* __restrict__ dg, const double* __restrict__ eg,
                                                 double* __restrict__ evals) {
     extern __shared__ double sm[];
@@ -455,117 +468,140 @@ __global__ __launch_bounds__(256) void k_bisect(int n, const double* __restrict_
         e2[k] = (k + 1 < n) ? eg[k] * eg[k] : 0.0;
     }
     __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);   // eigenvalue of this wave
+    if (i >= n) return;                                  // whole wave
     double lo = 1e300, hi = -1e300, emax = 0.0;
-    for (int k = 0; k < n; ++k) {
+    for (int k = lane; k < n; k += 64) {
         const double r = (k > 0 ? sqrt(e2[k - 1]) : 0.0) + (k + 1 < n ? sqrt(e2[k]) : 0.0);
         lo = fmin(lo, d[k] - r);
         hi = fmax(hi, d[k] + r);
         emax = fmax(emax, e2[k]);
     }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    emax = wave_max(emax);
     const double tnorm = fmax(fabs(lo), fabs(hi));
     const double pivmin = 1e-300 + 2.2250738585072014e-308 * fmax(1.0, emax);
     lo -= 2.2e-16 * tnorm * n + pivmin;
     hi += 2.2e-16 * tnorm * n + pivmin;
-    for (int it = 0; it < 200; ++it) {
-        const double mid = 0.5 * (lo + hi);
+    for (int it = 0; it < 64; ++it) {
         if (hi - lo <= 2.0 * 2.2204460492503131e-16 * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin) break;
-        if (mid <= lo || mid >= hi) break;
-        if (sturm_count(n, d, e2, mid, pivmin) > i) hi = mid; else lo = mid;
+        const double h = (hi - lo) / 65.0;
+        double x = lo + h * (lane + 1);
+        if (lane == 63 || x >= hi) x = fmin(x, hi);
+        // points past rounding collapse onto hi count like hi: the count there is > i
+        const bool above = x >= hi || sturm_count(n, d, e2, x, pivmin) > i;
+        const unsigned long long m = __ballot(above);
+        // counts are monotone in x: the first lane above brackets the eigenvalue
+        const int f = m ? __builtin_ctzll(m) : 64;
+        const double nlo = f == 0 ? lo : __shfl(x, f - 1, 64);
+        const double nhi = f == 64 ? hi : __shfl(x, f, 64);
+        if (!(nlo > lo || nhi < hi)) break;   // no progress left in double precision
+        lo = nlo;
+        hi = nhi;
     }
-    evals[i] = 0.5 * (lo + hi);
+    if (lane == 0) evals[i] = 0.5 * (lo + hi);
 }
 
 // ---------------------------------------------------------------------------
 // Inverse iteration: thread i solves (T - lambda_i I) x = b three times with a
 // partially pivoted LU of the tridiagonal (tiny pivots replaced by eps*|T|), from a
-// pseudo-random start vector seeded by i.  Work arrays are interleaved [k][thread]
-// so the lanes of a wave touch consecutive addresses.  Output: row i of Y.
+// pseudo-random start vector seeded by i.  Work arrays and the vectors are interleaved
+// [k][thread] so the lanes of a wave touch consecutive addresses: the output Yt holds
+// eigenvector i in column i (Yt[k n + i]).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_inviter(int n, const double* __restrict__ d, const double* __restrict__ e,
-                                               const double* __restrict__ evals, double* __restrict__ Y,
+                                               const double* __restrict__ evals, double* __restrict__ Yt,
                                                double* __restrict__ wk) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const long S = n;   // stride between k entries of one thread
-    double* U1 = wk;                  // diagonal of U
+    double* U1 = wk;                  // reciprocal of the diagonal of U
     double* U2 = wk + (long)n * S;    // first superdiagonal of U
     double* U3 = wk + 2L * n * S;     // second superdiagonal (from pivoting)
     double* L = wk + 3L * n * S;      // multipliers
-    double* x = Y + (long)i * n;
+    double* x = Yt + i;               // x[k S]
     double tnorm = 0.0;
     for (int k = 0; k < n; ++k) tnorm = fmax(tnorm, fabs(d[k]) + (k + 1 < n ? fabs(e[k]) : 0.0) + (k > 0 ? fabs(e[k - 1]) : 0.0));
     const double lam = evals[i];
     const double tiny = 2.2204460492503131e-16 * fmax(tnorm, 1e-300);
     int* piv = (int*)(wk + 4L * n * S);   // n*S ints fit in the space of n*S doubles
     // factor T - lam I = P L U, row by row
-    double a = d[0] - lam, b = (n > 1) ? e[0] : 0.0, c = 0.0;
+    double a = d[0] - lam, b = (n > 1) ? e[0] : 0.0;
+    auto pivot = [&](double u) {
+        if (fabs(u) < tiny) u = copysign(tiny, u == 0.0 ? 1.0 : u);
+        return 1.0 / u;
+    };
     for (int k = 0; k < n - 1; ++k) {
         const double sub = e[k], dn = d[k + 1] - lam, en = (k + 2 < n) ? e[k + 1] : 0.0;
         if (fabs(a) >= fabs(sub)) {
             const double aa = (a == 0.0) ? tiny : a;
             const double l = sub / aa;
-            U1[(long)k * S + i] = aa; U2[(long)k * S + i] = b; U3[(long)k * S + i] = 0.0;
+            U1[(long)k * S + i] = pivot(aa); U2[(long)k * S + i] = b; U3[(long)k * S + i] = 0.0;
             L[(long)k * S + i] = l; piv[(long)k * S + i] = 0;
             a = dn - l * b;
             b = en;
         } else {
             const double l = a / sub;
-            U1[(long)k * S + i] = sub; U2[(long)k * S + i] = dn; U3[(long)k * S + i] = en;
+            U1[(long)k * S + i] = pivot(sub); U2[(long)k * S + i] = dn; U3[(long)k * S + i] = en;
             L[(long)k * S + i] = l; piv[(long)k * S + i] = 1;
             a = b - l * dn;
             b = -l * en;
         }
-        (void)c;
     }
-    U1[(long)(n - 1) * S + i] = (a == 0.0) ? tiny : (fabs(a) < tiny ? copysign(tiny, a) : a);
+    U1[(long)(n - 1) * S + i] = pivot((a == 0.0) ? tiny : (fabs(a) < tiny ? copysign(tiny, a) : a));
     // start vector
     unsigned int h = 2654435761u * (unsigned)(i + 1);
     for (int k = 0; k < n; ++k) {
         h ^= h << 13; h ^= h >> 17; h ^= h << 5;
-        x[k] = ((double)(h & 0xffffff) / 16777216.0) - 0.5 + 1e-3;
+        x[k * S] = ((double)(h & 0xffffff) / 16777216.0) - 0.5 + 1e-3;
     }
     for (int it = 0; it < 3; ++it) {
         // forward: apply the row interchanges and multipliers
+        double xk = x[0];
         for (int k = 0; k < n - 1; ++k) {
             const double l = L[(long)k * S + i];
+            const double xn = x[(k + 1) * S];
             if (piv[(long)k * S + i]) {
-                const double t = x[k];
-                x[k] = x[k + 1];
-                x[k + 1] = t - l * x[k];
+                x[k * S] = xn;
+                xk = xk - l * xn;
             } else {
-                x[k + 1] -= l * x[k];
+                x[k * S] = xk;
+                xk = xn - l * xk;
             }
         }
-        // back substitution with the banded U
+        x[(long)(n - 1) * S] = xk;
+        // back substitution with the banded U, then the norm
+        double x1 = 0.0, x2 = 0.0, nrm = 0.0;
         for (int k = n - 1; k >= 0; --k) {
-            double v = x[k];
-            if (k + 1 < n) v -= U2[(long)k * S + i] * x[k + 1];
-            if (k + 2 < n) v -= U3[(long)k * S + i] * x[k + 2];
-            double u = U1[(long)k * S + i];
-            if (fabs(u) < tiny) u = copysign(tiny, u == 0.0 ? 1.0 : u);
-            x[k] = v / u;
+            double v = x[k * S];
+            v -= U2[(long)k * S + i] * x1;
+            v -= U3[(long)k * S + i] * x2;
+            v *= U1[(long)k * S + i];
+            x[k * S] = v;
+            nrm = fma(v, v, nrm);
+            x2 = x1;
+            x1 = v;
         }
-        double nrm = 0.0;
-        for (int k = 0; k < n; ++k) nrm = fma(x[k], x[k], nrm);
         nrm = 1.0 / sqrt(nrm);
-        for (int k = 0; k < n; ++k) x[k] *= nrm;
+        for (int k = 0; k < n; ++k) x[k * S] *= nrm;
     }
 }
 
-// modified Gram-Schmidt (two passes) inside each cluster of close eigenvalues:
-// clusters[2c], clusters[2c+1] = first and one-past-last index; one block per cluster
-__global__ __launch_bounds__(256) void k_cluster_mgs(int n, const int* __restrict__ clusters, double* __restrict__ Y) {
+// modified Gram-Schmidt (two passes) inside each cluster of close eigenvalues (columns
+// of Yt): clusters[2c], clusters[2c+1] = first and one-past-last index; one block per
+// cluster
+__global__ __launch_bounds__(256) void k_cluster_mgs(int n, const int* __restrict__ clusters, double* __restrict__ Yt) {
     __shared__ double red[256];
     const int c0 = clusters[2 * blockIdx.x], c1 = clusters[2 * blockIdx.x + 1];
     for (int j = c0; j < c1; ++j) {
-        double* yj = Y + (long)j * n;
+        double* yj = Yt + j;
         for (int pass = 0; pass < 2; ++pass) {
             for (int p = c0; p < j; ++p) {
-                const double* yp = Y + (long)p * n;
+                const double* yp = Yt + p;
                 double s = 0.0;
-                for (int k = threadIdx.x; k < n; k += blockDim.x) s = fma(yj[k], yp[k], s);
+                for (int k = threadIdx.x; k < n; k += blockDim.x) s = fma(yj[(long)k * n], yp[(long)k * n], s);
                 red[threadIdx.x] = s;
                 __syncthreads();
                 for (int o = 128; o > 0; o >>= 1) {
@@ -574,12 +610,12 @@ __global__ __launch_bounds__(256) void k_cluster_mgs(int n, const int* __restric
                 }
                 const double dot = red[0];
                 __syncthreads();
-                for (int k = threadIdx.x; k < n; k += blockDim.x) yj[k] -= dot * yp[k];
+                for (int k = threadIdx.x; k < n; k += blockDim.x) yj[(long)k * n] -= dot * yp[(long)k * n];
                 __syncthreads();
             }
         }
         double s = 0.0;
-        for (int k = threadIdx.x; k < n; k += blockDim.x) s = fma(yj[k], yj[k], s);
+        for (int k = threadIdx.x; k < n; k += blockDim.x) s = fma(yj[(long)k * n], yj[(long)k * n], s);
         red[threadIdx.x] = s;
         __syncthreads();
         for (int o = 128; o > 0; o >>= 1) {
@@ -588,7 +624,7 @@ __global__ __launch_bounds__(256) void k_cluster_mgs(int n, const int* __restric
         }
         const double inv = 1.0 / sqrt(red[0]);
         __syncthreads();
-        for (int k = threadIdx.x; k < n; k += blockDim.x) yj[k] *= inv;
+        for (int k = threadIdx.x; k < n; k += blockDim.x) yj[(long)k * n] *= inv;
         __syncthreads();
     }
 }
@@ -596,7 +632,7 @@ __global__ __launch_bounds__(256) void k_cluster_mgs(int n, const int* __restric
 hipError_t tridiag_eigen(int n, EigWork& w, int nclusters, const int* clusters, hipStream_t st) {
     if (nclusters < 0) {
         const size_t lds = 2 * (size_t)n * sizeof(double);
-        hipLaunchKernelGGL(k_bisect, dim3((unsigned)((n + 255) / 256)), dim3(256), lds, st, n, w.d, w.e, w.evals);
+        hipLaunchKernelGGL(k_bisect, dim3((unsigned)((n + 3) / 4)), dim3(256), lds, st, n, w.d, w.e, w.evals);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_inviter, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, n, w.d, w.e, w.evals, w.Y, w.wk);

@@ -3,11 +3,12 @@
 // in the columns of a row-major Z).  All steps run on the GPU:
 //   1. Householder tridiagonalisation A = Q T Q^T (one reflector per column: symv and
 //      rank-2 update kernels);
-//   2. eigenvalues of T by bisection on Sturm counts, one eigenvalue per thread;
-//   3. eigenvectors of T by inverse iteration, one per thread, then two-pass modified
-//      Gram-Schmidt inside clusters of close eigenvalues (exact degeneracies included);
+//   2. eigenvalues of T by multisection on Sturm counts, one eigenvalue per wave;
+//   3. eigenvectors of T by inverse iteration, one per thread (the vectors stored as the
+//      columns of Yt, so a wave's accesses coalesce), then two-pass modified Gram-Schmidt
+//      inside clusters of close eigenvalues (exact degeneracies included);
 //   4. Q accumulated from the reflectors in compact-WY blocks with MFMA GEMMs;
-//   5. Z = Q Y^T with one MFMA GEMM.
+//   5. Z = Q Yt with one MFMA GEMM.
 // The reference reduces with Householder and diagonalises T with implicit QL
 // (SymmPack.cpp:67-269); the contract (not the steps) is what the callers rely on.
 #include <algorithm>
@@ -91,7 +92,7 @@ static int eigh_impl(EigenWS& w, int n, double* dA, double* dZ, double* h_evals,
     if (ncl) UNO_HIP(w.clusters.upload(w.hcl.data(), w.hcl.size(), st));
     UNO_HIP(tridiag_eigen(n, ew, ncl, w.clusters.p, st));
     UNO_HIP(accumulate_q(n, ew, st));
-    UNO_HIP(gemm_f64(false, true, n, n, n, 1.0, w.Q.p, n, w.Y.p, n, 0.0, dZ, n, st));
+    UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, w.Q.p, n, w.Y.p, n, 0.0, dZ, n, st));   // Z = Q Yt
     std::copy(w.hev.begin(), w.hev.end(), h_evals);
     return UNO_OK;
 }
