@@ -282,103 +282,139 @@ hipError_t gemm_f64(bool ta, bool tb, int m, int n, int k, double alpha, const d
 // Householder tridiagonalisation A = Q T Q^T (one reflector per column k):
 //   x = A[k+1:,k], alpha = -sign(x0)|x|, v = x - alpha e1, tau = 2/|v|^2
 //   p = tau A22 v, K = tau/2 p.v, w = p - K v, A22 -= v w^T + w v^T
-// d[k] = A[k][k], e[k] = alpha.  Reflector k is stored in row k of V.
+// d[k] = A[k][k], e[k] = alpha.  Reflector k is stored in row k of V (absolute column
+// indices).  Two launches per column: the rank-2 update of step k-1 is applied in the same
+// pass over the trailing rows that forms p of step k (the matrix is read and written
+// once per column), and a one-block launch finishes w of step k and forms reflector k+1
+// from row k+1 updated on the fly.  The trailing matrix stays symmetric (full rows are
+// updated), so row k+1 is column k+1.
 // ---------------------------------------------------------------------------
-// Step k, launch 1 (one wave per row of the trailing matrix, four rows per block): every
-// block forms the reflector of column k from row k of A (the trailing matrix stays
-// symmetric), block 0 stores it (V row k, tau, d, e); each wave computes
-// p_row = tau A_row . v and the block adds its rows' p_row v_row to part[block].
-__global__ __launch_bounds__(256) void k_hh_vsymv(int n, int k, const double* A, double* V, double* tau, double* d,
-                                                  double* e, double* p, double* part) {
+__device__ __forceinline__ void hh_reflector(double nx2, double x0, double& alpha, double& v0, double& t) {
+    const double nx = sqrt(nx2);
+    alpha = x0 > 0.0 ? -nx : nx;
+    v0 = x0 - alpha;
+    const double vn2 = nx2 - x0 * x0 + v0 * v0;
+    t = (nx2 == 0.0 || vn2 == 0.0) ? 0.0 : 2.0 / vn2;
+}
+
+// reflector 0 from row 0 (one block)
+__global__ __launch_bounds__(256) void k_hh_first(int n, const double* A, double* V, double* tau, double* d, double* e) {
     __shared__ double sh[256];
+    const double* x = A + 1;   // row 0, columns 1..
+    double s = 0.0;
+    for (int j = threadIdx.x; j < n - 1; j += 256) s = fma(x[j], x[j], s);
+    const double nx2 = block_sum(s, sh), x0 = x[0];
+    double alpha, v0, t;
+    hh_reflector(nx2, x0, alpha, v0, t);
+    for (int j = threadIdx.x; j < n - 1; j += 256) V[1 + j] = j == 0 ? v0 : x[j];
+    if (threadIdx.x == 0) {
+        tau[0] = t;
+        d[0] = A[0];
+        e[0] = t == 0.0 ? x0 : alpha;
+    }
+}
+
+// step k, launch 1 (one wave per trailing row, four rows per block): for k > 0 the update
+// A -= v' w'^T + w' v'^T of step k-1 (v' = V row k-1, w' = Wprev) on the row's columns
+// k+1..n-1, written back; then p_r = tau_k a_r . v_k into P[r], and the block's sum of
+// p_r v_r into part[block]
+__global__ __launch_bounds__(256) void k_hh_usymv(int n, int k, double* A, const double* V, const double* tau,
+                                                  const double* Wprev, double* P, double* part) {
     __shared__ double pv[4];
     const int m = n - k - 1;
-    const double* rk = A + (long)k * n + k + 1;   // row k, columns k+1.. = the column below the diagonal
-    double s = 0.0;
-    for (int i = threadIdx.x; i < m; i += 256) s = fma(rk[i], rk[i], s);
-    const double nx2 = block_sum(s, sh);
-    const double x0 = rk[0];
-    const double nx = sqrt(nx2), alpha = x0 > 0.0 ? -nx : nx, v0 = x0 - alpha;
-    const double vn2 = nx2 - x0 * x0 + v0 * v0;
-    const double t = (nx2 == 0.0 || vn2 == 0.0) ? 0.0 : 2.0 / vn2;
-    if (blockIdx.x == 0) {
-        for (int i = threadIdx.x; i < m; i += 256) V[(long)k * n + k + 1 + i] = i == 0 ? v0 : rk[i];
-        if (threadIdx.x == 0) {
-            tau[k] = t;
-            d[k] = A[(long)k * n + k];
-            e[k] = t == 0.0 ? x0 : alpha;
-        }
-    }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + wave;
+    const double* v = V + (long)k * n;
     double pr = 0.0;
     if (row < m) {
-        const double* a = A + (long)(k + 1 + row) * n + k + 1;
+        const int r = k + 1 + row;
+        double* a = A + (long)r * n;
         double acc = 0.0;
-        for (int j = lane; j < m; j += 64) acc = fma(a[j], j == 0 ? v0 : rk[j], acc);
+        if (k > 0) {
+            const double* vp = V + (long)(k - 1) * n;
+            const double vr = vp[r], wr = Wprev[r];
+            for (int j = k + 1 + lane; j < n; j += 64) {
+                const double x = a[j] - (vr * Wprev[j] + wr * vp[j]);
+                a[j] = x;
+                acc = fma(x, v[j], acc);
+            }
+        } else {
+            for (int j = k + 1 + lane; j < n; j += 64) acc = fma(a[j], v[j], acc);
+        }
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-        pr = t * acc;
-        if (lane == 0) p[row] = pr;
+        pr = tau[k] * acc;
+        if (lane == 0) P[r] = pr;
     }
-    if (lane == 0) pv[wave] = row < m ? pr * (row == 0 ? v0 : rk[row]) : 0.0;
+    if (lane == 0) pv[wave] = row < m ? pr * v[k + 1 + row] : 0.0;
     __syncthreads();
     if (threadIdx.x == 0) part[blockIdx.x] = (pv[0] + pv[1]) + (pv[2] + pv[3]);
 }
 
-// Step k, launch 2 (eight rows per block): K = tau/2 p.v from the partial sums (fixed
-// order, every block), w = p - K v, and the rank-2 update A -= v w^T + w v^T of the
-// block's rows.
-__global__ __launch_bounds__(256) void k_hh_wrank2(int n, int k, double* A, const double* V, const double* tau,
-                                                   const double* p, const double* part, int nparts) {
-    extern __shared__ double lds[];
-    double* sv = lds;          // v, m entries
-    double* sw = lds + n;      // w
+// step k, launch 2 (one block): K = tau_k/2 p.v from the partial sums (fixed order),
+// w = p - K v into Wcur; unless k is the last reflector, row k+1 after the update of step
+// k gives d[k+1] and reflector k+1 (V row k+1, tau, e)
+__global__ __launch_bounds__(256) void k_hh_next(int n, int k, const double* A, double* V, double* tau, double* d,
+                                                 double* e, const double* P, const double* part, int nparts,
+                                                 double* Wcur) {
     __shared__ double sh[256];
-    const int m = n - k - 1;
     double s = 0.0;
     for (int b = threadIdx.x; b < nparts; b += 256) s += part[b];
     const double K = 0.5 * tau[k] * block_sum(s, sh);
-    const double* v = V + (long)k * n + k + 1;
-    for (int j = threadIdx.x; j < m; j += 256) {
-        const double vj = v[j];
-        sv[j] = vj;
-        sw[j] = p[j] - K * vj;
+    const double* v = V + (long)k * n;
+    for (int j = k + 1 + threadIdx.x; j < n; j += 256) Wcur[j] = P[j] - K * v[j];
+    if (k + 1 > n - 3) return;   // uniform: the tail launch finishes the last 2 x 2 block
+    const int r = k + 1;
+    const double vr = v[r], wr = P[r] - K * vr;
+    const double* a = A + (long)r * n;
+    double* vn = V + (long)r * n;
+    double s2 = 0.0;
+    for (int j = r + 1 + threadIdx.x; j < n; j += 256) {
+        const double wj = P[j] - K * v[j];
+        const double x = a[j] - (vr * wj + wr * v[j]);
+        vn[j] = x;
+        s2 = fma(x, x, s2);
     }
-    __syncthreads();
-    for (int rr = 0; rr < 8; ++rr) {
-        const int r = blockIdx.x * 8 + rr;
-        if (r >= m) break;
-        const double vi = sv[r], wi = sw[r];
-        double* a = A + (long)(k + 1 + r) * n + k + 1;
-        for (int j = threadIdx.x; j < m; j += 256) a[j] -= vi * sw[j] + wi * sv[j];
+    const double nx2 = block_sum(s2, sh);
+    const double w1 = P[r + 1] - K * v[r + 1];
+    const double x0 = a[r + 1] - (vr * w1 + wr * v[r + 1]);   // thread 0's first entry, recomputed
+    double alpha, v0, t;
+    hh_reflector(nx2, x0, alpha, v0, t);
+    if (threadIdx.x == 0) {
+        vn[r + 1] = v0;
+        tau[r] = t;
+        d[r] = a[r] - (vr * wr + wr * vr);
+        e[r] = t == 0.0 ? x0 : alpha;
     }
 }
 
-__global__ void k_hh_last(int n, double* A, double* d, double* e) {
-    if (n >= 2) {
-        d[n - 2] = A[(long)(n - 2) * n + n - 2];
-        e[n - 2] = A[(long)(n - 1) * n + n - 2];
-    }
-    d[n - 1] = A[(long)(n - 1) * n + n - 1];
-    e[n - 1] = 0.0;
+// the update of the last reflector (step n-3) on the trailing 2 x 2 block, then its d, e
+__global__ void k_hh_tail(int n, const double* A, const double* V, const double* W, double* d, double* e) {
+    const int k = n - 3, r0 = n - 2, r1 = n - 1;
+    const double* v = V + (long)k * n;
+    d[r0] = A[(long)r0 * n + r0] - (v[r0] * W[r0] + W[r0] * v[r0]);
+    e[r0] = A[(long)r1 * n + r0] - (v[r1] * W[r0] + W[r1] * v[r0]);
+    d[r1] = A[(long)r1 * n + r1] - (v[r1] * W[r1] + W[r1] * v[r1]);
+    e[r1] = 0.0;
 }
 
 hipError_t householder_tridiag(int n, EigWork& w, hipStream_t st) {
+    if (n < 3) return hipErrorInvalidValue;   // the callers solve n <= 2 in closed form
     // reflector rows must be zero outside their support for the blocked back-transform
     UNO_LAUNCH_FILL((long)n * n, w.V, st);
-    // partial dots of launch 1 live in the inverse-iteration work space (unused until the
-    // tridiagonal is done)
+    // the inverse-iteration work space is unused until the tridiagonal is done: the
+    // partial sums of launch 1 and the two w vectors (steps of even and odd k) live there
     double* part = w.wk;
-    const size_t lds = 2 * (size_t)n * sizeof(double);
+    double* Wb[2] = {w.wk + n, w.wk + 2L * n};
+    hipLaunchKernelGGL(k_hh_first, dim3(1), dim3(256), 0, st, n, w.A, w.V, w.tau, w.d, w.e);
     for (int k = 0; k + 2 < n; ++k) {
         const int m = n - k - 1;
-        const int nb1 = (m + 3) / 4, nb2 = (m + 7) / 8;
-        hipLaunchKernelGGL(k_hh_vsymv, dim3((unsigned)nb1), dim3(256), 0, st, n, k, w.A, w.V, w.tau, w.d, w.e, w.p,
-                           part);
-        hipLaunchKernelGGL(k_hh_wrank2, dim3((unsigned)nb2), dim3(256), lds, st, n, k, w.A, w.V, w.tau, w.p, part,
-                           nb1);
+        const int nb = (m + 3) / 4;
+        hipLaunchKernelGGL(k_hh_usymv, dim3((unsigned)nb), dim3(256), 0, st, n, k, w.A, w.V, w.tau, Wb[(k + 1) & 1],
+                           w.p, part);
+        hipLaunchKernelGGL(k_hh_next, dim3(1), dim3(256), 0, st, n, k, w.A, w.V, w.tau, w.d, w.e, w.p, part, nb,
+                           Wb[k & 1]);
     }
-    hipLaunchKernelGGL(k_hh_last, dim3(1), dim3(1), 0, st, n, w.A, w.d, w.e);
+    hipLaunchKernelGGL(k_hh_tail, dim3(1), dim3(1), 0, st, n, w.A, w.V, Wb[(n - 3) & 1], w.d, w.e);
     return hipGetLastError();
 }
 
