@@ -283,11 +283,12 @@ hipError_t gemm_f64(bool ta, bool tb, int m, int n, int k, double alpha, const d
 //   x = A[k+1:,k], alpha = -sign(x0)|x|, v = x - alpha e1, tau = 2/|v|^2
 //   p = tau A22 v, K = tau/2 p.v, w = p - K v, A22 -= v w^T + w v^T
 // d[k] = A[k][k], e[k] = alpha.  Reflector k is stored in row k of V (absolute column
-// indices).  Two launches per column: the rank-2 update of step k-1 is applied in the same
-// pass over the trailing rows that forms p of step k (the matrix is read and written
-// once per column), and a one-block launch finishes w of step k and forms reflector k+1
-// from row k+1 updated on the fly.  The trailing matrix stays symmetric (full rows are
-// updated), so row k+1 is column k+1.
+// indices).  One launch per column k: every block first finishes step k-1 for itself
+// (K and w of step k-1 from the previous launch's p and partial sums, row k updated on the
+// fly, reflector k from it: the same arithmetic in the same order in every block, so every
+// block holds the same reflector; block 0 stores it), then applies the rank-2 update of
+// step k-1 to its trailing rows and forms their p of step k.  The trailing matrix stays
+// symmetric (full rows are updated), so row k is column k.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void hh_reflector(double nx2, double x0, double& alpha, double& v0, double& t) {
     const double nx = sqrt(nx2);
@@ -297,124 +298,129 @@ __device__ __forceinline__ void hh_reflector(double nx2, double x0, double& alph
     t = (nx2 == 0.0 || vn2 == 0.0) ? 0.0 : 2.0 / vn2;
 }
 
-// reflector 0 from row 0 (one block)
-__global__ __launch_bounds__(256) void k_hh_first(int n, const double* A, double* V, double* tau, double* d, double* e) {
+// launch of column k (four trailing rows per block, one wave per row).  Pp/partp: p and
+// partial sums of step k-1 (read), Pc/partc: those of step k (written).  Dynamic LDS: 3 n
+// doubles (v of step k, v and w of step k-1).
+__global__ __launch_bounds__(256) void k_hh_col(int n, int k, double* A, double* V, double* tau, double* d, double* e,
+                                                const double* Pp, const double* partp, int npartp, double* Pc,
+                                                double* partc) {
+    extern __shared__ double lds[];
+    double* sv = lds;            // v_k (absolute indices k+1..n-1)
+    double* svp = lds + n;       // v_{k-1} (k..n-1)
+    double* swp = lds + 2 * n;   // w_{k-1} (k..n-1)
     __shared__ double sh[256];
-    const double* x = A + 1;   // row 0, columns 1..
-    double s = 0.0;
-    for (int j = threadIdx.x; j < n - 1; j += 256) s = fma(x[j], x[j], s);
-    const double nx2 = block_sum(s, sh), x0 = x[0];
+    __shared__ double pv[4];
+    double K = 0.0;
+    if (k > 0) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < npartp; b += 256) s += partp[b];
+        K = 0.5 * tau[k - 1] * block_sum(s, sh);
+        const double* vp = V + (long)(k - 1) * n;
+        for (int j = k + threadIdx.x; j < n; j += 256) {
+            const double vj = vp[j];
+            svp[j] = vj;
+            swp[j] = Pp[j] - K * vj;
+        }
+        __syncthreads();
+    }
+    // row k of the matrix after step k-1, and reflector k from it
+    const double* ak = A + (long)k * n;
+    const double vk = k > 0 ? svp[k] : 0.0, wk = k > 0 ? swp[k] : 0.0;
+    double s2 = 0.0;
+    for (int j = k + 1 + threadIdx.x; j < n; j += 256) {
+        const double x = k > 0 ? ak[j] - (vk * swp[j] + wk * svp[j]) : ak[j];
+        sv[j] = x;
+        s2 = fma(x, x, s2);
+    }
+    const double nx2 = block_sum(s2, sh);   // also orders the sv writes before the reads below
+    const double x0 = sv[k + 1];
     double alpha, v0, t;
     hh_reflector(nx2, x0, alpha, v0, t);
-    for (int j = threadIdx.x; j < n - 1; j += 256) V[1 + j] = j == 0 ? v0 : x[j];
-    if (threadIdx.x == 0) {
-        tau[0] = t;
-        d[0] = A[0];
-        e[0] = t == 0.0 ? x0 : alpha;
+    __syncthreads();   // every thread has read sv[k + 1]
+    if (threadIdx.x == 0) sv[k + 1] = v0;
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        double* vrow = V + (long)k * n;
+        for (int j = k + 1 + threadIdx.x; j < n; j += 256) vrow[j] = sv[j];
+        if (threadIdx.x == 0) {
+            tau[k] = t;
+            d[k] = k > 0 ? ak[k] - (vk * wk + wk * vk) : ak[k];
+            e[k] = t == 0.0 ? x0 : alpha;
+        }
     }
-}
-
-// step k, launch 1 (one wave per trailing row, four rows per block): for k > 0 the update
-// A -= v' w'^T + w' v'^T of step k-1 (v' = V row k-1, w' = Wprev) on the row's columns
-// k+1..n-1, written back; then p_r = tau_k a_r . v_k into P[r], and the block's sum of
-// p_r v_r into part[block]
-__global__ __launch_bounds__(256) void k_hh_usymv(int n, int k, double* A, const double* V, const double* tau,
-                                                  const double* Wprev, double* P, double* part) {
-    __shared__ double pv[4];
+    // the block's trailing rows: update of step k-1, then p of step k
     const int m = n - k - 1;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + wave;
-    const double* v = V + (long)k * n;
     double pr = 0.0;
     if (row < m) {
         const int r = k + 1 + row;
         double* a = A + (long)r * n;
         double acc = 0.0;
         if (k > 0) {
-            const double* vp = V + (long)(k - 1) * n;
-            const double vr = vp[r], wr = Wprev[r];
+            const double vr = svp[r], wr = swp[r];
             for (int j = k + 1 + lane; j < n; j += 64) {
-                const double x = a[j] - (vr * Wprev[j] + wr * vp[j]);
+                const double x = a[j] - (vr * swp[j] + wr * svp[j]);
                 a[j] = x;
-                acc = fma(x, v[j], acc);
+                acc = fma(x, sv[j], acc);
             }
         } else {
-            for (int j = k + 1 + lane; j < n; j += 64) acc = fma(a[j], v[j], acc);
+            for (int j = k + 1 + lane; j < n; j += 64) acc = fma(a[j], sv[j], acc);
         }
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-        pr = tau[k] * acc;
-        if (lane == 0) P[r] = pr;
+        pr = t * acc;
+        if (lane == 0) Pc[r] = pr;
     }
-    if (lane == 0) pv[wave] = row < m ? pr * v[k + 1 + row] : 0.0;
+    if (lane == 0) pv[wave] = row < m ? pr * sv[k + 1 + row] : 0.0;
     __syncthreads();
-    if (threadIdx.x == 0) part[blockIdx.x] = (pv[0] + pv[1]) + (pv[2] + pv[3]);
+    if (threadIdx.x == 0) partc[blockIdx.x] = (pv[0] + pv[1]) + (pv[2] + pv[3]);
 }
 
-// step k, launch 2 (one block): K = tau_k/2 p.v from the partial sums (fixed order),
-// w = p - K v into Wcur; unless k is the last reflector, row k+1 after the update of step
-// k gives d[k+1] and reflector k+1 (V row k+1, tau, e)
-__global__ __launch_bounds__(256) void k_hh_next(int n, int k, const double* A, double* V, double* tau, double* d,
-                                                 double* e, const double* P, const double* part, int nparts,
-                                                 double* Wcur) {
+// after the last reflector (step n-3): its w on the trailing 2 x 2 block, the update
+// there, and that block's d, e
+__global__ __launch_bounds__(256) void k_hh_tail(int n, const double* A, const double* V, const double* tau,
+                                                 const double* P, const double* part, int nparts, double* d, double* e) {
     __shared__ double sh[256];
     double s = 0.0;
     for (int b = threadIdx.x; b < nparts; b += 256) s += part[b];
-    const double K = 0.5 * tau[k] * block_sum(s, sh);
-    const double* v = V + (long)k * n;
-    for (int j = k + 1 + threadIdx.x; j < n; j += 256) Wcur[j] = P[j] - K * v[j];
-    if (k + 1 > n - 3) return;   // uniform: the tail launch finishes the last 2 x 2 block
-    const int r = k + 1;
-    const double vr = v[r], wr = P[r] - K * vr;
-    const double* a = A + (long)r * n;
-    double* vn = V + (long)r * n;
-    double s2 = 0.0;
-    for (int j = r + 1 + threadIdx.x; j < n; j += 256) {
-        const double wj = P[j] - K * v[j];
-        const double x = a[j] - (vr * wj + wr * v[j]);
-        vn[j] = x;
-        s2 = fma(x, x, s2);
-    }
-    const double nx2 = block_sum(s2, sh);
-    const double w1 = P[r + 1] - K * v[r + 1];
-    const double x0 = a[r + 1] - (vr * w1 + wr * v[r + 1]);   // thread 0's first entry, recomputed
-    double alpha, v0, t;
-    hh_reflector(nx2, x0, alpha, v0, t);
-    if (threadIdx.x == 0) {
-        vn[r + 1] = v0;
-        tau[r] = t;
-        d[r] = a[r] - (vr * wr + wr * vr);
-        e[r] = t == 0.0 ? x0 : alpha;
-    }
-}
-
-// the update of the last reflector (step n-3) on the trailing 2 x 2 block, then its d, e
-__global__ void k_hh_tail(int n, const double* A, const double* V, const double* W, double* d, double* e) {
     const int k = n - 3, r0 = n - 2, r1 = n - 1;
-    const double* v = V + (long)k * n;
-    d[r0] = A[(long)r0 * n + r0] - (v[r0] * W[r0] + W[r0] * v[r0]);
-    e[r0] = A[(long)r1 * n + r0] - (v[r1] * W[r0] + W[r1] * v[r0]);
-    d[r1] = A[(long)r1 * n + r1] - (v[r1] * W[r1] + W[r1] * v[r1]);
-    e[r1] = 0.0;
+    const double K = 0.5 * tau[k] * block_sum(s, sh);
+    if (threadIdx.x == 0) {
+        const double* v = V + (long)k * n;
+        const double w0 = P[r0] - K * v[r0], w1 = P[r1] - K * v[r1];
+        d[r0] = A[(long)r0 * n + r0] - (v[r0] * w0 + w0 * v[r0]);
+        e[r0] = A[(long)r1 * n + r0] - (v[r1] * w0 + w1 * v[r0]);
+        d[r1] = A[(long)r1 * n + r1] - (v[r1] * w1 + w1 * v[r1]);
+        e[r1] = 0.0;
+    }
 }
 
 hipError_t householder_tridiag(int n, EigWork& w, hipStream_t st) {
     if (n < 3) return hipErrorInvalidValue;   // the callers solve n <= 2 in closed form
     // reflector rows must be zero outside their support for the blocked back-transform
     UNO_LAUNCH_FILL((long)n * n, w.V, st);
-    // the inverse-iteration work space is unused until the tridiagonal is done: the
-    // partial sums of launch 1 and the two w vectors (steps of even and odd k) live there
-    double* part = w.wk;
-    double* Wb[2] = {w.wk + n, w.wk + 2L * n};
-    hipLaunchKernelGGL(k_hh_first, dim3(1), dim3(256), 0, st, n, w.A, w.V, w.tau, w.d, w.e);
+    // the inverse-iteration work space is unused until the tridiagonal is done: p and the
+    // partial sums of the steps of even and odd k live there
+    double* P[2] = {w.wk, w.wk + n};
+    double* part[2] = {w.wk + 2L * n, w.wk + 3L * n};
+    const size_t lds = 3 * (size_t)n * sizeof(double);
+    static size_t attr = 0;
+    if (lds > attr) {
+        hipError_t er = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_hh_col),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (er != hipSuccess) return er;
+        attr = lds;
+    }
+    int nbp = 0;
     for (int k = 0; k + 2 < n; ++k) {
         const int m = n - k - 1;
         const int nb = (m + 3) / 4;
-        hipLaunchKernelGGL(k_hh_usymv, dim3((unsigned)nb), dim3(256), 0, st, n, k, w.A, w.V, w.tau, Wb[(k + 1) & 1],
-                           w.p, part);
-        hipLaunchKernelGGL(k_hh_next, dim3(1), dim3(256), 0, st, n, k, w.A, w.V, w.tau, w.d, w.e, w.p, part, nb,
-                           Wb[k & 1]);
+        hipLaunchKernelGGL(k_hh_col, dim3((unsigned)nb), dim3(256), lds, st, n, k, w.A, w.V, w.tau, w.d, w.e,
+                           P[(k + 1) & 1], part[(k + 1) & 1], nbp, P[k & 1], part[k & 1]);
+        nbp = nb;
     }
-    hipLaunchKernelGGL(k_hh_tail, dim3(1), dim3(1), 0, st, n, w.A, w.V, Wb[(n - 3) & 1], w.d, w.e);
+    hipLaunchKernelGGL(k_hh_tail, dim3(1), dim3(256), 0, st, n, w.A, w.V, w.tau, P[(n - 3) & 1], part[(n - 3) & 1], nbp,
+                       w.d, w.e);
     return hipGetLastError();
 }
 

@@ -463,11 +463,17 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
         if (bytes) std::memcpy(hs.data() + off, v.data(), bytes);
         return off;
     };
-    for (QuartetClass& q : c->qcls) {
+    // the spaces are independent: their index arrays are built in parallel, then appended
+    // to the staging buffer in class order
+    const int nqc = (int)c->qcls.size();
+    std::vector<SpaceRec> built(2 * (size_t)nqc);
+    std::vector<char> used(2 * (size_t)nqc, 0);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int qi = 0; qi < nqc; ++qi) {
+        QuartetClass& q = c->qcls[qi];
         build_prefix(c, q, 4.0 * dmax, mixed ? c->fp32_thr : 0.0);
         const PairClassList& LX = c->cls[q.X];
         const PairClassList& LY = c->cls[q.Y];
-        const int qi = (int)(&q - c->qcls.data());
         for (int prec = 0; prec < 2; ++prec) {
             QuartetSpace& sp = prec ? q.s32 : q.s64;
             if (sp.nq == 0) continue;
@@ -494,20 +500,27 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
             } else {
                 build_blocks(sp, eri_quartets_per_block(LX.lcls, LY.lcls));
             }
-            if (!c->tuning) {
-                r.o_prefix = put(sp.prefix);
-                if (r.tasks) {
-                    r.o_wpre = put(sp.wpre);
-                    r.o_tblk = put(sp.tblk);
-                    r.o_rbra = put(sp.rbra);
-                    r.o_rpre = put(sp.rpre);
-                } else {
-                    r.o_blk = put(sp.blk);
-                }
-                if (prec) r.o_kstart = put(sp.kstart);
-            }
-            recs.push_back(r);
+            built[2 * (size_t)qi + prec] = r;
+            used[2 * (size_t)qi + prec] = 1;
         }
+    }
+    for (size_t k = 0; k < built.size(); ++k) {
+        if (!used[k]) continue;
+        SpaceRec r = built[k];
+        const QuartetSpace& sp = r.prec ? r.q->s32 : r.q->s64;
+        if (!c->tuning) {
+            r.o_prefix = put(sp.prefix);
+            if (r.tasks) {
+                r.o_wpre = put(sp.wpre);
+                r.o_tblk = put(sp.tblk);
+                r.o_rbra = put(sp.rbra);
+                r.o_rpre = put(sp.rpre);
+            } else {
+                r.o_blk = put(sp.blk);
+            }
+            if (r.prec) r.o_kstart = put(sp.kstart);
+        }
+        recs.push_back(r);
     }
     char* ds = nullptr;
     if (!c->tuning && !hs.empty()) {
