@@ -411,7 +411,7 @@ def main():
         return {"scf_s": round(secs, 4), "context_s": round(ctx, 4), "iterations": res.iterations,
                 "fock_builds": res.nfock,
                 "full_builds": res.nfull, "fock_s": round(res.fock_s, 4), "diag_s": round(res.diag_s, 4),
-                "guess_s": round(res.guess_s, 4), "shell_quartets": res.quartets, "fp32_quartets": res.quartets32,
+                "guess_s": round(res.guess_s, 4), "setup_s": round(res.setup_s, 4), "diis_s": round(res.diis_s, 4), "shell_quartets": res.quartets, "fp32_quartets": res.quartets32,
                 "fock_builds_per_s": round(res.nfock / res.fock_s, 4) if res.fock_s > 0 else None,
                 "shell_quartets_per_s": round(res.quartets / res.fock_s, 1) if res.fock_s > 0 else None}
 

@@ -129,6 +129,9 @@ typedef struct uno_scf_result {
     double guess_s;       /* seconds spent on the learned guess (guess = 2) */
     double wall99_s;      /* batched runs: seconds until 99 % of the molecules had stopped
                              (converged or at maxits); single runs: wall_s */
+    double setup_s;       /* seconds before the first update: buffers, one-electron
+                             matrices, X, the guess */
+    double diis_s;        /* seconds in the DIIS extrapolation (accel = 2) */
 } uno_scf_result;
 
 void uno_scf_defaults(uno_scf_opts* opts);

@@ -41,7 +41,8 @@ class ScfResult(C.Structure):
                 ("electronic", C.c_double), ("iterations", C.c_int), ("converged", C.c_int),
                 ("uhf", C.c_int), ("nfock", C.c_int), ("wall_s", C.c_double), ("fock_s", C.c_double),
                 ("diag_s", C.c_double), ("quartets", C.c_double), ("quartets32", C.c_double),
-                ("nfull", C.c_int), ("guess_s", C.c_double), ("wall99_s", C.c_double)]
+                ("nfull", C.c_int), ("guess_s", C.c_double), ("wall99_s", C.c_double),
+                ("setup_s", C.c_double), ("diis_s", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

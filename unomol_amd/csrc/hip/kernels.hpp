@@ -36,6 +36,8 @@ hipError_t k_pack_sym_add(int n, const double* W, double* packed, hipStream_t st
 hipError_t k_sym_w(int n, const double* W, double* G, hipStream_t st, bool accumulate = false);
 hipError_t k_axpby(long len, double a, const double* x, double b, const double* y, double* z, hipStream_t st);
 hipError_t k_fill(long len, double v, double* x, hipStream_t st);
+// X[i][j] = Z[i][j] f[j] (n x n, f on the device)
+hipError_t k_scale_cols(int n, const double* Z, const double* f, double* X, hipStream_t st);
 // out[0] += sum_ij A_ij B_ij (full matrices); deterministic two-pass reduction whose
 // workspace ws holds kRedBlocks partial sums (one workspace per stream)
 constexpr int kRedBlocks = 240;
@@ -54,7 +56,6 @@ hipError_t gemm_f64(bool ta, bool tb, int m, int n, int k, double alpha, const d
 // eigensolver pieces (linalg.hip)
 struct EigWork {
     double* A;      // n x n working copy (destroyed)
-    double* Q;      // n x n accumulated reflectors
     double* V;      // n x n reflector vectors (row k = v_k, zero outside its support)
     double* tau;    // n
     double* p;      // n
@@ -64,13 +65,10 @@ struct EigWork {
     double* evals;  // n  eigenvalues of T (ascending)
     double* Y;      // n x n eigenvectors of T, one per column (Yt)
     double* wk;     // 5 n^2 inverse-iteration work
-    double* wyS;    // 32 x 32
-    double* wyT;    // 32 x 32
-    double* wy1;    // 32 x n
-    double* wy2;    // 32 x n
 };
 hipError_t householder_tridiag(int n, EigWork& w, hipStream_t st);
-hipError_t accumulate_q(int n, EigWork& w, hipStream_t st);
+// Z <- Q Z with Q = H_0 ... H_{n-3} from the reflectors in w.V / w.tau (uses w.wk)
+hipError_t apply_q(int n, EigWork& w, double* Z, hipStream_t st);
 // nclusters < 0: multisection for w.evals; otherwise inverse iteration into the columns of
 // w.Y followed by Gram-Schmidt inside the given clusters ({first, end} pairs, device memory)
 hipError_t tridiag_eigen(int n, EigWork& w, int nclusters, const int* clusters, hipStream_t st);

@@ -82,6 +82,17 @@ hipError_t k_fill(long len, double v, double* x, hipStream_t st) {
     return hipGetLastError();
 }
 
+__global__ void k_scale_cols_(int n, const double* __restrict__ Z, const double* __restrict__ f, double* __restrict__ X) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (long)n * n) X[t] = Z[t] * f[t % n];
+}
+hipError_t k_scale_cols(int n, const double* Z, const double* f, double* X, hipStream_t st) {
+    const long nn = (long)n * n;
+    if (nn <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scale_cols_, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, n, Z, f, X);
+    return hipGetLastError();
+}
+
 // deterministic block reduction (fixed order; blockDim.x a power of two)
 __device__ double block_sum(double v, double* sh) {
     const int t = threadIdx.x;
@@ -425,47 +436,83 @@ hipError_t householder_tridiag(int n, EigWork& w, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
-// Q = H_0 H_1 ... H_{n-3} by blocks of NB reflectors in compact WY form
+// Back-transform Z <- H_0 H_1 ... H_{n-3} Z by blocks of kWy reflectors in compact WY form
 // (H_k0 ... H_k1-1 = I - Vb^T T Vb with the reflectors as the rows of Vb and T upper
-// triangular), applied from the left last block first: Q <- Q - Vb^T (T (Vb Q)).
+// triangular), the last block first: Z <- Z - Vb^T (M Z) with M = T Vb.  One launch forms
+// every block's M (S = Vb Vb^T, the T recurrence and T Vb, one workgroup per block); then
+// two MFMA GEMMs per block.
 // ---------------------------------------------------------------------------
-__global__ void k_eye(int n, double* Q) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < (long)n * n) Q[t] = (t / n == t % n) ? 1.0 : 0.0;
-}
+constexpr int kWy = 32;
 
-// T[j][j] = tau_j, T[0:j][j] = -tau_j T[0:j][0:j] S[0:j][j], S = Vb Vb^T (nb x nb)
-__global__ void k_wy_t(int nb, const double* __restrict__ S, const double* __restrict__ tau, double* __restrict__ T) {
-    __shared__ double sT[64][65];
-    const int i = threadIdx.x;
-    for (int q = i; q < 64 * 65; q += blockDim.x) sT[q / 65][q % 65] = 0.0;
-    __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        if (i < j) {
-            double s = 0.0;
-            for (int l = i; l < j; ++l) s = fma(sT[i][l], S[l * nb + j], s);
-            sT[i][j] = -tau[j] * s;
+// T[j][j] = tau_j, T[0:j][j] = -tau_j T[0:j][0:j] S[0:j][j], S = Vb Vb^T; M = T Vb
+__global__ __launch_bounds__(256) void k_wy_m(int n, int nref, const double* __restrict__ V,
+                                              const double* __restrict__ tau, double* __restrict__ M) {
+    __shared__ double sS[kWy][kWy + 1];
+    __shared__ double sT[kWy][kWy + 1];
+    __shared__ double sV[kWy][65];
+    const int k0 = blockIdx.x * kWy, nb = min(kWy, nref - k0);
+    const double* Vb = V + (long)k0 * n;
+    const int t = threadIdx.x;
+    // S, in column chunks of 64 staged in LDS; thread t owns entries (l, j) = t / 32 + 8 q, t % 32
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c0 = 0; c0 < n; c0 += 64) {
+        for (int q = t; q < kWy * 64; q += 256) {
+            const int l = q / 64, c = q % 64;
+            sV[l][c] = (l < nb && c0 + c < n) ? Vb[(long)l * n + c0 + c] : 0.0;
         }
-        if (i == j) sT[j][j] = tau[j];
+        __syncthreads();
+        const int j = t % kWy;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int l = t / kWy + 8 * q;
+            double s = acc[q];
+            for (int c = 0; c < 64; ++c) s = fma(sV[l][c], sV[j][c], s);
+            acc[q] = s;
+        }
         __syncthreads();
     }
-    for (int q = i; q < nb * nb; q += blockDim.x) T[q] = sT[q / nb][q % nb];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sS[t / kWy + 8 * q][t % kWy] = acc[q];
+    for (int q = t; q < kWy * (kWy + 1); q += 256) sT[q / (kWy + 1)][q % (kWy + 1)] = 0.0;
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        if (t < j) {
+            double s = 0.0;
+            for (int l = t; l < j; ++l) s = fma(sT[t][l], sS[l][j], s);
+            sT[t][j] = -tau[k0 + j] * s;
+        }
+        if (t == j) sT[j][j] = tau[k0 + j];
+        __syncthreads();
+    }
+    // M = T Vb (nb x n), thread per column
+    double* Mb = M + (long)k0 * n;
+    for (int c = t; c < n; c += 256) {
+        double v[kWy];
+#pragma unroll
+        for (int j2 = 0; j2 < kWy; ++j2) v[j2] = j2 < nb ? Vb[(long)j2 * n + c] : 0.0;
+#pragma unroll 4
+        for (int l = 0; l < nb; ++l) {
+            double s = 0.0;
+#pragma unroll
+            for (int j2 = 0; j2 < kWy; ++j2) s = fma(sT[l][j2], v[j2], s);
+            Mb[(long)l * n + c] = s;
+        }
+    }
 }
 
-hipError_t accumulate_q(int n, EigWork& w, hipStream_t st) {
-    const long nn = (long)n * n;
-    hipLaunchKernelGGL(k_eye, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, n, w.Q);
+hipError_t apply_q(int n, EigWork& w, double* Z, hipStream_t st) {
     const int nref = n - 2;
-    const int NB = 32;
-    for (int k0 = ((nref - 1) / NB) * NB; k0 >= 0 && nref > 0; k0 -= NB) {
-        const int nb = (k0 + NB <= nref) ? NB : nref - k0;
-        const double* Vb = w.V + (long)k0 * n;   // nb rows
+    if (nref <= 0) return hipSuccess;
+    const int nblk = (nref + kWy - 1) / kWy;
+    double* M = w.wk;              // nblk * kWy rows of n (within the 5 n^2 work space)
+    double* wy = w.wk + 2L * n * n;  // kWy x n
+    hipLaunchKernelGGL(k_wy_m, dim3((unsigned)nblk), dim3(256), 0, st, n, nref, w.V, w.tau, M);
+    for (int b = nblk - 1; b >= 0; --b) {
+        const int k0 = b * kWy, nb = std::min(kWy, nref - k0);
+        const double* Vb = w.V + (long)k0 * n;
         hipError_t e;
-        if ((e = gemm_f64(false, true, nb, nb, n, 1.0, Vb, n, Vb, n, 0.0, w.wyS, nb, st))) return e;
-        hipLaunchKernelGGL(k_wy_t, dim3(1), dim3(64), 0, st, nb, w.wyS, w.tau + k0, w.wyT);
-        if ((e = gemm_f64(false, false, nb, n, n, 1.0, Vb, n, w.Q, n, 0.0, w.wy1, n, st))) return e;
-        if ((e = gemm_f64(false, false, nb, n, nb, 1.0, w.wyT, nb, w.wy1, n, 0.0, w.wy2, n, st))) return e;
-        if ((e = gemm_f64(true, false, n, n, nb, -1.0, Vb, n, w.wy2, n, 1.0, w.Q, n, st))) return e;
+        if ((e = gemm_f64(false, false, nb, n, n, 1.0, M + (long)k0 * n, n, Z, n, 0.0, wy, n, st))) return e;
+        if ((e = gemm_f64(true, false, n, n, nb, -1.0, Vb, n, wy, n, 1.0, Z, n, st))) return e;
     }
     return hipGetLastError();
 }
@@ -599,35 +646,75 @@ __global__ __launch_bounds__(64) void k_inviter(int n, const double* __restrict_
         h ^= h << 13; h ^= h >> 17; h ^= h << 5;
         x[k * S] = ((double)(h & 0xffffff) / 16777216.0) - 0.5 + 1e-3;
     }
+    // the solves walk the vector in chunks of kCh entries: a chunk's loads are issued
+    // together ahead of its dependent chain (one memory latency per chunk, not per entry)
+    constexpr int kCh = 16;
     for (int it = 0; it < 3; ++it) {
         // forward: apply the row interchanges and multipliers
         double xk = x[0];
-        for (int k = 0; k < n - 1; ++k) {
-            const double l = L[(long)k * S + i];
-            const double xn = x[(k + 1) * S];
-            if (piv[(long)k * S + i]) {
-                x[k * S] = xn;
-                xk = xk - l * xn;
-            } else {
-                x[k * S] = xk;
-                xk = xn - l * xk;
+        for (int k0 = 0; k0 < n - 1; k0 += kCh) {
+            double xs[kCh], ls[kCh];
+            int ps[kCh];
+#pragma unroll
+            for (int c = 0; c < kCh; ++c) {
+                const int k = k0 + c;
+                if (k < n - 1) {
+                    xs[c] = x[(long)(k + 1) * S];
+                    ls[c] = L[(long)k * S + i];
+                    ps[c] = piv[(long)k * S + i];
+                }
             }
+#pragma unroll
+            for (int c = 0; c < kCh; ++c) {
+                if (k0 + c < n - 1) {
+                    const double xn = xs[c];
+                    if (ps[c]) {
+                        xs[c] = xn;
+                        xk = xk - ls[c] * xn;
+                    } else {
+                        xs[c] = xk;
+                        xk = xn - ls[c] * xk;
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < kCh; ++c)
+                if (k0 + c < n - 1) x[(long)(k0 + c) * S] = xs[c];
         }
         x[(long)(n - 1) * S] = xk;
         // back substitution with the banded U, then the norm
         double x1 = 0.0, x2 = 0.0, nrm = 0.0;
-        for (int k = n - 1; k >= 0; --k) {
-            double v = x[k * S];
-            v -= U2[(long)k * S + i] * x1;
-            v -= U3[(long)k * S + i] * x2;
-            v *= U1[(long)k * S + i];
-            x[k * S] = v;
-            nrm = fma(v, v, nrm);
-            x2 = x1;
-            x1 = v;
+        for (int k1 = n - 1; k1 >= 0; k1 -= kCh) {
+            double xs[kCh], u1[kCh], u2[kCh], u3[kCh];
+#pragma unroll
+            for (int c = 0; c < kCh; ++c) {
+                const int k = k1 - c;
+                if (k >= 0) {
+                    xs[c] = x[(long)k * S];
+                    u1[c] = U1[(long)k * S + i];
+                    u2[c] = U2[(long)k * S + i];
+                    u3[c] = U3[(long)k * S + i];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < kCh; ++c) {
+                if (k1 - c >= 0) {
+                    double v = xs[c];
+                    v -= u2[c] * x1;
+                    v -= u3[c] * x2;
+                    v *= u1[c];
+                    xs[c] = v;
+                    nrm = fma(v, v, nrm);
+                    x2 = x1;
+                    x1 = v;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < kCh; ++c)
+                if (k1 - c >= 0) x[(long)(k1 - c) * S] = xs[c];
         }
         nrm = 1.0 / sqrt(nrm);
-        for (int k = 0; k < n; ++k) x[k * S] *= nrm;
+        for (int k = 0; k < n; ++k) x[(long)k * S] *= nrm;
     }
 }
 

@@ -1,14 +1,14 @@
 // Dense symmetric eigensolver for device-resident matrices (contract of the
 // reference's SymmPack::rsp, SymmPack.cpp:272-288: eigenvalues ascending, eigenvectors
 // in the columns of a row-major Z).  All steps run on the GPU:
-//   1. Householder tridiagonalisation A = Q T Q^T (one reflector per column: symv and
-//      rank-2 update kernels);
+//   1. Householder tridiagonalisation A = Q T Q^T (one reflector per column, one launch
+//      per column: the previous rank-2 update fused with the next matrix-vector product);
 //   2. eigenvalues of T by multisection on Sturm counts, one eigenvalue per wave;
 //   3. eigenvectors of T by inverse iteration, one per thread (the vectors stored as the
 //      columns of Yt, so a wave's accesses coalesce), then two-pass modified Gram-Schmidt
 //      inside clusters of close eigenvalues (exact degeneracies included);
-//   4. Q accumulated from the reflectors in compact-WY blocks with MFMA GEMMs;
-//   5. Z = Q Yt with one MFMA GEMM.
+//   4. Z = Q Yt: the reflectors applied to Yt in compact-WY blocks of 32 (every block's
+//      T Vb formed in one launch, then two MFMA GEMMs per block).
 // The reference reduces with Householder and diagonalises T with implicit QL
 // (SymmPack.cpp:67-269); the contract (not the steps) is what the callers rely on.
 #include <algorithm>
@@ -20,20 +20,19 @@ namespace uno {
 
 struct EigenWS {
     int n = 0;
-    DBuf<double> A, Q, V, tau, p, d, e, scal, evals, Y, wk, wyS, wyT, wy1, wy2;
+    DBuf<double> V, tau, p, d, e, scal, evals, Y, wk;
     DBuf<int> clusters;
     std::vector<double> hev;
     std::vector<int> hcl;
     hipError_t ensure(int nn) {
-        if (nn <= n && Q.p) return hipSuccess;
+        if (nn <= n && V.p) return hipSuccess;
         n = nn;
         const size_t sq = (size_t)nn * nn;
         hipError_t r;
-        if ((r = Q.alloc(sq)) || (r = V.alloc(sq)) || (r = tau.alloc(nn)) || (r = p.alloc(nn)) ||
+        if ((r = V.alloc(sq)) || (r = tau.alloc(nn)) || (r = p.alloc(nn)) ||
             (r = d.alloc(nn)) ||
             (r = e.alloc(nn)) || (r = scal.alloc(4)) || (r = evals.alloc(nn)) || (r = Y.alloc(sq)) ||
-            (r = wk.alloc(5 * sq)) || (r = wyS.alloc(32 * 32)) || (r = wyT.alloc(32 * 32)) ||
-            (r = wy1.alloc(32 * (size_t)nn)) || (r = wy2.alloc(32 * (size_t)nn)) || (r = clusters.alloc(2 * (size_t)nn)))
+            (r = wk.alloc(5 * sq)) || (r = clusters.alloc(2 * (size_t)nn)))
             return r;
         return hipSuccess;
     }
@@ -67,8 +66,7 @@ static int eigh_impl(EigenWS& w, int n, double* dA, double* dZ, double* h_evals,
         UNO_HIP(hipStreamSynchronize(st));
         return UNO_OK;
     }
-    EigWork ew{dA, w.Q.p, w.V.p, w.tau.p, w.p.p, w.d.p, w.e.p, w.scal.p, w.evals.p, w.Y.p, w.wk.p,
-               w.wyS.p, w.wyT.p, w.wy1.p, w.wy2.p};
+    EigWork ew{dA, w.V.p, w.tau.p, w.p.p, w.d.p, w.e.p, w.scal.p, w.evals.p, w.Y.p, w.wk.p};
     UNO_HIP(householder_tridiag(n, ew, st));
     UNO_HIP(tridiag_eigen(n, ew, -1, nullptr, st));
     w.hev.resize(n);
@@ -91,8 +89,9 @@ static int eigh_impl(EigenWS& w, int n, double* dA, double* dZ, double* h_evals,
     const int ncl = (int)(w.hcl.size() / 2);
     if (ncl) UNO_HIP(w.clusters.upload(w.hcl.data(), w.hcl.size(), st));
     UNO_HIP(tridiag_eigen(n, ew, ncl, w.clusters.p, st));
-    UNO_HIP(accumulate_q(n, ew, st));
-    UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, w.Q.p, n, w.Y.p, n, 0.0, dZ, n, st));   // Z = Q Yt
+    // Z = Q Yt: the eigenvectors of T back-transformed by the reflectors in place
+    UNO_HIP(hipMemcpyAsync(dZ, w.Y.p, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    UNO_HIP(apply_q(n, ew, dZ, st));
     std::copy(w.hev.begin(), w.hev.end(), h_evals);
     return UNO_OK;
 }

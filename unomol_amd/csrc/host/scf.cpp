@@ -125,11 +125,9 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
             nuc += c->B.q[i] * c->B.q[j] / std::sqrt(dx * dx + dy * dy + dz * dz);
         }
     // X = U s^-1/2 keeping every vector (RHF.hpp:214-233)
-    std::vector<double> ev(n), hZ(nn);
+    std::vector<double> ev(n);
     UNO_HIP(hipMemcpyAsync(b.Tm.p, b.S.p, nn * sizeof(double), hipMemcpyDeviceToDevice, st));
     if (int rc = eigh_device(c, n, b.Tm.p, b.Cw.p, ev.data())) return rc;
-    UNO_HIP(hipMemcpyAsync(hZ.data(), b.Cw.p, nn * sizeof(double), hipMemcpyDeviceToHost, st));
-    UNO_HIP(hipStreamSynchronize(st));
     for (int j = 0; j < n; ++j) {
         double f = ev[j];
         if (std::fabs(f) < 1e-7 && f < 0.0) f = -f;
@@ -137,13 +135,13 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
         if (f < 0.0) { set_error("Negative Eigenvalue in Overlap Matrix"); return UNO_ESINGULAR; }
         ev[j] = 1.0 / std::sqrt(f);
     }
-    for (long i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) hZ[i * n + j] *= ev[j];
-    UNO_HIP(b.X.upload(hZ.data(), nn, st));
+    // column j of U scaled by s_j^-1/2 on the device
+    UNO_HIP(c->pk.upload(ev.data(), n, st));
+    UNO_HIP(k_scale_cols(n, b.Cw.p, c->pk.p, b.X.p, st));
 
     double* P[2] = {b.P[0].p, b.P[1].p};
     std::vector<double> evals(2 * (size_t)n, 0.0);
-    double diag_s = 0.0, fock_s = 0.0;
+    double diag_s = 0.0, fock_s = 0.0, diis_s = 0.0;
 
     // F (device, n x n) -> P[s] with nocc electrons; keeps C[s] and the eigenvalues
     auto diag_to_P = [&](const double* Fd, int s, int no) -> int {
@@ -251,6 +249,7 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
         double* Fs[2] = {b.F.p, b.Etmp.p};
         for (int s = 0; s < ns; ++s) UNO_HIP(k_axpby(nn, 1.0, b.H.p, 1.0, b.G[s].p, Fs[s], st));
         if (o.accel == 2) {
+            const auto td = clk::now();
             const int slot = diis_next;
             diis_next = (diis_next + 1) % kDiis;
             if (ndiis < kDiis) ++ndiis;
@@ -294,6 +293,7 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
                     }
                 }
             }
+            diis_s += secs(td, clk::now());
         }
         pdiff = 0.0;
         for (int s = 0; s < ns; ++s) {
@@ -332,6 +332,7 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
     };
     iteration = 0;
     eold = 0.0;
+    r.setup_s = secs(t_start, clk::now());
     if (int rc = update()) return rc;
     const double init_energy = energy + nuc;
     if (!o.single_start) {
@@ -375,6 +376,7 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
     r.uhf = uhf ? 1 : 0;
     r.fock_s = fock_s;
     r.diag_s = diag_s;
+    r.diis_s = diis_s;
     r.wall_s = secs(t_start, clk::now());
     r.wall99_s = r.wall_s;
     return conv ? UNO_OK : UNO_ENOCONV;
