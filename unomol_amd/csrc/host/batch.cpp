@@ -186,10 +186,10 @@ static int batch_fock(uno_batch* b, const std::vector<double>& dmax, double& qua
             const PairClassList& LX = c->cls[q.X];
             const PairClassList& LY = c->cls[q.Y];
             Ent& e = groups[std::make_tuple(LX.lcls, LY.lcls, x.uhf ? 1 : 0)];
-            for (int i = 0; i < LX.count; ++i) {
+            for (int i = 0; i + 1 < (int)q.s64.prefix.size(); ++i) {
                 const long long cnt = q.s64.prefix[i + 1] - q.s64.prefix[i];
                 if (cnt == 0) continue;
-                e.bra.push_back(x.pair_off + LX.off + i);
+                e.bra.push_back(x.pair_off + LX.off + q.s64.b0 + i);
                 e.ket.push_back(x.pair_off + LY.off);
                 e.mol.push_back(m);
                 e.cnt.push_back(cnt);

@@ -69,6 +69,24 @@ void build_prefix(uno_ctx* c, QuartetClass& q, double dmax, double fp32_thr) {
     }
     q.s64.nq = q.s64.prefix[nb];
     q.s32.nq = mixed ? q.s32.prefix[nb] : 0;
+    // drop the empty bras at either end of each space (with a small density most bras of
+    // an incremental build have no ket left): the arrays shrink, b0 records the offset
+    auto trim = [](QuartetSpace& sp, bool has_kstart) {
+        sp.b0 = 0;
+        const int nbf = (int)sp.prefix.size() - 1;
+        int lo = 0, hi = nbf;
+        while (lo < hi && sp.prefix[lo + 1] == sp.prefix[lo]) ++lo;
+        while (hi > lo && sp.prefix[hi] == sp.prefix[hi - 1]) --hi;
+        if (lo == 0 && hi == nbf) return;
+        std::vector<long long> np(sp.prefix.begin() + lo, sp.prefix.begin() + hi + 1);
+        const long long base = np[0];
+        for (long long& v : np) v -= base;
+        sp.prefix.swap(np);
+        if (has_kstart) sp.kstart = std::vector<int>(sp.kstart.begin() + lo, sp.kstart.begin() + hi);
+        sp.b0 = lo;
+    };
+    trim(q.s64, false);
+    if (mixed) trim(q.s32, true);
 }
 
 // wave tasks: bra i owns ceil(count_i / 64) consecutive tasks from wpre[i]; tblk[b] is
@@ -524,8 +542,21 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
     }
     char* ds = nullptr;
     if (!c->tuning && !hs.empty()) {
+        // the previous build's upload must have read the pinned copy before it is reused
+        // (the device buffer is reused in stream order)
+        if (c->stage_ev) UNO_HIP(hipEventSynchronize(c->stage_ev));
+        else UNO_HIP(hipEventCreateWithFlags(&c->stage_ev, hipEventDisableTiming));
+        if (c->stage_pin_cap < hs.size()) {
+            if (c->stage_pin) UNO_HIP(hipHostFree(c->stage_pin));
+            c->stage_pin = nullptr;
+            c->stage_pin_cap = 0;
+            UNO_HIP(hipHostMalloc((void**)&c->stage_pin, hs.size()));
+            c->stage_pin_cap = hs.size();
+        }
+        std::memcpy(c->stage_pin, hs.data(), hs.size());
         UNO_HIP(c->stage_d.alloc(hs.size()));
-        UNO_HIP(hipMemcpyAsync(c->stage_d.p, hs.data(), hs.size(), hipMemcpyHostToDevice, st));
+        UNO_HIP(hipMemcpyAsync(c->stage_d.p, c->stage_pin, hs.size(), hipMemcpyHostToDevice, st));
+        UNO_HIP(hipEventRecord(c->stage_ev, st));
         ds = c->stage_d.p;
     }
 
@@ -539,7 +570,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
         const int qi = (int)(&q - c->qcls.data());
         const bool tasks = r.tasks;
         EriArgs a = base_args(c);
-        a.bra_off = LX.off;
+        a.bra_off = LX.off + sp.b0;
         a.ket_off = LY.off;
         a.pairQ = c->d_pairQ.p;
         a.rseg = kRowSeg;
@@ -547,7 +578,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
         a.colinv = c->d_colinv.p + (size_t)q.Y * c->n;
         a.ncols = c->run_ncols[q.Y];
         a.qpb = eri_quartets_per_block(LX.lcls, LY.lcls);
-        a.nbra = LX.count;
+        a.nbra = (int)sp.prefix.size() - 1;
         a.nquart = sp.nq;
         a.same_class = q.X == q.Y;
         a.dsh = c->Dsh.p;
@@ -686,8 +717,6 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
                 c->prof_shell[ci] += (double)cnt[2 * (kCounterSlots * ci + sl) + 1];
             }
     }
-    // the staging buffer is pageable host memory: its copy must be done before it is reused
-    if (ds) UNO_HIP(hipStreamSynchronize(st));
     return UNO_OK;
 }
 

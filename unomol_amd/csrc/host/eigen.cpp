@@ -114,5 +114,8 @@ void delete_eigen(EigenWS* w) { delete w; }
 
 uno_ctx::~uno_ctx() {
     uno::delete_eigen(eig);
+    if (stage_ev) (void)hipEventSynchronize(stage_ev);
+    if (stage_pin) (void)hipHostFree(stage_pin);
+    if (stage_ev) (void)hipEventDestroy(stage_ev);
     if (st) (void)hipStreamDestroy(st);
 }

@@ -69,6 +69,9 @@ struct PairClassList {
 // One quartet space of a launch: per bra, a contiguous range of kets (prefix counts, the
 // first ket of each range when it does not start at 0) and the block table.
 struct QuartetSpace {
+    // bras b0 .. b0 + prefix.size() - 2 of the bra run (the empty bras at either end are
+    // trimmed); prefix[i] = first quartet of bra b0 + i
+    int b0 = 0;
     std::vector<long long> prefix, wpre, rpre;
     std::vector<int> kstart, blk, tblk, rbra;
     DBuf<long long> d_prefix, d_wpre, d_rpre;
@@ -138,6 +141,9 @@ struct uno_ctx {
     std::vector<int> run_ncols;
     uno::DBuf<int> d_colmap, d_colinv;
     std::vector<char> stage_h;   // a Fock build's index arrays, uploaded once per build
+    char* stage_pin = nullptr;   // pinned copy of stage_h that the (asynchronous) upload reads
+    size_t stage_pin_cap = 0;
+    hipEvent_t stage_ev = nullptr;   // recorded after the upload: stage_pin is free again
     uno::DBuf<char> stage_d;
     long long njp = 0;
     uno::EigenWS* eig = nullptr;
