@@ -181,7 +181,7 @@ static int batch_fock(uno_batch* b, const std::vector<double>& dmax, double& qua
         if (!x.active) continue;
         uno_ctx* c = x.ctx;
         for (QuartetClass& q : c->qcls) {
-            build_prefix(c, q, 4.0 * dmax[m], 0.0);
+            build_prefix(c, q, 4.0 * dmax[m], 0.0, c->tau);
             if (q.s64.nq == 0) continue;
             const PairClassList& LX = c->cls[q.X];
             const PairClassList& LY = c->cls[q.Y];

@@ -283,6 +283,7 @@ int uno_ctx_set(uno_ctx* c, const char* key, double v) {
     else if (k == "prim_cut") c->primcut = v;
     else if (k == "skip_center") c->skip_center = (int)v;
     else if (k == "fp32_threshold") c->fp32_thr = v;
+    else if (k == "schwarz_mixed") c->tau_mixed = v;
     else if (k == "precision") c->precision = (int)v;
     else if (k == "row_merge") c->row_merge = (int)v;
     else if (k == "row_min_kets") c->row_min_kets = (long long)v;
@@ -298,6 +299,7 @@ int uno_ctx_get(uno_ctx* c, const char* key, double* v) {
     else if (k == "prim_cut") *v = c->primcut;
     else if (k == "skip_center") *v = c->skip_center;
     else if (k == "fp32_threshold") *v = c->fp32_thr;
+    else if (k == "schwarz_mixed") *v = c->tau_mixed;
     else if (k == "precision") *v = c->precision;
     else if (k == "row_merge") *v = c->row_merge;
     else if (k == "row_min_kets") *v = (double)c->row_min_kets;

@@ -31,7 +31,7 @@ static int ensure_device() {
 // Quartet spaces of one launch.  Both runs are sorted by Schwarz bound (descending), so
 // for bra i the kets passing Qi Qj dmax >= tau form a prefix of the ket run, and within
 // it the kets with Qi Qj >= fp32_thr (FP64 in a mixed build) form a shorter prefix.
-void build_prefix(uno_ctx* c, QuartetClass& q, double dmax, double fp32_thr) {
+void build_prefix(uno_ctx* c, QuartetClass& q, double dmax, double fp32_thr, double tau) {
     const PairClassList& Lb = c->cls[q.X];
     const PairClassList& Lk = c->cls[q.Y];
     const int nb = (int)Lb.Q.size();
@@ -48,7 +48,7 @@ void build_prefix(uno_ctx* c, QuartetClass& q, double dmax, double fp32_thr) {
         long long cnt = 0, c64 = 0;
         const double qd = Lb.Q[i] * dmax;
         if (qd > 0.0) {
-            const double thr = c->tau / qd;
+            const double thr = tau / qd;
             while (p_all > 0 && !(Lk.Q[p_all - 1] >= thr)) --p_all;
             cnt = p_all;
             if (q.X == q.Y) cnt = std::min<long long>(cnt, i + 1);
@@ -489,7 +489,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
 #pragma omp parallel for schedule(dynamic, 4)
     for (int qi = 0; qi < nqc; ++qi) {
         QuartetClass& q = c->qcls[qi];
-        build_prefix(c, q, 4.0 * dmax, mixed ? c->fp32_thr : 0.0);
+        build_prefix(c, q, 4.0 * dmax, mixed ? c->fp32_thr : 0.0, c->tau);
         const PairClassList& LX = c->cls[q.X];
         const PairClassList& LY = c->cls[q.Y];
         for (int prec = 0; prec < 2; ++prec) {

@@ -119,6 +119,9 @@ struct uno_ctx {
     hipStream_t st = nullptr;
     // options
     double tau = 1e-13, primcut = 1e-12, fp32_thr = 1e-6;
+    // screening threshold of the mixed-precision builds (never below tau): their error
+    // only steers the early iterations; the FP64 phase starts with a full rebuild at tau
+    double tau_mixed = 1e-11;
     int precision = 0;
     // pair data
     std::vector<uno::PairHdr> pairs;
@@ -181,7 +184,7 @@ namespace uno {
 
 int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell);
 // quartet-space helpers of the Fock build (ctx.cpp)
-void build_prefix(uno_ctx* c, QuartetClass& q, double dmax, double fp32_thr);
+void build_prefix(uno_ctx* c, QuartetClass& q, double dmax, double fp32_thr, double tau);
 void build_tasks(QuartetSpace& s, long long row_min_kets = 0);
 void build_rows(QuartetSpace& s, long long min_kets, int seg);
 void build_blocks(QuartetSpace& s, int qpb);

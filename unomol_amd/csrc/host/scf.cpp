@@ -223,7 +223,15 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
             else UNO_HIP(k_axpby(nn, 1.0, P[s], -1.0, b.Pref[s].p, Din[s], st));
         }
         if (uhf) UNO_HIP(k_axpby(nn, 1.0, c->D1.p, 1.0, c->D2.p, c->D0.p, st));
-        if (int rc = fock_build(c, uhf, -1.0, mixed)) return rc;
+        {
+            // the mixed phase screens at tau_mixed (its error only steers the early
+            // iterations; the FP64 phase starts with a full rebuild at the context's tau)
+            const double tau0 = c->tau;
+            if (mixed) c->tau = std::max(tau0, c->tau_mixed);
+            const int rc = fock_build(c, uhf, -1.0, mixed);
+            c->tau = tau0;
+            if (rc) return rc;
+        }
         for (int s = 0; s < ns; ++s) {
             UNO_HIP(k_sym_w(n, s == 0 ? c->W0.p : c->W1.p, b.G[s].p, st, !full));
             if (o.incremental)
