@@ -556,6 +556,11 @@ __global__ __launch_bounds__(kRowThreads) void k_eri_rows(const EriArgs a) {
     }
 }
 
+template <typename V>
+__device__ __forceinline__ V sel3(int ax, V a0, V a1, V a2) {
+    return ax == 0 ? a0 : (ax == 1 ? a1 : a2);
+}
+
 // ---------------------------------------------------------------------------
 // one 64-lane wave (one block) per shell quartet, already resolved (and screened) to its
 // bra pair, ket pair, entry and bound
@@ -629,8 +634,11 @@ __device__ __forceinline__ unsigned wave_one(const EriArgs& a, const long long g
                 const int r = lane / 3, ax = lane % 3;
                 const T fff = sUW[r] * it;
                 const T B00 = (T)0.5 * fff, B1 = ((T)0.5 - B00 * q) * ip_, B1p = ((T)0.5 - B00 * p) * iq_;
-                const T C = (T)(pp.P[ax] - P1.A[ax]) - q * PQ[ax] * fff;
-                const T Cp = (T)(qq.P[ax] - P2.A[ax]) + p * PQ[ax] * fff;
+                // axis picked by selects, not by a dynamic index (which would put the pair
+                // records in scratch memory)
+                const T pqa = sel3(ax, PQ[0], PQ[1], PQ[2]);
+                const T C = (T)sel3(ax, pp.P[0] - P1.A[0], pp.P[1] - P1.A[1], pp.P[2] - P1.A[2]) - q * pqa * fff;
+                const T Cp = (T)sel3(ax, qq.P[0] - P2.A[0], qq.P[1] - P2.A[1], qq.P[2] - P2.A[2]) + p * pqa * fff;
                 T* G = sG[lane];
                 T Gl[E + 1][F + 1];
                 Gl[0][0] = 1;
@@ -659,7 +667,7 @@ __device__ __forceinline__ unsigned wave_one(const EriArgs& a, const long long g
                 const int d = t2 % (LD + 1), c = (t2 / (LD + 1)) % (LC + 1), b = (t2 / ((LD + 1) * (LC + 1))) % (LB + 1),
                           aa = t2 / ((LD + 1) * (LC + 1) * (LB + 1));
                 const int ax = ra % 3;
-                const T ab = (T)P1.AB[ax], cd = (T)P2.AB[ax];
+                const T ab = (T)sel3(ax, P1.AB[0], P1.AB[1], P1.AB[2]), cd = (T)sel3(ax, P2.AB[0], P2.AB[1], P2.AB[2]);
                 const T* G = sG[ra];
                 T v = 0, abp = 1;
                 for (int i = 0; i <= b; ++i) {
@@ -786,8 +794,14 @@ __device__ __forceinline__ unsigned wave_one(const EriArgs& a, const long long g
 // wave evaluates the survivors one after another.  qpb stays small: the blocks are many,
 // so enough waves are in flight to cover each one's serial latency (64 per block was
 // measured 2.5x slower)
+// at least two waves per SIMD (registers capped at 256) for every class but (dd|dd); a cap
+// for three waves made the compiler spill for (dp|dp) and (dd|pp) and was slower
+template <int LA, int LB, int LC, int LD>
+constexpr int wave_min_waves() {
+    return ClassInfo<LA, LB, LC, LD>::NCOMP <= 648 ? 2 : 1;
+}
 template <int LA, int LB, int LC, int LD, int MODE, typename T>
-__global__ __launch_bounds__(64) void k_eri_wave(const EriArgs a) {
+__global__ __launch_bounds__(64, (wave_min_waves<LA, LB, LC, LD>())) void k_eri_wave(const EriArgs a) {
     const long long g0 = (long long)blockIdx.x * a.qpb;
     const int lane = threadIdx.x;
     int pab = 0, pcd = 0, entry = 0;
