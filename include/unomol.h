@@ -75,7 +75,9 @@ int uno_ctx_destroy(uno_ctx* ctx);
  * row-merged; default 192), "row_variant" (LDS layout of the row-merged blocks: 1 every
  * bra row and the Rys table, 2 every bra row, 3 the first bra shell's rows; 0, the
  * default, takes the first that fits the LDS budget; a forced one that does not fit
- * falls back to that choice). */
+ * falls back to that choice), "row_density_lds" (the bra's exchange density rows staged
+ * in LDS beside the merged rows: -1, the default, wherever they fit the budget and the
+ * autotuner does not prefer otherwise; 0 never; 1 wherever they fit). */
 int uno_ctx_set(uno_ctx* ctx, const char* key, double value);
 /* Reads the knobs above and, read-only: "last_quartets" (shell quartets evaluated by the
  * last Fock build), "last_quartets32" (of which in FP32), "pairs" (significant shell pairs), "pair_runs" (runs of the pair

@@ -332,13 +332,17 @@ def test_fock_rows_vs_waves_config_b(uhf):
         Gwave = build()
         scale = np.abs(Gwave).max()
         t.set("row_merge", 1)
-        # LDS variants: rows + Rys table, every bra row, the first bra shell's rows
+        # LDS variants: rows + Rys table, every bra row, the first bra shell's rows; each
+        # with the bra's exchange density rows read from global memory and staged in LDS
         for variant in (1, 2, 3):
-            t.set("row_variant", variant)
-            Grow = build()
-            assert np.abs(Grow - Gwave).max() < tol * max(scale, 1.0), (prec, variant, np.abs(Grow - Gwave).max(),
-                                                                         scale)
+            for dens in (0, 1):
+                t.set("row_variant", variant)
+                t.set("row_density_lds", dens)
+                Grow = build()
+                assert np.abs(Grow - Gwave).max() < tol * max(scale, 1.0), (prec, variant, dens,
+                                                                             np.abs(Grow - Gwave).max(), scale)
         t.set("row_variant", 0)
+        t.set("row_density_lds", -1)
 
 
 def test_autotune_keeps_fock_matrix():

@@ -84,6 +84,9 @@ struct EriArgs {
     const int* colmap;
     const int* colinv;
     int ncols;
+    // 1: the bra's exchange density rows (every bra function, each spin density, the same
+    // columns) are staged in LDS after the accumulator rows
+    int dlds;
     // batched launches: entry i (a bra with one ket range) names its bra pair, its first
     // ket pair and its molecule (null: the run offsets above and molecule view n/D/W)
     const int* e_bra;

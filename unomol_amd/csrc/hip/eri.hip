@@ -348,6 +348,11 @@ template <int LA, int LB, int MODE, bool FB>
 __host__ __device__ constexpr int row_count() {
     return (ncart_c(LA) + (FB ? ncart_c(LB) : 0)) * (MODE == kDigestUHF ? 2 : 1);
 }
+// rows of the staged exchange density (a.dlds): every bra function, per spin density
+template <int LA, int LB, int MODE>
+__host__ __device__ constexpr int dens_row_count() {
+    return (ncart_c(LA) + ncart_c(LB)) * (MODE == kDigestUHF ? 2 : 1);
+}
 
 // threads of a row-merged block (512 was measured slower: fewer blocks for the bras with
 // few kets, and the same per-wave time)
@@ -378,17 +383,32 @@ __global__ __launch_bounds__(kRowThreads) void k_eri_rows(const EriArgs a) {
     const long long k0 = seg * a.rseg, k1 = min(cnt, k0 + (long long)a.rseg);
     const MolView mv = mol_view(a, entry);
     const int n = mv.n, nc = a.ncols;
+    const int pab = __builtin_amdgcn_readfirstlane(entry_bra(a, entry));
+    const PairHdr P1 = a.pairs[pab];
+    const int ket0 = entry_ket(a, entry);
+    const int fa = P1.f1, fb = P1.f2;
     for (int i = threadIdx.x; i < NROW * nc; i += blockDim.x) rows[i] = 0.0;
+    // with a.dlds, the exchange density rows of the bra (rows fa.., fb.. of each spin
+    // density, at the same columns) follow the accumulator rows: the exchange digestion
+    // then reads LDS instead of gathering from global memory
+    constexpr int NDR = NA + NB;
+    const bool dl = a.dlds != 0;
+    double* dlr = rows + NROW * nc;
+    if (dl) {
+        for (int i = threadIdx.x; i < NDR * NSP * nc; i += blockDim.x) {
+            const int r = i / nc, col = i - r * nc;
+            const int sp = r / NDR, rr = r - sp * NDR;
+            const double* DK = (MODE == kDigestUHF) ? (sp == 0 ? mv.D1 : mv.D2) : mv.D0;
+            const int grow = rr < NA ? fa + rr : fb + rr - NA;
+            dlr[i] = DK[(long)grow * n + a.colinv[col]];
+        }
+    }
     const T* tab = rys_table_global<NR, T>();
     if constexpr (TL) {
         for (int i = threadIdx.x; i < rys_table_size<NR, T>(); i += blockDim.x) tab_lds[i] = tab[i];
         tab = tab_lds;
     }
     __syncthreads();
-    const int pab = __builtin_amdgcn_readfirstlane(entry_bra(a, entry));
-    const PairHdr P1 = a.pairs[pab];
-    const int ket0 = entry_ket(a, entry);
-    const int fa = P1.f1, fb = P1.f2;
     constexpr T jf = (MODE == kDigestRHF) ? 4 : 2;
     double jacc[NA * NB];
 #pragma unroll
@@ -446,19 +466,37 @@ __global__ __launch_bounds__(kRowThreads) void k_eri_rows(const EriArgs a) {
             double* R = rows + sp * NAB * nc;
             double* WB = (MODE == kDigestUHF && sp) ? mv.W1 : mv.W0;
             T dbd[NB * ND], dbc[NB * NC], dad[NA * ND], dac[NA * NC];
+            if (dl) {
+                const double* DR = dlr + sp * NDR * nc;
 #pragma unroll
-            for (int j = 0; j < NB; ++j) {
+                for (int j = 0; j < NB; ++j) {
 #pragma unroll
-                for (int l = 0; l < ND; ++l) dbd[j * ND + l] = (T)DK[(long)(fb + j) * n + fd + l];
+                    for (int l = 0; l < ND; ++l) dbd[j * ND + l] = (T)DR[(NA + j) * nc + ld + l];
 #pragma unroll
-                for (int k = 0; k < NC; ++k) dbc[j * NC + k] = (T)DK[(long)(fb + j) * n + fc + k];
-            }
+                    for (int k = 0; k < NC; ++k) dbc[j * NC + k] = (T)DR[(NA + j) * nc + lc + k];
+                }
 #pragma unroll
-            for (int i = 0; i < NA; ++i) {
+                for (int i = 0; i < NA; ++i) {
 #pragma unroll
-                for (int l = 0; l < ND; ++l) dad[i * ND + l] = (T)DK[(long)(fa + i) * n + fd + l];
+                    for (int l = 0; l < ND; ++l) dad[i * ND + l] = (T)DR[i * nc + ld + l];
 #pragma unroll
-                for (int k = 0; k < NC; ++k) dac[i * NC + k] = (T)DK[(long)(fa + i) * n + fc + k];
+                    for (int k = 0; k < NC; ++k) dac[i * NC + k] = (T)DR[i * nc + lc + k];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+#pragma unroll
+                    for (int l = 0; l < ND; ++l) dbd[j * ND + l] = (T)DK[(long)(fb + j) * n + fd + l];
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) dbc[j * NC + k] = (T)DK[(long)(fb + j) * n + fc + k];
+                }
+#pragma unroll
+                for (int i = 0; i < NA; ++i) {
+#pragma unroll
+                    for (int l = 0; l < ND; ++l) dad[i * ND + l] = (T)DK[(long)(fa + i) * n + fd + l];
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) dac[i * NC + k] = (T)DK[(long)(fa + i) * n + fc + k];
+                }
             }
 #pragma unroll
             for (int i = 0; i < NA; ++i) {
@@ -859,7 +897,8 @@ static hipError_t launch_cls(const EriArgs& a, hipStream_t st) {
 // row-merged Fock-build launch of a lane class: one block per row task
 template <int LA, int LB, int LC, int LD, int MODE, typename T, bool TL, bool FB>
 static hipError_t launch_rows_v(const EriArgs& a, hipStream_t st) {
-    const size_t dyn = (size_t)row_count<LA, LB, MODE, FB>() * a.ncols * sizeof(double);
+    const size_t dyn = (size_t)(row_count<LA, LB, MODE, FB>() + (a.dlds ? dens_row_count<LA, LB, MODE>() : 0)) *
+                       a.ncols * sizeof(double);
     static size_t attr = 0;   // dynamic LDS limit set for this instantiation
     if (dyn > attr) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eri_rows<LA, LB, LC, LD, MODE, T, TL, FB>),
@@ -914,8 +953,9 @@ hipError_t eri_launch_rows(int mode, int cb, int ck, const EriArgs& a, int varia
     return hipErrorInvalidValue;
 }
 
-int eri_rows_variant(int cb, int ck, int mode, int ncols, bool fp32, int force) {
-    if (!eri_uses_wave_tasks(cb, ck)) return 0;
+// the LDS a row-merged block of this variant needs at ncols columns, with or without the
+// staged density rows (0: the variant does not apply)
+static size_t rows_lds_bytes(int cb, int ck, int mode, int ncols, bool fp32, int variant, bool dlds) {
     static const int na[6] = {1, 3, 3, 6, 6, 6};          // functions of the first bra shell
     static const int nb[6] = {1, 1, 3, 1, 3, 6};          // and of the second
     static const int lsum[6] = {0, 1, 2, 2, 3, 4};
@@ -925,8 +965,24 @@ int eri_rows_variant(int cb, int ck, int mode, int ncols, bool fp32, int force) 
     const size_t sp = mode == kDigestUHF ? 2 : 1, reserve = 1024;
     const size_t all = (size_t)(na[cb] + nb[cb]) * sp * ncols * sizeof(double),
                  first = (size_t)na[cb] * sp * ncols * sizeof(double);
-    const bool fits[4] = {true, all + tab + reserve <= kRowLdsBudget, all + reserve <= kRowLdsBudget,
-                          first + reserve <= kRowLdsBudget};
+    const size_t dens = dlds ? all : 0;
+    switch (variant) {
+        case 1: return all + dens + tab + reserve;
+        case 2: return all + dens + reserve;
+        case 3: return first + dens + reserve;
+        default: return 0;
+    }
+}
+
+bool eri_rows_dlds_fits(int cb, int ck, int mode, int ncols, bool fp32, int variant) {
+    const size_t b = rows_lds_bytes(cb, ck, mode, ncols, fp32, variant, true);
+    return b > 0 && b <= kRowLdsBudget;
+}
+
+int eri_rows_variant(int cb, int ck, int mode, int ncols, bool fp32, int force) {
+    if (!eri_uses_wave_tasks(cb, ck)) return 0;
+    bool fits[4] = {true, false, false, false};
+    for (int v = 1; v <= 3; ++v) fits[v] = rows_lds_bytes(cb, ck, mode, ncols, fp32, v, false) <= kRowLdsBudget;
     if (force >= 1 && force <= 3 && fits[force]) return force;
     for (int v = 1; v <= 3; ++v)
         if (fits[v]) return v;

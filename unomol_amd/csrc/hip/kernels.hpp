@@ -22,6 +22,9 @@ bool eri_uses_wave_tasks(int bra_class, int ket_class);
 hipError_t eri_launch_rows(int mode, int bra_class, int ket_class, const EriArgs& a, int variant, hipStream_t st,
                            bool fp32);
 int eri_rows_variant(int bra_class, int ket_class, int mode, int ncols, bool fp32, int force);
+// whether that variant still fits the LDS budget with the bra's exchange density rows
+// staged as well (a.dlds = 1)
+bool eri_rows_dlds_fits(int bra_class, int ket_class, int mode, int ncols, bool fp32, int variant);
 // W[f1 + k][f2 + l] += jp[jo + k nd + l] (and W1) for every pair of the array (pair blocks
 // are disjoint, no atomics); the block size of a pair follows from the next pair's jo
 hipError_t jp_scatter(int npairs, const PairHdr* pairs, long long njp, const double* jp, double* W0, double* W1, int n,

@@ -288,6 +288,7 @@ int uno_ctx_set(uno_ctx* c, const char* key, double v) {
     else if (k == "row_merge") c->row_merge = (int)v;
     else if (k == "row_min_kets") c->row_min_kets = (long long)v;
     else if (k == "row_variant") c->row_variant = (int)v;
+    else if (k == "row_density_lds") c->row_dens = (int)v;
     else { set_error("unknown option " + k); return UNO_EINVAL; }
     return UNO_OK;
 }
@@ -304,6 +305,7 @@ int uno_ctx_get(uno_ctx* c, const char* key, double* v) {
     else if (k == "row_merge") *v = c->row_merge;
     else if (k == "row_min_kets") *v = (double)c->row_min_kets;
     else if (k == "row_variant") *v = c->row_variant;
+    else if (k == "row_density_lds") *v = c->row_dens;
     else if (k == "last_quartets") *v = c->last_quartets;
     else if (k == "last_quartets32") *v = c->last_quartets32;
     else if (k == "pair_runs") *v = (double)c->cls.size();

@@ -467,7 +467,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
     // staging buffer (uploaded once per build; the tuner uploads per candidate instead)
     struct SpaceRec {
         QuartetClass* q;
-        int prec, variant;
+        int prec, variant, dlds;
         bool tasks;
         size_t o_prefix, o_wpre, o_tblk, o_rbra, o_rpre, o_blk, o_kstart;
     };
@@ -495,12 +495,14 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
         for (int prec = 0; prec < 2; ++prec) {
             QuartetSpace& sp = prec ? q.s32 : q.s64;
             if (sp.nq == 0) continue;
-            SpaceRec r{&q, prec, 0, eri_uses_wave_tasks(LX.lcls, LY.lcls), 0, 0, 0, 0, 0, 0, 0};
+            SpaceRec r{&q, prec, 0, 0, eri_uses_wave_tasks(LX.lcls, LY.lcls), 0, 0, 0, 0, 0, 0, 0};
             // row-merge configuration: the tuned one for this space if any, else the defaults
             int variant = r.tasks && c->row_merge
                               ? eri_rows_variant(LX.lcls, LY.lcls, mode, c->run_ncols[q.Y], prec == 1, c->row_variant)
                               : 0;
             long long min_kets = std::max(1LL, c->row_min_kets);
+            int dlds = variant && c->row_dens != 0 &&
+                       eri_rows_dlds_fits(LX.lcls, LY.lcls, mode, c->run_ncols[q.Y], prec == 1, variant);
             if (r.tasks && c->row_merge && !c->tuning) {
                 auto it = c->tune.find(std::make_tuple(qi, prec, uhf ? 1 : 0, mixed ? 1 : 0));
                 if (it != c->tune.end()) {
@@ -508,10 +510,13 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
                     if (v == 0 || eri_rows_variant(LX.lcls, LY.lcls, mode, c->run_ncols[q.Y], prec == 1, v) == v) {
                         variant = v;
                         min_kets = it->second.min_kets;
+                        dlds = v && it->second.dlds &&
+                               eri_rows_dlds_fits(LX.lcls, LY.lcls, mode, c->run_ncols[q.Y], prec == 1, v);
                     }
                 }
             }
             r.variant = variant;
+            r.dlds = dlds;
             if (r.tasks) {
                 build_rows(sp, variant ? min_kets : 0, kRowSeg);
                 build_tasks(sp, variant ? min_kets : 0);
@@ -577,6 +582,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
         a.colmap = c->d_colmap.p + (size_t)q.Y * c->n;
         a.colinv = c->d_colinv.p + (size_t)q.Y * c->n;
         a.ncols = c->run_ncols[q.Y];
+        a.dlds = r.dlds;
         a.qpb = eri_quartets_per_block(LX.lcls, LY.lcls);
         a.nbra = (int)sp.prefix.size() - 1;
         a.nquart = sp.nq;
@@ -657,11 +663,17 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
         };
         if (c->tuning && tasks && c->row_merge) {
             // time every candidate on this space (each one run twice, the faster counts)
-            std::vector<TuneChoice> cand{{0, 0}};
+            std::vector<TuneChoice> cand{{0, 0, 0}};
             for (int v = 1; v <= 3; ++v)
-                if (eri_rows_variant(LX.lcls, LY.lcls, mode, c->run_ncols[q.Y], prec == 1, v) == v)
-                    for (long long mk : {64LL, 192LL, 512LL}) cand.push_back({v, mk});
-            TuneChoice best{r.variant, std::max(1LL, c->row_min_kets)};
+                if (eri_rows_variant(LX.lcls, LY.lcls, mode, c->run_ncols[q.Y], prec == 1, v) == v) {
+                    const bool df = c->row_dens != 0 &&
+                                    eri_rows_dlds_fits(LX.lcls, LY.lcls, mode, c->run_ncols[q.Y], prec == 1, v);
+                    for (long long mk : {64LL, 192LL, 512LL}) {
+                        if (c->row_dens != 1 || !df) cand.push_back({v, mk, 0});
+                        if (df) cand.push_back({v, mk, 1});
+                    }
+                }
+            TuneChoice best{r.variant, std::max(1LL, c->row_min_kets), r.dlds};
             float best_ms = 1e30f;
             hipEvent_t t0, t1;
             UNO_HIP(hipEventCreate(&t0));
@@ -669,6 +681,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
             for (const TuneChoice& tc : cand) {
                 build_rows(sp, tc.variant ? tc.min_kets : 0, kRowSeg);
                 build_tasks(sp, tc.variant ? tc.min_kets : 0);
+                a.dlds = tc.dlds;
                 if (int rc = set_index()) return rc;
                 float ms_min = 1e30f;
                 for (int rep = 0; rep < 2; ++rep) {

@@ -88,6 +88,7 @@ constexpr int kRowSeg = 1024;
 struct TuneChoice {
     int variant;
     long long min_kets;
+    int dlds = 0;   // the bra's exchange density rows staged in LDS as well
 };
 
 struct QuartetClass {
@@ -158,6 +159,7 @@ struct uno_ctx {
     int row_merge = 1;
     long long row_min_kets = uno::kRowMinKets;
     int row_variant = 0;     // row-merged LDS variant: 0 the best that fits, 1-3 forced
+    int row_dens = -1;       // exchange density rows in LDS: -1 when they fit, 0 never, 1 forced where they fit
     // autotuned row-merge configuration per quartet space (run-pair index, precision, UHF,
     // mixed build): LDS variant (0 = wave tasks only) and the ket count from which a bra is
     // row-merged.  tuning = 1 while uno_autotune times the candidates.
