@@ -640,6 +640,10 @@ __global__ __launch_bounds__(64) void k_inviter(int n, const double* __restrict_
         }
     }
     U1[(long)(n - 1) * S + i] = pivot((a == 0.0) ? tiny : (fabs(a) < tiny ? copysign(tiny, a) : a));
+    // the last row's superdiagonals do not exist: stored as zeros, since the solves below
+    // multiply them (by zero) without a branch and the work space is not cleared
+    U2[(long)(n - 1) * S + i] = 0.0;
+    U3[(long)(n - 1) * S + i] = 0.0;
     // start vector
     unsigned int h = 2654435761u * (unsigned)(i + 1);
     for (int k = 0; k < n; ++k) {
