@@ -100,6 +100,10 @@ struct uno_batch {
     DBuf<MolDesc> mdesc;
     // stores (each sum_m N_m^2 doubles); index 0 alpha / closed shell, 1 beta
     DBuf<double> H, S, X, P[2], Pold[2], F[2], G[2], C[2], Fp, Tm, Tm2, V, Z, D0, D1, D2, W0, W1, Dsh, ev[2];
+    // per spin: the last eigenvectors of X^T F X (orthogonal basis), the next eigensolve's
+    // starting basis once warm[spin] is set
+    DBuf<double> Wp[2];
+    bool warm[2] = {false, false};
     DBuf<double> dF[kBatchDiis][2], dE[kBatchDiis][2];
     DBuf<double> red;
     Scratch scratch;
@@ -327,7 +331,7 @@ int uno_batch_create(int nmol, const uno_basis* bases, const int* nelec, uno_bat
     const size_t S = (size_t)b->store;
     DBuf<double>* stores[] = {&b->H, &b->S, &b->X, &b->P[0], &b->P[1], &b->Pold[0], &b->Pold[1], &b->F[0], &b->F[1],
                               &b->G[0], &b->G[1], &b->C[0], &b->C[1], &b->Fp, &b->Tm, &b->Tm2, &b->V, &b->Z,
-                              &b->D0, &b->D1, &b->D2, &b->W0, &b->W1};
+                              &b->D0, &b->D1, &b->D2, &b->W0, &b->W1, &b->Wp[0], &b->Wp[1]};
     for (DBuf<double>* p : stores)
         if (p->alloc(S) != hipSuccess) { set_error("batched SCF: device allocation failed"); delete b; return UNO_ENOMEM; }
     for (int k = 0; k < kBatchDiis; ++k)
@@ -365,6 +369,7 @@ int uno_batch_run(uno_batch* b, const uno_scf_opts* opts, uno_scf_result* res, d
     const auto t0 = clk::now();
     hipStream_t st = b->st;
     b->scratch.reset();
+    b->warm[0] = b->warm[1] = false;
     const int nm = (int)b->mol.size();
     const long long S = b->store;
     // per-molecule eigenvalue offsets into ev stores: reuse the matrix offsets (n <= n^2)
@@ -427,9 +432,54 @@ int uno_batch_run(uno_batch* b, const uno_scf_opts* opts, uno_scf_result* res, d
         if (int rc = g1.run(1.0, 0.0, st)) return rc;
         for (const MatSlot& m : sl[s]) g2.add(m.n, m.n, m.n, true, false, b->X.p + m.off, b->Tm.p + m.off, b->Fp.p + m.off, m.n, m.n, m.n);
         if (int rc = g2.run(1.0, 0.0, st)) return rc;
-        if (int rc = batch_eigh(b, sl[s], b->Fp.p, b->ev[s].p, evoff(s))) return rc;
+        // after their first eigensolve the molecules start from their last eigenvectors Wp:
+        // in that basis the matrix is nearly diagonal and the Jacobi sweeps settle in a few
+        // rounds (W = Wp Z after)
+        std::vector<char> warm(sl[s].size(), 0);
+        GemmBatch w1(b->scratch), w2(b->scratch), w3(b->scratch);
+        for (size_t k = 0; k < sl[s].size(); ++k) {
+            const MatSlot& m = sl[s][k];
+            warm[k] = b->warm[s];
+            if (warm[k]) {
+                w1.add(m.n, m.n, m.n, false, false, b->Fp.p + m.off, b->Wp[s].p + m.off, b->Tm.p + m.off, m.n, m.n, m.n);
+                w2.add(m.n, m.n, m.n, true, false, b->Wp[s].p + m.off, b->Tm.p + m.off, b->Tm2.p + m.off, m.n, m.n, m.n);
+                w3.add(m.n, m.n, m.n, false, false, b->Wp[s].p + m.off, b->Z.p + m.off, b->Tm.p + m.off, m.n, m.n, m.n);
+            }
+        }
+        if (int rc = w1.run(1.0, 0.0, st)) return rc;
+        if (int rc = w2.run(1.0, 0.0, st)) return rc;
+        {
+            std::vector<BatchMat> bm(sl[s].size());
+            for (size_t k = 0; k < sl[s].size(); ++k) {
+                const MatSlot& m = sl[s][k];
+                bm[k] = BatchMat{m.n, (warm[k] ? b->Tm2.p : b->Fp.p) + m.off, b->V.p + m.off, b->ev[s].p + m.off,
+                                 b->Z.p + m.off};
+            }
+            BatchMat* d = nullptr;
+            UNO_HIP(b->scratch.upload(bm.data(), bm.size(), st, &d));
+            UNO_HIP(batched_jacobi((int)bm.size(), d, 50, nullptr, st));
+            // bm is pageable host memory: its upload must be done before it goes
+            UNO_HIP(hipStreamSynchronize(st));
+        }
+        if (int rc = w3.run(1.0, 0.0, st)) return rc;
+        // the new starting bases in one launch: Wp Z (warm, in Tm) or Z (cold)
+        {
+            std::vector<double> coef(2 * sl[s].size());
+            for (size_t k = 0; k < sl[s].size(); ++k) {
+                coef[2 * k] = warm[k] ? 1.0 : 0.0;
+                coef[2 * k + 1] = warm[k] ? 0.0 : 1.0;
+            }
+            const double* src[2] = {b->Tm.p, b->Z.p};
+            double* dcoef = nullptr;
+            const double** dsrc = nullptr;
+            UNO_HIP(b->scratch.upload(coef.data(), coef.size(), st, &dcoef));
+            UNO_HIP(b->scratch.upload(src, 2, st, &dsrc));
+            UNO_HIP(batch_lincomb((int)sl[s].size(), dsl[s].p, 2, dcoef, dsrc, b->Wp[s].p, st));
+            UNO_HIP(hipStreamSynchronize(st));
+        }
+        b->warm[s] = true;
         GemmBatch g3(b->scratch), g4(b->scratch);
-        for (const MatSlot& m : sl[s]) g3.add(m.n, m.n, m.n, false, false, b->X.p + m.off, b->Z.p + m.off, b->C[s].p + m.off, m.n, m.n, m.n);
+        for (const MatSlot& m : sl[s]) g3.add(m.n, m.n, m.n, false, false, b->X.p + m.off, b->Wp[s].p + m.off, b->C[s].p + m.off, m.n, m.n, m.n);
         if (int rc = g3.run(1.0, 0.0, st)) return rc;
         for (size_t k = 0; k < sl[s].size(); ++k) {
             const MatSlot& m = sl[s][k];
