@@ -6,7 +6,8 @@
 // binomial transfer to (la,lb|lc,ld).  The primitive cut drops a primitive quartet
 // whose bare prefactor 2 pi^(5/2) e_ab e_cd / (p q sqrt(p+q)) is below primcut, as the
 // reference does with 1e-12 (TwoElectronInts.cpp:477-479).  The Rys roots/weights come
-// from this project's own Chebyshev tables (rys_table.inc, gen_rys_table.py).
+// from this project's own Chebyshev tables (rys_table.inc, gen_rys_table.py), evaluated in
+// their power-basis re-expansion (rys_mono.inc, gen_rys_monomial.py).
 //
 // Two execution shapes:
 //   * one lane per shell quartet (classes with <= 162 components): VRR, transfer and the
@@ -72,7 +73,7 @@ __device__ __forceinline__ T rsqrt_nr(T s) {
 }
 
 constexpr int kRysAsym[6] = {0, 40, 48, 56, 62, 68};
-// entries of the Chebyshev table of NR roots: intervals x 2 NR functions x coefficients
+// entries of the table of NR roots: intervals x 2 NR functions x coefficients
 template <int NR, typename T>
 __host__ __device__ constexpr int rys_table_size() {
     return kRysAsym[NR] * 2 * NR * (sizeof(T) == 4 ? UNO_RYS_NCOEF_F : UNO_RYS_NCOEF);
