@@ -194,8 +194,9 @@ __global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
     bool active = jk < cnt;
     PairHdr P2;
     if (active) {
-        P2 = a.pairs[pcd];
-        active = !screened(a, mv, P1, P2, a.pairQ[pab] * a.pairQ[pcd]);
+        // the screen reads only the ket's shell indices; the whole record for survivors
+        active = !screened(a, mv, P1, a.pairs[pcd], a.pairQ[pab] * a.pairQ[pcd]);
+        if (active) P2 = a.pairs[pcd];
     }
     T e[NCOMP];
     unsigned nprimq = 0;
@@ -425,8 +426,9 @@ __global__ __launch_bounds__(kRowThreads) void k_eri_rows(const EriArgs a) {
         PairHdr P2;
         const int pcd = ket0 + (int)jk;
         if (active) {
-            P2 = a.pairs[pcd];
-            active = !screened(a, mv, P1, P2, a.pairQ[pab] * a.pairQ[pcd]);
+            // the screen reads only the ket's shell indices; the whole record for survivors
+            active = !screened(a, mv, P1, a.pairs[pcd], a.pairQ[pab] * a.pairQ[pcd]);
+            if (active) P2 = a.pairs[pcd];
         }
         if (!active) continue;
         T e[NCOMP];
