@@ -617,8 +617,16 @@ __device__ __forceinline__ unsigned wave_one(const EriArgs& a, const long long g
     __shared__ T sI[NR * 3][TBL];
     __shared__ T sE[NCOMP];
     const int lane = threadIdx.x;
-    const PairHdr P1 = a.pairs[pab];
-    const PairHdr P2 = a.pairs[pcd];
+    // the two pair records live in LDS (read where needed) rather than in 40 registers
+    __shared__ PairHdr sP[2];
+    static_assert(sizeof(PairHdr) % 4 == 0 && sizeof(PairHdr) / 4 <= 32, "pair record copy by lanes");
+    constexpr int kHdrWords = sizeof(PairHdr) / 4;
+    if (lane < kHdrWords) reinterpret_cast<int*>(&sP[0])[lane] = reinterpret_cast<const int*>(a.pairs + pab)[lane];
+    else if (lane >= 32 && lane < 32 + kHdrWords)
+        reinterpret_cast<int*>(&sP[1])[lane - 32] = reinterpret_cast<const int*>(a.pairs + pcd)[lane - 32];
+    __syncthreads();
+    const PairHdr& P1 = sP[0];
+    const PairHdr& P2 = sP[1];
     const MolView mv = mol_view(a, entry);
     // each lane owns components lane + 64 m; precompute their table indices
     int ix[PER], iy[PER], iz[PER];
@@ -834,11 +842,12 @@ __device__ __forceinline__ unsigned wave_one(const EriArgs& a, const long long g
 // wave evaluates the survivors one after another.  qpb stays small: the blocks are many,
 // so enough waves are in flight to cover each one's serial latency (64 per block was
 // measured 2.5x slower)
-// at least two waves per SIMD (registers capped at 256) for every class but (dd|dd); a cap
-// for three waves made the compiler spill for (dp|dp) and (dd|pp) and was slower
+// waves per SIMD the register allocation must allow: three up to 324 components ((dp|dp),
+// (dd|pp) then fit without spilling, with the pair records in LDS), two up to 648, one for
+// (dd|dd)
 template <int LA, int LB, int LC, int LD>
 constexpr int wave_min_waves() {
-    return ClassInfo<LA, LB, LC, LD>::NCOMP <= 648 ? 2 : 1;
+    return ClassInfo<LA, LB, LC, LD>::NCOMP <= 324 ? 3 : ClassInfo<LA, LB, LC, LD>::NCOMP <= 648 ? 2 : 1;
 }
 template <int LA, int LB, int LC, int LD, int MODE, typename T>
 __global__ __launch_bounds__(64, (wave_min_waves<LA, LB, LC, LD>())) void k_eri_wave(const EriArgs a) {
