@@ -662,7 +662,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
             return UNO_OK;
         };
         if (c->tuning && tasks && c->row_merge) {
-            // time every candidate on this space (each one run twice, the faster counts)
+            // time every candidate on this space (each one run three times, the fastest counts)
             std::vector<TuneChoice> cand{{0, 0, 0}};
             for (int v = 1; v <= 3; ++v)
                 if (eri_rows_variant(LX.lcls, LY.lcls, mode, c->run_ncols[q.Y], prec == 1, v) == v) {
@@ -684,7 +684,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
                 a.dlds = tc.dlds;
                 if (int rc = set_index()) return rc;
                 float ms_min = 1e30f;
-                for (int rep = 0; rep < 2; ++rep) {
+                for (int rep = 0; rep < 3; ++rep) {
                     UNO_HIP(hipEventRecord(t0, st));
                     if (int rc = run(tc.variant)) return rc;
                     UNO_HIP(hipEventRecord(t1, st));
