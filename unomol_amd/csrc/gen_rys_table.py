@@ -10,7 +10,8 @@ Method (this project's own; no fitted constants are taken from anywhere):
     (Golub-Welsch), all in 40-digit arithmetic.
   * Each root u_i(x) = t_i^2 and weight w_i(x) is tabulated on unit intervals
     [k, k+1) as a degree-DEG Chebyshev series in s = 2(x-k)-1 (interpolation at the
-    Chebyshev points of the first kind), evaluated on the device with Clenshaw.
+    Chebyshev points of the first kind); the device evaluates their exact power-basis
+    re-expansion (gen_rys_monomial.py) with Horner.
   * For x >= XASYM[n] the large-x limit is used: the integral over [0,1] becomes the
     half-line Gauss-Hermite rule with 2n points, u_i = h_i / x, w_i = omega_i / sqrt(x)
     (h_i = squared positive roots of H_2n, omega_i their Gauss-Hermite weights).
