@@ -428,6 +428,26 @@ def test_scf_mixed_precision_energy(tmp_path):
     t.set("schwarz_mixed", 1e-10)
 
 
+def test_scf_mixed_precision_energy_uhf(tmp_path):
+    """The same for the open-shell path: a 4-water 6-31G* cation (UHF doublet), mixed
+    precision + incremental builds + the mixed-phase screening against the plain FP64 UHF
+    SCF: the final energy within 1e-8 Eh, FP32 quartets used on the way."""
+    say("scf mixed precision 4-water 6-31G* cation (UHF)")
+    deck = str(tmp_path / "w4p.dat")
+    unomol_amd.generate_deck(deck, "water_cluster", 4, 4001, "6-31G*", 1, 1e-9, 150)
+    b = unomol_amd.Basis(deck)
+    assert b.deck["nelec"] % 2 == 1
+    t = unomol_amd.TwoElectronInts(b)
+    ref = unomol_amd.UnRestrictedHartreeFock(b, t)
+    e64 = ref.findEnergy(accel=2, eps=1e-9, cflag=3)
+    assert ref.result.converged
+    hf = unomol_amd.UnRestrictedHartreeFock(b, t)
+    e = hf.findEnergy(accel=2, eps=1e-9, cflag=3, precision=1, incremental=1)
+    assert hf.result.converged
+    assert hf.result.quartets32 > 0
+    assert abs(e - e64) < 1e-8, (e, e64)
+
+
 B_TAGS = [t for t in ("B_w16_631gs", "B_w16_631gs_cation")
           if os.path.exists(os.path.join(R.GOLDEN, "ref_run_x2", t, "short.gs.out"))]
 
