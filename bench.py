@@ -18,16 +18,19 @@ iteration.
 The learned guess loads unomol_amd/models/guess_631gs.bin; without it, a network is
 trained first on small clusters (outside the timed region, reported).
 
-Prints ONE JSON line with the metric, a roofline object for the dominant ERI kernel of
+Prints ONE compact JSON line (stdout) with the metric, the errors of the timed path
+against the plain FP64 run (energy, orbital energies, density), an ablation (FP64 +
+learned guess, mixed + core-H guess), a roofline object for the dominant ERI kernel of
 the timed steps (the class/precision kernel with the largest summed time; per-launch
 time from HIP events recorded on the engine's stream around each launch inside the timed
 region; algorithmic FLOPs = unomol_amd.flopmodel's per-primitive-quartet and
 per-shell-quartet operation counts of the kernel's own arithmetic times the quartets the
-kernel evaluated, counted by the kernel; HBM
-traffic per launch from the committed FETCH_SIZE/WRITE_SIZE passes when they cover the
-same kernel) and a cpu_baseline object (the reference's Rys ERI code compiled from its
-sources into oracle/_ref, timed on a random sample of the same workload's shell quartets
-on one host core).
+kernel evaluated, counted by the kernel; HBM traffic per launch from the committed
+FETCH_SIZE/WRITE_SIZE passes when they cover the same kernel), the 16-water cluster's
+plain FP64 GPU time beside the reference program's measured time on it, and a
+cpu_baseline object (the reference's own code compiled from its sources into
+oracle/_ref, timed on samples of the workload on one host core, extrapolated to the
+SCF).  The per-class kernel table goes to stderr.
 """
 import argparse
 import ctypes as C
@@ -60,26 +63,94 @@ def ncart(l):
     return (l + 1) * (l + 2) // 2
 
 
-def cpu_baseline(deck_path, budget_s=15.0):
-    """Reference ERI code (oracle/_ref/libref_harness.so) on a random sample of canonical
-    shell quartets of the deck, one host core; extrapolated to one full ERI pass."""
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(deck_path, fp64_iterations, budget_s=15.0):
+    """The reference's CPU path, compiled from its sources into oracle/_ref/libref_harness.so,
+    on one host core.  The 64-water workload cannot run through the reference whole (its
+    integral store would hold 2.7e11 ERIs, 6.6 TB), so its SCF time is extrapolated from two
+    measured samples:
+      * its Rys ERI code on a random sample of the deck's canonical shell quartets (about
+        budget_s seconds): ERI-pass time and the fraction of ERIs it would store;
+      * its direct-store integral pass and Fock replay (formGmatrix) on a 4-water 6-31G*
+        cluster: seconds per stored ERI per Fock build (the in-memory rate, no disk spill,
+        which favours the reference).
+    Estimate = ERI pass + (FP64 GPU iteration count) x replay of the estimated store."""
     lib_path = os.path.join(ROOT, "oracle", "_ref", "libref_harness.so")
     if not os.path.exists(lib_path):
         return None
     lib = C.CDLL(lib_path)
-    secs, nf, tot = C.c_double(), C.c_long(), C.c_double()
+    secs, nf, tot, nk = C.c_double(), C.c_long(), C.c_double(), C.c_long()
     n = 2000
-    lib.ref_eri_sample(deck_path.encode(), C.c_long(n), C.c_ulong(7), C.byref(secs), C.byref(nf), C.byref(tot))
+    lib.ref_eri_sample(deck_path.encode(), C.c_long(n), C.c_ulong(7), C.byref(secs), C.byref(nf), C.byref(tot),
+                       C.byref(nk))
     if secs.value > 0:
         n = max(n, int(n * budget_s / max(secs.value, 1e-3)))
-    lib.ref_eri_sample(deck_path.encode(), C.c_long(n), C.c_ulong(11), C.byref(secs), C.byref(nf), C.byref(tot))
+    lib.ref_eri_sample(deck_path.encode(), C.c_long(n), C.c_ulong(11), C.byref(secs), C.byref(nf), C.byref(tot),
+                       C.byref(nk))
     rate = n / secs.value
-    return {"value": tot.value / rate, "unit": "s (one full reference ERI pass, extrapolated)", "cores": 1,
+    eri_pass = tot.value / rate
+    stored = tot.value * nk.value / n
+    import unomol_amd
+    tmp = tempfile.mkdtemp()
+    w4 = os.path.join(tmp, "w4.dat")
+    unomol_amd.generate_deck(w4, "water_cluster", 4, 4001, "6-31G*", 0, 1e-8, 50)
+    es, gs, ns_ = C.c_double(), C.c_double(), C.c_double()
+    lib.ref_fock_sample(w4.encode(), os.path.join(tmp, "MINTS.DAT").encode(), 5, C.byref(es), C.byref(gs),
+                        C.byref(ns_))
+    per_eri = gs.value / ns_.value
+    replay = stored * per_eri
+    est = eri_pass + fp64_iterations * replay
+    return {"value": round(est, 1), "unit": "s (reference SCF of this workload, extrapolated)", "cores": 1,
             "kind": "reference",
-            "sample": "%d random canonical shell quartets (%d function quartets) of the same deck through the "
-                      "reference's Rys ERI kernel in %.1f s; %.3e shell quartets/s; the deck has %.3e canonical "
-                      "shell quartets" % (n, nf.value, secs.value, rate, tot.value),
-            "shell_quartets_per_s": rate}
+            "sample": "reference Rys ERI code on %d random canonical shell quartets of the deck (%.1f s): "
+                      "%.3e shell quartets/s, ERI pass %.0f s, %.3e ERIs stored (|val| > 1e-14); reference "
+                      "formGmatrix replay on a 4-water 6-31G* cluster (%.3e stored ERIs, %.1f ms per build): "
+                      "%.2f ns per stored ERI, %.0f s per Fock build here; SCF = ERI pass + %d iterations "
+                      "(the FP64 GPU run's count) of replay" %
+                      (n, secs.value, rate, eri_pass, stored, ns_.value, 1e3 * gs.value, 1e9 * per_eri, replay,
+                       fp64_iterations),
+            "eri_pass_s": round(eri_pass, 1), "replay_s_per_build": round(replay, 2),
+            "shell_quartets_per_s": round(rate, 1), "host_cores": host_cores()}
+
+
+def reference_config_b():
+    """The reference program itself on the 16-water 6-31G* cluster (three SCF iterations
+    from the core-Hamiltonian guess, serial Unomol and 8-rank UnomolMPI), measured once by
+    profiles/cpu_reference_config_b.md (its last line is JSON), or None."""
+    path = os.path.join(ROOT, "profiles", "cpu_reference_config_b.md")
+    if not os.path.exists(path):
+        return None
+    rec = None
+    for line in open(path):
+        if line.strip().startswith("{"):
+            rec = json.loads(line)
+    return rec
+
+
+def run_config_b(unomol_amd):
+    """Plain FP64 GPU path on the 16-water 6-31G* cluster (304 functions): context build,
+    guess and three SCF iterations with the reference's damping, the same trajectory the
+    reference runs in profiles/cpu_reference_config_b.md (energies equal to 1e-9 Eh in the
+    parity tests).  Returns wall seconds of the best of three runs and the energy."""
+    tmp = tempfile.mkdtemp()
+    path = os.path.join(tmp, "b.dat")
+    unomol_amd.generate_deck(path, "water_cluster", 16, 16001, "6-31G*", 0, 1e-10, 3)
+    best, e = None, None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        b = unomol_amd.Basis(path)
+        t = unomol_amd.TwoElectronInts(b)
+        hf = unomol_amd.RestrictedHartreeFock(b, t)
+        e = hf.findEnergy(maxits=3, eps=1e-10, cflag=2, accel=0, guess=0)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return best, e
 
 
 def read_profile(N, t):
@@ -163,7 +234,7 @@ def roofline_of(classes):
     tot_fl = sum(c["gflop"] for c in classes) * 1e9
     tot_ms = sum(c["ms"] for c in classes)
     name = kernel_name(dom)
-    return {"bound": "mfma", "pipe": "%s VALU: no dense contraction; MI355X's published dense %s peak, "
+    return {"bound": "valu", "pipe": "%s VALU (no dense contraction); MI355X's published dense %s peak, "
                                      "%.1f TFLOP/s, is the bound" % (dom["precision"].upper(), dom["precision"].upper(), peak),
             "kernel": name, "class": dom["class"], "precision": dom["precision"],
             "launches": int(dom["launches"]), "avg_launch_ms": round(dom["ms"] / max(dom["launches"], 1), 4),
@@ -185,33 +256,15 @@ def get_model(unomol_amd):
                  "loss": loss}
 
 
-def batched_decks(tmp, count, seed):
-    """The batched workload: seeded distortions of H2, HF, H2O, NH3, CH4, N2, CO, CO2, C2H2
-    and 1-8-water clusters, 6-31G*, every other one a +1 cation (UHF)."""
-    import unomol_amd
-    rng = np.random.default_rng(seed)
-    paths = []
-    for i in range(count):
-        charge = i % 2
-        path = os.path.join(tmp, "m%03d.dat" % i)
-        if rng.uniform() < 0.5:
-            unomol_amd.generate_deck(path, "molecule", int(rng.integers(0, 9)), seed * 10000 + i, "6-31G*", charge,
-                                     1e-8, 200)
-        else:
-            unomol_amd.generate_deck(path, "water_cluster", int(rng.integers(1, 9)), seed * 10000 + i, "6-31G*",
-                                     charge, 1e-8, 200)
-        paths.append(path)
-    return paths
-
-
 def run_batched(args):
     """Batched SCF: all molecules iterate in lockstep on the device (one ERI launch per class
     pair over every molecule, grouped GEMMs, batched Jacobi eigensolver).  Timed: building
     the batch (every molecule's pairs and Schwarz bounds) and the SCF to |dE| < 1e-8.
     Beside it: the single-molecule engine run molecule after molecule on a sample."""
     import unomol_amd
+    from unomol_amd.workloads import batched_pool
     tmp = tempfile.mkdtemp()
-    paths = batched_decks(tmp, args.molecules, 5)
+    paths = batched_pool(tmp, args.molecules, 5)
     opts = dict(maxits=200, eps=1e-8, cflag=3, accel=2)
     from unomol_amd.deck import read_deck
     sizes = [read_deck(p)["norb"] for p in paths]
@@ -256,12 +309,12 @@ def run_batched(args):
     log("single-molecule engine: %.3f s per molecule on %d molecules; max |E_batch - E_single| %.2e" %
         (single, nsample, worst))
     print(json.dumps({
-        "metric": "batched SCF throughput, %d mixed RHF/UHF molecules (6-31G*), |dE|<1e-8 Eh" % len(paths),
+        "metric": "batched SCF throughput, %d mixed RHF/UHF molecules (6-31G / 6-31G*), |dE|<1e-8 Eh" % len(paths),
         "value": round(len(paths) / wall, 2), "unit": "molecules/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(1e3 * wall, 1), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "%d molecules: distorted H2 HF H2O NH3 CH4 N2 CO CO2 C2H2 and 1-8-water clusters, "
-                               "%d-%d functions, every other one a +1 cation (UHF), seed 5" %
+                               "6-31G or 6-31G*, %d-%d functions, every other one a +1 cation (UHF), seed 5" %
                                (len(paths), min(sizes), max(sizes)), "global_batch": len(paths),
                    "parallelism": "single-gpu"},
         "converged": nconv, "lockstep_iterations": iters,
@@ -361,12 +414,22 @@ def main():
                 "tuned_build_ms_fp64": round(ms_fp64, 3)}
         log("autotune: %.2f s (tuned build %.1f ms mixed, %.1f ms FP64)" % (tune["seconds"], ms_mixed, ms_fp64))
         del tt, hft
-    # the plain FP64 reference run (core-H guess, full FP64 builds), before the timed steps
-    # so that the timed steps' launches are the last of their kernels in the process
+    # the plain FP64 reference run (core-H guess, full FP64 builds) and the ablation runs,
+    # before the timed steps so that the timed steps' launches are the last of their
+    # kernels in the process
+    ablation = {}
     if not args.fp64_only:
         t64, hf64, e64, dt64 = one_scf(fp64_opts)
         ctx64_s = ctx_s[0]
-        log("fp64 reference: E=%.10f in %.2f s, %d iterations" % (e64, dt64, hf64.result.iterations))
+        r64 = hf64.result
+        ev64, P64 = hf64.evals.copy(), hf64.P.copy()
+        log("fp64 reference: E=%.10f in %.2f s, %d iterations" % (e64, dt64, r64.iterations))
+        for name, opts in (("fp64+learned", dict(fp64_opts, guess=2)),
+                           ("mixed+coreH", dict(fp64_opts, precision=1, incremental=1, fp64_final=3))):
+            _, hfa, ea, dta = one_scf(opts)
+            ablation[name] = {"scf_s": round(dta, 4), "iterations": hfa.result.iterations,
+                              "energy_diff_vs_fp64": ea - e64}
+            log("ablation %s: E=%.10f in %.2f s, %d iterations" % (name, ea, dta, hfa.result.iterations))
     for i in range(args.warmup):
         _, _, e, dt = one_scf(step_opts)
         log("warmup %d: E=%.10f in %.2f s" % (i, e, dt))
@@ -394,24 +457,49 @@ def main():
         roofline = roofline_of(classes)
         log("dominant kernel %s: %d launches per step, %.3f ms per launch, %.3f TFLOP/s" %
             (roofline["kernel"], roofline["launches"], roofline["avg_launch_ms"], roofline["achieved"]))
+        # the per-class table is long: stderr only, one JSON line
+        print("[bench] eri_classes " + json.dumps(classes), file=sys.stderr, flush=True)
 
     if args.fp64_only:
         t64, hf64, e64, dt64 = t, hf, energy, dt
         ctx64_s = step_ctx_s
-    r64 = hf64.result
+        r64 = hf64.result
+        ev64, P64 = hf64.evals.copy(), hf64.P.copy()
+    # errors of the timed path against the plain FP64 GPU run (same deck, same threshold)
+    nocc = basis.deck["nelec"] // 2
+    err = {"energy": energy - e64,
+           "orbital_energy_max_abs": float(np.abs(hf.evals - ev64).max()),
+           "occupied_orbital_energy_max_abs": float(np.abs(hf.evals[:nocc] - ev64[:nocc]).max()),
+           "density_max_abs": float(np.abs(hf.P - P64).max())}
+    log("errors vs FP64: dE %.2e, max |d eps| %.2e (occupied %.2e), max |dP| %.2e" %
+        (err["energy"], err["orbital_energy_max_abs"], err["occupied_orbital_energy_max_abs"],
+         err["density_max_abs"]))
     # streaming copy of 1 GiB: the HBM reference point for the traffic counters
     cms = C.c_double()
     N.check(N.lib().uno_copy_probe(C.c_ulonglong(1 << 30), 1, C.byref(cms)))
     copy = {"bytes_each_way": 1 << 30, "ms": round(cms.value, 4),
             "GBps": round(2 * (1 << 30) / (cms.value * 1e-3) / 1e9, 1)}
+    # the 16-water cluster: plain FP64 GPU against the reference program's measured run
+    log("config b: 16-water 6-31G*, three FP64 iterations")
+    b_s, b_e = run_config_b(unomol_amd)
+    refb = reference_config_b()
+    cfgb = {"workload": "16-water 6-31G* RHF (304 functions), seed 16001, core-H guess, 3 SCF iterations with "
+                        "the reference's damping", "gpu_fp64_s": round(b_s, 4), "energy": b_e}
+    if refb:
+        cfgb["reference"] = refb
+        for k in ("serial_s", "mpi8_s"):
+            if refb.get(k):
+                cfgb["speedup_vs_reference_" + k[:-2]] = round(refb[k] / b_s, 1)
+    log("config b: %.3f s on the GPU%s" % (b_s, "" if not refb else "; reference serial %.1f s" %
+                                                                       refb.get("serial_s", float("nan"))))
     log("cpu baseline")
-    cpu = None if args.no_cpu_baseline else cpu_baseline(deck)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(deck, r64.iterations)
 
     def summary(res, secs, ctx):
         return {"scf_s": round(secs, 4), "context_s": round(ctx, 4), "iterations": res.iterations,
-                "fock_builds": res.nfock,
-                "full_builds": res.nfull, "fock_s": round(res.fock_s, 4), "diag_s": round(res.diag_s, 4),
-                "guess_s": round(res.guess_s, 4), "setup_s": round(res.setup_s, 4), "diis_s": round(res.diis_s, 4), "shell_quartets": res.quartets, "fp32_quartets": res.quartets32,
+                "fock_builds": res.nfock, "full_builds": res.nfull, "fock_s": round(res.fock_s, 4),
+                "diag_s": round(res.diag_s, 4), "guess_s": round(res.guess_s, 4), "setup_s": round(res.setup_s, 4),
+                "diis_s": round(res.diis_s, 4), "shell_quartets": res.quartets, "fp32_quartets": res.quartets32,
                 "fock_builds_per_s": round(res.nfock / res.fock_s, 4) if res.fock_s > 0 else None,
                 "shell_quartets_per_s": round(res.quartets / res.fock_s, 1) if res.fock_s > 0 else None}
 
@@ -435,19 +523,21 @@ def main():
                    "global_batch": 1, "parallelism": "single-gpu"},
         "energy": energy,
         "energy_fp64": e64,
-        "energy_diff_vs_fp64": energy - e64,
+        "errors_vs_fp64": err,
         "fp64_s": round(dt64, 4),
         "speedup_vs_fp64": round(dt64 / (wall / args.steps), 3),
+        "fock_builds_per_s": round(r.nfock / r.fock_s, 3) if r.fock_s > 0 else None,
         "steps_detail": summary(r, wall / args.steps, step_ctx_s),
         "fp64_detail": summary(r64, dt64, ctx64_s),
+        "ablation": ablation,
         "guess_model": model_info,
         "autotune": tune,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "config_b": cfgb,
         "hbm_copy_probe": copy,
-        "eri_classes": classes,
     }
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -77,7 +77,10 @@ int uno_ctx_destroy(uno_ctx* ctx);
  * default, takes the first that fits the LDS budget; a forced one that does not fit
  * falls back to that choice), "row_density_lds" (the bra's exchange density rows staged
  * in LDS beside the merged rows: -1, the default, wherever they fit the budget and the
- * autotuner does not prefer otherwise; 0 never; 1 wherever they fit). */
+ * autotuner does not prefer otherwise; 0 never; 1 wherever they fit), "rys2_exact" (0, the
+ * default: the two-root classes use the reference's own two-root quadrature,
+ * Rys.cpp:413-631, which for 15 < x <= 33 departs from the exact quadrature by up to
+ * 3.5e-4 relative, so that ERIs match the reference to 1e-12; 1: the exact quadrature). */
 int uno_ctx_set(uno_ctx* ctx, const char* key, double value);
 /* Reads the knobs above and, read-only: "last_quartets" (shell quartets evaluated by the
  * last Fock build), "last_quartets32" (of which in FP32), "pairs" (significant shell pairs), "pair_runs" (runs of the pair
@@ -118,6 +121,11 @@ typedef struct uno_scf_opts {
     int single_start;    /* 1: one update before the loop and no UHF energy reset, as the
                             reference's finite-field and DPM SCFs (RHF.hpp:246-255); 2: the
                             same with eps used as given (no DBL_EPSILON N^2/2 floor) */
+    int guess_check;     /* guess 2: 1 = also form the core-Hamiltonian guess (RHF.hpp:205-211)
+                            and keep whichever has the lower iteration-0 energy (two extra
+                            Fock builds); 0 = keep the learned guess unless it fails its own
+                            checks (electron count off by more than 10 %, purification
+                            diverging, non-finite values) */
 } uno_scf_opts;
 
 typedef struct uno_scf_result {
@@ -136,6 +144,10 @@ typedef struct uno_scf_result {
     double setup_s;       /* seconds before the first update: buffers, one-electron
                              matrices, X, the guess */
     double diis_s;        /* seconds in the DIIS extrapolation (accel = 2) */
+    int guess_fallback;   /* guess 2: 1 when the core-Hamiltonian guess replaced the learned one */
+    double guess_energy;  /* guess 2: iteration-0 energy of the learned guess when guess_check
+                             evaluated it (else 0) */
+    double guess_energy_coreh;  /* the same for the core-Hamiltonian guess */
 } uno_scf_result;
 
 void uno_scf_defaults(uno_scf_opts* opts);
@@ -155,6 +167,10 @@ int uno_rys_eval(int n, int npts, const double* x, double* u, double* w);
 /* Debug/parity: the same from the single-precision tables of the FP32 quartets
  * (x rounded to float; results widened to double). */
 int uno_rys_eval_f32(int n, int npts, const double* x, double* u, double* w);
+/* Debug/parity: two roots from the reference-compatible table that the ERI kernels use
+ * by default (the reference's Rys::root2, Rys.cpp:413-631, including its 15 < x <= 33
+ * behaviour); fp32 != 0 selects the single-precision table. */
+int uno_rys_eval_ref2(int fp32, int npts, const double* x, double* u, double* w);
 /* Debug/parity: Schwarz bounds sqrt(max|(ab|ab)|) per shell pair a>=b, packed order. */
 int uno_schwarz(uno_ctx* ctx, double* q);
 
@@ -262,6 +278,9 @@ typedef struct uno_batch uno_batch;
 int uno_batch_create(int nmol, const uno_basis* bases, const int* nelec, uno_batch** out);
 int uno_batch_destroy(uno_batch* batch);
 int uno_batch_nmol(const uno_batch* batch);
+/* The final density of molecule m after uno_batch_run, dense row-major N x N: the RHF half
+ * density, or alpha then beta (2 N x N) for a UHF molecule. */
+int uno_batch_density(const uno_batch* batch, int m, double* P);
 /* res[nmol] per molecule (wall_s, fock_s and quartets are the batch totals);
  * energies (optional) [nmol] total energies. */
 int uno_batch_run(uno_batch* batch, const uno_scf_opts* opts, uno_scf_result* res, double* energies);

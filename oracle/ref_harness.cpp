@@ -7,6 +7,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <chrono>
+#include <cmath>
+#include <algorithm>
 #include <random>
 #include <vector>
 #include "Basis.hpp"
@@ -122,16 +124,17 @@ int ref_oneint(const char* deck, double* S, double* T, double* H) {
 
 // Times the reference ERI kernel over a random sample of shell quartets of the deck
 // (canonical quartets drawn uniformly with replacement by the given seed).
-// Returns the seconds spent, the number of function quartets evaluated and the
-// number of canonical shell quartets in the whole deck.
+// Returns the seconds spent, the number of function quartets evaluated, how many of
+// them the reference would store, and the number of canonical shell quartets in the
+// whole deck.
 int ref_eri_sample(const char* deck, long nsample, unsigned long seed, double* seconds,
-                   long* nfunc, double* total_shell_quartets) {
+                   long* nfunc, double* total_shell_quartets, long* nstored) {
     Basis B{std::string(deck)};
     Ctx X(B);
     const int ns = B.number_of_shells();
     std::mt19937_64 rng(seed);
     std::uniform_int_distribution<int> pick(0, ns - 1);
-    long nf = 0;
+    long nf = 0, nk = 0;
     auto t0 = std::chrono::steady_clock::now();
     for (long s = 0; s < nsample; ++s) {
         int i = pick(rng), j = pick(rng), k = pick(rng), l = pick(rng);
@@ -139,14 +142,56 @@ int ref_eri_sample(const char* deck, long nsample, unsigned long seed, double* s
         if (l > k) std::swap(k, l);
         if (k > i) { std::swap(i, k); std::swap(j, l); }
         if (j > i) std::swap(i, j);
-        nf += X.quartet(i, j, k, l);
+        const int m = X.quartet(i, j, k, l);
+        nf += m;
+        // the records the reference would store (|val| > 1e-14, TwoElectronInts.cpp:668)
+        for (int c = 0; c < m; ++c) nk += std::fabs(X.buf[c].val) > 1e-14 ? 1 : 0;
     }
     auto t1 = std::chrono::steady_clock::now();
     *seconds = std::chrono::duration<double>(t1 - t0).count();
     *nfunc = nf;
-    double m = (double)ns * (ns + 1) / 2;
+    if (nstored) *nstored = nk;
     *total_shell_quartets = (double)ns * (ns + 1) / 2 * ((double)ns * (ns + 1) / 2 + 1) / 2;
-    (void)m;
+    return 0;
+}
+
+// The reference's own direct-store integral pass and Fock replay on a deck: builds the
+// reference TwoElectronInts (every canonical ERI with |val| > 1e-14 into its in-memory
+// store, TwoElectronInts.cpp:511-697) and then times `reps` calls of its
+// formGmatrix(P, G) (TwoElectronInts.cpp:822-844) on a fixed symmetric density.  The store
+// lives in the reference's 8 GiB in-memory cache (no disk spill for decks that fit it).
+// store_label is the cache's file name (the reference removes it at exit).  Returns the
+// seconds of the integral pass, the average seconds per G build and the stored ERI count.
+int ref_fock_sample(const char* deck, const char* store_label, int reps, double* eri_seconds, double* g_seconds,
+                    double* nstored) {
+    Basis B{std::string(deck)};
+    const int n = B.number_of_orbitals();
+    const long n2 = (long)n * (n + 1) / 2;
+    auto t0 = std::chrono::steady_clock::now();
+    TwoElectronInts ti(B, 0, std::string(store_label));
+    auto t1 = std::chrono::steady_clock::now();
+    std::vector<double> P(n2), G(n2);
+    for (long i = 0; i < n2; ++i) P[i] = 0.01 * (double)((i * 7919) % 101) / 101.0;
+    auto t2 = std::chrono::steady_clock::now();
+    for (int r = 0; r < reps; ++r) {
+        std::fill(G.begin(), G.end(), 0.0);
+        ti.formGmatrix(P.data(), G.data());
+    }
+    auto t3 = std::chrono::steady_clock::now();
+    *eri_seconds = std::chrono::duration<double>(t1 - t0).count();
+    *g_seconds = std::chrono::duration<double>(t3 - t2).count() / (reps > 0 ? reps : 1);
+    // stored records: count the ERIs above the reference's 1e-14 drop
+    Ctx X(B);
+    const int ns = B.number_of_shells();
+    double cnt = 0;
+    for (int i = 0; i < ns; ++i)
+        for (int j = 0; j <= i; ++j)
+            for (int k = 0; k <= i; ++k)
+                for (int l = 0; l <= k; ++l) {
+                    int m = X.quartet(i, j, k, l);
+                    for (int c = 0; c < m; ++c) cnt += std::fabs(X.buf[c].val) > 1e-14 ? 1 : 0;
+                }
+    *nstored = cnt;
     return 0;
 }
 

@@ -54,8 +54,58 @@ static int sturm(int n, const q* a, const q* b2, q lam) {
     return cnt;
 }
 
+// The reference's two-root quadrature (Rys::root2, Rys.cpp:413-631) departs from the
+// exact rule for 15 < x <= 40: there it evaluates one form over the whole range
+// (Rys.cpp:615-624),
+//   r_i = (a_i x + b_i) e^-x + h_i / (x - h_i)          (r = t^2 / (1 - t^2)),
+//   w_2 = (a_3 x + b_3) e^-x + (2 omega_2 / sqrt(pi)) W,  w_1 = W - w_2,  W = sqrt(pi / (4 x)),
+// with h_i, omega_i the squared positive nodes and the weights of the 4-point
+// Gauss-Hermite rule (its large-x limit).  The oracle restates that form; the six
+// coefficients (a_i, b_i) are set by orc_set_rys2_ref from a least-squares fit to the
+// reference's own outputs (the test harness fits them to the committed samples).
+// Off (the default), the exact rule is used everywhere.
+static int g_ref2 = 0;
+static double g_ref2c[6];
+
+extern "C" int orc_set_rys2_ref(int on, const double* coef) {
+    if (on && !coef) return 1;
+    g_ref2 = on;
+    if (on)
+        for (int i = 0; i < 6; ++i) g_ref2c[i] = coef[i];
+    return 0;
+}
+
+// the large-x constants above from the 4-point Gauss-Hermite rule: positive nodes
+// s^2 = (3 -+ sqrt 6) / 2, weights 12 sqrt(pi) / H_3(s)^2 with H_3(s) = 8 s^3 - 12 s
+static void hermite4(double h[2], double om[2]) {
+    const double r6 = sqrt(6.0);
+    h[0] = (3.0 - r6) / 2;
+    h[1] = (3.0 + r6) / 2;
+    for (int i = 0; i < 2; ++i) {
+        const double s = sqrt(h[i]), H3 = 8 * s * s * s - 12 * s;
+        om[i] = 12 * sqrt(M_PI) / (H3 * H3);
+    }
+}
+
+static void rys2_ref(double x, double* u, double* w) {
+    double h[2], om[2];
+    hermite4(h, om);
+    const double e = exp(-x), W = sqrt(M_PI / (4 * x));
+    const double r1 = (g_ref2c[0] * x + g_ref2c[1]) * e + h[0] / (x - h[0]);
+    const double r2 = (g_ref2c[2] * x + g_ref2c[3]) * e + h[1] / (x - h[1]);
+    const double w2 = (g_ref2c[4] * x + g_ref2c[5]) * e + (2 * om[1] / sqrt(M_PI)) * W;
+    u[0] = r1 / (1 + r1);
+    u[1] = r2 / (1 + r2);
+    w[0] = W - w2;
+    w[1] = w2;
+}
+
 extern "C" int orc_rys(int n, double xd, double* u, double* w) {
     if (n < 1 || n > 9 || !(xd >= 0)) return 1;
+    if (n == 2 && g_ref2 && xd > 15.0 && xd <= 40.0) {
+        rys2_ref(xd, u, w);
+        return 0;
+    }
     q x = xd;
     q mu[20];
     boys_q(2 * n - 1, x, mu);

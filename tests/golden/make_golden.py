@@ -7,7 +7,8 @@ Sources, all of them the reference's own data or its own code:
                    (test/patin.dat.*, scfout.dat.*, short.dat.*, moments.dat.*), copied as data;
   * ref_run/<tag>/ outputs of the reference rebuilt from its sources (oracle/_ref/Unomol)
                    on those decks and on this project's generated decks (config A water
-                   6-31G, water 6-31G*, its +1 cation, a 2-water cluster);
+                   6-31G, water 6-31G*, its +1 cation, a 2-water cluster, the 16-water
+                   cluster and its cation, the DPM scans);
   * ref_run_x2/    the same runs with only the reference's 2-root fit (Rys.cpp:413-631)
                    replaced by exact roots (oracle/_ref/Unomol_x2);
   * eri_<tag>.npz  every canonical ERI of small decks from the reference's ERI code
@@ -19,6 +20,8 @@ usage: make -C oracle && make -C oracle ref && python tests/golden/make_golden.p
        python tests/golden/make_golden.py --config-b    (the 16-water runs only, a few minutes)
        python tests/golden/make_golden.py --extra       (finite-field decks and the N atom)
        python tests/golden/make_golden.py --dpm         (DPM scans of water and its cation)
+       python tests/golden/make_golden.py --outputs     (every output file of a few decks, with
+                                                         printing switched on, and a restart)
 """
 import ctypes as C
 import os
@@ -102,6 +105,7 @@ def make_config_b():
     for tag, (kind, n, seed, basis, charge) in B_DECKS.items():
         path = os.path.join(gdir, tag + ".dat")
         unomol_amd.generate_deck(path, kind, n, seed, basis, charge, 1e-10, B_MAXITS)
+        run_ref(open(path).read(), os.path.join(HERE, "ref_run", tag))
         run_ref(open(path).read(), os.path.join(HERE, "ref_run_x2", tag), exe="Unomol_x2")
         print("config B reference run done:", tag, flush=True)
 
@@ -111,6 +115,7 @@ def make_extra():
     for tag in EXTRA_TAGS:
         src = os.path.join(REF, "test", "patin.dat." + tag)
         shutil.copy(src, os.path.join(dst, "patin.dat." + tag))
+        run_ref(open(src).read(), os.path.join(HERE, "ref_run", tag))
         run_ref(open(src).read(), os.path.join(HERE, "ref_run_x2", tag), exe="Unomol_x2")
         print("extra reference run done:", tag, flush=True)
 
@@ -127,12 +132,58 @@ def make_dpm():
         lines[3] = "1 0"
         text = "\n".join(lines)
         open(path, "w").write(text)
-        run_ref(text, os.path.join(HERE, "ref_run_x2", tag), exe="Unomol_x2",
-                extra_files={"posin.bas": POSIN, "pos.grid.dat": GRID})
+        for sub, exe in (("ref_run", "Unomol"), ("ref_run_x2", "Unomol_x2")):
+            run_ref(text, os.path.join(HERE, sub, tag), exe=exe, extra_files={"posin.bas": POSIN, "pos.grid.dat": GRID})
         print("dpm reference run done:", tag, flush=True)
 
 
+# full output sets of the reference (every file it writes, binary ones included) for the
+# byte/format checks: the deck with eigenvector and Mulliken-overlap printing switched on
+# (prt_flag[0] = prt_flag[2] = 1) and a restart from the reference's own PMATRIX.DAT
+# (scf_flag[2] = 1, RHF.hpp:120-123)
+OUT_TAGS = ["631.h2o", "431.co", "water_631gs_cation", "b.dhdz"]
+OUT_FILES = ("short.gs.out", "scfout.gs.out", "moments.out", "intsout.out", "mol_dipmom.out", "overlaps.out",
+             "PMATRIX.DAT", "MOMINTS.DAT", "RMOM.DAT")
+
+
+def _set_line(text, idx, value):
+    lines = text.split("\n")
+    lines[idx] = value
+    return "\n".join(lines)
+
+
+def make_outputs():
+    for tag in OUT_TAGS:
+        src = os.path.join(HERE, "ref_run", tag, "patin.dat")
+        text = open(src).read()
+        assert len(open(src).read().split("\n")[5].split()) == 3
+        text = _set_line(text, 5, " 1 0 1")
+        for sub, deck, extra in (("print", text, None), ("restart", None, True)):
+            outdir = os.path.join(HERE, "ref_out", tag, sub)
+            os.makedirs(outdir, exist_ok=True)
+            with tempfile.TemporaryDirectory() as tmp:
+                d = deck
+                if extra:
+                    flags = text.split("\n")[4].split()
+                    d = _set_line(text, 4, " %s %s 1" % (flags[0], flags[1]))
+                    shutil.copy(os.path.join(HERE, "ref_out", tag, "print", "PMATRIX.DAT"),
+                                os.path.join(tmp, "PMATRIX.DAT"))
+                    shutil.copy(os.path.join(tmp, "PMATRIX.DAT"), os.path.join(outdir, "PMATRIX.in"))
+                with open(os.path.join(tmp, "patin.dat"), "w") as f:
+                    f.write(d)
+                shutil.copy(os.path.join(tmp, "patin.dat"), os.path.join(outdir, "patin.dat"))
+                subprocess.run([os.path.join(ROOT, "oracle", "_ref", "Unomol")], cwd=tmp, check=True,
+                               stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+                for fn in OUT_FILES:
+                    if os.path.exists(os.path.join(tmp, fn)):
+                        shutil.copy(os.path.join(tmp, fn), os.path.join(outdir, fn))
+        print("output set done:", tag, flush=True)
+
+
 def main():
+    if "--outputs" in sys.argv:
+        make_outputs()
+        return
     if "--dpm" in sys.argv:
         make_dpm()
         return

@@ -27,7 +27,49 @@ def lib():
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
         _lib = C.CDLL(path)
+        set_rys2_mode(True, _lib)
     return _lib
+
+
+# Golden samples of the reference's two-root quadrature (tests/golden/rys2_ref.npz, written
+# by unomol_amd/csrc/gen_rys_ref2.py from the reference compiled from its sources).
+RYS2_NPZ = os.path.join(ROOT, "tests", "golden", "rys2_ref.npz")
+_rys2_coef = None
+
+
+def rys2_ref_coef():
+    """Least-squares (a_i, b_i) of the reference's 15 < x <= 40 two-root form
+    (Rys.cpp:615-624): r - h/(x-h) and w_2 - c W are (a x + b) e^-x.  Fitted on
+    15 < x <= 22, where that term is largest, minimising the error of r and w_2 themselves."""
+    global _rys2_coef
+    if _rys2_coef is None:
+        z = np.load(RYS2_NPZ)
+        x, r, w = z["x"], z["r"], z["w"]
+        m = (x > 15.0) & (x <= 22.0)
+        x, r, w = x[m], r[m], w[m]
+        r6 = np.sqrt(6.0)
+        h = np.array([(3 - r6) / 2, (3 + r6) / 2])
+        s = np.sqrt(h[1])
+        om2 = 12 * np.sqrt(np.pi) / (8 * s ** 3 - 12 * s) ** 2
+        W = np.sqrt(np.pi / (4 * x))
+        e = np.exp(-x)
+        A = np.stack([x * e, e], axis=1)
+        coef = []
+        for y in (r[:, 0] - h[0] / (x - h[0]), r[:, 1] - h[1] / (x - h[1]), w[:, 1] - 2 * om2 / np.sqrt(np.pi) * W):
+            coef += list(np.linalg.lstsq(A, y, rcond=None)[0])
+        _rys2_coef = np.array(coef)
+    return _rys2_coef
+
+
+def set_rys2_mode(reference, L=None):
+    """reference=True: the oracle's two-root quadrature is the reference's (the default,
+    as in the product); False: the exact rule."""
+    L = L or lib()
+    if reference:
+        c = np.ascontiguousarray(rys2_ref_coef())
+        assert L.orc_set_rys2_ref(1, _p(c)) == 0
+    else:
+        assert L.orc_set_rys2_ref(0, None) == 0
 
 
 def _p(a):

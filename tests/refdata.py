@@ -100,3 +100,101 @@ def read_finitefield(path):
             out["alpha"].append(float(t[1]))
             out["polnrg"].append(float(t[2]))
     return out
+
+
+# ---- the other output files (RHF.hpp:172-174,494-531,571-660; Moments.cpp:95-186,365-469) --
+MOM_DTYPE = np.dtype([("v", "<f8", (9,)), ("ijr", "<u4"), ("pad", "<u4")])   # 80-byte MomInts record
+
+
+def read_momints(path):
+    return np.fromfile(path, dtype=MOM_DTYPE)
+
+
+def read_intsout(path):
+    """(text of the header and shell dump, [S, T, H] packed arrays) of intsout.out."""
+    head, mats, cur = [], [], None
+    for ln in open(path):
+        if ln.strip().endswith("Matrix"):
+            cur = []
+            mats.append(cur)
+        elif cur is not None:
+            p = ln.split()
+            if len(p) == 3:
+                cur.append(float(p[2]))
+        else:
+            head.append(ln.rstrip())
+    return "\n".join(head).strip(), [np.array(m) for m in mats]
+
+
+def read_overlaps(path):
+    rows = [ln.split() for ln in open(path)][2:]
+    return np.array([[float(v) for v in r[2:]] for r in rows if r])
+
+
+def read_mulliken(path):
+    """(per-orbital populations [n x 1 or n x 2], per-centre net charges) from scfout."""
+    pops, charges, mode = [], [], None
+    for ln in open(path):
+        if "Mulliken Populations" in ln:
+            mode = "pop"
+        elif "Mulliken Atomic Charges" in ln:
+            mode = "chg"
+        elif ln.startswith("xxxx") or "Eigenvectors" in ln:
+            if mode == "chg":
+                mode = None
+        elif mode == "pop":
+            p = ln.split()
+            if p and p[0].isdigit():
+                pops.append([float(v) for v in p[1:]])
+        elif mode == "chg":
+            p = ln.split()
+            if len(p) == 3 and p[0].isdigit():
+                charges.append(float(p[2]))
+    return np.array(pops), np.array(charges)
+
+
+def read_eigvecs(path, n):
+    """Printed eigenvector blocks of scfout (prt_flag[0]): list per spin of (energies [n],
+    coefficients [n x n], column i = vector i)."""
+    out, cur, vals = [], None, []
+    for ln in open(path):
+        if "Eigenvectors" in ln:
+            cur = {"e": [], "c": []}
+            out.append(cur)
+        elif cur is not None and ln.startswith("Column"):
+            cur["e"].append(float(ln.split("EigenEnergy=")[1]))
+            vals = []
+            cur["c"].append(vals)
+        elif cur is not None and ln.strip() and not ln.startswith("xxxx"):
+            vals.extend(float(v) for v in ln.split())
+    return [(np.array(b["e"]), np.array(b["c"]).T) for b in out]
+
+
+def read_mol_dipmom(path):
+    """Blocks of MO transition dipoles: list per spin of {(i, j): (dx, dy, dz)}."""
+    blocks, cur = [], None
+    for ln in open(path):
+        if "TRANSISITION DIPOLE MOMENTS" in ln:
+            cur = {}
+            blocks.append(cur)
+        elif cur is not None:
+            p = ln.split()
+            if len(p) == 5 and p[0].isdigit():
+                cur[(int(p[0]), int(p[1]))] = tuple(float(v) for v in p[2:])
+    return blocks
+
+
+def assert_output_files(got_dir, norb=None, uhf=False):
+    """The drop-in run wrote every file the reference writes, with the reference's record
+    layouts: PMATRIX.DAT packed doubles (alpha then beta), MOMINTS.DAT / RMOM.DAT one 80-byte
+    record per packed function pair with its packed index, intsout.out / mol_dipmom.out."""
+    got_dir = str(got_dir)
+    for fn in ("PMATRIX.DAT", "MOMINTS.DAT", "RMOM.DAT", "intsout.out", "mol_dipmom.out"):
+        assert os.path.exists(os.path.join(got_dir, fn)), fn
+    if norb is not None:
+        n2 = norb * (norb + 1) // 2
+        assert os.path.getsize(os.path.join(got_dir, "PMATRIX.DAT")) == 8 * n2 * (2 if uhf else 1)
+        for fn in ("MOMINTS.DAT", "RMOM.DAT"):
+            m = read_momints(os.path.join(got_dir, fn))
+            assert len(m) == n2
+            assert sorted(m["ijr"].tolist()) == list(range(n2))

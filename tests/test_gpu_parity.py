@@ -60,20 +60,53 @@ def test_rys_moment_identity(n):
         np.testing.assert_allclose(mom, f, rtol=1e-13)
 
 
+def test_rys_ref2_table_vs_reference(golden):
+    """The two-root table the ERI kernels use by default against the reference's own
+    two-root quadrature (Rys::root2, sampled from the reference build): 2e-15 relative
+    where it is the reference's fit (15 < x <= 40, including its 15 < x <= 33 departure
+    from the exact rule), within the reference's own fit error (5e-13) elsewhere, where
+    the table is the exact rule; the float table to float rounding.  x = 15 itself is
+    left out: the reference takes its x <= 15 branch there, the table its [15, 16) interval."""
+    say("rys reference-compatible 2-root table vs the reference's root2")
+    z = np.load(os.path.join(golden, "rys2_ref.npz"))
+    keep = z["x"] != 15.0
+    x, r, w = z["x"][keep], z["r"][keep], z["w"][keep]
+    u_ref = r / (1 + r)
+    inside = (x > 15.0) & (x <= 40.0)
+    u, ww = unomol_amd.rys_eval_ref2(x)
+    for got, ref in ((u, u_ref), (ww, w)):
+        rel = np.abs(got - ref) / np.abs(ref)
+        assert rel[inside].max() < 2e-15
+        assert rel[~inside].max() < 5e-13
+    uf, wf = unomol_amd.rys_eval_ref2(x, fp32=True)
+    np.testing.assert_allclose(uf, u_ref, rtol=2e-6)
+    np.testing.assert_allclose(wf, w, rtol=2e-6)
+    # the exact table departs from the reference there by far more than 1e-12
+    ue, _ = unomol_amd.rys_eval(2, x[inside])
+    assert (np.abs(ue - u_ref[inside]) / u_ref[inside]).max() > 1e-6
+
+
 @pytest.mark.parametrize("tag", ["A_water_631g", "water_631gs"])
 def test_eri_vs_reference(golden, tag):
+    """Every canonical ERI of the deck against the reference's own ERI code: 1e-12
+    absolute in every class with the default (reference-compatible) two-root quadrature;
+    with the exact quadrature (rys2_exact = 1), against the reference with its two-root
+    fit replaced by the exact rule."""
     say("eri emit vs reference " + tag)
     z = np.load(os.path.join(golden, "eri_%s.npz" % tag))
     b, t = _tints(os.path.join(golden, "decks", tag + ".dat"))
     v, f = t.eri_emit()
     assert (f == z["ijkl"]).all()
-    # the reference ERI code with exact 2-root quadrature: 1e-12 absolute everywhere
-    assert np.abs(v - z["ref_x2"]).max() < 1e-12
-    # plain reference: all classes but the 2-root ones (its fit error, Rys.cpp:615-624)
+    assert np.abs(v - z["ref"]).max() < 1e-12
+    t.set("rys2_exact", 1)
+    v2, f2 = t.eri_emit()
+    assert (f2 == z["ijkl"]).all()
+    assert np.abs(v2 - z["ref_x2"]).max() < 1e-12
+    # the two modes differ only in the two-root classes
     fl = np.concatenate([[l] * ((l + 1) * (l + 2) // 2) for l in b.deck["sh_l"]])
     L = fl[f].sum(axis=1)
     two = (L == 2) | (L == 3)
-    assert np.abs(v - z["ref"])[~two].max() < 1e-12
+    assert np.abs(v2 - v)[~two].max() == 0.0
 
 
 @pytest.mark.parametrize("tag", ["631.h2o", "631.ch4", "631.co", "w2_631gs"])
@@ -189,29 +222,51 @@ SCF_TAGS = [t for t in R.RHF_TAGS if t not in R.DEGENERATE_TAGS] + R.UHF_TAGS + 
 
 @pytest.mark.parametrize("tag", SCF_TAGS)
 def test_scf_vs_reference(tmp_path, tag):
+    """The drop-in run (patin.dat in, the reference's output files out) against the
+    reference rebuilt from its sources on the same deck: initial and final energies and
+    every orbital energy to 1e-9 Eh, moments to 1e-7."""
     say("scf drop-in run " + tag)
     deck = open(R.deck_path(tag)).read()
     (tmp_path / "patin.dat").write_text(deck)
     r = unomol_amd.run_deck(tmp_path)
     assert r.converged
-    ref = R.read_scfout(os.path.join(R.GOLDEN, "ref_run_x2", tag, "scfout.gs.out"))
-    short = R.read_short(os.path.join(R.GOLDEN, "ref_run_x2", tag, "short.gs.out"))
-    plain = R.read_short(os.path.join(R.GOLDEN, "ref_run", tag, "short.gs.out"))
+    src = os.path.join(R.GOLDEN, "ref_run", tag)
+    ref = R.read_scfout(os.path.join(src, "scfout.gs.out"))
+    short = R.read_short(os.path.join(src, "short.gs.out"))
     got = R.read_short(str(tmp_path / "short.gs.out"))
     mine = R.read_scfout(str(tmp_path / "scfout.gs.out"))
-    # total energies 1e-9 Eh against the reference with exact 2-root quadrature
     assert abs(got["energy"] - short["energy"]) < 1e-9
     assert abs(got["init"] - short["init"]) < 1e-9
-    # against the plain reference: within the effect of its 2-root fit
-    assert abs(got["energy"] - plain["energy"]) <= abs(short["energy"] - plain["energy"]) + 1e-9
     np.testing.assert_allclose(mine["evals"], ref["evals"], atol=1e-9)
     if len(ref["evals_b"]):
         np.testing.assert_allclose(mine["evals_b"], ref["evals_b"], atol=1e-9)
-    mref = R.read_moments(os.path.join(R.GOLDEN, "ref_run_x2", tag, "moments.out"))
+    mref = R.read_moments(os.path.join(src, "moments.out"))
     mgot = R.read_moments(str(tmp_path / "moments.out"))
     R.assert_moments(tag, [mgot[k] for k in R.MOM_KEYS], mref)
-    for fn in ("PMATRIX.DAT", "MOMINTS.DAT", "RMOM.DAT", "intsout.out", "mol_dipmom.out"):
-        assert (tmp_path / fn).exists()
+    # the reference's published goldens of its own test decks (test/short.dat.*)
+    pub = os.path.join(R.GOLDEN, "ref_test", "short.dat." + tag)
+    if os.path.exists(pub):
+        p = R.read_short(pub)
+        assert abs(got["energy"] - p["energy"]) < 1e-9
+        assert abs(got["init"] - p["init"]) < 1e-9
+    R.assert_output_files(tmp_path, read_deck(R.deck_path(tag))["norb"], uhf=bool(len(ref["evals_b"])))
+
+
+@pytest.mark.parametrize("tag", ["631.co", "w2_631gs", "water_631gs_cation"])
+def test_scf_exact_roots_vs_reference_x2(tag):
+    """The opt-in exact two-root quadrature (rys2_exact = 1) against the reference with
+    its two-root fit replaced by the exact rule (oracle/ref_root2_exact.cpp)."""
+    say("scf with exact 2-root quadrature " + tag)
+    path = R.deck_path(tag)
+    b, t = _tints(path)
+    t.set("rys2_exact", 1)
+    cls = unomol_amd.UnRestrictedHartreeFock if b.deck["nelec"] % 2 else unomol_amd.RestrictedHartreeFock
+    hf = cls(b, t)
+    e = hf.findEnergy()
+    assert hf.result.converged
+    ref = R.read_short(os.path.join(R.GOLDEN, "ref_run_x2", tag, "short.gs.out"))
+    assert abs(e - ref["energy"]) < 1e-9
+    assert abs(hf.result.init_energy - ref["init"]) < 1e-9
 
 
 def test_scf_diis_same_energy(tmp_path):
@@ -448,17 +503,16 @@ def test_scf_mixed_precision_energy_uhf(tmp_path):
     assert abs(e - e64) < 1e-8, (e, e64)
 
 
-B_TAGS = [t for t in ("B_w16_631gs", "B_w16_631gs_cation")
-          if os.path.exists(os.path.join(R.GOLDEN, "ref_run_x2", t, "short.gs.out"))]
+B_TAGS = ["B_w16_631gs", "B_w16_631gs_cation"]
 
 
 @pytest.mark.parametrize("tag", B_TAGS)
 def test_config_b_trajectory_vs_reference(tmp_path, tag):
     """16-water 6-31G* cluster (304 functions), RHF and the +1 UHF doublet: three SCF
     iterations from the core-Hamiltonian guess with the reference's damping, against the
-    reference (exact 2-root quadrature) on the same deck."""
+    reference rebuilt from its sources on the same deck."""
     say("config B trajectory " + tag)
-    src = os.path.join(R.GOLDEN, "ref_run_x2", tag)
+    src = os.path.join(R.GOLDEN, "ref_run", tag)
     (tmp_path / "patin.dat").write_text(open(os.path.join(src, "patin.dat")).read())
     unomol_amd.run_deck(tmp_path)
     ref = R.read_short(os.path.join(src, "short.gs.out"))
@@ -568,17 +622,16 @@ def test_oneint_device_vs_reference(golden, tag):
 # energy, depends on the rotation the eigensolver picks inside that set; the converged
 # energy, orbital energies and polarizabilities do not.
 FF_DEGENERATE_CORE_H = ("tz2p.sf6",)
-FF_TAGS = [t for t in ("n.dhdz", "d6s3p.h2", "dh95.h2", "dh95.c2h2", "dh95.co2", "tzp.ch4", "tz2p.sf6")
-           if os.path.exists(os.path.join(R.GOLDEN, "ref_run_x2", t, "short.gs.out"))]
+FF_TAGS = ["n.dhdz", "d6s3p.h2", "dh95.h2", "dh95.c2h2", "dh95.co2", "tzp.ch4", "tz2p.sf6"]
 
 
 @pytest.mark.parametrize("tag", FF_TAGS)
 def test_finite_field_decks_vs_reference(tmp_path, tag):
     """Reference decks with the finite-field analysis (int_flag[1] = 1) and the nitrogen
     atom: ground state and, where requested, the three field SCFs and polarizabilities
-    against the reference (exact 2-root quadrature)."""
+    against the reference rebuilt from its sources."""
     say("finite-field deck " + tag)
-    src = os.path.join(R.GOLDEN, "ref_run_x2", tag)
+    src = os.path.join(R.GOLDEN, "ref_run", tag)
     (tmp_path / "patin.dat").write_text(open(os.path.join(R.GOLDEN, "ref_test", "patin.dat." + tag)).read())
     r = unomol_amd.run_deck(tmp_path)
     ref = R.read_short(os.path.join(src, "short.gs.out"))
@@ -610,8 +663,7 @@ def test_finite_field_decks_vs_reference(tmp_path, tag):
         np.testing.assert_allclose(b["alpha"], a["alpha"], atol=2e-9 * 2 / a["field"] ** 2)
 
 
-DPM_TAGS = [t for t in ("dpm_water", "dpm_water_cation")
-            if os.path.exists(os.path.join(R.GOLDEN, "ref_run_x2", t, "vpol.out"))]
+DPM_TAGS = ["dpm_water", "dpm_water_cation"]
 
 
 @pytest.mark.parametrize("tag", DPM_TAGS)
@@ -619,7 +671,7 @@ def test_dpm_scan_vs_reference(tmp_path, tag):
     """DPM polarization-potential scan (int_flag[0] = 1): positron basis on a moving centre,
     the six-Gaussian model potential and one SCF per grid point, against the reference."""
     say("dpm scan " + tag)
-    src = os.path.join(R.GOLDEN, "ref_run_x2", tag)
+    src = os.path.join(R.GOLDEN, "ref_run", tag)
     for fn in ("patin.dat", "posin.bas", "pos.grid.dat"):
         (tmp_path / fn).write_text(open(os.path.join(src, fn)).read())
     unomol_amd.run_deck(tmp_path)
@@ -672,3 +724,142 @@ def test_batched_scf_matches_single(tmp_path):
         e1 = hf.findEnergy(maxits=100, eps=1e-10, cflag=3, accel=2)
         assert res[i].converged and hf.result.converged
         assert abs(e[i] - e1) < 1e-8, (i, p, e[i], e1)
+
+
+OUT_TAGS = ["631.h2o", "431.co", "water_631gs_cation", "b.dhdz"]
+
+
+def _nondegenerate(e, tol=1e-6):
+    """Orbitals whose energy is at least tol away from every other one (their vectors are
+    fixed up to sign; vectors inside a degenerate level are not)."""
+    d = np.abs(e[:, None] - e[None, :]) + np.eye(len(e)) * 1e9
+    return d.min(axis=1) > tol
+
+
+@pytest.mark.parametrize("tag", OUT_TAGS)
+def test_output_files_vs_reference(tmp_path, tag):
+    """Every file the drop-in run writes against the reference's own files on the same deck
+    (printing switched on: eigenvectors and Mulliken overlap populations), then a restart
+    from the reference's PMATRIX.DAT (scf_flag[2] = 1, RHF.hpp:120-123)."""
+    say("output files vs reference " + tag)
+    src = os.path.join(R.GOLDEN, "ref_out", tag, "print")
+    (tmp_path / "patin.dat").write_text(open(os.path.join(src, "patin.dat")).read())
+    r = unomol_amd.run_deck(tmp_path)
+    assert r.converged
+    n = read_deck(str(tmp_path / "patin.dat"))["norb"]
+    n2 = n * (n + 1) // 2
+    uhf = bool(r.uhf)
+    g = lambda fn: str(tmp_path / fn)              # noqa: E731
+    s = lambda fn: os.path.join(src, fn)            # noqa: E731
+    # intsout.out: the header and shell dump as text, S T H to 1e-13
+    hg, mg = R.read_intsout(g("intsout.out"))
+    hs, ms = R.read_intsout(s("intsout.out"))
+    assert hg == hs
+    for a, b in zip(mg, ms):
+        assert a.shape == b.shape == (n2,)
+        assert np.abs(a - b).max() < 1e-13 * max(1.0, np.abs(b).max())
+    # MOMINTS.DAT / RMOM.DAT: 80-byte records, same order and packed indices, values 1e-12
+    for fn in ("MOMINTS.DAT", "RMOM.DAT"):
+        a, b = R.read_momints(g(fn)), R.read_momints(s(fn))
+        assert os.path.getsize(g(fn)) == os.path.getsize(s(fn)) == 80 * n2
+        assert (a["ijr"] == b["ijr"]).all()
+        assert np.abs(a["v"] - b["v"]).max() < 1e-12 * max(1.0, np.abs(b["v"]).max())
+    # PMATRIX.DAT: packed doubles, alpha then beta
+    pg, ps = np.fromfile(g("PMATRIX.DAT")), np.fromfile(s("PMATRIX.DAT"))
+    assert pg.shape == ps.shape == ((2 if uhf else 1) * n2,)
+    assert np.abs(pg - ps).max() < 1e-8
+    # scfout: energies, Mulliken populations and charges, printed eigenvectors up to sign
+    ms_, mg_ = R.read_scfout(s("scfout.gs.out")), R.read_scfout(g("scfout.gs.out"))
+    np.testing.assert_allclose(mg_["evals"], ms_["evals"], atol=1e-9)
+    popg, chg = R.read_mulliken(g("scfout.gs.out"))
+    pops, chs = R.read_mulliken(s("scfout.gs.out"))
+    assert popg.shape == pops.shape
+    np.testing.assert_allclose(popg, pops, atol=1e-8)
+    np.testing.assert_allclose(chg, chs, atol=2e-9)
+    vg, vs = R.read_eigvecs(g("scfout.gs.out"), n), R.read_eigvecs(s("scfout.gs.out"), n)
+    assert len(vg) == len(vs) == (2 if uhf else 1)
+    keep_all = []
+    for (eg, cg), (es, cs) in zip(vg, vs):
+        assert cg.shape == cs.shape == (n, n)
+        np.testing.assert_allclose(eg, es, atol=1e-9)
+        keep = _nondegenerate(es)
+        keep_all.append(keep)
+        sign = np.sign(np.sum(cg * cs, axis=0))
+        assert np.abs(cg * sign - cs)[:, keep].max() < 1e-7
+    # overlaps.out: Mulliken overlap populations per packed pair (and spin)
+    og, os_ = R.read_overlaps(g("overlaps.out")), R.read_overlaps(s("overlaps.out"))
+    assert og.shape == os_.shape == (n2, 2 if uhf else 1)
+    assert np.abs(og - os_).max() < 1e-8
+    # mol_dipmom.out: MO transition dipoles (printed to 7 digits) up to the vectors' signs,
+    # for pairs of non-degenerate orbitals
+    dg, ds = R.read_mol_dipmom(g("mol_dipmom.out")), R.read_mol_dipmom(s("mol_dipmom.out"))
+    assert len(dg) == len(ds) == (2 if uhf else 1)
+    for bg, bs, keep in zip(dg, ds, keep_all):
+        assert bg.keys() == bs.keys()
+        for (i, j), v in bs.items():
+            if keep[i] and keep[j]:
+                np.testing.assert_allclose(np.abs(bg[(i, j)]), np.abs(v), rtol=2e-6, atol=2e-8)
+    # restart from the reference's converged density
+    say("restart from PMATRIX.DAT " + tag)
+    rs = os.path.join(R.GOLDEN, "ref_out", tag, "restart")
+    rd = tmp_path / "restart"
+    rd.mkdir()
+    (rd / "patin.dat").write_text(open(os.path.join(rs, "patin.dat")).read())
+    (rd / "PMATRIX.DAT").write_bytes(open(os.path.join(rs, "PMATRIX.in"), "rb").read())
+    r2 = unomol_amd.run_deck(rd)
+    assert r2.converged
+    a, b = R.read_short(str(rd / "short.gs.out")), R.read_short(os.path.join(rs, "short.gs.out"))
+    assert abs(a["init"] - b["init"]) < 1e-9
+    assert abs(a["energy"] - b["energy"]) < 1e-9
+    assert r2.iterations <= R.read_scfout(os.path.join(rs, "scfout.gs.out"))["iterations"] + 2
+
+
+def test_error_paths(tmp_path):
+    """Failures the reference turns into exit() come back as status codes with the
+    reference's message: f shells (l = 3) rejected by the device context (the ERI kernels
+    cover s, p, d), and a run that reaches maxits returns UNO_ENOCONV with its results and
+    writes the reference's banner (RHF.hpp:413) into scfout.gs.out."""
+    say("error paths")
+    deck = open(R.deck_path("631.h2o")).read().split("\n")
+    # an f shell on the oxygen: one primitive, l = 3
+    hdr = deck[0].split()
+    nsh, norb, ncen = int(hdr[0]), int(hdr[1]), int(hdr[2])
+    deck[0] = "%7d%7d%7d  3" % (nsh + 1, norb + 10, ncen)
+    fdeck = "\n".join(deck).rstrip("\n") + "\n  1  3  1\n  0.8000000000   1.0000000000\n"
+    b = unomol_amd.Basis(text=fdeck)
+    with pytest.raises(N.UnoError) as ei:
+        unomol_amd.TwoElectronInts(b)
+    assert ei.value.code == N.UNO_EINVAL
+    # non-convergence: two iterations on a deck that needs ~45
+    lines = open(R.deck_path("631.h2o")).read().split("\n")
+    lines[1] = "     10      2"
+    (tmp_path / "patin.dat").write_text("\n".join(lines))
+    r = unomol_amd.run_deck(tmp_path)
+    assert not r.converged and r.iterations == 2
+    text = (tmp_path / "scfout.gs.out").read_text()
+    assert text.startswith("WARNING CONVERGENCE _NOT_ REACHED!")
+    assert R.read_scfout(str(tmp_path / "scfout.gs.out"))["converged"] is False
+
+
+SHIM_EXE = os.path.join(R.GOLDEN, "..", "..", "oracle", "_ref", "Unomol_shim")
+
+
+@pytest.mark.skipif(not os.path.exists(SHIM_EXE), reason="the shim program is built where the reference is mounted")
+@pytest.mark.parametrize("tag", ["631.h2o", "water_631gs_cation"])
+def test_reference_drivers_on_shim_vs_reference(tmp_path, tag):
+    """The reference's own program (its unchanged Unomol.cc / RHF.hpp / UHF.hpp) built on
+    the drop-in shim, i.e. every formGmatrix / rsp / sp_trans call going through the C ABI
+    to the GPU, against the reference's outputs on the same deck: energies and orbital
+    energies to 1e-9 Eh."""
+    import subprocess
+    say("reference drivers on the shim " + tag)
+    (tmp_path / "patin.dat").write_text(open(R.deck_path(tag)).read())
+    subprocess.run([os.path.abspath(SHIM_EXE)], cwd=str(tmp_path), check=True, timeout=300,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    src = os.path.join(R.GOLDEN, "ref_run", tag)
+    a = R.read_short(str(tmp_path / "short.gs.out"))
+    b = R.read_short(os.path.join(src, "short.gs.out"))
+    assert abs(a["init"] - b["init"]) < 1e-9
+    assert abs(a["energy"] - b["energy"]) < 1e-9
+    ma, mb = R.read_scfout(str(tmp_path / "scfout.gs.out")), R.read_scfout(os.path.join(src, "scfout.gs.out"))
+    np.testing.assert_allclose(ma["evals"], mb["evals"], atol=1e-9)

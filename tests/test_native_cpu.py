@@ -277,3 +277,23 @@ def test_guess_descriptors_and_blocks_rotation_invariant(tmp_path):
     Xr, Yr, _ = unomol_amd.guess_samples(dr, Dr[np.tril_indices(n)])
     np.testing.assert_allclose(Xr, X, atol=2e-5)
     np.testing.assert_allclose(Yr, Y, atol=2e-5 * np.abs(Y).max())
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference"), reason="needs the reference sources")
+def test_reference_drivers_build_on_shim(tmp_path):
+    """The reference's unchanged drivers (Unomol.cc, RHF.hpp, UHF.hpp and every other
+    reference source but TwoElectronInts and SymmPack.cpp) compile and link against the
+    drop-in shim (unomol_amd/shim) and libunomol_hip.so; the program's undefined symbols
+    that the library provides are exactly C-ABI entry points declared in include/unomol.h."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "shim", "OUT=%s" % tmp_path], check=True,
+                   stdout=subprocess.DEVNULL)
+    exe = tmp_path / "Unomol_shim"
+    assert exe.exists()
+    und = subprocess.run(["nm", "-D", "--undefined-only", str(exe)], check=True, capture_output=True,
+                         text=True).stdout
+    used = set(re.findall(r"\b(uno_[a-z_0-9]+)\b", und))
+    hdr = open(os.path.join(ROOT, "include", "unomol.h")).read()
+    declared = set(re.findall(r"\b(uno_[a-z_0-9]+)\s*\(", hdr))
+    assert {"uno_ctx_create", "uno_fock_rhf", "uno_fock_uhf", "uno_eigh_packed", "uno_sp_trans"} <= used
+    assert used <= declared

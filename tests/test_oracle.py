@@ -19,13 +19,21 @@ def rysmp(golden):
     return np.load(os.path.join(golden, "rys_mp.npz"))
 
 
+@pytest.fixture
+def exact_rys2():
+    """The oracle's exact two-root quadrature for one test (the default is the reference's)."""
+    O.set_rys2_mode(False)
+    yield
+    O.set_rys2_mode(True)
+
+
 def test_oracle_boys_vs_mpmath(rysmp):
     for i, x in enumerate(rysmp["x"]):
         f = O.boys(11, x)
         np.testing.assert_allclose(f, rysmp["boys"][i], rtol=2e-15, atol=0)
 
 
-def test_oracle_rys_vs_mpmath(rysmp):
+def test_oracle_rys_vs_mpmath(rysmp, exact_rys2):
     for n in range(1, 6):
         for i, x in enumerate(rysmp["x"]):
             u, w = O.rys(n, x)
@@ -34,7 +42,7 @@ def test_oracle_rys_vs_mpmath(rysmp):
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7])
-def test_oracle_rys_moment_identity(n):
+def test_oracle_rys_moment_identity(n, exact_rys2):
     # sum_i w_i u_i^k = F_k(x) for k < 2n: fixes roots and weights completely
     rng = np.random.default_rng(n)
     for x in np.concatenate([rng.uniform(0, 80, 20), [0.0, 1e-6, 15.0, 33.0]]):
@@ -50,21 +58,91 @@ def test_oracle_boys_closed_form():
         assert abs(O.boys(0, x)[0] - f0) < 2e-16 * max(1.0, f0) * 4
 
 
+def test_oracle_rys2_reference_form(golden):
+    """The oracle's reference-compatible two-root quadrature (its six coefficients fitted
+    to the reference's outputs on 15 < x <= 22) against every committed sample of the
+    reference's root2 on (15, 40], to 1e-14 relative; the fit recovers a form whose
+    large-x limit is the 4-point Gauss-Hermite rule (checked by the exact rule at x = 40,
+    where the reference's e^-x terms are 4e-17)."""
+    z = np.load(O.RYS2_NPZ)
+    x, r, w = z["x"], z["r"], z["w"]
+    m = (x > 15.0) & (x <= 40.0)
+    worst = 0.0
+    for xi, ri, wi in zip(x[m], r[m], w[m]):
+        u, ww = O.rys(2, float(xi))
+        worst = max(worst, np.max(np.abs(u - ri / (1 + ri)) / (ri / (1 + ri))), np.max(np.abs(ww - wi) / wi))
+    assert worst < 1e-14
+    # at 16 the reference departs from the exact rule by ~1e-4: the mode matters
+    O.set_rys2_mode(False)
+    ue, we = O.rys(2, 16.0)
+    ue40, we40 = O.rys(2, 40.0)
+    O.set_rys2_mode(True)
+    ur, wr = O.rys(2, 16.0)
+    ur40, wr40 = O.rys(2, 40.0)
+    assert np.max(np.abs(ur - ue) / ue) > 1e-5
+    np.testing.assert_allclose(ur40, ue40, rtol=1e-14)
+    np.testing.assert_allclose(wr40, we40, rtol=1e-14)
+
+
+def _horner_table(path, name, k, nc):
+    import re
+    text = open(path).read()
+    body = re.search(r"%s\[(\d+)\] = \{(.*?)\};" % name, text, re.S).group(2)
+    vals = np.array([float(v.rstrip("f")) for v in body.replace("\n", "").split(",") if v.strip()])
+    return vals.reshape(-1, 4, nc)
+
+
+def test_product_rys2_table_vs_reference_samples(golden):
+    """The product's reference-compatible two-root table (unomol_amd/csrc/rys_ref2.inc),
+    evaluated here the way the kernels do (Horner in s = 2(x-k)-1 on unit intervals),
+    against the reference's own root2 samples: 2e-15 relative on (15, 40) in double,
+    2e-6 in float; outside, the exact table agrees with the reference's fit to 5e-13.
+    x = 15 itself is left out: there the reference takes its x <= 15 branch and the table
+    its [15, 16) interval (3.5e-4 apart at that one argument)."""
+    path = os.path.join(ROOT, "unomol_amd", "csrc", "rys_ref2.inc")
+    z = np.load(O.RYS2_NPZ)
+    x, r, w = z["x"], z["r"], z["w"]
+    m = (x < 48.0) & (x != 15.0)
+    x, r, w = x[m], r[m], w[m]
+    ref = np.concatenate([r / (1 + r), w], axis=1)
+    for name, nc, tol_in in (("uno_rys_ref2_mono", 13, 2e-15), ("uno_rys_ref2_mono_f", 7, 2e-6)):
+        tab = _horner_table(path, name, 0, nc)
+        assert tab.shape[0] == 48
+        k = np.floor(x).astype(int)
+        s = 2 * (x - k) - 1
+        got = np.zeros_like(ref)
+        for f in range(4):
+            c = tab[k, f, :]
+            v = c[:, nc - 1].copy()
+            for j in range(nc - 2, -1, -1):
+                v = v * s + c[:, j]
+            got[:, f] = v
+        rel = np.abs(got - ref) / np.abs(ref)
+        inside = (x > 15.0) & (x < 40.0)
+        assert rel[inside].max() < tol_in, name
+        assert rel[~inside].max() < max(5e-13, tol_in), name
+
+
 @pytest.mark.parametrize("tag", ["A_water_631g", "water_631gs"])
 def test_oracle_eri_vs_reference(golden, tag):
+    """Every canonical ERI against the reference's own ERI code at 1e-12 in every class;
+    with the exact two-root rule, against the reference with its fit replaced by it."""
     z = np.load(os.path.join(golden, "eri_%s.npz" % tag))
     d = read_deck(os.path.join(golden, "decks", tag + ".dat"))
     v, f = O.eri_all(d)
     assert (f == z["ijkl"]).all(), "canonical order differs from the reference"
-    # the reference's ERI code with exact 2-root quadrature: parity at 1e-12
-    assert np.abs(v - z["ref_x2"]).max() < 1e-12
-    # plain reference: its 2-root fit (Rys.cpp:615-624) is off by <= 3.5e-4 relative for
-    # 15 < x < 31, which moves ERIs of the 2-root classes by up to ~1e-7; all others 1e-12
-    err = np.abs(v - z["ref"])
+    assert np.abs(v - z["ref"]).max() < 1e-12
+    O.set_rys2_mode(False)
+    try:
+        v2, f2 = O.eri_all(d)
+    finally:
+        O.set_rys2_mode(True)
+    assert np.abs(v2 - z["ref_x2"]).max() < 1e-12
+    # the reference's two-root fit moves the ERIs of the 2-root classes by up to ~1e-9
     L = _fun_l(d)[f].sum(axis=1)
     two = (L == 2) | (L == 3)
-    assert err[~two].max() < 1e-12
-    assert err[two].max() < 1e-6
+    assert np.abs(v2 - v)[~two].max() == 0.0
+    assert np.abs(v2 - v)[two].max() > 1e-12
 
 
 def _fun_l(d):
@@ -169,22 +247,24 @@ SCF_TAGS = [t for t in R.RHF_TAGS if t not in R.DEGENERATE_TAGS] + R.UHF_TAGS + 
 
 @pytest.mark.parametrize("tag", SCF_TAGS)
 def test_oracle_scf_vs_reference_run(tag):
+    """The oracle's SCF against the reference rebuilt from its sources: initial and final
+    energies and every orbital energy to 1e-9 Eh, moments to 1e-7; and the reference's
+    published goldens of its own decks (test/short.dat.*) to 1e-9 Eh."""
     d = read_deck(R.deck_path(tag))
     out, ev, P, mom = O.scf(d)
-    # reference with exact 2-root quadrature: total and orbital energies at 1e-9 Eh
-    ref = R.read_scfout(os.path.join(R.GOLDEN, "ref_run_x2", tag, "scfout.gs.out"))
-    short = R.read_short(os.path.join(R.GOLDEN, "ref_run_x2", tag, "short.gs.out"))
+    ref = R.read_scfout(os.path.join(R.GOLDEN, "ref_run", tag, "scfout.gs.out"))
+    short = R.read_short(os.path.join(R.GOLDEN, "ref_run", tag, "short.gs.out"))
     assert abs(out.energy - short["energy"]) < 1e-9
     assert abs(out.init_energy - short["init"]) < 1e-9
     n = d["norb"]
     np.testing.assert_allclose(ev[:n], ref["evals"], atol=1e-9)
     if out.uhf:
         np.testing.assert_allclose(ev[n:], ref["evals_b"], atol=1e-9)
-    # plain reference: its 2-root fit (Rys.cpp:615-624) moves its energies by up to ~1e-6
-    # Eh; the deviation from it must be no more than that fit's own effect
-    plain = R.read_short(os.path.join(R.GOLDEN, "ref_run", tag, "short.gs.out"))
-    assert abs(out.energy - plain["energy"]) <= abs(short["energy"] - plain["energy"]) + 1e-9
-    mref = R.read_moments(os.path.join(R.GOLDEN, "ref_run_x2", tag, "moments.out"))
+    pub = os.path.join(R.GOLDEN, "ref_test", "short.dat." + tag)
+    if os.path.exists(pub):
+        p = R.read_short(pub)
+        assert abs(out.energy - p["energy"]) < 1e-9
+    mref = R.read_moments(os.path.join(R.GOLDEN, "ref_run", tag, "moments.out"))
     R.assert_moments(tag, mom, mref)
 
 
@@ -204,7 +284,7 @@ def test_oracle_scf_degenerate_start(tag):
     is required here; the reference's ground-state energy is a lower bound."""
     d = read_deck(R.deck_path(tag))
     out, ev, P, mom = O.scf(d)
-    short = R.read_short(os.path.join(R.GOLDEN, "ref_run_x2", tag, "short.gs.out"))
+    short = R.read_short(os.path.join(R.GOLDEN, "ref_run", tag, "short.gs.out"))
     assert out.iterations < d["maxits"]
     assert out.energy > short["energy"] - 1e-9
 

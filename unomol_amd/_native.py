@@ -32,7 +32,7 @@ class ScfOpts(C.Structure):
     _fields_ = [("maxits", C.c_int), ("eps", C.c_double), ("cflag", C.c_int), ("accel", C.c_int),
                 ("guess", C.c_int), ("precision", C.c_int), ("fp64_final", C.c_int),
                 ("incremental", C.c_int), ("verbose", C.c_int), ("field", C.c_double * 3),
-                ("single_start", C.c_int)]
+                ("single_start", C.c_int), ("guess_check", C.c_int)]
 
 
 class ScfResult(C.Structure):
@@ -42,7 +42,8 @@ class ScfResult(C.Structure):
                 ("uhf", C.c_int), ("nfock", C.c_int), ("wall_s", C.c_double), ("fock_s", C.c_double),
                 ("diag_s", C.c_double), ("quartets", C.c_double), ("quartets32", C.c_double),
                 ("nfull", C.c_int), ("guess_s", C.c_double), ("wall99_s", C.c_double),
-                ("setup_s", C.c_double), ("diis_s", C.c_double)]
+                ("setup_s", C.c_double), ("diis_s", C.c_double), ("guess_fallback", C.c_int),
+                ("guess_energy", C.c_double), ("guess_energy_coreh", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -53,12 +54,12 @@ _lib = None
 # every symbol include/unomol.h declares
 EXPORTS = ["uno_last_error", "uno_ctx_create", "uno_ctx_rebuild", "uno_ctx_destroy", "uno_ctx_set",
            "uno_ctx_get", "uno_ctx_norb", "uno_fock_rhf", "uno_fock_uhf", "uno_eigh_packed",
-           "uno_sp_trans", "uno_scf_defaults", "uno_scf_run", "uno_eri_emit", "uno_rys_eval", "uno_rys_eval_f32",
+           "uno_sp_trans", "uno_scf_defaults", "uno_scf_run", "uno_eri_emit", "uno_rys_eval", "uno_rys_eval_f32", "uno_rys_eval_ref2",
            "uno_schwarz", "uno_profile", "uno_copy_probe", "uno_oneint", "uno_oneint_device", "uno_moment_ints", "uno_deck_generate", "uno_run_deck",
            "uno_mlp_create", "uno_mlp_destroy", "uno_mlp_nparams", "uno_mlp_dims", "uno_mlp_get_params",
            "uno_mlp_set_params", "uno_mlp_forward", "uno_mlp_loss_grad", "uno_mlp_train", "uno_mlp_save",
            "uno_mlp_load", "uno_guess_features", "uno_guess_samples", "uno_guess_assemble", "uno_ctx_set_guess",
-           "uno_shell_rotation", "uno_autotune", "uno_batch_create", "uno_batch_destroy", "uno_batch_nmol", "uno_batch_run"]
+           "uno_shell_rotation", "uno_autotune", "uno_batch_create", "uno_batch_destroy", "uno_batch_nmol", "uno_batch_run", "uno_batch_density"]
 
 
 def lib():

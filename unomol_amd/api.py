@@ -215,6 +215,15 @@ def rys_eval(n, x, fp32=False):
     return u, w
 
 
+def rys_eval_ref2(x, fp32=False):
+    """Two roots from the reference-compatible table the ERI kernels use by default."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    u = np.zeros((len(x), 2))
+    w = np.zeros((len(x), 2))
+    N.check(N.lib().uno_rys_eval_ref2(1 if fp32 else 0, len(x), N.ptr(x), N.ptr(u), N.ptr(w)))
+    return u, w
+
+
 def oneint(deck):
     b = N.basis_struct(deck)
     n = deck["norb"]
@@ -369,3 +378,12 @@ class Batch:
         e = np.zeros(len(self.decks))
         N.check(N.lib().uno_batch_run(self.h, C.byref(o), res, N.ptr(e)), allow=(N.UNO_ENOCONV,))
         return e, list(res)
+
+    def density(self, m):
+        """Final density of molecule m: dense (N, N) RHF half density, or (2, N, N) alpha/beta."""
+        d = self.decks[m]
+        n = d["norb"]
+        ns = 2 if d["nelec"] % 2 else 1
+        P = np.zeros(ns * n * n)
+        N.check(N.lib().uno_batch_density(self.h, m, N.ptr(P)))
+        return P.reshape(ns, n, n) if ns == 2 else P.reshape(n, n)

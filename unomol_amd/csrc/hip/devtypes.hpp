@@ -94,6 +94,8 @@ struct EriArgs {
     const int* e_mol;
     const MolDesc* mols;
     int qpb;                  // quartets per block of the launch
+    int rys2_exact;           // 1: exact two-root Rys quadrature; 0: the reference's
+                              // two-root quadrature (Rys.cpp:413-631), the default
     int nbra;
     long long nquart;
     const int2* explicit_q;   // explicit (bra, ket) list (block / Schwarz modes) or null

@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void k_eri_lane(const EriArgs a) {
     const PairHdr P1 = a.pairs[pab];
     const PairHdr P2 = a.pairs[pcd];
     T e[NCOMP];
-    const unsigned nprimq = eri_block<LA, LB, LC, LD, T>(a, P1, P2, e, rys_table_global<CI::NR, T>());
+    const unsigned nprimq = eri_block<LA, LB, LC, LD, T>(a, P1, P2, e, rys_table_of<CI::NR, T>(a));
     if (a.counter) {
         unsigned long long* slot = a.counter + 2 * (blockIdx.x & (kCounterSlots - 1));
         atomicAdd(slot, (unsigned long long)nprimq);
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
     constexpr int NR = CI::NR;
     constexpr bool kLdsTab = NR <= kLdsTabMaxRoots;
     __shared__ T tab_lds[kLdsTab ? rys_table_size<NR, T>() : 1];
-    const T* tab = rys_table_global<NR, T>();
+    const T* tab = rys_table_of<NR, T>(a);
     if constexpr (kLdsTab) {
         for (int i = threadIdx.x; i < rys_table_size<NR, T>(); i += blockDim.x) tab_lds[i] = tab[i];
         __syncthreads();
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(kRowThreads) void k_eri_rows(const EriArgs a) {
             dlr[i] = DK[(long)grow * n + a.colinv[col]];
         }
     }
-    const T* tab = rys_table_global<NR, T>();
+    const T* tab = rys_table_of<NR, T>(a);
     if constexpr (TL) {
         for (int i = threadIdx.x; i < rys_table_size<NR, T>(); i += blockDim.x) tab_lds[i] = tab[i];
         tab = tab_lds;
@@ -673,7 +673,7 @@ __device__ __forceinline__ unsigned wave_one(const EriArgs& a, const long long g
             const T x = p * q * it * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
             if (lane == 0) {
                 T u[NR], w[NR];
-                rys_roots<NR, T>(x, u, w);
+                rys_roots<NR, T>(x, u, w, rys_table_of<NR, T>(a));
                 for (int r = 0; r < NR; ++r) { sUW[r] = u[r]; sUW[NR + r] = w[r] * pref; }
             }
             __syncthreads();
@@ -1085,30 +1085,33 @@ hipError_t jp_scatter(int npairs, const PairHdr* pairs, long long njp, const dou
 // Rys-table evaluation for the parity tests
 // ---------------------------------------------------------------------------
 template <int NR, typename T>
-__global__ void k_rys_eval(int npts, const double* x, double* u, double* w) {
+__global__ void k_rys_eval(int npts, const double* x, double* u, double* w, int ref2) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npts) return;
     T uu[NR], ww[NR];
-    rys_roots<NR, T>((T)x[i], uu, ww);
+    EriArgs a{};
+    a.rys2_exact = !ref2;
+    rys_roots<NR, T>((T)x[i], uu, ww, rys_table_of<NR, T>(a));
     for (int r = 0; r < NR; ++r) { u[i * NR + r] = (double)uu[r]; w[i * NR + r] = (double)ww[r]; }
 }
 
 template <typename T>
-static hipError_t rys_eval_t(int n, int npts, const double* x, double* u, double* w, hipStream_t st) {
+static hipError_t rys_eval_t(int n, int npts, const double* x, double* u, double* w, hipStream_t st, int ref2) {
     const dim3 g((npts + 255) / 256), b(256);
     switch (n) {
-        case 1: hipLaunchKernelGGL((k_rys_eval<1, T>), g, b, 0, st, npts, x, u, w); break;
-        case 2: hipLaunchKernelGGL((k_rys_eval<2, T>), g, b, 0, st, npts, x, u, w); break;
-        case 3: hipLaunchKernelGGL((k_rys_eval<3, T>), g, b, 0, st, npts, x, u, w); break;
-        case 4: hipLaunchKernelGGL((k_rys_eval<4, T>), g, b, 0, st, npts, x, u, w); break;
-        case 5: hipLaunchKernelGGL((k_rys_eval<5, T>), g, b, 0, st, npts, x, u, w); break;
+        case 1: hipLaunchKernelGGL((k_rys_eval<1, T>), g, b, 0, st, npts, x, u, w, ref2); break;
+        case 2: hipLaunchKernelGGL((k_rys_eval<2, T>), g, b, 0, st, npts, x, u, w, ref2); break;
+        case 3: hipLaunchKernelGGL((k_rys_eval<3, T>), g, b, 0, st, npts, x, u, w, ref2); break;
+        case 4: hipLaunchKernelGGL((k_rys_eval<4, T>), g, b, 0, st, npts, x, u, w, ref2); break;
+        case 5: hipLaunchKernelGGL((k_rys_eval<5, T>), g, b, 0, st, npts, x, u, w, ref2); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t rys_eval_launch(int n, int npts, const double* x, double* u, double* w, hipStream_t st, bool fp32) {
-    return fp32 ? rys_eval_t<float>(n, npts, x, u, w, st) : rys_eval_t<double>(n, npts, x, u, w, st);
+hipError_t rys_eval_launch(int n, int npts, const double* x, double* u, double* w, hipStream_t st, bool fp32,
+                           bool ref2) {
+    return fp32 ? rys_eval_t<float>(n, npts, x, u, w, st, ref2) : rys_eval_t<double>(n, npts, x, u, w, st, ref2);
 }
 
 }  // namespace uno

@@ -7,7 +7,10 @@
 // whose bare prefactor 2 pi^(5/2) e_ab e_cd / (p q sqrt(p+q)) is below primcut, as the
 // reference does with 1e-12 (TwoElectronInts.cpp:477-479).  The Rys roots/weights come
 // from this project's own Chebyshev tables (rys_table.inc, gen_rys_table.py), evaluated in
-// their power-basis re-expansion (rys_mono.inc, gen_rys_monomial.py).
+// their power-basis re-expansion (rys_mono.inc, gen_rys_monomial.py).  Two roots use the
+// reference-compatible table by default (rys_ref2.inc, gen_rys_ref2.py): the reference's
+// own two-root quadrature, which departs from the exact one for 15 < x <= 33
+// (Rys.cpp:615-624); EriArgs::rys2_exact selects the exact table instead.
 //
 // Two execution shapes:
 //   * one lane per shell quartet (classes with <= 162 components): VRR, transfer and the
@@ -28,6 +31,7 @@ namespace uno {
 #include "../rys_table.inc"
 #include "../rys_table_f32.inc"
 #include "../rys_mono.inc"
+#include "../rys_ref2.inc"
 #undef UNO_RYS_QUAL
 
 __host__ __device__ constexpr int ncart_c(int l) { return (l + 1) * (l + 2) / 2; }
@@ -124,6 +128,18 @@ __device__ __forceinline__ void rys_roots(T x, T* u, T* w, const T* tab) {
 template <int NR, typename T>
 __device__ __forceinline__ void rys_roots(T x, T* u, T* w) {
     rys_roots<NR, T>(x, u, w, rys_table_global<NR, T>());
+}
+// the table of NR roots a launch evaluates: the exact tables, except for two roots, where
+// the reference-compatible table is the default (same layout and size)
+template <int NR, typename T>
+__device__ __forceinline__ const T* rys_table_of(const EriArgs& a) {
+    if constexpr (NR == 2) {
+        if (!a.rys2_exact) {
+            if constexpr (sizeof(T) == 4) return uno_rys_ref2_mono_f;
+            else return uno_rys_ref2_mono;
+        }
+    }
+    return rys_table_global<NR, T>();
 }
 
 // One axis of one root: Rys 2-D recurrence G[e][f] (e <= LA+LB, f <= LC+LD) then the

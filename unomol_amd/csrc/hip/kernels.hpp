@@ -29,7 +29,8 @@ bool eri_rows_dlds_fits(int bra_class, int ket_class, int mode, int ncols, bool 
 // are disjoint, no atomics); the block size of a pair follows from the next pair's jo
 hipError_t jp_scatter(int npairs, const PairHdr* pairs, long long njp, const double* jp, double* W0, double* W1, int n,
                       hipStream_t st);
-hipError_t rys_eval_launch(int n, int npts, const double* x, double* u, double* w, hipStream_t st, bool fp32 = false);
+hipError_t rys_eval_launch(int n, int npts, const double* x, double* u, double* w, hipStream_t st, bool fp32 = false,
+                           bool ref2 = false);
 
 // dense helpers (linalg.hip); matrices are row-major n x n
 hipError_t k_unpack(int n, const double* packed, double* dense, hipStream_t st);       // symmetric fill

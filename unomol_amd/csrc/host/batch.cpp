@@ -234,6 +234,7 @@ static int batch_fock(uno_batch* b, const std::vector<double>& dmax, double& qua
         a.pairQ = b->pairQ.p;
         a.tau = b->mol[0].ctx->tau;
         a.primcut = b->mol[0].ctx->primcut;
+        a.rys2_exact = b->mol[0].ctx->rys2_exact;
         a.prefix = dprefix;
         a.blk = tasks ? nullptr : dblk;
         a.wpre = tasks ? dwpre : nullptr;
@@ -357,6 +358,17 @@ int uno_batch_destroy(uno_batch* b) {
 }
 
 int uno_batch_nmol(const uno_batch* b) { return b ? (int)b->mol.size() : 0; }
+
+int uno_batch_density(const uno_batch* b, int m, double* P) {
+    if (!b || !P || m < 0 || m >= (int)b->mol.size()) { set_error("bad argument"); return UNO_EINVAL; }
+    const BatchMol& x = b->mol[m];
+    const size_t nn = (size_t)x.n * x.n;
+    if (!b->P[0].p) { set_error("the batch has not run"); return UNO_EINVAL; }
+    UNO_HIP(hipMemcpyAsync(P, b->P[0].p + x.moff, nn * sizeof(double), hipMemcpyDeviceToHost, b->st));
+    if (x.uhf) UNO_HIP(hipMemcpyAsync(P + nn, b->P[1].p + x.moff, nn * sizeof(double), hipMemcpyDeviceToHost, b->st));
+    UNO_HIP(hipStreamSynchronize(b->st));
+    return UNO_OK;
+}
 
 int uno_batch_run(uno_batch* b, const uno_scf_opts* opts, uno_scf_result* res, double* energies) {
     if (!b || !res) { set_error("null argument"); return UNO_EINVAL; }

@@ -163,6 +163,22 @@ int write_outputs(const std::string& dir, const Deck& d, const Basis& B, const u
     }
     std::fprintf(f, "%sMulliken Atomic Charges \nOrbital   Nuclear Charge   Net Charge \n", kRule);
     for (int i = 0; i < d.ncen; ++i) std::fprintf(f, "%7u %15.10lf %15.10lf \n", i + 1, d.charge[i], net[i]);
+    if (d.prt_flag[2]) {
+        // Mulliken overlap populations (RHF.hpp:647-657, UHF.hpp:742-753)
+        FILE* fo = create(dir, "overlaps.out");
+        if (!fo) { std::fclose(f); set_error("could not create overlaps.out"); return UNO_EIO; }
+        std::fprintf(fo, "   MULLIKEN OVERLAP POPULATIONS\n");
+        std::fprintf(fo, uhf ? "  i      j        Alpha value                 Beta Value\n" : "  i      j                   value \n");
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j <= i; ++j) {
+                if (uhf)
+                    std::fprintf(fo, "%5d %5d  %24.16le  %24.16le\n", i, j, ovl[(size_t)i * n + j],
+                                 ovl[((size_t)n + i) * n + j]);
+                else
+                    std::fprintf(fo, "%5d %5d  %24.16le\n", i, j, ovl[(size_t)i * n + j]);
+            }
+        std::fclose(fo);
+    }
     if (d.prt_flag[0]) {
         for (int s = 0; s < ns; ++s) {
             std::fprintf(f, "xxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxxx\n");
@@ -289,6 +305,7 @@ int uno_ctx_set(uno_ctx* c, const char* key, double v) {
     else if (k == "row_min_kets") c->row_min_kets = (long long)v;
     else if (k == "row_variant") c->row_variant = (int)v;
     else if (k == "row_density_lds") c->row_dens = (int)v;
+    else if (k == "rys2_exact") c->rys2_exact = (int)v;
     else { set_error("unknown option " + k); return UNO_EINVAL; }
     return UNO_OK;
 }
@@ -306,6 +323,7 @@ int uno_ctx_get(uno_ctx* c, const char* key, double* v) {
     else if (k == "row_min_kets") *v = (double)c->row_min_kets;
     else if (k == "row_variant") *v = c->row_variant;
     else if (k == "row_density_lds") *v = c->row_dens;
+    else if (k == "rys2_exact") *v = c->rys2_exact;
     else if (k == "last_quartets") *v = c->last_quartets;
     else if (k == "last_quartets32") *v = c->last_quartets32;
     else if (k == "pair_runs") *v = (double)c->cls.size();
@@ -434,23 +452,24 @@ int uno_eri_emit(uno_ctx* c, long cap, double* val, int* ijkl, long* count) {
     return eri_emit(c, cap, val, ijkl, count);
 }
 
-static int rys_eval_on(hipStream_t st, int n, int npts, const double* x, double* u, double* w, bool fp32) {
+static int rys_eval_on(hipStream_t st, int n, int npts, const double* x, double* u, double* w, bool fp32,
+                       bool ref2) {
     DBuf<double> dx, du, dw;
     UNO_HIP(dx.upload(x, npts, st));
     UNO_HIP(du.alloc((size_t)npts * n));
     UNO_HIP(dw.alloc((size_t)npts * n));
-    UNO_HIP(rys_eval_launch(n, npts, dx.p, du.p, dw.p, st, fp32));
+    UNO_HIP(rys_eval_launch(n, npts, dx.p, du.p, dw.p, st, fp32, ref2));
     UNO_HIP(hipMemcpyAsync(u, du.p, (size_t)npts * n * sizeof(double), hipMemcpyDeviceToHost, st));
     UNO_HIP(hipMemcpyAsync(w, dw.p, (size_t)npts * n * sizeof(double), hipMemcpyDeviceToHost, st));
     UNO_HIP(hipStreamSynchronize(st));
     return UNO_OK;
 }
 
-static int rys_eval_impl(int n, int npts, const double* x, double* u, double* w, bool fp32) {
+static int rys_eval_impl(int n, int npts, const double* x, double* u, double* w, bool fp32, bool ref2 = false) {
     if (n < 1 || n > 5 || npts <= 0 || !x || !u || !w) { set_error("bad argument"); return UNO_EINVAL; }
     hipStream_t st;
     UNO_HIP(hipStreamCreate(&st));
-    const int rc = rys_eval_on(st, n, npts, x, u, w, fp32);
+    const int rc = rys_eval_on(st, n, npts, x, u, w, fp32, ref2);
     (void)hipStreamDestroy(st);
     return rc;
 }
@@ -461,6 +480,10 @@ int uno_rys_eval(int n, int npts, const double* x, double* u, double* w) {
 
 int uno_rys_eval_f32(int n, int npts, const double* x, double* u, double* w) {
     return rys_eval_impl(n, npts, x, u, w, true);
+}
+
+int uno_rys_eval_ref2(int fp32, int npts, const double* x, double* u, double* w) {
+    return rys_eval_impl(2, npts, x, u, w, fp32 != 0, true);
 }
 
 int uno_schwarz(uno_ctx* c, double* q) {

@@ -165,6 +165,7 @@ EriArgs base_args(uno_ctx* c) {
     a.primcut = c->primcut;
     a.n = c->n;
     a.nsh = c->nsh;
+    a.rys2_exact = c->rys2_exact;
     return a;
 }
 

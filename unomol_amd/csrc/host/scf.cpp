@@ -155,10 +155,57 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
         return UNO_OK;
     };
 
+    // core-Hamiltonian guess (RHF.hpp:205-211; UHF.hpp:206-215: beta reuses alpha's orbitals)
+    auto core_guess = [&]() -> int {
+        if (int rc = diag_to_P(b.H.p, 0, nocc[0])) return rc;
+        if (uhf) {
+            UNO_HIP(gemm_f64(false, true, n, n, nocc[1], 1.0, b.C[0].p, n, b.C[0].p, n, 0.0, P[1], n, st));
+            std::memcpy(evals.data() + n, evals.data(), n * sizeof(double));
+        }
+        return UNO_OK;
+    };
+    // iteration-0 electronic energy of the densities in P (one full Fock build in the
+    // precision of the first SCF build), without touching the SCF's state
+    auto energy_of = [&](double& e) -> int {
+        double* Din[2] = {uhf ? c->D1.p : c->D0.p, c->D2.p};
+        for (int s = 0; s < ns; ++s) UNO_HIP(hipMemcpyAsync(Din[s], P[s], nn * sizeof(double), hipMemcpyDeviceToDevice, st));
+        if (uhf) UNO_HIP(k_axpby(nn, 1.0, c->D1.p, 1.0, c->D2.p, c->D0.p, st));
+        const bool mixed = o.precision == 1;
+        const double tau0 = c->tau;
+        if (mixed) c->tau = std::max(tau0, c->tau_mixed);
+        const int rc = fock_build(c, uhf, -1.0, mixed);
+        c->tau = tau0;
+        if (rc) return rc;
+        double h[2];
+        UNO_HIP(k_fill(2, 0.0, c->scal.p, st));
+        for (int s = 0; s < ns; ++s) {
+            UNO_HIP(k_sym_w(n, s == 0 ? c->W0.p : c->W1.p, b.G[s].p, st, false));
+            UNO_HIP(k_dot(nn, P[s], b.H.p, c->scal.p, c->red.p, st));
+            UNO_HIP(k_dot(nn, P[s], b.G[s].p, c->scal.p + 1, c->red.p, st));
+        }
+        UNO_HIP(hipMemcpyAsync(h, c->scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        UNO_HIP(hipStreamSynchronize(st));
+        e = uhf ? h[0] + 0.5 * h[1] : 2.0 * h[0] + h[1];
+        return UNO_OK;
+    };
+    // tr(P S) of spin s
+    auto trace_ps = [&](int s, double& t) -> int {
+        UNO_HIP(k_fill(1, 0.0, c->scal.p + 3, st));
+        UNO_HIP(k_dot(nn, P[s], b.S.p, c->scal.p + 3, c->red.p, st));
+        UNO_HIP(hipMemcpyAsync(&t, c->scal.p + 3, sizeof(double), hipMemcpyDeviceToHost, st));
+        UNO_HIP(hipStreamSynchronize(st));
+        return UNO_OK;
+    };
+
     // guess
     if (o.guess == 2) {
         // learned: pair descriptors (host) -> network (GPU) -> blocks assembled in the lab
-        // frame; UHF spins get the closed-shell density scaled to their occupations
+        // frame, symmetrised; each spin scaled to its electron count (tr(P S) = nocc); the
+        // closed-shell density then McWeeny-purified, P <- 3 PSP - 2 PSPSP, twice, and
+        // scaled again.  A prediction whose electron count is off by more than 10 %, or
+        // whose purification moves away from idempotency, or that is not finite, is
+        // replaced by the core-Hamiltonian guess; with guess_check the core-H guess is
+        // formed as well and the lower iteration-0 energy wins.
         const auto t0 = clk::now();
         std::vector<float> X, Y;
         std::vector<int> pairs;
@@ -170,14 +217,56 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
             if (int rc = uno_mlp_forward(c->guess_model, np, X.data(), Y.data())) return rc;
         std::vector<double> D;
         if (int rc = guess_assemble(c->B, pairs, Y.data(), D, err)) { set_error(err); return rc; }
-        for (int s = 0; s < ns; ++s) {
-            const double f = uhf ? (double)nocc[s] / (0.5 * nelec) : 1.0;
-            if (f != 1.0)
-                for (double& v : D) v *= f;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < i; ++j) {
+                const double v = 0.5 * (D[(size_t)i * n + j] + D[(size_t)j * n + i]);
+                D[(size_t)i * n + j] = D[(size_t)j * n + i] = v;
+            }
+        bool ok = true;
+        for (double v : D) ok = ok && std::isfinite(v);
+        for (int s = 0; s < ns && ok; ++s) {
             UNO_HIP(hipMemcpyAsync(P[s], D.data(), nn * sizeof(double), hipMemcpyHostToDevice, st));
-            UNO_HIP(hipStreamSynchronize(st));
-            if (f != 1.0)
-                for (double& v : D) v /= f;
+            double t = 0.0;
+            if (int rc = trace_ps(s, t)) return rc;
+            const double want = nocc[s];
+            if (!(std::fabs(t - 0.5 * nelec) <= 0.1 * 0.5 * nelec) || !(t > 0.0)) { ok = false; break; }
+            UNO_HIP(k_axpby(nn, want / t, P[s], 0.0, P[s], P[s], st));
+            if (!uhf) {
+                double idem[2] = {0.0, 0.0};
+                for (int it = 0; it < 2; ++it) {
+                    UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, b.S.p, n, P[s], n, 0.0, b.Tm.p, n, st));     // S P
+                    UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, P[s], n, b.Tm.p, n, 0.0, b.Fp.p, n, st));    // P S P
+                    UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, b.Fp.p, n, b.Tm.p, n, 0.0, b.Cw.p, n, st));  // P S P S P
+                    UNO_HIP(k_axpby(nn, 1.0, b.Fp.p, -1.0, P[s], b.Tm.p, st));                              // PSP - P
+                    UNO_HIP(k_absmax(nn, b.Tm.p, c->scal.p + 3, st));
+                    UNO_HIP(hipMemcpyAsync(&idem[it], c->scal.p + 3, sizeof(double), hipMemcpyDeviceToHost, st));
+                    UNO_HIP(k_axpby(nn, 3.0, b.Fp.p, -2.0, b.Cw.p, P[s], st));
+                }
+                UNO_HIP(hipStreamSynchronize(st));
+                if (!(idem[1] <= idem[0]) || !(idem[1] < 0.25)) ok = false;
+                if (ok) {
+                    if (int rc = trace_ps(s, t)) return rc;
+                    if (!(t > 0.0)) ok = false;
+                    else UNO_HIP(k_axpby(nn, want / t, P[s], 0.0, P[s], P[s], st));
+                }
+            }
+        }
+        if (ok && o.guess_check) {
+            double el = 0.0, ec = 0.0;
+            if (int rc = energy_of(el)) return rc;
+            for (int s = 0; s < ns; ++s)
+                UNO_HIP(hipMemcpyAsync(b.Pold[s].p, P[s], nn * sizeof(double), hipMemcpyDeviceToDevice, st));
+            if (int rc = core_guess()) return rc;
+            if (int rc = energy_of(ec)) return rc;
+            r.guess_energy = el + nuc;
+            r.guess_energy_coreh = ec + nuc;
+            if (el <= ec)
+                for (int s = 0; s < ns; ++s)
+                    UNO_HIP(hipMemcpyAsync(P[s], b.Pold[s].p, nn * sizeof(double), hipMemcpyDeviceToDevice, st));
+            else r.guess_fallback = 1;
+        } else if (!ok) {
+            r.guess_fallback = 1;
+            if (int rc = core_guess()) return rc;
         }
         r.guess_s = secs(t0, clk::now());
     } else if (o.guess == 1 && P_io) {
@@ -186,12 +275,7 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
             UNO_HIP(k_unpack(n, c->pk.p, P[s], st));
         }
     } else {
-        // core Hamiltonian (RHF.hpp:205-211; UHF.hpp:206-215: beta reuses alpha's orbitals)
-        if (int rc = diag_to_P(b.H.p, 0, nocc[0])) return rc;
-        if (uhf) {
-            UNO_HIP(gemm_f64(false, true, n, n, nocc[1], 1.0, b.C[0].p, n, b.C[0].p, n, 0.0, P[1], n, st));
-            std::memcpy(evals.data() + n, evals.data(), n * sizeof(double));
-        }
+        if (int rc = core_guess()) return rc;
     }
 
     // DIIS state
