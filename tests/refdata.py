@@ -155,7 +155,10 @@ def read_mulliken(path):
 
 def read_eigvecs(path, n):
     """Printed eigenvector blocks of scfout (prt_flag[0]): list per spin of (energies [n],
-    coefficients [n x n], column i = vector i)."""
+    coefficients [n x n], column i = vector i).  Only the first n values of a column are
+    its coefficients: when n is not a multiple of 4 the reference's print loop
+    (RHF.hpp:439-451) runs on past the column's last row and appends 4 - n % 4 + n % 4 values
+    read beyond it."""
     out, cur, vals = [], None, []
     for ln in open(path):
         if "Eigenvectors" in ln:
@@ -167,7 +170,7 @@ def read_eigvecs(path, n):
             cur["c"].append(vals)
         elif cur is not None and ln.strip() and not ln.startswith("xxxx"):
             vals.extend(float(v) for v in ln.split())
-    return [(np.array(b["e"]), np.array(b["c"]).T) for b in out]
+    return [(np.array(b["e"]), np.array([c[:n] for c in b["c"]]).T) for b in out]
 
 
 def read_mol_dipmom(path):

