@@ -141,7 +141,10 @@ def make_dpm():
 # byte/format checks: the deck with eigenvector and Mulliken-overlap printing switched on
 # (prt_flag[0] = prt_flag[2] = 1) and a restart from the reference's own PMATRIX.DAT
 # (scf_flag[2] = 1, RHF.hpp:120-123)
-OUT_TAGS = ["631.h2o", "431.co", "water_631gs_cation", "b.dhdz"]
+# (the open-shell atoms b.dhdz / f.dhdz are left out: their unpaired electron sits along
+# an arbitrary axis of a degenerate p level, so P, the vectors and the overlaps are fixed
+# only up to a rotation of the atom)
+OUT_TAGS = ["631.h2o", "431.co", "water_631gs_cation"]
 OUT_FILES = ("short.gs.out", "scfout.gs.out", "moments.out", "intsout.out", "mol_dipmom.out", "overlaps.out",
              "PMATRIX.DAT", "MOMINTS.DAT", "RMOM.DAT")
 

@@ -72,12 +72,15 @@ def test_config_d_batched_pool_512(tmp_path):
     final density holds its electron count (tr(PS) = N, both spins), and for molecules of up
     to 20 functions spread over the pool the oracle (the reference's energy expression and
     brute-force Fock build on the CPU) evaluated at the batched density gives the batched
-    energy to 1e-8 Eh and a density that commutes with its Fock matrix (SCF stationarity)."""
+    energy to 1e-8 Eh and a density that commutes with its Fock matrix (SCF stationarity).
+    The run converges energy and density (cflag 2, eps 1e-9), so the final density is the
+    one the final energy belongs to; an energy-only test (|dE| < 1e-8) can stop where the
+    DIIS energies pause while the density still moves."""
     paths = batched_pool(tmp_path, 512, 5)
     decks = [read_deck(p) for p in paths]
     say("config D: batched SCF of %d molecules" % len(paths))
     batch = unomol_amd.Batch(paths)
-    e, res = batch.run(maxits=200, eps=1e-8, cflag=3, accel=2)
+    e, res = batch.run(maxits=300, eps=1e-9, cflag=2, accel=2)
     nconv = sum(1 for r in res if r.converged)
     say("config D: %d/%d converged" % (nconv, len(paths)))
     assert nconv == len(paths), [i for i, r in enumerate(res) if not r.converged]

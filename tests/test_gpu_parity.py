@@ -726,14 +726,24 @@ def test_batched_scf_matches_single(tmp_path):
         assert abs(e[i] - e1) < 1e-8, (i, p, e[i], e1)
 
 
-OUT_TAGS = ["631.h2o", "431.co", "water_631gs_cation", "b.dhdz"]
+# (the open-shell atoms b.dhdz / f.dhdz are left out: their unpaired electron sits along
+# an arbitrary axis of a degenerate p level, so P, the vectors and the overlaps are fixed
+# only up to a rotation of the atom)
+OUT_TAGS = ["631.h2o", "431.co", "water_631gs_cation"]
 
 
-def _nondegenerate(e, tol=1e-6):
-    """Orbitals whose energy is at least tol away from every other one (their vectors are
-    fixed up to sign; vectors inside a degenerate level are not)."""
-    d = np.abs(e[:, None] - e[None, :]) + np.eye(len(e)) * 1e9
-    return d.min(axis=1) > tol
+def _levels(e, tol=1e-5):
+    """Groups of orbitals whose energies lie within tol of a neighbour (degenerate levels:
+    only the subspace they span is fixed, not the vectors)."""
+    groups, cur = [], [0]
+    for i in range(1, len(e)):
+        if e[i] - e[i - 1] < tol:
+            cur.append(i)
+        else:
+            groups.append(cur)
+            cur = [i]
+    groups.append(cur)
+    return groups
 
 
 @pytest.mark.parametrize("tag", OUT_TAGS)
@@ -776,16 +786,21 @@ def test_output_files_vs_reference(tmp_path, tag):
     assert popg.shape == pops.shape
     np.testing.assert_allclose(popg, pops, atol=1e-8)
     np.testing.assert_allclose(chg, chs, atol=2e-9)
+    # printed eigenvectors: the projector onto every level's vectors (sign- and
+    # rotation-free), to the printed 10 digits
     vg, vs = R.read_eigvecs(g("scfout.gs.out"), n), R.read_eigvecs(s("scfout.gs.out"), n)
     assert len(vg) == len(vs) == (2 if uhf else 1)
-    keep_all = []
+    single_all = []
     for (eg, cg), (es, cs) in zip(vg, vs):
         assert cg.shape == cs.shape == (n, n)
         np.testing.assert_allclose(eg, es, atol=1e-9)
-        keep = _nondegenerate(es)
-        keep_all.append(keep)
-        sign = np.sign(np.sum(cg * cs, axis=0))
-        assert np.abs(cg * sign - cs)[:, keep].max() < 1e-7
+        single = np.zeros(n, dtype=bool)
+        for lev in _levels(es):
+            Pg, Ps = cg[:, lev] @ cg[:, lev].T, cs[:, lev] @ cs[:, lev].T
+            assert np.abs(Pg - Ps).max() < 1e-7, lev
+            if len(lev) == 1:
+                single[lev[0]] = True
+        single_all.append(single)
     # overlaps.out: Mulliken overlap populations per packed pair (and spin)
     og, os_ = R.read_overlaps(g("overlaps.out")), R.read_overlaps(s("overlaps.out"))
     assert og.shape == os_.shape == (n2, 2 if uhf else 1)
@@ -794,7 +809,7 @@ def test_output_files_vs_reference(tmp_path, tag):
     # for pairs of non-degenerate orbitals
     dg, ds = R.read_mol_dipmom(g("mol_dipmom.out")), R.read_mol_dipmom(s("mol_dipmom.out"))
     assert len(dg) == len(ds) == (2 if uhf else 1)
-    for bg, bs, keep in zip(dg, ds, keep_all):
+    for bg, bs, keep in zip(dg, ds, single_all):
         assert bg.keys() == bs.keys()
         for (i, j), v in bs.items():
             if keep[i] and keep[j]:
