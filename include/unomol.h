@@ -98,6 +98,13 @@ int uno_fock_uhf(uno_ctx* ctx, const double* PA, const double* PB, double* GA, d
 
 /* Replaces SymmPack::rsp(n, a, z, evals, tmp) (SymmPack.cpp:272-288). */
 int uno_eigh_packed(int n, double* a, double* z, double* evals);
+/* No counterpart in the reference: uno_eigh_packed for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and errors as uno_eigh_packed, for n <= 256; its device work
+ * space is a fixed 5.1 MiB array per GPU inside the library (no hipMalloc/hipFree, no
+ * stream creation: it runs on the null stream).  Larger n returns UNO_EINVAL.  Calls that
+ * overlap on one GPU share that array and race; it is not re-entrant. */
+int uno_eigh_packed_noalloc(int n, double* a, double* z, double* evals);
 /* Replaces SymmPack::sp_trans(n, a, z, tmp) (SymmPack.cpp:294-348): a <- X^T a X. */
 int uno_sp_trans(int n, double* a, const double* x);
 

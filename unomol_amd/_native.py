@@ -53,7 +53,7 @@ _lib = None
 
 # every symbol include/unomol.h declares
 EXPORTS = ["uno_last_error", "uno_ctx_create", "uno_ctx_rebuild", "uno_ctx_destroy", "uno_ctx_set",
-           "uno_ctx_get", "uno_ctx_norb", "uno_fock_rhf", "uno_fock_uhf", "uno_eigh_packed",
+           "uno_ctx_get", "uno_ctx_norb", "uno_fock_rhf", "uno_fock_uhf", "uno_eigh_packed", "uno_eigh_packed_noalloc",
            "uno_sp_trans", "uno_scf_defaults", "uno_scf_run", "uno_eri_emit", "uno_rys_eval", "uno_rys_eval_f32", "uno_rys_eval_ref2",
            "uno_schwarz", "uno_profile", "uno_copy_probe", "uno_oneint", "uno_oneint_device", "uno_moment_ints", "uno_deck_generate", "uno_run_deck",
            "uno_mlp_create", "uno_mlp_destroy", "uno_mlp_nparams", "uno_mlp_dims", "uno_mlp_get_params",

@@ -18,6 +18,12 @@
 
 namespace uno {
 
+// device work buffers of one solve
+struct EigBufs {
+    double *V, *tau, *p, *d, *e, *scal, *evals, *Y, *wk;
+    int* clusters;
+};
+
 struct EigenWS {
     int n = 0;
     DBuf<double> V, tau, p, d, e, scal, evals, Y, wk;
@@ -38,8 +44,9 @@ struct EigenWS {
     }
 };
 
-static int eigh_impl(EigenWS& w, int n, double* dA, double* dZ, double* h_evals, hipStream_t st) {
-    UNO_HIP(w.ensure(n));
+// The solver proper on caller-provided work buffers (device: b; host: hev[n], hcl[n]).
+static int eigh_core(int n, double* dA, double* dZ, double* h_evals, const EigBufs& b, double* hev, int* hcl,
+                     hipStream_t st) {
     if (n <= 2) {
         // closed form for the trivial sizes
         double a[4] = {0, 0, 0, 0};
@@ -66,39 +73,79 @@ static int eigh_impl(EigenWS& w, int n, double* dA, double* dZ, double* h_evals,
         UNO_HIP(hipStreamSynchronize(st));
         return UNO_OK;
     }
-    EigWork ew{dA, w.V.p, w.tau.p, w.p.p, w.d.p, w.e.p, w.scal.p, w.evals.p, w.Y.p, w.wk.p};
+    EigWork ew{dA, b.V, b.tau, b.p, b.d, b.e, b.scal, b.evals, b.Y, b.wk};
     UNO_HIP(householder_tridiag(n, ew, st));
     UNO_HIP(tridiag_eigen(n, ew, -1, nullptr, st));
-    w.hev.resize(n);
-    UNO_HIP(hipMemcpyAsync(w.hev.data(), w.evals.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    UNO_HIP(hipMemcpyAsync(hev, b.evals, n * sizeof(double), hipMemcpyDeviceToHost, st));
     UNO_HIP(hipStreamSynchronize(st));
     // clusters of eigenvalues closer than 1e-6 of the spectral scale get orthogonalised
-    // together; farther pairs are separated by inverse iteration itself
-    const double scale = std::max(1.0, std::max(std::fabs(w.hev.front()), std::fabs(w.hev.back())));
-    w.hcl.clear();
-    int start = 0;
+    // together; farther pairs are separated by inverse iteration itself ({first, end} pairs,
+    // at most n ints)
+    const double scale = std::max(1.0, std::max(std::fabs(hev[0]), std::fabs(hev[n - 1])));
+    int ncl = 0, start = 0;
     for (int j = 1; j <= n; ++j) {
-        if (j == n || w.hev[j] - w.hev[j - 1] > 1e-6 * scale) {
+        if (j == n || hev[j] - hev[j - 1] > 1e-6 * scale) {
             if (j - start > 1) {
-                w.hcl.push_back(start);
-                w.hcl.push_back(j);
+                hcl[2 * ncl] = start;
+                hcl[2 * ncl + 1] = j;
+                ++ncl;
             }
             start = j;
         }
     }
-    const int ncl = (int)(w.hcl.size() / 2);
-    if (ncl) UNO_HIP(w.clusters.upload(w.hcl.data(), w.hcl.size(), st));
-    UNO_HIP(tridiag_eigen(n, ew, ncl, w.clusters.p, st));
+    if (ncl) UNO_HIP(hipMemcpyAsync(b.clusters, hcl, 2 * (size_t)ncl * sizeof(int), hipMemcpyHostToDevice, st));
+    UNO_HIP(tridiag_eigen(n, ew, ncl, b.clusters, st));
     // Z = Q Yt: the eigenvectors of T back-transformed by the reflectors in place
-    UNO_HIP(hipMemcpyAsync(dZ, w.Y.p, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    UNO_HIP(hipMemcpyAsync(dZ, b.Y, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, st));
     UNO_HIP(apply_q(n, ew, dZ, st));
-    std::copy(w.hev.begin(), w.hev.end(), h_evals);
+    std::copy(hev, hev + n, h_evals);
     return UNO_OK;
+}
+
+static int eigh_impl(EigenWS& w, int n, double* dA, double* dZ, double* h_evals, hipStream_t st) {
+    UNO_HIP(w.ensure(n));
+    w.hev.resize(std::max(n, 1));
+    w.hcl.resize(std::max(n, 2));
+    const EigBufs b{w.V.p, w.tau.p, w.p.p, w.d.p, w.e.p, w.scal.p, w.evals.p, w.Y.p, w.wk.p, w.clusters.p};
+    return eigh_core(n, dA, dZ, h_evals, b, w.hev.data(), w.hcl.data(), st);
 }
 
 int eigh_device(uno_ctx* c, int n, double* dA, double* dZ, double* h_evals) {
     if (!c->eig) c->eig = new EigenWS();
     return eigh_impl(*c->eig, n, dA, dZ, h_evals, c->st);
+}
+
+size_t eigh_ws_doubles(int n) {
+    const size_t sq = (size_t)n * n, pad = 32;
+    // V, Y, wk (5 n^2), tau, p, d, e, evals, scal, clusters (n ints), each 256-byte aligned
+    return 7 * sq + 5 * (n + pad) + (4 + pad) + (n / 2 + 1 + pad) + 10 * pad;
+}
+
+int eigh_in_workspace(int n, double* dA, double* dZ, double* h_evals, double* ws, double* hev, int* hcl,
+                      hipStream_t st) {
+    const size_t sq = (size_t)n * n;
+    size_t off = 0;
+    auto take = [&](size_t count) {
+        double* p = ws + off;
+        off += (count + 31) / 32 * 32;
+        return p;
+    };
+    EigBufs b;
+    b.V = take(sq);
+    b.Y = take(sq);
+    b.wk = take(5 * sq);
+    b.tau = take(n);
+    b.p = take(n);
+    b.d = take(n);
+    b.e = take(n);
+    b.evals = take(n);
+    b.scal = take(4);
+    b.clusters = reinterpret_cast<int*>(take(n / 2 + 1));
+    if (off > eigh_ws_doubles(n)) {
+        set_error("eigensolver work space layout exceeds its size");
+        return UNO_EINVAL;
+    }
+    return eigh_core(n, dA, dZ, h_evals, b, hev, hcl, st);
 }
 
 int eigh_standalone(int n, double* dA, double* dZ, double* h_evals, hipStream_t st) {

@@ -206,6 +206,11 @@ int eri_emit(uno_ctx* c, long cap, double* val, int* ijkl, long* count);
 // columns), evals ascending (host)
 int eigh_device(uno_ctx* c, int n, double* dA, double* dZ, double* h_evals);
 int eigh_standalone(int n, double* dA, double* dZ, double* h_evals, hipStream_t st);
+// the same solve on one caller-provided device work space of eigh_ws_doubles(n) doubles
+// (256-byte aligned) and host arrays hev[n], hcl[n]: no allocation
+size_t eigh_ws_doubles(int n);
+int eigh_in_workspace(int n, double* dA, double* dZ, double* h_evals, double* ws, double* hev, int* hcl,
+                      hipStream_t st);
 
 // dh (optional, host packed): added to the core Hamiltonian (model potentials)
 int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, double* evals, double* P, double* C,
