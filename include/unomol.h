@@ -80,7 +80,11 @@ int uno_ctx_destroy(uno_ctx* ctx);
  * autotuner does not prefer otherwise; 0 never; 1 wherever they fit), "rys2_exact" (0, the
  * default: the two-root classes use the reference's own two-root quadrature,
  * Rys.cpp:413-631, which for 15 < x <= 33 departs from the exact quadrature by up to
- * 3.5e-4 relative, so that ERIs match the reference to 1e-12; 1: the exact quadrature). */
+ * 3.5e-4 relative, so that ERIs match the reference to 1e-12; 1: the exact quadrature),
+ * "purify" (uno_scf_run's densities between Fock builds: 1 by trace-correcting
+ * purification of X^T F X with FP64 MFMA GEMMs, the eigensolve only for the final orbital
+ * energies and vectors or where purification does not settle; 0 always the eigensolve;
+ * -1, the default, purification from 256 functions). */
 int uno_ctx_set(uno_ctx* ctx, const char* key, double value);
 /* Reads the knobs above and, read-only: "last_quartets" (shell quartets evaluated by the
  * last Fock build), "last_quartets32" (of which in FP32), "pairs" (significant shell pairs), "pair_runs" (runs of the pair
@@ -155,6 +159,8 @@ typedef struct uno_scf_result {
     double guess_energy;  /* guess 2: iteration-0 energy of the learned guess when guess_check
                              evaluated it (else 0) */
     double guess_energy_coreh;  /* the same for the core-Hamiltonian guess */
+    int purified;         /* densities formed by purification instead of an eigensolve */
+    int purify_steps;     /* purification steps (one GEMM each) over all of them */
 } uno_scf_result;
 
 void uno_scf_defaults(uno_scf_opts* opts);

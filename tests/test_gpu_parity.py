@@ -269,6 +269,34 @@ def test_scf_exact_roots_vs_reference_x2(tag):
     assert abs(hf.result.init_energy - ref["init"]) < 1e-9
 
 
+@pytest.mark.parametrize("tag,uhf", [("B_w16_631gs", False), ("B_w16_631gs_cation", True), ("631.h2o", False)])
+def test_scf_purification_matches_eigensolve(tag, uhf):
+    """SCF densities by trace-correcting purification (FP64 MFMA GEMMs, no eigensolve
+    between Fock builds) against the eigensolve path: the same energy (1e-10 Eh), orbital
+    energies (1e-9 Eh, from the final eigensolve) and density (1e-9), with every iteration's
+    density purified."""
+    say("scf purification vs eigensolve " + tag)
+    path = os.path.join(R.GOLDEN, "decks", tag + ".dat") if tag.startswith("B_") else R.deck_path(tag)
+    b, t = _tints(path)
+    cls = unomol_amd.UnRestrictedHartreeFock if uhf else unomol_amd.RestrictedHartreeFock
+    t.set("purify", 0)
+    ref = cls(b, t)
+    e0 = ref.findEnergy(accel=2, eps=1e-10, cflag=3, maxits=100)
+    assert ref.result.converged and ref.result.purified == 0
+    t.set("purify", 1)
+    hf = cls(b, t)
+    e1 = hf.findEnergy(accel=2, eps=1e-10, cflag=3, maxits=100)
+    r = hf.result
+    say("purification: %d densities purified in %d iterations, |dE| %.2e, diag %.3f s vs %.3f s" %
+        (r.purified, r.iterations, abs(e1 - e0), r.diag_s, ref.result.diag_s))
+    assert r.converged
+    assert r.purified >= r.iterations * (2 if uhf else 1) - 2
+    assert abs(e1 - e0) < 1e-10
+    np.testing.assert_allclose(hf.evals, ref.evals, atol=1e-9)
+    assert np.abs(hf.P - ref.P).max() < 1e-9
+    t.set("purify", -1)
+
+
 def test_scf_diis_same_energy(tmp_path):
     say("scf diis water 6-31G*")
     path = R.deck_path("w2_631gs")

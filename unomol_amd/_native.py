@@ -43,7 +43,7 @@ class ScfResult(C.Structure):
                 ("diag_s", C.c_double), ("quartets", C.c_double), ("quartets32", C.c_double),
                 ("nfull", C.c_int), ("guess_s", C.c_double), ("wall99_s", C.c_double),
                 ("setup_s", C.c_double), ("diis_s", C.c_double), ("guess_fallback", C.c_int),
-                ("guess_energy", C.c_double), ("guess_energy_coreh", C.c_double)]
+                ("guess_energy", C.c_double), ("guess_energy_coreh", C.c_double), ("purified", C.c_int), ("purify_steps", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -53,6 +53,20 @@ hipError_t k_shell_max(int n, int nsh, const int* sh_off, const int* sh_n, const
                        const double* D2, double* Dsh, hipStream_t st);
 // out[0] = max |x|
 hipError_t k_absmax(long len, const double* x, double* out, hipStream_t st);
+// purification: Gershgorin bounds of F (lohi: 2n work, bounds: 2 doubles) and
+// X = (hi I - F) / (hi - lo); out[0] = tr(A)
+hipError_t sp2_init(int n, const double* F, double* X, double* lohi, double* bounds, hipStream_t st);
+hipError_t k_trace(int n, const double* A, double* out, hipStream_t st);
+// launch-chained purification steps (linalg.hip): states is an array of step states
+// (sp2_state_bytes() each); sp2_start fills state 0 and X0's tile sums (part0: 2 doubles per
+// upper-triangle 64 x 64 tile); sp2_steps enqueues steps k0 .. k0+count-1; sp2_state_read
+// waits for the stream and reads state k (cur = buffer holding X, done 1 = converged,
+// 2 = failed, steps = steps taken, t1 = tr(X))
+size_t sp2_state_bytes();
+hipError_t sp2_start(int n, const double* X0, double* part0, void* st0, hipStream_t st);
+hipError_t sp2_steps(int n, int nocc, double thr, void* states, int k0, int count, double* buf0, double* buf1,
+                     double* part0, double* part1, hipStream_t st);
+hipError_t sp2_state_read(const void* states, int k, int* cur, int* done, int* steps, double* t1, hipStream_t st);
 // C = alpha op(A) op(B) + beta C  (FP64 MFMA 16x16x4)
 hipError_t gemm_f64(bool ta, bool tb, int m, int n, int k, double alpha, const double* A, int lda, const double* B,
                     int ldb, double beta, double* C, int ldc, hipStream_t st);

@@ -144,6 +144,83 @@ hipError_t k_diffnorm(long len, int n, const double* A, const double* B, double*
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Density-matrix purification (SCF densities without an eigensolve): Gershgorin bounds
+// of F', the initial X = (hi I - F') / (hi - lo) with the bounds read on the device, and
+// the traces tr(X) and tr(X^2) = sum X_ij^2 (symmetric X) per step.
+// ---------------------------------------------------------------------------
+// one block per row: lohi[2i] = F_ii - sum_{j != i} |F_ij|, lohi[2i+1] = F_ii + the same
+__global__ __launch_bounds__(256) void k_gersh_rows(int n, const double* __restrict__ F, double* __restrict__ lohi) {
+    __shared__ double sh[256];
+    const int i = blockIdx.x;
+    const double* row = F + (long)i * n;
+    double r = 0.0;
+    for (int j = threadIdx.x; j < n; j += 256)
+        if (j != i) r += fabs(row[j]);
+    r = block_sum(r, sh);
+    if (threadIdx.x == 0) {
+        lohi[2 * i] = row[i] - r;
+        lohi[2 * i + 1] = row[i] + r;
+    }
+}
+// bounds[0] = min lo, bounds[1] = max hi over the rows (one block)
+__global__ __launch_bounds__(1024) void k_gersh_final(int n, const double* __restrict__ lohi, double* bounds) {
+    __shared__ double slo[1024], shi[1024];
+    double lo = 1e300, hi = -1e300;
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        lo = fmin(lo, lohi[2 * i]);
+        hi = fmax(hi, lohi[2 * i + 1]);
+    }
+    slo[threadIdx.x] = lo;
+    shi[threadIdx.x] = hi;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            slo[threadIdx.x] = fmin(slo[threadIdx.x], slo[threadIdx.x + s]);
+            shi[threadIdx.x] = fmax(shi[threadIdx.x], shi[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        bounds[0] = slo[0];
+        bounds[1] = shi[0];
+    }
+}
+__global__ void k_sp2_init_(int n, const double* __restrict__ F, const double* __restrict__ bounds,
+                            double* __restrict__ X) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)n * n) return;
+    const double lo = bounds[0], hi = bounds[1];
+    const double inv = 1.0 / (hi - lo);
+    const long i = t / n, j = t - i * n;
+    X[t] = ((i == j ? hi : 0.0) - F[t]) * inv;
+}
+hipError_t sp2_init(int n, const double* F, double* X, double* lohi, double* bounds, hipStream_t st) {
+    hipLaunchKernelGGL(k_gersh_rows, dim3((unsigned)n), dim3(256), 0, st, n, F, lohi);
+    hipLaunchKernelGGL(k_gersh_final, dim3(1), dim3(1024), 0, st, n, (const double*)lohi, bounds);
+    const long len = (long)n * n;
+    hipLaunchKernelGGL(k_sp2_init_, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, n, F,
+                       (const double*)bounds, X);
+    return hipGetLastError();
+}
+// out[0] = tr(A)
+__global__ __launch_bounds__(1024) void k_trace_(int n, const double* __restrict__ A, double* out) {
+    __shared__ double sh[1024];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) s += A[(long)i * n + i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 512; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+hipError_t k_trace(int n, const double* A, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_trace_, dim3(1), dim3(1024), 0, st, n, A, out);
+    return hipGetLastError();
+}
+
 // one block per shell pair (s, t): max |D| over its function block of up to three matrices
 __global__ void k_shell_max_(int n, int nsh, const int* off, const int* nf, const double* D, const double* D1,
                              const double* D2, double* Dsh) {
@@ -191,44 +268,64 @@ hipError_t k_absmax(long len, const double* x, double* out, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // Split K (gridDim.z > 1): block z covers k in [z kchunk, (z+1) kchunk) and adds
 // alpha * its partial product into C, which the launcher has already scaled by beta.
+// One 64 x 64 output tile of C = op(A) op(B) over k in [kb, ke), accumulated in acc (four
+// waves, each a 32 x 32 quarter of 2 x 2 MFMA tiles).  K advances in chunks of 32: the next
+// chunk's operands are loaded from global memory into registers while the current chunk's
+// MFMAs run on LDS, then stored into the other LDS buffer (one barrier per chunk).
+constexpr int kGemmKc = 32;
 template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, double alpha, const double* __restrict__ A,
-                                              int lda, const double* __restrict__ B, int ldb, double beta,
-                                              double* __restrict__ C, int ldc, int kchunk) {
-    __shared__ double sA[16][64 + 2];
-    __shared__ double sB[16][64 + 2];
+__device__ __forceinline__ void gemm_tile(int M, int N, int kb, int ke, const double* __restrict__ A, int lda,
+                                          const double* __restrict__ B, int ldb, int bi, int bj,
+                                          double (*sA)[kGemmKc][64 + 2], double (*sB)[kGemmKc][64 + 2],
+                                          double4_t acc[2][2]) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int bi = blockIdx.y * 64, bj = blockIdx.x * 64;
     const int wi = (wave >> 1) * 32, wj = (wave & 1) * 32;
-    const int kb = blockIdx.z * kchunk, ke = kb + kchunk < K ? kb + kchunk : K;
-    double4_t acc[2][2];
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) acc[a][b] = (double4_t){0.0, 0.0, 0.0, 0.0};
-    for (int k0 = kb; k0 < ke; k0 += 16) {
-        for (int e = tid; e < 1024; e += 256) {
-            int i, kk;
-            if (TA) { kk = e / 64; i = e % 64; } else { i = e / 16; kk = e % 16; }
-            const int gi = bi + i, gk = k0 + kk;
-            double v = 0.0;
-            if (gi < M && gk < ke) v = TA ? A[(long)gk * lda + gi] : A[(long)gi * lda + gk];
-            sA[kk][i] = v;
-        }
-        for (int e = tid; e < 1024; e += 256) {
-            int j, kk;
-            if (TB) { j = e / 16; kk = e % 16; } else { kk = e / 64; j = e % 64; }
-            const int gj = bj + j, gk = k0 + kk;
-            double v = 0.0;
-            if (gj < N && gk < ke) v = TB ? B[(long)gj * ldb + gk] : B[(long)gk * ldb + gj];
-            sB[kk][j] = v;
-        }
-        __syncthreads();
+    double ra[8], rb[8];
+    // element e = tid + 256 m of a chunk: A as (i, kk), B as (kk, j); the fastest index runs
+    // along the operand's contiguous dimension
+    auto load = [&](int k0) {
 #pragma unroll
-        for (int kk = 0; kk < 16; kk += 4) {
+        for (int m = 0; m < 8; ++m) {
+            const int e = tid + 256 * m;
+            int i, kk;
+            if (TA) { kk = e / 64; i = e % 64; } else { i = e / kGemmKc; kk = e % kGemmKc; }
+            const int gi = bi + i, gk = k0 + kk;
+            ra[m] = (gi < M && gk < ke) ? (TA ? A[(long)gk * lda + gi] : A[(long)gi * lda + gk]) : 0.0;
+            int j, kj;
+            if (TB) { j = e / kGemmKc; kj = e % kGemmKc; } else { kj = e / 64; j = e % 64; }
+            const int gj = bj + j, gk2 = k0 + kj;
+            rb[m] = (gj < N && gk2 < ke) ? (TB ? B[(long)gj * ldb + gk2] : B[(long)gk2 * ldb + gj]) : 0.0;
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int e = tid + 256 * m;
+            int i, kk;
+            if (TA) { kk = e / 64; i = e % 64; } else { i = e / kGemmKc; kk = e % kGemmKc; }
+            sA[buf][kk][i] = ra[m];
+            int j, kj;
+            if (TB) { j = e / kGemmKc; kj = e % kGemmKc; } else { kj = e / 64; j = e % 64; }
+            sB[buf][kj][j] = rb[m];
+        }
+    };
+    if (kb >= ke) return;
+    load(kb);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = kb; k0 < ke; k0 += kGemmKc) {
+        const bool more = k0 + kGemmKc < ke;
+        if (more) load(k0 + kGemmKc);
+#pragma unroll
+        for (int kk = 0; kk < kGemmKc; kk += 4) {
             double av[2], bv[2];
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
-                av[t] = sA[kk + (lane >> 4)][wi + t * 16 + (lane & 15)];
-                bv[t] = sB[kk + (lane >> 4)][wj + t * 16 + (lane & 15)];
+                av[t] = sA[buf][kk + (lane >> 4)][wi + t * 16 + (lane & 15)];
+                bv[t] = sB[buf][kk + (lane >> 4)][wj + t * 16 + (lane & 15)];
             }
 #pragma unroll
             for (int ta = 0; ta < 2; ++ta)
@@ -236,8 +333,24 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, double alpha,
                 for (int tb = 0; tb < 2; ++tb)
                     acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ta], bv[tb], acc[ta][tb], 0, 0, 0);
         }
+        if (more) store(buf ^ 1);
         __syncthreads();
+        buf ^= 1;
     }
+}
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, double alpha, const double* __restrict__ A,
+                                              int lda, const double* __restrict__ B, int ldb, double beta,
+                                              double* __restrict__ C, int ldc, int kchunk) {
+    __shared__ double sA[2][kGemmKc][64 + 2];
+    __shared__ double sB[2][kGemmKc][64 + 2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int bi = blockIdx.y * 64, bj = blockIdx.x * 64;
+    const int wi = (wave >> 1) * 32, wj = (wave & 1) * 32;
+    const int kb = blockIdx.z * kchunk, ke = kb + kchunk < K ? kb + kchunk : K;
+    double4_t acc[2][2];
+    gemm_tile<TA, TB>(M, N, kb, ke, A, lda, B, ldb, bi, bj, sA, sB, acc);
 #pragma unroll
     for (int ta = 0; ta < 2; ++ta)
 #pragma unroll
@@ -271,7 +384,7 @@ hipError_t gemm_f64(bool ta, bool tb, int m, int n, int k, double alpha, const d
     if (tiles < 128 && k >= 256) {
         ksplit = (int)std::min<long>(512 / tiles, k / 64);
         if (ksplit > 1) {
-            kchunk = ((k + ksplit - 1) / ksplit + 15) / 16 * 16;
+            kchunk = ((k + ksplit - 1) / ksplit + kGemmKc - 1) / kGemmKc * kGemmKc;
             ksplit = (k + kchunk - 1) / kchunk;
         } else {
             ksplit = 1;
@@ -287,6 +400,180 @@ hipError_t gemm_f64(bool ta, bool tb, int m, int n, int k, double alpha, const d
     else if (!ta && tb) hipLaunchKernelGGL((k_gemm<false, true>), grid, blk, 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kchunk);
     else hipLaunchKernelGGL((k_gemm<true, true>), grid, blk, 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kchunk);
     return hipGetLastError();
+}
+
+// One purification step per launch, the host never waiting in between.  The step's state
+// (which buffer holds X, the countdown after idempotency, done) passes from launch k to
+// launch k+1 through st[k] -> st[k+1]; every block derives the same decision from the
+// same data in the same order.  Per step: tr(X) and tr(X^2) = sum X_ij^2 from the per-tile
+// partial sums the previous step left (fixed-order sum), the choice X^2 or 2X - X^2
+// (whichever trace lies nearer nocc), then X^2 on the upper-triangle 64 x 64 tiles only
+// (FP64 MFMA, X symmetric) with the choice applied in the epilogue, each tile also
+// written mirrored, and the new X's partial sums per tile.  Once tr(X) - tr(X^2) is below
+// thr, two more steps run and the rest of the launches return at once.
+struct Sp2State {
+    int cur, extra, done, steps;
+    double t1, t2;
+};
+
+__device__ __forceinline__ void sp2_tile_of(int t, int nb, int& ti, int& tj) {
+    // upper-triangle tiles in row order: row ti holds tiles tj = ti .. nb-1
+    int r = 0, base = 0;
+    while (base + (nb - r) <= t) {
+        base += nb - r;
+        ++r;
+    }
+    ti = r;
+    tj = r + (t - base);
+}
+
+// partial sums of X over tile t: [0] = sum X_ij^2 (mirror counted), [1] = sum of the diagonal
+__device__ __forceinline__ void sp2_tile_sums(double s2, double s1, double* sh, double* out) {
+    const int tid = threadIdx.x;
+    sh[tid] = s2;
+    sh[256 + tid] = s1;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (tid < k) {
+            sh[tid] += sh[tid + k];
+            sh[256 + tid] += sh[256 + tid + k];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        out[0] = sh[0];
+        out[1] = sh[256];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sp2_stats0(int n, int nb, const double* __restrict__ X, double* part,
+                                                     Sp2State* st0) {
+    __shared__ double sh[512];
+    int ti, tj;
+    sp2_tile_of(blockIdx.x, nb, ti, tj);
+    const int bi = ti * 64, bj = tj * 64;
+    const double w = tj > ti ? 2.0 : 1.0;
+    double s2 = 0.0, s1 = 0.0;
+    for (int e = threadIdx.x; e < 4096; e += 256) {
+        const int gi = bi + e / 64, gj = bj + e % 64;
+        if (gi < n && gj < n) {
+            const double v = X[(long)gi * n + gj];
+            s2 += w * v * v;
+            if (gi == gj) s1 += v;
+        }
+    }
+    sp2_tile_sums(s2, s1, sh, part + 2 * blockIdx.x);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *st0 = Sp2State{0, -1, 0, 0, 0.0, 0.0};
+}
+
+__global__ __launch_bounds__(256) void k_sp2_step(int n, int nb, int nocc, double thr, const Sp2State* __restrict__ sin,
+                                                  Sp2State* __restrict__ sout, double* buf0, double* buf1,
+                                                  double* part0, double* part1) {
+    __shared__ double sA[2][kGemmKc][64 + 2];
+    __shared__ double sB[2][kGemmKc][64 + 2];
+    __shared__ double sh[512];
+    const int tid = threadIdx.x;
+    const Sp2State s = *sin;
+    if (s.done) {
+        if (blockIdx.x == 0 && tid == 0) *sout = s;
+        return;
+    }
+    // traces of the current X from the previous step's tile sums, in a fixed order
+    const int ntile = nb * (nb + 1) / 2;
+    const double* pin = s.cur ? part1 : part0;
+    double a2 = 0.0, a1 = 0.0;
+    for (int t = tid; t < ntile; t += 256) {
+        a2 += pin[2 * t];
+        a1 += pin[2 * t + 1];
+    }
+    sh[tid] = a2;
+    sh[256 + tid] = a1;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (tid < k) {
+            sh[tid] += sh[tid + k];
+            sh[256 + tid] += sh[256 + tid + k];
+        }
+        __syncthreads();
+    }
+    const double t2 = sh[0], t1 = sh[256];
+    __syncthreads();
+    const double idem = t1 - t2;
+    int extra = s.extra, done = 0;
+    if (!isfinite(idem)) done = 2;
+    else {
+        if (extra < 0 && fabs(idem) < thr) extra = 2;
+        if (extra == 0) done = 1;
+    }
+    if (done) {
+        if (blockIdx.x == 0 && tid == 0) *sout = Sp2State{s.cur, extra, done, s.steps, t1, t2};
+        return;
+    }
+    if (extra > 0) --extra;
+    const bool square = fabs(t2 - nocc) < fabs(2.0 * t1 - t2 - nocc);
+    const double* X = s.cur ? buf1 : buf0;
+    double* Y = s.cur ? buf0 : buf1;
+    double* pout = s.cur ? part0 : part1;
+    int ti, tj;
+    sp2_tile_of(blockIdx.x, nb, ti, tj);
+    const int wave = tid >> 6, lane = tid & 63;
+    const int bi = ti * 64, bj = tj * 64;
+    const int wi = (wave >> 1) * 32, wj = (wave & 1) * 32;
+    double4_t acc[2][2];
+    gemm_tile<false, false>(n, n, 0, n, X, n, X, n, bi, bj, sA, sB, acc);
+    const double w = tj > ti ? 2.0 : 1.0;
+    double s2 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = bi + wi + ta * 16 + 4 * r + (lane >> 4);
+                const int gj = bj + wj + tb * 16 + (lane & 15);
+                if (gi < n && gj < n) {
+                    const double x2 = acc[ta][tb][r];
+                    const double v = square ? x2 : 2.0 * X[(long)gi * n + gj] - x2;
+                    Y[(long)gi * n + gj] = v;
+                    if (tj > ti) Y[(long)gj * n + gi] = v;
+                    s2 += w * v * v;
+                    if (gi == gj) s1 += v;
+                }
+            }
+    sp2_tile_sums(s2, s1, sh, pout + 2 * blockIdx.x);
+    if (blockIdx.x == 0 && tid == 0) *sout = Sp2State{1 - s.cur, extra, 0, s.steps + 1, t1, t2};
+}
+
+size_t sp2_state_bytes() { return sizeof(Sp2State); }
+
+hipError_t sp2_start(int n, const double* X0, double* part0, void* st0, hipStream_t st) {
+    const int nb = (n + 63) / 64, ntile = nb * (nb + 1) / 2;
+    hipLaunchKernelGGL(k_sp2_stats0, dim3((unsigned)ntile), dim3(256), 0, st, n, nb, X0, part0,
+                       reinterpret_cast<Sp2State*>(st0));
+    return hipGetLastError();
+}
+
+hipError_t sp2_steps(int n, int nocc, double thr, void* states, int k0, int count, double* buf0, double* buf1,
+                     double* part0, double* part1, hipStream_t st) {
+    const int nb = (n + 63) / 64, ntile = nb * (nb + 1) / 2;
+    Sp2State* S = reinterpret_cast<Sp2State*>(states);
+    for (int k = k0; k < k0 + count; ++k)
+        hipLaunchKernelGGL(k_sp2_step, dim3((unsigned)ntile), dim3(256), 0, st, n, nb, nocc, thr,
+                           (const Sp2State*)(S + k), S + k + 1, buf0, buf1, part0, part1);
+    return hipGetLastError();
+}
+
+hipError_t sp2_state_read(const void* states, int k, int* cur, int* done, int* steps, double* t1, hipStream_t st) {
+    Sp2State h;
+    hipError_t e = hipMemcpyAsync(&h, reinterpret_cast<const Sp2State*>(states) + k, sizeof(h),
+                                  hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    *cur = h.cur;
+    *done = h.done;
+    *steps = h.steps;
+    *t1 = h.t1;
+    return hipSuccess;
 }
 
 // ---------------------------------------------------------------------------

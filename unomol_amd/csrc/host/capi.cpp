@@ -306,6 +306,7 @@ int uno_ctx_set(uno_ctx* c, const char* key, double v) {
     else if (k == "row_variant") c->row_variant = (int)v;
     else if (k == "row_density_lds") c->row_dens = (int)v;
     else if (k == "rys2_exact") c->rys2_exact = (int)v;
+    else if (k == "purify") c->purify = (int)v;
     else { set_error("unknown option " + k); return UNO_EINVAL; }
     return UNO_OK;
 }
@@ -324,6 +325,7 @@ int uno_ctx_get(uno_ctx* c, const char* key, double* v) {
     else if (k == "row_variant") *v = c->row_variant;
     else if (k == "row_density_lds") *v = c->row_dens;
     else if (k == "rys2_exact") *v = c->rys2_exact;
+    else if (k == "purify") *v = c->purify;
     else if (k == "last_quartets") *v = c->last_quartets;
     else if (k == "last_quartets32") *v = c->last_quartets32;
     else if (k == "pair_runs") *v = (double)c->cls.size();

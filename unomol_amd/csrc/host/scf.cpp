@@ -65,6 +65,9 @@ struct ScfBufs {
 };
 
 constexpr int kRebuild = 10;            // incremental builds between full rebuilds
+constexpr int kSp2MinN = 256;           // purification instead of the eigensolve from this size (purify = -1)
+constexpr int kSp2MaxIt = 100;
+constexpr double kSp2Idem = 1e-8;
 constexpr double kMixedSwitch = 1e-5;   // |dE| (Eh) ending the mixed-precision phase
 
 int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, double* evals_out, double* P_io,
@@ -144,14 +147,86 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
     double diag_s = 0.0, fock_s = 0.0, diis_s = 0.0;
 
     // F (device, n x n) -> P[s] with nocc electrons; keeps C[s] and the eigenvalues
-    auto diag_to_P = [&](const double* Fd, int s, int no) -> int {
+    // (write_P false: only C[s] and the eigenvalues)
+    auto diag_to_P = [&](const double* Fd, int s, int no, bool write_P = true) -> int {
         const auto t0 = clk::now();
         UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, Fd, n, b.X.p, n, 0.0, b.Tm.p, n, st));       // F X
         UNO_HIP(gemm_f64(true, false, n, n, n, 1.0, b.X.p, n, b.Tm.p, n, 0.0, b.Fp.p, n, st));   // X^T F X
         if (int rc = eigh_device(c, n, b.Fp.p, b.Cw.p, evals.data() + (size_t)s * n)) return rc;
         UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, b.X.p, n, b.Cw.p, n, 0.0, b.C[s].p, n, st));  // C = X W
-        UNO_HIP(gemm_f64(false, true, n, n, no, 1.0, b.C[s].p, n, b.C[s].p, n, 0.0, P[s], n, st)); // C_occ C_occ^T
+        if (write_P)
+            UNO_HIP(gemm_f64(false, true, n, n, no, 1.0, b.C[s].p, n, b.C[s].p, n, 0.0, P[s], n, st)); // C_occ C_occ^T
         diag_s += secs(t0, clk::now());
+        return UNO_OK;
+    };
+
+    // F (device) -> P[s] by purification, without an eigensolve: F' = X^T F X; from
+    // X_0 = (hi I - F') / (hi - lo) (Gershgorin bounds of F'), X <- X^2 or 2X - X^2,
+    // whichever trace lies nearer nocc (Niklasson's trace-correcting recursion), until
+    // tr(X) - tr(X^2) = sum l(1-l) over X's eigenvalues is below kSp2Idem, then two more
+    // steps (each squares the distance of every eigenvalue to 0 or 1); P = X_S P' X_S^T.
+    // The result is the projector onto F's nocc lowest eigenvectors, the density the
+    // eigensolve gives.  ok = false (P untouched) when the recursion does not settle
+    // within kSp2MaxIt steps or ends with the wrong electron count (a level degenerate at
+    // the occupation boundary): the caller then diagonalises.  Each step is one launch
+    // (hip/linalg.hip, k_sp2_step: the symmetric MFMA product on the upper-triangle tiles
+    // with the choice and the next traces fused in); the steps pass their state from launch
+    // to launch on the device and the host reads it only after a batch of launches (the
+    // previous density's step count plus a margin; steps after convergence return at once).
+    DBuf<double> sp2w;
+    const bool use_sp2 = c->purify == 1 || (c->purify < 0 && n >= kSp2MinN);
+    const int sp2_nb = (n + 63) / 64, sp2_tiles = sp2_nb * (sp2_nb + 1) / 2;
+    const size_t sp2_state_d = (sp2_state_bytes() * (kSp2MaxIt + 2) + 7) / 8;
+    double *sp2_lohi = nullptr, *sp2_bounds = nullptr, *sp2_part[2] = {nullptr, nullptr};
+    void* sp2_states = nullptr;
+    if (use_sp2) {
+        UNO_HIP(sp2w.alloc(2 * (size_t)n + 4 + 4 * (size_t)sp2_tiles + sp2_state_d));
+        sp2_lohi = sp2w.p;
+        sp2_bounds = sp2w.p + 2 * (size_t)n;
+        sp2_part[0] = sp2_bounds + 4;
+        sp2_part[1] = sp2_part[0] + 2 * (size_t)sp2_tiles;
+        sp2_states = sp2_part[1] + 2 * (size_t)sp2_tiles;
+    }
+    bool last_sp2[2] = {false, false};
+    int sp2_budget = 64;   // steps enqueued before the first look at the state; then last count + 6
+    auto density_sp2 = [&](const double* Fd, int s, int no, bool& ok) -> int {
+        ok = false;
+        const auto t0 = clk::now();
+        UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, Fd, n, b.X.p, n, 0.0, b.Tm.p, n, st));       // F X
+        UNO_HIP(gemm_f64(true, false, n, n, n, 1.0, b.X.p, n, b.Tm.p, n, 0.0, b.Fp.p, n, st));   // X^T F X
+        UNO_HIP(sp2_init(n, b.Fp.p, b.Cw.p, sp2_lohi, sp2_bounds, st));
+        UNO_HIP(sp2_start(n, b.Cw.p, sp2_part[0], sp2_states, st));
+        int k = 0, cur = 0, done = 0, steps = 0, budget = sp2_budget;
+        double t1 = 0.0;
+        while (k < kSp2MaxIt) {
+            const int cnt = std::min(budget, kSp2MaxIt - k);
+            UNO_HIP(sp2_steps(n, no, kSp2Idem, sp2_states, k, cnt, b.Cw.p, b.Tm.p, sp2_part[0], sp2_part[1], st));
+            k += cnt;
+            UNO_HIP(sp2_state_read(sp2_states, k, &cur, &done, &steps, &t1, st));
+            if (done) break;
+            budget = 16;
+        }
+        r.purify_steps += steps;
+        ok = done == 1 && std::fabs(t1 - no) < 1e-6;
+        if (ok) {
+            sp2_budget = steps + 6;
+            double* Xk = cur ? b.Tm.p : b.Cw.p;
+            double* T = cur ? b.Cw.p : b.Tm.p;
+            UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, b.X.p, n, Xk, n, 0.0, T, n, st));    // X_S P'
+            UNO_HIP(gemm_f64(false, true, n, n, n, 1.0, T, n, b.X.p, n, 0.0, P[s], n, st));   // (X_S P') X_S^T
+            ++r.purified;
+        }
+        diag_s += secs(t0, clk::now());
+        return UNO_OK;
+    };
+    // the density of spin s from F: purification where enabled, else (or when it does not
+    // settle) the eigensolve
+    auto density_of = [&](const double* Fd, int s, int no) -> int {
+        bool ok = false;
+        if (use_sp2)
+            if (int rc = density_sp2(Fd, s, no, ok)) return rc;
+        last_sp2[s] = ok;
+        if (!ok) return diag_to_P(Fd, s, no);
         return UNO_OK;
     };
 
@@ -391,7 +466,7 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
         for (int s = 0; s < ns; ++s) {
             UNO_HIP(hipMemcpyAsync(b.Pold[s].p, P[s], nn * sizeof(double), hipMemcpyDeviceToDevice, st));
             // spin-beta Fock lives in Etmp; copy to F's slot is unnecessary: diag reads Fs[s]
-            if (int rc = diag_to_P(Fs[s], s, nocc[s])) return rc;
+            if (int rc = density_of(Fs[s], s, nocc[s])) return rc;
             UNO_HIP(k_diffnorm(nn, n, P[s], b.Pold[s].p, c->scal.p + 2 + s, c->red.p, st));
         }
         UNO_HIP(hipMemcpyAsync(hs + 2, c->scal.p + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -438,6 +513,14 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
             for (int s = 0; s < ns; ++s) UNO_HIP(k_axpby(nn, 0.5, P[s], 0.5, b.Pold[s].p, P[s], st));
         if (int rc = update()) return rc;
         if (converged()) { conv = true; break; }
+    }
+    // orbital energies and vectors of the last Fock matrix where its density came from
+    // purification (the density itself is not touched)
+    {
+        double* Flast[2] = {b.F.p, b.Etmp.p};
+        for (int s = 0; s < ns; ++s)
+            if (last_sp2[s])
+                if (int rc = diag_to_P(Flast[s], s, nocc[s], false)) return rc;
     }
     // kinetic energy and outputs
     UNO_HIP(k_fill(1, 0.0, c->scal.p, st));
