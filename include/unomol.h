@@ -277,6 +277,13 @@ int uno_guess_samples(const uno_basis* basis, const double* P, long cap, float* 
 int uno_guess_assemble(const uno_basis* basis, long count, const int* pairs, const float* Y, double* P);
 /* Network used by uno_scf_run with guess = 2 (not owned; NULL clears). */
 int uno_ctx_set_guess(uno_ctx* ctx, uno_mlp* model);
+/* Pulay DIIS coefficients (accel = 2; no reference counterpart: the reference damps).
+ * B[i*K + j] = <e_i, e_j> over a ring of K slots, `filled` in use, `newest` just written;
+ * coef[K] minimises |sum c_i e_i| with sum c_i = 1 over the most recent vectors whose
+ * differences from the newest are linearly independent (older ones are dropped until they
+ * are); *used = the number of vectors kept (1: coef selects the newest alone). */
+int uno_diis_coefficients(const double* B, int K, int filled, int newest, double* coef, int* used);
+
 /* Debug/parity: the (ncart x ncart) map of one shell's Cartesian functions under a
  * rotation Q of the molecule (row-major 3x3; l <= 2): phi'_mu = sum_nu M_mu,nu phi_nu. */
 int uno_shell_rotation(int l, const double* Q, double* M);
@@ -286,11 +293,22 @@ int uno_shell_rotation(int l, const double* Q, double* M);
  * and open-shell mixed (odd electron counts run UHF), iterated in lockstep on the
  * device: one ERI launch per class pair across all molecules, grouped GEMMs and a
  * batched Jacobi eigensolver for the dense steps, Pulay DIIS per molecule.  FP64 full
- * builds from the core-Hamiltonian guess; opts->cflag / eps / maxits as for uno_scf_run. */
+ * builds from the core-Hamiltonian guess; opts->cflag / eps / maxits as for uno_scf_run.
+ * The Fock build's quartet lists are built once per run against a density bound (twice each
+ * molecule's largest shell-block density; rebuilt if a density outgrows it) with the
+ * costliest bras first, the launches skip converged molecules by a device flag (the wave
+ * classes take their work from a device queue), and the whole build is captured once as a
+ * HIP graph and replayed every iteration. */
 typedef struct uno_batch uno_batch;
 int uno_batch_create(int nmol, const uno_basis* bases, const int* nelec, uno_batch** out);
 int uno_batch_destroy(uno_batch* batch);
 int uno_batch_nmol(const uno_batch* batch);
+/* Options of the batched Fock build: "graph" (1: capture and replay, 0: plain launches),
+ * "queue" (0: one block per task; 1: the one-quartet-per-wave classes take their quartet
+ * blocks from a device work queue; 2: the lane classes' 64-ket tasks as well).  uno_batch_get also reads the
+ * last run's "space_builds", "graph_builds", "graph_replays" and "launches_per_build". */
+int uno_batch_set(uno_batch* batch, const char* key, double value);
+int uno_batch_get(const uno_batch* batch, const char* key, double* value);
 /* The final density of molecule m after uno_batch_run, dense row-major N x N: the RHF half
  * density, or alpha then beta (2 N x N) for a UHF molecule. */
 int uno_batch_density(const uno_batch* batch, int m, double* P);

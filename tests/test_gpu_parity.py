@@ -752,6 +752,40 @@ def test_batched_scf_matches_single(tmp_path):
         e1 = hf.findEnergy(maxits=100, eps=1e-10, cflag=3, accel=2)
         assert res[i].converged and hf.result.converged
         assert abs(e[i] - e1) < 1e-8, (i, p, e[i], e1)
+    assert batch.get("graph_builds") >= 1 and batch.get("graph_replays") >= batch.get("graph_builds")
+
+
+def test_batched_graph_and_queue_match_plain_launches(tmp_path):
+    """The batched Fock build captured as a HIP graph and replayed, with work-queue
+    launches, against plain launches of the same static quartet lists (no graph, no queue):
+    the same energies, iteration counts (to one) and densities on every molecule; the graph replays
+    every build of a run and the second run of the same batch (a fresh capture) agrees with
+    the first."""
+    say("batched graph/queue vs plain: 16 mixed molecules")
+    paths = _batch_decks(tmp_path, 16, 11)
+    runs = []
+    for graph, queue in ((1, 1), (1, 1), (0, 0), (0, 2)):
+        batch = unomol_amd.Batch(paths) if not runs or graph == 0 and queue == 0 else batch
+        batch.set("graph", graph)
+        batch.set("queue", queue)
+        e, res = batch.run(maxits=100, eps=1e-10, cflag=2, accel=2)
+        say("  graph %d queue %d: builds %d replays %d spaces %d launches/build %d" %
+            (graph, queue, batch.get("graph_builds"), batch.get("graph_replays"), batch.get("space_builds"),
+             batch.get("launches_per_build")))
+        assert all(r.converged for r in res)
+        if graph:
+            assert batch.get("graph_builds") == batch.get("space_builds")
+            assert batch.get("graph_replays") >= max(r.iterations for r in res)
+        else:
+            assert batch.get("graph_builds") == 0
+        runs.append((e, [r.iterations for r in res], [batch.density(i) for i in range(len(paths))]))
+    e0, it0, P0 = runs[0]
+    for e, it, P in runs[1:]:
+        assert np.abs(e - e0).max() < 1e-9
+        # atomics add in a different order from run to run: a molecule at its threshold may
+        # take one iteration more or fewer
+        assert max(abs(a - b) for a, b in zip(it, it0)) <= 1
+        assert max(np.abs(a - b).max() for a, b in zip(P, P0)) < 1e-7
 
 
 # (the open-shell atoms b.dhdz / f.dhdz are left out: their unpaired electron sits along

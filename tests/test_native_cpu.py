@@ -200,6 +200,58 @@ def _dense(p, n):
     return A + np.tril(A, -1).T
 
 
+def _diis_ring(E, K, newest):
+    """Error vectors E (rows, oldest first) placed in a ring of K slots ending at `newest`:
+    the ring's B matrix."""
+    nd = len(E)
+    slots = [(newest - nd + 1 + i) % K for i in range(nd)]
+    B = np.zeros((K, K))
+    for a, sa in enumerate(slots):
+        for b, sb in enumerate(slots):
+            B[sa, sb] = E[a] @ E[b]
+    return B, slots
+
+
+def test_diis_coefficients_minimise_the_error():
+    """Well-conditioned histories: the coefficients sum to one and satisfy the Lagrange
+    conditions of min |sum c_i e_i| (E^T E c equal in every component), i.e. the bordered
+    system solved by numpy; in a wrapped ring too."""
+    rng = np.random.default_rng(3)
+    for K, nd, newest in ((8, 2, 1), (8, 5, 4), (8, 8, 2), (6, 6, 5)):
+        E = rng.standard_normal((nd, 30)) * np.logspace(0, -3, nd)[:, None]
+        B, slots = _diis_ring(E, K, newest)
+        c, used = unomol_amd.diis_coefficients(B, nd, newest)
+        assert used == nd
+        A = np.zeros((nd + 1, nd + 1))
+        A[:nd, :nd] = E @ E.T
+        A[:nd, nd] = A[nd, :nd] = -1.0
+        rhs = np.zeros(nd + 1)
+        rhs[nd] = -1.0
+        ref = np.linalg.solve(A, rhs)[:nd]
+        np.testing.assert_allclose(c[slots], ref, rtol=1e-9, atol=1e-12)
+        assert abs(c.sum() - 1.0) < 1e-13
+        assert np.count_nonzero(c) == nd
+        g = E @ (E.T @ c[slots])
+        assert np.ptp(g) < 1e-9 * np.abs(g).max()
+
+
+def test_diis_coefficients_drop_dependent_history():
+    """Error vectors that are all parallel (H2: one occupied-virtual rotation): with three
+    or more of them the minimiser is not unique, so the two newest are used and their
+    combination vanishes; identical vectors leave the newest Fock matrix alone."""
+    u = np.random.default_rng(4).standard_normal(12)
+    a = np.array([3e-2, -4e-3, 5e-4, 7e-5])
+    B, slots = _diis_ring(a[:, None] * u[None, :], 8, 3)
+    c, used = unomol_amd.diis_coefficients(B, 4, 3)
+    assert used == 2
+    assert np.count_nonzero(c) == 2 and c[3] != 0 and c[2] != 0
+    assert abs(c.sum() - 1.0) < 1e-13
+    assert abs(c[2] * a[2] + c[3] * a[3]) < 1e-14
+    B1, _ = _diis_ring(np.tile(u, (3, 1)), 8, 2)
+    c1, used1 = unomol_amd.diis_coefficients(B1, 3, 2)
+    assert used1 == 1 and c1[2] == 1.0 and np.count_nonzero(c1) == 1
+
+
 def test_shell_rotation_is_a_representation():
     Q1, Q2 = _rot(1), _rot(2)
     for l in range(3):

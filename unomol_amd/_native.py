@@ -59,7 +59,8 @@ EXPORTS = ["uno_last_error", "uno_ctx_create", "uno_ctx_rebuild", "uno_ctx_destr
            "uno_mlp_create", "uno_mlp_destroy", "uno_mlp_nparams", "uno_mlp_dims", "uno_mlp_get_params",
            "uno_mlp_set_params", "uno_mlp_forward", "uno_mlp_loss_grad", "uno_mlp_train", "uno_mlp_save",
            "uno_mlp_load", "uno_guess_features", "uno_guess_samples", "uno_guess_assemble", "uno_ctx_set_guess",
-           "uno_shell_rotation", "uno_autotune", "uno_batch_create", "uno_batch_destroy", "uno_batch_nmol", "uno_batch_run", "uno_batch_density"]
+           "uno_shell_rotation", "uno_autotune", "uno_batch_create", "uno_batch_destroy", "uno_batch_nmol", "uno_batch_run", "uno_batch_density",
+           "uno_batch_set", "uno_batch_get", "uno_diis_coefficients"]
 
 
 def lib():

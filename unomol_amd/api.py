@@ -224,6 +224,17 @@ def rys_eval_ref2(x, fp32=False):
     return u, w
 
 
+def diis_coefficients(B, filled, newest):
+    """Pulay DIIS coefficients of the SCF's accel = 2 from the K x K error-overlap ring B:
+    (coefficients[K], number of vectors used)."""
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    K = B.shape[0]
+    c = np.zeros(K)
+    used = C.c_int()
+    N.check(N.lib().uno_diis_coefficients(N.ptr(B), K, int(filled), int(newest), N.ptr(c), C.byref(used)))
+    return c, used.value
+
+
 def oneint(deck):
     b = N.basis_struct(deck)
     n = deck["norb"]
@@ -378,6 +389,18 @@ class Batch:
         e = np.zeros(len(self.decks))
         N.check(N.lib().uno_batch_run(self.h, C.byref(o), res, N.ptr(e)), allow=(N.UNO_ENOCONV,))
         return e, list(res)
+
+    def set(self, key, value):
+        """Fock-build option: "graph" (capture and replay the build) or "queue" (work-queue
+        launches), 1 or 0."""
+        N.check(N.lib().uno_batch_set(self.h, key.encode(), C.c_double(value)))
+
+    def get(self, key):
+        """An option, or a counter of the last run: space_builds, graph_builds, graph_replays,
+        launches_per_build."""
+        v = C.c_double()
+        N.check(N.lib().uno_batch_get(self.h, key.encode(), C.byref(v)))
+        return v.value
 
     def density(self, m):
         """Final density of molecule m: dense (N, N) RHF half density, or (2, N, N) alpha/beta."""

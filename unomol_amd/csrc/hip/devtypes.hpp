@@ -116,6 +116,13 @@ struct EriArgs {
                               // of the per-lane kernels; null: they go straight to W
     unsigned long long* counter;  // evaluated-quartet counters (may be null): kCounterSlots
                                   // (primitive, shell) pairs, summed by the host
+    // work-queue launches (batched builds): a fixed grid whose waves (or one-wave blocks)
+    // take the next work item from *queue (zeroed before the launch) until it passes the
+    // item count (ntask wave tasks, or nqblk quartet blocks of the wave kernel); null: one
+    // item per wave / block as launched
+    unsigned long long* queue;
+    long long nqblk;
+    const int* mol_active;    // per molecule: 0 skips its items (null: every molecule)
 };
 
 }  // namespace uno

@@ -1,6 +1,7 @@
 // ERI + fused J/K kernels (one instantiation per quartet class and digest mode) and the
 // host-side launcher.  See eri_core.hpp for the mathematics and execution shapes.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdio>
 #include "eri_core.hpp"
 #include "kernels.hpp"
@@ -161,22 +162,10 @@ __device__ __forceinline__ void jket_add(const EriArgs& a, const MolView& mv, co
 }
 
 template <int LA, int LB, int LC, int LD, int MODE, typename T>
-__global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
+__device__ __forceinline__ void bra_task(const EriArgs& a, const long long t, const T* tab) {
     using CI = ClassInfo<LA, LB, LC, LD>;
     constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND, NCOMP = CI::NCOMP;
     const int lane = threadIdx.x & 63;
-    const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    // the Rys table of this root count, copied to LDS by the block (up to 35 KB)
-    constexpr int NR = CI::NR;
-    constexpr bool kLdsTab = NR <= kLdsTabMaxRoots;
-    __shared__ T tab_lds[kLdsTab ? rys_table_size<NR, T>() : 1];
-    const T* tab = rys_table_of<NR, T>(a);
-    if constexpr (kLdsTab) {
-        for (int i = threadIdx.x; i < rys_table_size<NR, T>(); i += blockDim.x) tab_lds[i] = tab[i];
-        __syncthreads();
-        tab = tab_lds;
-    }
-    if (t >= a.ntask) return;   // whole wave
     // bra of task t: the largest i with wpre[i] <= t, bracketed by the task block table
     int lo = a.tblk[t >> 6], hi = a.tblk[(t >> 6) + 1];
     while (lo < hi) {
@@ -184,6 +173,7 @@ __global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
         if (a.wpre[mid] <= t) lo = mid; else hi = mid - 1;
     }
     const int bra = __builtin_amdgcn_readfirstlane(lo);   // entry of this task
+    if (a.mol_active && !a.mol_active[a.e_mol[bra]]) return;   // a molecule already converged
     const int chunk = __builtin_amdgcn_readfirstlane((int)(t - a.wpre[bra]));
     const long long cnt = a.prefix[bra + 1] - a.prefix[bra];
     const int jk = chunk * 64 + lane;
@@ -331,7 +321,37 @@ __global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
                     for (int k = 0; k < NC; ++k) s = fma(e[((i * NB + j) * NC + k) * ND + l], dac[i * NC + k], s);
                 atomic_add_f64(W + (long)(fb + j) * n + fd + l, (double)(-s));
             }
+    }}
+
+template <int LA, int LB, int LC, int LD, int MODE, typename T>
+__global__ __launch_bounds__(256) void k_eri_bra(const EriArgs a) {
+    using CI = ClassInfo<LA, LB, LC, LD>;
+    const int lane = threadIdx.x & 63;
+    // the Rys table of this root count, copied to LDS by the block (up to 35 KB)
+    constexpr int NR = CI::NR;
+    constexpr bool kLdsTab = NR <= kLdsTabMaxRoots;
+    __shared__ T tab_lds[kLdsTab ? rys_table_size<NR, T>() : 1];
+    const T* tab = rys_table_of<NR, T>(a);
+    if constexpr (kLdsTab) {
+        for (int i = threadIdx.x; i < rys_table_size<NR, T>(); i += blockDim.x) tab_lds[i] = tab[i];
+        __syncthreads();
+        tab = tab_lds;
     }
+    if (a.queue) {
+        // work queue: each wave takes the next task until none is left (every wave ends
+        // after at most one grab past the end)
+        for (;;) {
+            unsigned long long g = 0;
+            if (lane == 0) g = atomicAdd(a.queue, 1ULL);
+            const long long t = (long long)__shfl(g, 0, 64);
+            if (t >= a.ntask) break;
+            bra_task<LA, LB, LC, LD, MODE, T>(a, t, tab);
+        }
+        return;
+    }
+    const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (t >= a.ntask) return;   // whole wave
+    bra_task<LA, LB, LC, LD, MODE, T>(a, t, tab);
 }
 
 // ---------------------------------------------------------------------------
@@ -851,23 +871,35 @@ constexpr int wave_min_waves() {
 }
 template <int LA, int LB, int LC, int LD, int MODE, typename T>
 __global__ __launch_bounds__(64, (wave_min_waves<LA, LB, LC, LD>())) void k_eri_wave(const EriArgs a) {
-    const long long g0 = (long long)blockIdx.x * a.qpb;
     const int lane = threadIdx.x;
-    int pab = 0, pcd = 0, entry = 0;
-    double q12 = -1.0;
-    bool live = lane < a.qpb && quartet_of(a, g0 + lane, pab, pcd, q12, entry);
-    if (MODE <= kDigestUHF && live && q12 >= 0.0)
-        live = !screened(a, mol_view(a, entry), a.pairs[pab], a.pairs[pcd], q12);
-    unsigned long long todo = __ballot(live);
     unsigned long long nprim = 0, nshell = 0;
-    while (todo) {
-        const int src = __builtin_ctzll(todo);
-        todo &= todo - 1;
-        const int qab = __builtin_amdgcn_readlane(pab, src), qcd = __builtin_amdgcn_readlane(pcd, src);
-        const int qe = __builtin_amdgcn_readlane(entry, src);
-        nprim += wave_one<LA, LB, LC, LD, MODE, T>(a, g0 + src, qab, qcd, qe);
-        ++nshell;
-        __syncthreads();
+    // one quartet block per launched block, or (work queue) blocks taken one after another
+    // until none is left (every block ends after at most one grab past the end)
+    for (long long item = blockIdx.x;;) {
+        if (a.queue) {
+            unsigned long long g = 0;
+            if (lane == 0) g = atomicAdd(a.queue, 1ULL);
+            item = (long long)__shfl(g, 0, 64);
+            if (item >= a.nqblk) break;
+        }
+        const long long g0 = item * a.qpb;
+        int pab = 0, pcd = 0, entry = 0;
+        double q12 = -1.0;
+        bool live = lane < a.qpb && quartet_of(a, g0 + lane, pab, pcd, q12, entry);
+        if (live && a.mol_active && !a.mol_active[a.e_mol[entry]]) live = false;
+        if (MODE <= kDigestUHF && live && q12 >= 0.0)
+            live = !screened(a, mol_view(a, entry), a.pairs[pab], a.pairs[pcd], q12);
+        unsigned long long todo = __ballot(live);
+        while (todo) {
+            const int src = __builtin_ctzll(todo);
+            todo &= todo - 1;
+            const int qab = __builtin_amdgcn_readlane(pab, src), qcd = __builtin_amdgcn_readlane(pcd, src);
+            const int qe = __builtin_amdgcn_readlane(entry, src);
+            nprim += wave_one<LA, LB, LC, LD, MODE, T>(a, g0 + src, qab, qcd, qe);
+            ++nshell;
+            __syncthreads();
+        }
+        if (!a.queue) break;
     }
     if (a.counter && lane == 0 && nshell) {
         unsigned long long* slot = a.counter + 2 * (blockIdx.x & (kCounterSlots - 1));
@@ -884,6 +916,9 @@ __global__ __launch_bounds__(64, (wave_min_waves<LA, LB, LC, LD>())) void k_eri_
 // registers); larger ones one quartet per wave
 static constexpr int kLaneMaxComp = 162;
 static constexpr int quartets_per_block(int ncomp) { return ncomp <= kLaneMaxComp ? 256 : 8; }
+// work-queue launches: 256-thread blocks for the wave tasks (eight per CU on 256 CUs; the
+// wave kernel's one-wave blocks four times as many)
+static constexpr long long kQueueBlocks = 2048;
 
 template <int LA, int LB, int LC, int LD, int MODE, typename T = double>
 static hipError_t launch_cls(const EriArgs& a, hipStream_t st) {
@@ -895,12 +930,18 @@ static hipError_t launch_cls(const EriArgs& a, hipStream_t st) {
     if constexpr (CI::NCOMP <= kLaneMaxComp && MODE <= kDigestUHF) {
         // Fock builds: wave tasks (bra, chunk of 64 kets), four per block
         if (a.ntask <= 0 || !a.wpre || !a.tblk) return hipErrorInvalidValue;
-        const long long nt = (a.ntask + 3) / 4;
+        long long nt = (a.ntask + 3) / 4;
+        if (a.queue) nt = std::min<long long>(nt, kQueueBlocks);
         hipLaunchKernelGGL((k_eri_bra<LA, LB, LC, LD, MODE, T>), dim3((unsigned)nt), dim3(256), 0, st, b);
     } else if constexpr (CI::NCOMP <= kLaneMaxComp) {
         hipLaunchKernelGGL((k_eri_lane<LA, LB, LC, LD, MODE, T>), dim3((unsigned)nb), dim3(256), 0, st, b);
     } else {
-        hipLaunchKernelGGL((k_eri_wave<LA, LB, LC, LD, MODE, T>), dim3((unsigned)nb), dim3(64), 0, st, b);
+        long long grid = nb;
+        if (a.queue) {
+            b.nqblk = nb;
+            grid = std::min<long long>(nb, 4 * kQueueBlocks);
+        }
+        hipLaunchKernelGGL((k_eri_wave<LA, LB, LC, LD, MODE, T>), dim3((unsigned)grid), dim3(64), 0, st, b);
     }
     return hipGetLastError();
 }

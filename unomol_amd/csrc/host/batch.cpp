@@ -44,29 +44,6 @@ constexpr int kBatchDiis = 8;
 using clk = std::chrono::steady_clock;
 double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
 
-bool solve_small(int m, std::vector<double> A, std::vector<double>& b) {
-    for (int k = 0; k < m; ++k) {
-        int piv = k;
-        for (int i = k + 1; i < m; ++i)
-            if (std::fabs(A[i * m + k]) > std::fabs(A[piv * m + k])) piv = i;
-        if (A[piv * m + k] == 0.0) return false;
-        if (piv != k) {
-            for (int j = 0; j < m; ++j) std::swap(A[k * m + j], A[piv * m + j]);
-            std::swap(b[k], b[piv]);
-        }
-        for (int i = k + 1; i < m; ++i) {
-            const double f = A[i * m + k] / A[k * m + k];
-            for (int j = k; j < m; ++j) A[i * m + j] -= f * A[k * m + j];
-            b[i] -= f * b[k];
-        }
-    }
-    for (int k = m - 1; k >= 0; --k) {
-        double s = b[k];
-        for (int j = k + 1; j < m; ++j) s -= A[k * m + j] * b[j];
-        b[k] = s / A[k * m + k];
-    }
-    return true;
-}
 
 // Device copies of the small per-iteration arrays (GEMM groups and tiles, matrix lists,
 // quartet entries, DIIS coefficients): the k-th upload of an iteration reuses slot k,
@@ -107,7 +84,33 @@ struct uno_batch {
     DBuf<double> dF[kBatchDiis][2], dE[kBatchDiis][2];
     DBuf<double> red;
     Scratch scratch;
+    // the Fock build's static quartet spaces (one per bra class, ket class, open shell),
+    // their work-queue counters, the per-molecule active flags, and the build captured as
+    // a HIP graph (replayed every iteration while the spaces stay)
+    struct Group {
+        int cx, cy, uhf, tasks, qpb, ne;
+        long long nq, ntask;
+        int *bra, *ket, *mol, *tblk, *blk;
+        long long *prefix, *wpre;
+    };
+    std::vector<Group> groups;
+    DBuf<char> gstore;
+    DBuf<unsigned long long> queues;
+    DBuf<int> active_d;
+    std::vector<double> dbound;     // per molecule: the density bound the spaces were built with
+    std::vector<double> mol_quartets;   // per molecule: quartets in the static spaces
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int use_graph = 1, use_queue = 1;
+    int graph_builds = 0, graph_replays = 0, space_builds = 0;
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+    }
     ~uno_batch() {
+        drop_graph();
         for (BatchMol& m : mol)
             if (m.ctx) uno_ctx_destroy(m.ctx);
         if (st) (void)hipStreamDestroy(st);
@@ -172,21 +175,28 @@ int batch_eigh(uno_batch* b, const std::vector<MatSlot>& sl, double* A, double* 
 
 namespace uno {
 
-// batched Fock build on the densities in D0 (D1, D2): W0 (W1) accumulators
-static int batch_fock(uno_batch* b, const std::vector<double>& dmax, double& quartets) {
-    hipStream_t st = b->st;
-    UNO_HIP(k_fill(b->store, 0.0, b->W0.p, st));
-    UNO_HIP(k_fill(b->store, 0.0, b->W1.p, st));
-    // entries grouped by (bra class, ket class, open shell)
+// Static quartet spaces of the batched Fock build: every molecule's screened prefixes built
+// with a density bound (twice the largest |D| block maximum seen so far, per molecule)
+// instead of the current density, so that the launches -- their index arrays, counts and
+// pointers -- stay the same from build to build.  The kernels apply the exact per-shell-block
+// density screen, which selects the same quartets prefixes built at the current density
+// would; a density that exceeds its bound rebuilds the spaces.  Entries are sorted by
+// their ket count (largest first) so that the work queue hands out the costly bras first.
+static int build_static(uno_batch* b, const std::vector<double>& dmax) {
+    b->drop_graph();
+    const int nm = (int)b->mol.size();
+    if ((int)b->dbound.size() != nm) b->dbound.assign(nm, 0.0);
+    for (int m = 0; m < nm; ++m) b->dbound[m] = std::max(2.0 * dmax[m], b->dbound[m]);
     struct Ent { std::vector<int> bra, ket, mol; std::vector<long long> cnt; };
     std::map<std::tuple<int, int, int>, Ent> groups;
-    for (int m = 0; m < (int)b->mol.size(); ++m) {
+    b->mol_quartets.assign(nm, 0.0);
+    for (int m = 0; m < nm; ++m) {
         BatchMol& x = b->mol[m];
-        if (!x.active) continue;
         uno_ctx* c = x.ctx;
         for (QuartetClass& q : c->qcls) {
-            build_prefix(c, q, 4.0 * dmax[m], 0.0, c->tau);
+            build_prefix(c, q, 4.0 * b->dbound[m], 0.0, c->tau);
             if (q.s64.nq == 0) continue;
+            b->mol_quartets[m] += (double)q.s64.nq;
             const PairClassList& LX = c->cls[q.X];
             const PairClassList& LY = c->cls[q.Y];
             Ent& e = groups[std::make_tuple(LX.lcls, LY.lcls, x.uhf ? 1 : 0)];
@@ -200,34 +210,90 @@ static int batch_fock(uno_batch* b, const std::vector<double>& dmax, double& qua
             }
         }
     }
-    // host arrays stay alive until the final synchronisation (their uploads are async)
-    std::vector<QuartetSpace> spaces(groups.size());
-    size_t gi = 0;
+    // host layout of every group's arrays in one buffer (16-byte aligned pieces)
+    std::vector<char> hs;
+    auto put = [&](const auto& v) -> size_t {
+        const size_t off = (hs.size() + 15) / 16 * 16;
+        const size_t bytes = v.size() * sizeof(v[0]);
+        hs.resize(off + std::max<size_t>(bytes, 16));
+        if (bytes) std::memcpy(hs.data() + off, v.data(), bytes);
+        return off;
+    };
+    struct Offs { size_t bra, ket, mol, prefix, wpre, tblk, blk; };
+    std::vector<Offs> offs;
+    b->groups.clear();
     for (auto& kv : groups) {
-        const int cx = std::get<0>(kv.first), cy = std::get<1>(kv.first), uhf = std::get<2>(kv.first);
         Ent& e = kv.second;
         const int ne = (int)e.bra.size();
-        QuartetSpace& sp = spaces[gi++];
+        std::vector<int> order(ne);
+        for (int i = 0; i < ne; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int u, int v) { return e.cnt[u] > e.cnt[v]; });
+        std::vector<int> bra(ne), ket(ne), mol(ne);
+        QuartetSpace sp;
         sp.prefix.assign(ne + 1, 0);
-        for (int i = 0; i < ne; ++i) sp.prefix[i + 1] = sp.prefix[i] + e.cnt[i];
-        sp.nq = sp.prefix[ne];
-        const bool tasks = eri_uses_wave_tasks(cx, cy);
-        const int qpb = eri_quartets_per_block(cx, cy);
-        int *dbra = nullptr, *dket = nullptr, *dmol = nullptr, *dtblk = nullptr, *dblk = nullptr;
-        long long *dprefix = nullptr, *dwpre = nullptr;
-        Scratch& sc = b->scratch;
-        UNO_HIP(sc.upload(e.bra.data(), ne, st, &dbra));
-        UNO_HIP(sc.upload(e.ket.data(), ne, st, &dket));
-        UNO_HIP(sc.upload(e.mol.data(), ne, st, &dmol));
-        UNO_HIP(sc.upload(sp.prefix.data(), sp.prefix.size(), st, &dprefix));
-        if (tasks) {
-            build_tasks(sp);
-            UNO_HIP(sc.upload(sp.wpre.data(), sp.wpre.size(), st, &dwpre));
-            UNO_HIP(sc.upload(sp.tblk.data(), sp.tblk.size(), st, &dtblk));
-        } else {
-            build_blocks(sp, qpb);
-            UNO_HIP(sc.upload(sp.blk.data(), sp.blk.size(), st, &dblk));
+        for (int i = 0; i < ne; ++i) {
+            bra[i] = e.bra[order[i]];
+            ket[i] = e.ket[order[i]];
+            mol[i] = e.mol[order[i]];
+            sp.prefix[i + 1] = sp.prefix[i] + e.cnt[order[i]];
         }
+        sp.nq = sp.prefix[ne];
+        uno_batch::Group g{};
+        g.cx = std::get<0>(kv.first);
+        g.cy = std::get<1>(kv.first);
+        g.uhf = std::get<2>(kv.first);
+        g.tasks = eri_uses_wave_tasks(g.cx, g.cy) ? 1 : 0;
+        g.qpb = eri_quartets_per_block(g.cx, g.cy);
+        g.ne = ne;
+        g.nq = sp.nq;
+        Offs o{};
+        o.bra = put(bra);
+        o.ket = put(ket);
+        o.mol = put(mol);
+        o.prefix = put(sp.prefix);
+        if (g.tasks) {
+            build_tasks(sp);
+            g.ntask = sp.ntask;
+            o.wpre = put(sp.wpre);
+            o.tblk = put(sp.tblk);
+        } else {
+            build_blocks(sp, g.qpb);
+            o.blk = put(sp.blk);
+        }
+        b->groups.push_back(g);
+        offs.push_back(o);
+    }
+    UNO_HIP(b->gstore.alloc(std::max<size_t>(hs.size(), 16)));
+    UNO_HIP(hipMemcpyAsync(b->gstore.p, hs.data(), hs.size(), hipMemcpyHostToDevice, b->st));
+    UNO_HIP(b->queues.alloc(std::max<size_t>(b->groups.size(), 1)));
+    UNO_HIP(b->active_d.alloc(std::max(nm, 1)));
+    for (size_t k = 0; k < b->groups.size(); ++k) {
+        uno_batch::Group& g = b->groups[k];
+        const Offs& o = offs[k];
+        char* base = b->gstore.p;
+        g.bra = reinterpret_cast<int*>(base + o.bra);
+        g.ket = reinterpret_cast<int*>(base + o.ket);
+        g.mol = reinterpret_cast<int*>(base + o.mol);
+        g.prefix = reinterpret_cast<long long*>(base + o.prefix);
+        g.wpre = g.tasks ? reinterpret_cast<long long*>(base + o.wpre) : nullptr;
+        g.tblk = g.tasks ? reinterpret_cast<int*>(base + o.tblk) : nullptr;
+        g.blk = g.tasks ? nullptr : reinterpret_cast<int*>(base + o.blk);
+    }
+    UNO_HIP(hipStreamSynchronize(b->st));   // hs is freed on return
+    ++b->space_builds;
+    return UNO_OK;
+}
+
+// the launches of one build: accumulators and queue counters cleared by fill kernels (no
+// memset nodes when captured), then one launch per static space
+static int enqueue_fock(uno_batch* b) {
+    hipStream_t st = b->st;
+    UNO_HIP(k_fill(b->store, 0.0, b->W0.p, st));
+    UNO_HIP(k_fill(b->store, 0.0, b->W1.p, st));
+    if (b->use_queue && !b->groups.empty())
+        UNO_HIP(k_fill((long)b->groups.size(), 0.0, reinterpret_cast<double*>(b->queues.p), st));
+    for (size_t k = 0; k < b->groups.size(); ++k) {
+        const uno_batch::Group& g = b->groups[k];
         EriArgs a{};
         a.pairs = b->pairs.p;
         a.prims = b->prims.p;
@@ -235,28 +301,72 @@ static int batch_fock(uno_batch* b, const std::vector<double>& dmax, double& qua
         a.tau = b->mol[0].ctx->tau;
         a.primcut = b->mol[0].ctx->primcut;
         a.rys2_exact = b->mol[0].ctx->rys2_exact;
-        a.prefix = dprefix;
-        a.blk = tasks ? nullptr : dblk;
-        a.wpre = tasks ? dwpre : nullptr;
-        a.tblk = tasks ? dtblk : nullptr;
-        a.ntask = tasks ? sp.ntask : 0;
-        a.qpb = qpb;
-        a.nquart = sp.nq;
-        a.e_bra = dbra;
-        a.e_ket = dket;
-        a.e_mol = dmol;
+        a.prefix = g.prefix;
+        a.blk = g.blk;
+        a.wpre = g.wpre;
+        a.tblk = g.tblk;
+        a.ntask = g.tasks ? g.ntask : 0;
+        a.qpb = g.qpb;
+        a.nquart = g.nq;
+        a.e_bra = g.bra;
+        a.e_ket = g.ket;
+        a.e_mol = g.mol;
         a.mols = b->mdesc.p;
         a.dsh = b->Dsh.p;
         a.dmax_global = 0.0;
         a.D0 = b->D0.p;
         a.W0 = b->W0.p;
-        if (uhf) {
+        if (g.uhf) {
             a.D1 = b->D1.p;
             a.D2 = b->D2.p;
             a.W1 = b->W1.p;
         }
-        UNO_HIP(eri_launch(uhf ? kDigestUHF : kDigestRHF, cx, cy, a, st));
-        quartets += (double)sp.nq;
+        // queue 1: the wave-per-quartet classes take their blocks from the queue; queue 2:
+        // the lane classes' 64-ket tasks too (one atomic per task)
+        const bool q = b->use_queue >= 2 || (b->use_queue == 1 && !g.tasks);
+        a.queue = q ? b->queues.p + k : nullptr;
+        a.mol_active = b->active_d.p;
+        UNO_HIP(eri_launch(g.uhf ? kDigestUHF : kDigestRHF, g.cx, g.cy, a, st));
+    }
+    return UNO_OK;
+}
+
+
+// batched Fock build on the densities in D0 (D1, D2): W0 (W1) accumulators.  dmax: each
+// molecule's largest shell-block density maximum.
+static int batch_fock(uno_batch* b, const std::vector<double>& dmax, double& quartets) {
+    hipStream_t st = b->st;
+    const int nm = (int)b->mol.size();
+    bool rebuild = b->groups.empty() && b->space_builds == 0;
+    for (int m = 0; m < nm && !rebuild; ++m)
+        if (b->mol[m].active && dmax[m] > b->dbound[m]) rebuild = true;
+    if (rebuild)
+        if (int rc = build_static(b, dmax)) return rc;
+    std::vector<int> act(nm);
+    for (int m = 0; m < nm; ++m) {
+        act[m] = b->mol[m].active ? 1 : 0;
+        if (act[m]) quartets += b->mol_quartets[m];   // before the kernels' density screen
+    }
+    UNO_HIP(hipMemcpyAsync(b->active_d.p, act.data(), nm * sizeof(int), hipMemcpyHostToDevice, st));
+    if (b->use_graph) {
+        if (!b->gexec) {
+            UNO_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue_fock(b);
+            hipGraph_t g = nullptr;
+            const hipError_t ee = hipStreamEndCapture(st, &g);
+            if (rc || ee != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                if (rc) return rc;
+                UNO_HIP(ee);
+            }
+            b->graph = g;
+            UNO_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+            ++b->graph_builds;
+        }
+        UNO_HIP(hipGraphLaunch(b->gexec, st));
+        ++b->graph_replays;
+    } else {
+        if (int rc = enqueue_fock(b)) return rc;
     }
     UNO_HIP(hipStreamSynchronize(st));
     return UNO_OK;
@@ -359,6 +469,29 @@ int uno_batch_destroy(uno_batch* b) {
 
 int uno_batch_nmol(const uno_batch* b) { return b ? (int)b->mol.size() : 0; }
 
+int uno_batch_set(uno_batch* b, const char* key, double v) {
+    if (!b || !key) { set_error("null argument"); return UNO_EINVAL; }
+    const std::string k(key);
+    if (k == "graph") b->use_graph = (int)v;
+    else if (k == "queue") b->use_queue = (int)v;
+    else { set_error("unknown batch option " + k); return UNO_EINVAL; }
+    b->drop_graph();
+    return UNO_OK;
+}
+
+int uno_batch_get(const uno_batch* b, const char* key, double* v) {
+    if (!b || !key || !v) { set_error("null argument"); return UNO_EINVAL; }
+    const std::string k(key);
+    if (k == "graph") *v = b->use_graph;
+    else if (k == "queue") *v = b->use_queue;
+    else if (k == "space_builds") *v = b->space_builds;
+    else if (k == "graph_builds") *v = b->graph_builds;
+    else if (k == "graph_replays") *v = b->graph_replays;
+    else if (k == "launches_per_build") *v = (double)b->groups.size();
+    else { set_error("unknown batch option " + k); return UNO_EINVAL; }
+    return UNO_OK;
+}
+
 int uno_batch_density(const uno_batch* b, int m, double* P) {
     if (!b || !P || m < 0 || m >= (int)b->mol.size()) { set_error("bad argument"); return UNO_EINVAL; }
     const BatchMol& x = b->mol[m];
@@ -382,6 +515,11 @@ int uno_batch_run(uno_batch* b, const uno_scf_opts* opts, uno_scf_result* res, d
     hipStream_t st = b->st;
     b->scratch.reset();
     b->warm[0] = b->warm[1] = false;
+    // static spaces and the captured build belong to one run (its density bounds)
+    b->drop_graph();
+    b->groups.clear();
+    b->dbound.clear();
+    b->space_builds = b->graph_builds = b->graph_replays = 0;
     const int nm = (int)b->mol.size();
     const long long S = b->store;
     // per-molecule eigenvalue offsets into ev stores: reuse the matrix offsets (n <= n^2)
@@ -601,19 +739,8 @@ int uno_batch_run(uno_batch* b, const uno_scf_opts* opts, uno_scf_result* res, d
             double* Bmm = &Bm[(size_t)m * kBatchDiis * kBatchDiis];
             for (int j = 0; j < ndiis; ++j) Bmm[slot * kBatchDiis + j] = Bmm[j * kBatchDiis + slot] = row[(size_t)m * kBatchDiis + j];
             coef[(size_t)k * kBatchDiis + slot] = 1.0;
-            if (ndiis >= 2) {
-                const int q = ndiis + 1;
-                std::vector<double> A(q * q, 0.0), rhs(q, 0.0);
-                for (int i = 0; i < ndiis; ++i) {
-                    for (int j = 0; j < ndiis; ++j) A[i * q + j] = Bmm[i * kBatchDiis + j];
-                    A[i * q + ndiis] = A[ndiis * q + i] = -1.0;
-                }
-                rhs[ndiis] = -1.0;
-                if (solve_small(q, A, rhs)) {
-                    for (int j = 0; j < kBatchDiis; ++j) coef[(size_t)k * kBatchDiis + j] = j < ndiis ? rhs[j] : 0.0;
-                    any = true;
-                }
-            }
+            if (ndiis >= 2 && diis_coefficients(Bmm, kBatchDiis, ndiis, slot, &coef[(size_t)k * kBatchDiis]) >= 2)
+                any = true;
         }
         if (any) {
             for (int s = 0; s < 2; ++s) {

@@ -210,6 +210,12 @@ int eigh_standalone(int n, double* dA, double* dZ, double* h_evals, hipStream_t 
 // the same solve on one caller-provided device work space of eigh_ws_doubles(n) doubles
 // (256-byte aligned) and host arrays hev[n], hcl[n]: no allocation
 size_t eigh_ws_doubles(int n);
+
+// Pulay DIIS coefficients over a ring of K slots (`filled` in use, `newest` just written)
+// from B[i*K + j] = <e_i, e_j>; the oldest vectors are dropped while the differences from
+// the newest are near linearly dependent.  Returns the number of vectors used (1: coef
+// selects the newest alone).
+int diis_coefficients(const double* B, int K, int filled, int newest, double* coef);
 int eigh_in_workspace(int n, double* dA, double* dZ, double* h_evals, double* ws, double* hev, int* hcl,
                       hipStream_t st);
 

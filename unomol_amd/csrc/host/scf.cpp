@@ -34,34 +34,10 @@ namespace {
 using clk = std::chrono::steady_clock;
 double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
 
-// small dense solve (Gaussian elimination with partial pivoting) for DIIS
-bool solve_small(int m, std::vector<double> A, std::vector<double>& b) {
-    for (int k = 0; k < m; ++k) {
-        int piv = k;
-        for (int i = k + 1; i < m; ++i)
-            if (std::fabs(A[i * m + k]) > std::fabs(A[piv * m + k])) piv = i;
-        if (A[piv * m + k] == 0.0) return false;
-        if (piv != k) {
-            for (int j = 0; j < m; ++j) std::swap(A[k * m + j], A[piv * m + j]);
-            std::swap(b[k], b[piv]);
-        }
-        for (int i = k + 1; i < m; ++i) {
-            const double f = A[i * m + k] / A[k * m + k];
-            for (int j = k; j < m; ++j) A[i * m + j] -= f * A[k * m + j];
-            b[i] -= f * b[k];
-        }
-    }
-    for (int k = m - 1; k >= 0; --k) {
-        double s = b[k];
-        for (int j = k + 1; j < m; ++j) s -= A[k * m + j] * b[j];
-        b[k] = s / A[k * m + k];
-    }
-    return true;
-}
 }  // namespace
 
 struct ScfBufs {
-    DBuf<double> H, T, S, X, F, Fp, Tm, Cw, G[2], C[2], Pold[2], diisF, diisE, Etmp, P[2], Pref[2];
+    DBuf<double> H, T, S, X, F, Fp, Tm, Cw, G[2], C[2], Pold[2], diisF, diisE, diisRow, Etmp, P[2], Pref[2];
 };
 
 constexpr int kRebuild = 10;            // incremental builds between full rebuilds
@@ -406,7 +382,12 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
         }
         UNO_HIP(hipMemcpyAsync(hs, c->scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
         UNO_HIP(hipStreamSynchronize(st));
-        fock_s += secs(t0, clk::now());
+        const double build_s = secs(t0, clk::now());
+        fock_s += build_s;
+        if (o.verbose > 1)
+            std::fprintf(stderr, "[scf] build %d: %s %s, %.2f ms, %.3e shell quartets (%.3e FP32)\n", r.nfock + 1,
+                         full ? "full" : "incremental", mixed ? "mixed" : "FP64", 1e3 * build_s, c->last_quartets,
+                         c->last_quartets32);
         r.quartets += c->last_quartets;
         ++r.nfock;
         energy = uhf ? hs[0] + 0.5 * hs[1] : 2.0 * hs[0] + hs[1];
@@ -434,7 +415,7 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
             }
             // B matrix row/column of the new slot
             std::vector<double> row(kDiis, 0.0);
-            DBuf<double> dr;
+            DBuf<double>& dr = b.diisRow;
             UNO_HIP(dr.alloc(kDiis));
             UNO_HIP(k_fill(kDiis, 0.0, dr.p, st));
             for (int j = 0; j < ndiis; ++j)
@@ -445,18 +426,13 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
             UNO_HIP(hipStreamSynchronize(st));
             for (int j = 0; j < ndiis; ++j) Bm[slot * kDiis + j] = Bm[j * kDiis + slot] = row[j];
             if (ndiis >= 2) {
-                const int m = ndiis + 1;
-                std::vector<double> A(m * m, 0.0), rhs(m, 0.0);
-                for (int i = 0; i < ndiis; ++i) {
-                    for (int j = 0; j < ndiis; ++j) A[i * m + j] = Bm[i * kDiis + j];
-                    A[i * m + ndiis] = A[ndiis * m + i] = -1.0;
-                }
-                rhs[ndiis] = -1.0;
-                if (solve_small(m, A, rhs)) {
+                double cf[kDiis];
+                if (diis_coefficients(Bm.data(), kDiis, ndiis, slot, cf) >= 2) {
                     for (int s = 0; s < ns; ++s) {
                         UNO_HIP(k_fill(nn, 0.0, Fs[s], st));
-                        for (int i = 0; i < ndiis; ++i)
-                            UNO_HIP(k_axpby(nn, 1.0, Fs[s], rhs[i], b.diisF.p + ((size_t)i * ns + s) * nn, Fs[s], st));
+                        for (int i = 0; i < kDiis; ++i)
+                            if (cf[i] != 0.0)
+                                UNO_HIP(k_axpby(nn, 1.0, Fs[s], cf[i], b.diisF.p + ((size_t)i * ns + s) * nn, Fs[s], st));
                     }
                 }
             }

@@ -13,10 +13,10 @@
         ``--copy-bytes`` each way), and the kernel's bytes per dispatch scaled by
         (copy bytes / copy reading).
 
-    python -m unomol_amd.profsum flops <dir>/run_counter_collection.csv <bench.json>
-        per ERI kernel of the bench's timed step: executed VALU FLOPs from
-        SQ_INSTS_VALU_FLOPS_FP64 / _FP32 (plus the _TRANS counters) over the kernel's last
-        L dispatches (L = the step's launches of that class) against the bench's
+    python -m unomol_amd.profsum flops <dir>/run_counter_collection.csv <bench stderr>
+        per ERI kernel of the bench's timed step (exact kernel names): executed VALU FLOPs
+        from SQ_INSTS_VALU_FLOPS_FP64 / _FP32 (plus the _TRANS counters) over the kernel's
+        last L dispatches (L = the step's launches of that kernel) against the bench's
         algorithmic FLOPs for the same launches.
 
 All print Markdown; ``pmc`` ends with one JSON line holding the numbers.
@@ -74,31 +74,60 @@ def pmc(counter_csv, counter, kernel, last, copy_bytes):
                       "copy_kib": copy_kib, "scale": scale, "bytes_per_dispatch": kib * 1024.0 * scale}))
 
 
-def flops(counter_csv, bench_json):
-    bench = json.loads(open(bench_json).read().strip().splitlines()[-1])
+def _bench_classes(path):
+    """The per-class table bench.py writes to stderr ('[bench] eri_classes [...]'), or the
+    eri_classes field of a JSON line (bench.py --fock-only)."""
+    for line in open(path):
+        if line.startswith("[bench] eri_classes "):
+            return json.loads(line[len("[bench] eri_classes "):])
+    for line in open(path):
+        line = line.strip()
+        if line.startswith("{"):
+            rec = json.loads(line)
+            if "eri_classes" in rec:
+                return rec["eri_classes"]
+    sys.exit("no eri_classes in %s" % path)
+
+
+def flops(counter_csv, bench_out, lanes=64):
+    """Per kernel (exact demangled name; every (class, precision, launch kind) entry of the
+    bench table names one kernel instantiation): the bench's algorithmic FLOPs of the timed
+    step's launches against the executed FP64/FP32 VALU FLOP counters over the kernel's last
+    L dispatches (L = the step's launches of that kernel), counter x lanes = lane-slot FLOPs
+    (inactive lanes of a wave included)."""
+    classes = _bench_classes(bench_out)
     by_kernel = {}
     for r in csv.DictReader(open(counter_csv)):
         by_kernel.setdefault(r["Kernel_Name"], []).append(r)
-    print("| kernel | class | launches | model GFLOP | counter GFLOP | counter/model |")
+    model = {}
+    for c in classes:
+        m = model.setdefault(c["kernel"], {"gflop": 0.0, "launches": 0, "label": [], "f32": c["precision"] == "f32"})
+        m["gflop"] += c["gflop"]
+        m["launches"] += int(round(c["launches"]))
+        m["label"].append("%s %s k%d" % (c["class"], c["precision"], c.get("kind", 0)))
+    print("| kernel | entries | launches | model GFLOP | counter GFLOP (x%d lanes) | counter/model |" % lanes)
     print("|---|---|---|---|---|---|")
     tot_m = tot_c = 0.0
-    for c in sorted(bench["eri_classes"], key=lambda c: -c["gflop"]):
-        rows = by_kernel.get(c["kernel"], [])
-        f32 = c["precision"] == "f32"
-        names = ("SQ_INSTS_VALU_FLOPS_FP32", "SQ_INSTS_VALU_FLOPS_FP32_TRANS") if f32 else \
-                ("SQ_INSTS_VALU_FLOPS_FP64", "SQ_INSTS_VALU_FLOPS_FP64_TRANS")
-        disp = sorted({int(r["Dispatch_Id"]) for r in rows})[-int(round(c["launches"])):]
-        keep = set(disp)
-        cnt = sum(float(r["Counter_Value"]) for r in rows
-                  if int(r["Dispatch_Id"]) in keep and r["Counter_Name"] in names)
-        if not disp:
+    rows_out = []
+    for name, m in sorted(model.items(), key=lambda kv: -kv[1]["gflop"]):
+        rows = by_kernel.get(name, [])
+        if not rows:
             continue
-        tot_m += c["gflop"]
+        names = ("SQ_INSTS_VALU_FLOPS_FP32", "SQ_INSTS_VALU_FLOPS_FP32_TRANS") if m["f32"] else \
+                ("SQ_INSTS_VALU_FLOPS_FP64", "SQ_INSTS_VALU_FLOPS_FP64_TRANS")
+        disp = sorted({int(r["Dispatch_Id"]) for r in rows})[-m["launches"]:]
+        keep = set(disp)
+        cnt = lanes * sum(float(r["Counter_Value"]) for r in rows
+                          if int(r["Dispatch_Id"]) in keep and r["Counter_Name"] in names)
+        tot_m += m["gflop"]
         tot_c += cnt / 1e9
-        print("| `%s` | %s %s | %d | %.3f | %.3f | %.3f |" % (c["kernel"], c["class"], c["precision"], len(disp),
-                                                          c["gflop"], cnt / 1e9, cnt / 1e9 / max(c["gflop"], 1e-30)))
+        rows_out.append({"kernel": name, "model_gflop": m["gflop"], "counter_gflop": cnt / 1e9,
+                         "launches": len(disp)})
+        print("| `%s` | %s | %d | %.3f | %.3f | %.3f |" % (name, ", ".join(m["label"]), len(disp), m["gflop"],
+                                                         cnt / 1e9, cnt / 1e9 / max(m["gflop"], 1e-30)))
     print()
     print("all ERI kernels: model %.2f GFLOP, counters %.2f GFLOP, ratio %.3f" % (tot_m, tot_c, tot_c / max(tot_m, 1e-30)))
+    print(json.dumps({"model_gflop": tot_m, "counter_gflop": tot_c, "kernels": rows_out[:12]}))
 
 
 def main(argv=None):
@@ -117,12 +146,13 @@ def main(argv=None):
     p.add_argument("--copy-bytes", type=float, default=float(1 << 30))
     f = sub.add_parser("flops")
     f.add_argument("counter_csv")
-    f.add_argument("bench_json")
+    f.add_argument("bench_out")
+    f.add_argument("--lanes", type=int, default=64)
     a = ap.parse_args(argv)
     if a.cmd == "trace":
         trace(a.stats_csv, a.trace_csv, a.kernel, a.last)
     elif a.cmd == "flops":
-        flops(a.counter_csv, a.bench_json)
+        flops(a.counter_csv, a.bench_out, a.lanes)
     else:
         pmc(a.counter_csv, a.counter, a.kernel, a.last, a.copy_bytes)
 
