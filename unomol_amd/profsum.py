@@ -19,6 +19,12 @@
         last L dispatches (L = the step's launches of that kernel) against the bench's
         algorithmic FLOPs for the same launches.
 
+    python -m unomol_amd.profsum bytes <fetch counters csv> <write counters csv> \
+        <dir>/run_kernel_trace.csv --match k_gemm --match k_sp2_step
+        per kernel whose name contains one of the --match strings: dispatches, FETCH_SIZE
+        and WRITE_SIZE bytes per dispatch (each scaled by its copy-kernel reading), the
+        average duration from a kernel trace of the same command, and the HBM rate.
+
 All print Markdown; ``pmc`` ends with one JSON line holding the numbers.
 """
 import argparse
@@ -130,6 +136,47 @@ def flops(counter_csv, bench_out, lanes=64):
     print(json.dumps({"model_gflop": tot_m, "counter_gflop": tot_c, "kernels": rows_out[:12]}))
 
 
+def _copy_scale(rows, copy_bytes):
+    cr = [r for r in rows if COPY_KERNEL in r["Kernel_Name"]]
+    if not cr:
+        sys.exit("no copy-kernel dispatch in the counter file")
+    return copy_bytes / (sum(float(r["Counter_Value"]) for r in cr) / len(cr) * 1024.0)
+
+
+def hbm_bytes(fetch_csv, write_csv, trace_csv, match, copy_bytes, peak_gbps=8000.0):
+    """Per-kernel HBM bytes (scaled counters) and rate against the average duration."""
+    per = {}
+    for path, counter in ((fetch_csv, "FETCH_SIZE"), (write_csv, "WRITE_SIZE")):
+        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+        scale = _copy_scale(rows, copy_bytes)
+        for r in rows:
+            name = r["Kernel_Name"]
+            if not any(m in name for m in match):
+                continue
+            e = per.setdefault(name, {})
+            e.setdefault(counter, []).append(float(r["Counter_Value"]) * 1024.0 * scale)
+    dur = {}
+    for r in csv.DictReader(open(trace_csv)):
+        name = r["Kernel_Name"]
+        if any(m in name for m in match):
+            dur.setdefault(name, []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9)
+    print("| kernel | dispatches | fetched B / dispatch | written B / dispatch | avg us | GB/s | of 8 TB/s |")
+    print("|---|---|---|---|---|---|---|")
+    out = []
+    for name, e in sorted(per.items()):
+        f = e.get("FETCH_SIZE", [0.0])
+        w = e.get("WRITE_SIZE", [0.0])
+        fb, wb = sum(f) / len(f), sum(w) / len(w)
+        d = dur.get(name)
+        avg = sum(d) / len(d) if d else float("nan")
+        rate = (fb + wb) / avg / 1e9 if d else float("nan")
+        print("| `%s` | %d | %.4e | %.4e | %.1f | %.1f | %.3f |" % (name[:90], len(f), fb, wb, avg * 1e6, rate,
+                                                                rate / peak_gbps))
+        out.append({"kernel": name, "dispatches": len(f), "fetch_bytes": fb, "write_bytes": wb, "avg_s": avg,
+                    "GBps": rate})
+    print(json.dumps(out))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -148,8 +195,16 @@ def main(argv=None):
     f.add_argument("counter_csv")
     f.add_argument("bench_out")
     f.add_argument("--lanes", type=int, default=64)
+    hb = sub.add_parser("bytes")
+    hb.add_argument("fetch_csv")
+    hb.add_argument("write_csv")
+    hb.add_argument("trace_csv")
+    hb.add_argument("--match", action="append", required=True)
+    hb.add_argument("--copy-bytes", type=float, default=float(1 << 30))
     a = ap.parse_args(argv)
-    if a.cmd == "trace":
+    if a.cmd == "bytes":
+        hbm_bytes(a.fetch_csv, a.write_csv, a.trace_csv, a.match, a.copy_bytes)
+    elif a.cmd == "trace":
         trace(a.stats_csv, a.trace_csv, a.kernel, a.last)
     elif a.cmd == "flops":
         flops(a.counter_csv, a.bench_out, a.lanes)
