@@ -63,7 +63,7 @@ int uno_ctx_rebuild(uno_ctx* ctx, const uno_basis* basis);
 int uno_ctx_destroy(uno_ctx* ctx);
 /* Tuning knobs (no reference counterpart): "schwarz" (quartet screening threshold on
  * Qab*Qcd*max|D|, default 1e-13), "schwarz_mixed" (the threshold of uno_scf_run's
- * mixed-precision phase, never below "schwarz"; default 1e-10: that phase ends with a
+ * mixed-precision phase, never below "schwarz"; default 3e-9: that phase ends with a
  * full FP64 rebuild at "schwarz"), "fp32_threshold" (quartets whose bound is below it
  * run in FP32 when "precision" is 1; default 1e-6), "precision" (0 = FP64, 1 = mixed, for the
  * builds of uno_fock_rhf/uno_fock_uhf; uno_scf_run takes its own option),

@@ -499,16 +499,16 @@ def test_scf_mixed_precision_energy(tmp_path):
     assert abs(e - e64) < 1e-8
     ev64 = ref.evals.copy()
     np.testing.assert_allclose(hf.evals, ev64, atol=1e-6)
-    # the mixed phase screens at its own threshold (default 1e-10); at the context's own
+    # the mixed phase screens at its own threshold (default 3e-9); at the context's own
     # threshold (1e-13) it evaluates more quartets and lands on the same energy
-    assert t.get("schwarz_mixed") == 1e-10
+    assert t.get("schwarz_mixed") == 3e-9
     t.set("schwarz_mixed", 1e-13)
     hf2 = unomol_amd.RestrictedHartreeFock(b, t)
     e2 = hf2.findEnergy(accel=2, eps=1e-9, cflag=3, precision=1, incremental=1)
     assert hf2.result.converged
     assert hf2.result.quartets > r.quartets
     assert abs(e2 - e64) < 1e-8
-    t.set("schwarz_mixed", 1e-10)
+    t.set("schwarz_mixed", 3e-9)
 
 
 def test_scf_mixed_precision_energy_uhf(tmp_path):

@@ -122,7 +122,7 @@ struct uno_ctx {
     double tau = 1e-13, primcut = 1e-12, fp32_thr = 1e-6;
     // screening threshold of the mixed-precision builds (never below tau): their error
     // only steers the early iterations; the FP64 phase starts with a full rebuild at tau
-    double tau_mixed = 1e-10;
+    double tau_mixed = 3e-9;
     int precision = 0;
     // pair data
     std::vector<uno::PairHdr> pairs;
