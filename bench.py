@@ -379,7 +379,7 @@ def main():
         return
 
     model, model_info = (None, None) if args.fp64_only else get_model(unomol_amd)
-    fast_opts = dict(fp64_opts, precision=1, incremental=1, guess=2, fp64_final=3)
+    fast_opts = dict(fp64_opts, precision=1, incremental=1, guess=2, fp64_final=2)
 
     ctx_s = [0.0]
 
@@ -425,7 +425,7 @@ def main():
         ev64, P64 = hf64.evals.copy(), hf64.P.copy()
         log("fp64 reference: E=%.10f in %.2f s, %d iterations" % (e64, dt64, r64.iterations))
         for name, opts in (("fp64+learned", dict(fp64_opts, guess=2)),
-                           ("mixed+coreH", dict(fp64_opts, precision=1, incremental=1, fp64_final=3))):
+                           ("mixed+coreH", dict(fp64_opts, precision=1, incremental=1, fp64_final=2))):
             _, hfa, ea, dta = one_scf(opts)
             ablation[name] = {"scf_s": round(dta, 4), "iterations": hfa.result.iterations,
                               "energy_diff_vs_fp64": ea - e64}

@@ -64,7 +64,9 @@ int uno_ctx_destroy(uno_ctx* ctx);
 /* Tuning knobs (no reference counterpart): "schwarz" (quartet screening threshold on
  * Qab*Qcd*max|D|, default 1e-13), "schwarz_mixed" (the threshold of uno_scf_run's
  * mixed-precision phase, never below "schwarz"; default 3e-9: that phase ends with a
- * full FP64 rebuild at "schwarz"), "fp32_threshold" (quartets whose bound is below it
+ * full FP64 rebuild at "schwarz"), "mixed_switch" (|dE| in Eh ending that phase; default
+ * 3e-6), "rebuild_every" (incremental builds between full rebuilds; default 10),
+ * "fp32_threshold" (quartets whose bound is below it
  * run in FP32 when "precision" is 1; default 1e-6), "precision" (0 = FP64, 1 = mixed, for the
  * builds of uno_fock_rhf/uno_fock_uhf; uno_scf_run takes its own option),
  * "prim_cut" (bare primitive prefactor cut, default 1e-12 as TwoElectronInts.cpp:428),
@@ -124,7 +126,8 @@ typedef struct uno_scf_opts {
                             network set by uno_ctx_set_guess) */
     int precision;       /* 0 FP64, 1 mixed: FP32 for quartets with Schwarz bound below
                             "fp32_threshold" until |dE| < 1e-5, then FP64 */
-    int fp64_final;      /* mixed: number of final iterations in full FP64 (default 3) */
+    int fp64_final;      /* mixed: number of final iterations in full FP64 (default 2: the
+                            last two energies are both FP64 energies) */
     int incremental;     /* 1: build F from the density change (Delta-D) */
     int verbose;         /* 1: per-iteration lines on stderr like the reference */
     double field[3];     /* uniform electric field (au): H -= field . dipole integrals

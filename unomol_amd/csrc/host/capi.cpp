@@ -300,6 +300,8 @@ int uno_ctx_set(uno_ctx* c, const char* key, double v) {
     else if (k == "skip_center") c->skip_center = (int)v;
     else if (k == "fp32_threshold") c->fp32_thr = v;
     else if (k == "schwarz_mixed") c->tau_mixed = v;
+    else if (k == "mixed_switch") c->mixed_switch = v;
+    else if (k == "rebuild_every") c->rebuild_every = std::max(1, (int)v);
     else if (k == "precision") c->precision = (int)v;
     else if (k == "row_merge") c->row_merge = (int)v;
     else if (k == "row_min_kets") c->row_min_kets = (long long)v;
@@ -319,6 +321,8 @@ int uno_ctx_get(uno_ctx* c, const char* key, double* v) {
     else if (k == "skip_center") *v = c->skip_center;
     else if (k == "fp32_threshold") *v = c->fp32_thr;
     else if (k == "schwarz_mixed") *v = c->tau_mixed;
+    else if (k == "mixed_switch") *v = c->mixed_switch;
+    else if (k == "rebuild_every") *v = c->rebuild_every;
     else if (k == "precision") *v = c->precision;
     else if (k == "row_merge") *v = c->row_merge;
     else if (k == "row_min_kets") *v = (double)c->row_min_kets;
@@ -436,7 +440,7 @@ void uno_scf_defaults(uno_scf_opts* o) {
     o->accel = 0;
     o->guess = 0;
     o->precision = 0;
-    o->fp64_final = 3;
+    o->fp64_final = 2;
     o->incremental = 0;
     o->verbose = 0;
 }

@@ -123,6 +123,8 @@ struct uno_ctx {
     // screening threshold of the mixed-precision builds (never below tau): their error
     // only steers the early iterations; the FP64 phase starts with a full rebuild at tau
     double tau_mixed = 3e-9;
+    double mixed_switch = 3e-6;   // |dE| (Eh) ending uno_scf_run's mixed-precision phase
+    int rebuild_every = 10;       // incremental builds between full rebuilds
     int precision = 0;
     // pair data
     std::vector<uno::PairHdr> pairs;

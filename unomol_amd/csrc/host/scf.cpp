@@ -14,11 +14,12 @@
 //
 // Incremental builds (incremental = 1): G is linear in the density, so after a full
 // build G[P_k] = G[P_ref] + G[P_k - P_ref]; the change is small late in the SCF and the
-// density screen then drops most quartets.  A full build is redone every kRebuild
-// incremental builds and whenever the precision changes.
+// density screen then drops most quartets.  A full build is redone every rebuild_every
+// (context knob, default 10) incremental builds and whenever the precision changes.
 // Mixed precision (precision = 1): Fock builds evaluate quartets with Schwarz bound
 // Qab Qcd below fp32_threshold in FP32 (accumulating in FP64) until |dE| falls below
-// kMixedSwitch or the convergence test passes; then a full FP64 build starts the final
+// the context's mixed_switch (default 3e-6 Eh) or the convergence test passes; then a
+// full FP64 build starts the final
 // phase, and convergence is accepted only after fp64_final FP64 iterations.
 #include <chrono>
 #include <cfloat>
@@ -40,11 +41,9 @@ struct ScfBufs {
     DBuf<double> H, T, S, X, F, Fp, Tm, Cw, G[2], C[2], Pold[2], diisF, diisE, diisRow, Etmp, P[2], Pref[2];
 };
 
-constexpr int kRebuild = 10;            // incremental builds between full rebuilds
 constexpr int kSp2MinN = 256;           // purification instead of the eigensolve from this size (purify = -1)
 constexpr int kSp2MaxIt = 100;
 constexpr double kSp2Idem = 1e-8;
-constexpr double kMixedSwitch = 1e-5;   // |dE| (Eh) ending the mixed-precision phase
 
 int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, double* evals_out, double* P_io,
             double* C_out, const double* dh_extra) {
@@ -349,7 +348,7 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
     int fp64_updates = 0;
     auto update = [&]() -> int {
         const auto t0 = clk::now();
-        const bool full = need_full || !o.incremental || nincr >= kRebuild;
+        const bool full = need_full || !o.incremental || nincr >= c->rebuild_every;
         const bool mixed = phase == 0;
         // densities of this build: P, or its change since the last build (G is linear in P)
         double* Din[2] = {uhf ? c->D1.p : c->D0.p, c->D2.p};
@@ -465,7 +464,7 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
         const bool pass = test();
         if (phase == 0) {
             // leave the mixed phase: the next build is a full FP64 one
-            if (pass || std::fabs(ediff) < kMixedSwitch) {
+            if (pass || std::fabs(ediff) < c->mixed_switch) {
                 phase = 1;
                 need_full = true;
             }
