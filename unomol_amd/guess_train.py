@@ -26,11 +26,17 @@ import numpy as np
 from . import api
 
 
-def converged_density(deck_path, eps=1e-9):
+def converged_density(deck_path, eps=1e-9, guess_net=None):
+    """FP64 SCF (DIIS, full builds) converged to |dE| < eps; with guess_net the SCF starts
+    from that network's guess (only the start changes, not the converged density)."""
     b = api.Basis(deck_path)
     t = api.TwoElectronInts(b)
+    opts = dict(accel=2, eps=eps, cflag=3, maxits=100)
+    if guess_net is not None:
+        api.set_guess_model(t, guess_net)
+        opts["guess"] = 2
     hf = api.RestrictedHartreeFock(b, t)
-    hf.findEnergy(accel=2, eps=eps, cflag=3, maxits=100)
+    hf.findEnergy(**opts)
     return b, hf
 
 
@@ -64,12 +70,15 @@ def _keep(rng, X, Y, M, cap):
 
 
 def make_training_set_batched(n_small=4000, n_large=1000, small_w=(2, 13), large_w=(14, 32), seed=20000,
-                              basis="6-31G*", cap=300, batch=512, log=None):
+                              basis="6-31G*", cap=300, batch=512, log=None, n_xl=0, xl_w=(40, 64),
+                              guess_net=None):
     """A larger training set from the batched SCF: n_small clusters of small_w waters run
     `batch` at a time through one batched SCF (FP64, DIIS, density and energy converged to
     1e-9), n_large clusters of large_w waters one by one; each cluster's atom pairs are
     sub-sampled to at most `cap` samples (uniformly), so that many clusters contribute.
-    Seeds seed, seed+1, ... (the held-out 64-water clusters use other seeds)."""
+    n_xl clusters of xl_w waters (bulk-like environments) follow one by one, their SCF
+    started from guess_net when given.  Seeds seed, seed+1, ... (the held-out 64-water
+    clusters use other seeds)."""
     rng = np.random.default_rng(seed)
     tmp = tempfile.mkdtemp()
     Xs, Ys, Ms = [], [], []
@@ -116,6 +125,21 @@ def make_training_set_batched(n_small=4000, n_large=1000, small_w=(2, 13), large
         if log and (j + 1) % 100 == 0:
             log("training set: %d/%d larger clusters, %d samples, %.1f s" %
                 (j + 1, n_large, sum(len(x) for x in Xs), time.time() - t0))
+    for j in range(n_xl):
+        nw = int(rng.integers(xl_w[0], xl_w[1] + 1))
+        path = os.path.join(tmp, "x%d.dat" % k)
+        api.generate_deck(path, "water_cluster", nw, seed + k, basis, 0, 1e-9, 150)
+        k += 1
+        bb, hf = converged_density(path, guess_net=guess_net)
+        if not hf.result.converged:
+            continue
+        X, Y, M = _keep(rng, *api.guess_samples(bb.deck, hf.P), cap)
+        Xs.append(X)
+        Ys.append(Y)
+        Ms.append(M)
+        if log and (j + 1) % 20 == 0:
+            log("training set: %d/%d cluster-scale clusters, %d samples, %.1f s" %
+                (j + 1, n_xl, sum(len(x) for x in Xs), time.time() - t0))
     return np.concatenate(Xs), np.concatenate(Ys), np.concatenate(Ms)
 
 
@@ -136,6 +160,8 @@ def main():
     ap.add_argument("--batched", action="store_true")
     ap.add_argument("--small", type=int, default=4000)
     ap.add_argument("--large", type=int, default=1000)
+    ap.add_argument("--xl", type=int, default=0, help="clusters of 40-64 waters")
+    ap.add_argument("--xl-guess", default="", help="network starting the SCFs of those clusters")
     ap.add_argument("--cap", type=int, default=300)
     ap.add_argument("--batch", type=int, default=1024, help="training minibatch")
     ap.add_argument("--lr", type=float, default=2e-3)
@@ -145,7 +171,9 @@ def main():
         print("[guess-train] " + m, flush=True)
 
     if args.batched:
-        X, Y, M = make_training_set_batched(args.small, args.large, seed=args.seed, cap=args.cap, log=log)
+        gn = api.MLP(path=args.xl_guess) if args.xl_guess else None
+        X, Y, M = make_training_set_batched(args.small, args.large, seed=args.seed, cap=args.cap, log=log,
+                                            n_xl=args.xl, guess_net=gn)
     else:
         X, Y, M = make_training_set(args.clusters, args.min_w, args.max_w, args.seed, log=log)
     log("samples %d" % len(X))
