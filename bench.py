@@ -50,8 +50,9 @@ PAIR_NAME = ["ss", "ps", "pp", "ds", "dp", "dd"]
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (and matrix) dense peak
 FP32_PEAK_TFLOPS = 157.3
 LANE_MAX_COMP = 162       # classes up to this many components run one quartet per lane
-PROF_SLOTS = 288          # profiling slots: 72 kind + 36 precision + 6 X + Y
+PROF_SLOTS = 432          # profiling slots: 72 kind + 36 precision + 6 X + Y
 ROW_KINDS = {1: "true, true", 2: "false, true", 3: "false, false"}   # k_eri_rows <TL, FB>
+GROUP_KINDS = {4: 2, 5: 4}                                            # k_eri_rows_mb <NBR>
 MODEL = os.path.join(ROOT, "unomol_amd", "models", "guess_631gs.bin")
 
 
@@ -176,7 +177,7 @@ def class_table(ms, launches, prim, shell):
     removed) times the quartets the kernel counted."""
     from unomol_amd import flopmodel
     classes = []
-    for kind in range(4):
+    for kind in range(PROF_SLOTS // 72):
         for prec in range(2):
             for X in range(6):
                 for Y in range(X + 1):
@@ -203,6 +204,9 @@ def kernel_name(cls):
     la, lb = PAIR_L[X]
     lc, ld = PAIR_L[Y]
     t = "float" if cls["precision"] == "f32" else "double"
+    if cls.get("kind", 0) in GROUP_KINDS:
+        return "void uno::k_eri_rows_mb<%d, %d, %d, %d, 0, %s, %d>(uno::EriArgs)" % (la, lb, lc, ld, t,
+                                                                                     GROUP_KINDS[cls["kind"]])
     if cls.get("kind", 0):
         return "void uno::k_eri_rows<%d, %d, %d, %d, 0, %s, %s>(uno::EriArgs)" % (la, lb, lc, ld, t,
                                                                                   ROW_KINDS[cls["kind"]])

@@ -204,11 +204,11 @@ int uno_autotune(uno_ctx* ctx, const double* P, const double* PB, int mixed, dou
  * profiling mode in which every Fock build records, per quartet class (bra pair class
  * X, ket pair class Y, index 6*X+Y; pair classes ss ps pp ds dp dd), the kernel time
  * (HIP events, ms), launches, and the primitive and shell quartets evaluated.
- * enable=0 turns it off; stats are read with enable=-1.  Arrays hold 288 entries,
+ * enable=0 turns it off; stats are read with enable=-1.  Arrays hold 432 entries,
  * index 72*kind + 36*prec + 6*X + Y: kind 0 the per-wave-task (or per-quartet) kernels,
  * kinds 1-3 the row-merged kernels (1: every bra row and the Rys table in LDS, 2: every
- * bra row, 3: the first bra shell's rows); prec 0 FP64, 1 the FP32 launches of mixed
- * builds. */
+ * bra row, 3: the first bra shell's rows), kinds 4-5 the bra-group row kernels (groups
+ * of 2 and 4 bras); prec 0 FP64, 1 the FP32 launches of mixed builds. */
 int uno_profile(uno_ctx* ctx, int enable, double* ms, double* launches, double* prim_quartets,
                 double* shell_quartets);
 

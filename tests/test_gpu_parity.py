@@ -367,6 +367,38 @@ def test_fock_rows_rhf_vs_oracle(tag):
     assert np.abs(G - 2 * Gref).max() < 2e-11
 
 
+@pytest.mark.parametrize("uhf", [False, True])
+@pytest.mark.parametrize("variant", [4, 5])
+def test_fock_rows_bra_groups_vs_oracle(uhf, variant):
+    """Every bra row-merged in groups of 2 (variant 4) or 4 (variant 5) bras that load each
+    ket's records once for the whole group, against the oracle's brute-force G (RHF on the
+    two-water deck, UHF on the water cation); a second build accumulates."""
+    tag = "water_631gs_cation" if uhf else "w2_631gs"
+    say("fock %s, bra groups (variant %d), vs oracle %s" % ("uhf" if uhf else "rhf", variant, tag))
+    path = R.deck_path(tag)
+    deck = read_deck(path)
+    n = deck["norb"]
+    rng = np.random.default_rng(15 + variant)
+    A = rng.standard_normal((n, n)) * 0.1
+    B = rng.standard_normal((n, n)) * 0.1
+    PA, PB = _pack(A + A.T), _pack(B + B.T)
+    b, t = _tints(path, schwarz=1e-16)
+    t.set("row_min_kets", 1)
+    t.set("row_variant", variant)
+    if uhf:
+        GA, GB = np.zeros_like(PA), np.zeros_like(PB)
+        t.formGmatrix(PA, PB, GA, GB)
+        RA, RB = _oracle_g(deck, PA, PB)
+        assert np.abs(GA - RA).max() < 1e-11 and np.abs(GB - RB).max() < 1e-11
+    else:
+        G = np.zeros_like(PA)
+        t.formGmatrix(PA, G)
+        Gref = _oracle_g(deck, PA)
+        assert np.abs(G - Gref).max() < 1e-11
+        t.formGmatrix(PA, G)
+        assert np.abs(G - 2 * Gref).max() < 2e-11
+
+
 def test_fock_rows_uhf_vs_oracle():
     say("fock uhf, all bras row-merged, vs oracle water_631gs_cation")
     path = R.deck_path("water_631gs_cation")
@@ -416,8 +448,10 @@ def test_fock_rows_vs_waves_config_b(uhf):
         scale = np.abs(Gwave).max()
         t.set("row_merge", 1)
         # LDS variants: rows + Rys table, every bra row, the first bra shell's rows; each
-        # with the bra's exchange density rows read from global memory and staged in LDS
-        for variant in (1, 2, 3):
+        # with the bra's exchange density rows read from global memory and staged in LDS;
+        # then groups of 2 and 4 bras sharing their kets (FP64 spaces of the small classes;
+        # the others keep the default variant)
+        for variant in (1, 2, 3, 4, 5):
             for dens in (0, 1):
                 t.set("row_variant", variant)
                 t.set("row_density_lds", dens)

@@ -137,6 +137,11 @@ constexpr int kLdsTabMaxRoots = 3;
 // LDS a row-merged block may take (rows, then the Rys table if it still fits): two blocks
 // per compute unit
 constexpr size_t kRowLdsBudget = 80 * 1024;
+// classes the bra-group row kernels cover (components per shell quartet): groups of two
+// bras up to 9, groups of four up to 3 (their component blocks and Coulomb sums for every
+// bra of the group stay in registers)
+constexpr int kRowsMbMaxComp2 = 9;
+constexpr int kRowsMbMaxComp4 = 3;
 
 // ---------------------------------------------------------------------------
 // Fock builds: one wave per (bra pair, chunk of up to 64 kets).  The bra is the same in
@@ -616,6 +621,318 @@ __global__ __launch_bounds__(kRowThreads) void k_eri_rows(const EriArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fock builds, groups of bras with many kets: one block per (group of up to NBR bras of a
+// space, segment of rseg kets).  The kets of an FP64 space are the same run for every bra,
+// so each ket's pair record and primitive-pair records are loaded once for the whole group
+// (ket primitives outside, the group's bras and their primitives inside) instead of once
+// per bra: the row kernels above wait on those loads.  Digestion per bra as in k_eri_rows
+// with every bra row merged in LDS (NBR x rows x ncols doubles) and the Rys table in LDS.
+// ---------------------------------------------------------------------------
+// one primitive quartet into the component block e (the body of eri_block's loops); false
+// when the bare-prefactor cut drops it
+template <int LA, int LB, int LC, int LD, typename T>
+__device__ __forceinline__ bool prim_quartet(const EriArgs& a, const double* A1, const PrimPair& pp, const PrimPair& qq,
+                                             const T* QC, const T* ab, const T* cd, const T* tab, T* e) {
+    using CI = ClassInfo<LA, LB, LC, LD>;
+    constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND, NR = CI::NR;
+    const T q = (T)qq.p, iq_ = (T)qq.ip;
+    const T p = (T)pp.p, ip_ = (T)pp.ip;
+    const T rt = rsqrt_nr(p + q), it = rt * rt;
+    const T bare = (T)kTwoPi25 * (T)pp.s * (T)qq.s * ip_ * iq_ * rt;
+    if (bare < (T)a.primcut) return false;
+    const T pref = (T)kTwoPi25 * (T)pp.K * (T)qq.K * ip_ * iq_ * rt;
+    const T PA[3] = {(T)(pp.P[0] - A1[0]), (T)(pp.P[1] - A1[1]), (T)(pp.P[2] - A1[2])};
+    const T PQ[3] = {(T)(pp.P[0] - qq.P[0]), (T)(pp.P[1] - qq.P[1]), (T)(pp.P[2] - qq.P[2])};
+    const T x = p * q * it * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+    T u[NR], w[NR];
+    rys_roots<NR, T>(x, u, w, tab);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const T fff = u[r] * it;
+        const T B00 = (T)0.5 * fff;
+        const T B1 = ((T)0.5 - B00 * q) * ip_;
+        const T B1p = ((T)0.5 - B00 * p) * iq_;
+        T Ix[CI::TBL], Iy[CI::TBL], Iz[CI::TBL];
+        axis_table<LA, LB, LC, LD, T>(PA[0] - q * PQ[0] * fff, QC[0] + p * PQ[0] * fff, B00, B1, B1p, ab[0], cd[0], Ix);
+        axis_table<LA, LB, LC, LD, T>(PA[1] - q * PQ[1] * fff, QC[1] + p * PQ[1] * fff, B00, B1, B1p, ab[1], cd[1], Iy);
+        axis_table<LA, LB, LC, LD, T>(PA[2] - q * PQ[2] * fff, QC[2] + p * PQ[2] * fff, B00, B1, B1p, ab[2], cd[2], Iz);
+        const T wr = w[r] * pref;
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+                for (int k = 0; k < NC; ++k)
+#pragma unroll
+                    for (int l = 0; l < ND; ++l) {
+                        const int c = ((i * NB + j) * NC + k) * ND + l;
+                        e[c] = fma(wr * Ix[CI::tix(i, j, k, l, 0)], Iy[CI::tix(i, j, k, l, 1)] * Iz[CI::tix(i, j, k, l, 2)], e[c]);
+                    }
+    }
+    return true;
+}
+
+template <int LA, int LB, int MODE>
+__host__ __device__ constexpr int rows_mb_count() {
+    return (ncart_c(LA) + ncart_c(LB)) * (MODE == kDigestUHF ? 2 : 1);
+}
+
+// waves per SIMD the bra-group kernels' registers must allow: the single-bra kernel's
+// occupancy for the smallest classes, one wave less for the others
+template <int LA, int LB, int LC, int LD, int NBR>
+constexpr int rows_mb_min_waves() {
+    constexpr int nc = ClassInfo<LA, LB, LC, LD>::NCOMP;
+    return nc == 1 ? (NBR == 2 ? 5 : 4) : (nc == 3 ? (NBR == 2 ? 4 : 3) : 2);
+}
+template <int LA, int LB, int LC, int LD, int MODE, typename T, int NBR>
+__global__ __launch_bounds__(kRowThreads, (rows_mb_min_waves<LA, LB, LC, LD, NBR>())) void k_eri_rows_mb(const EriArgs a) {
+    using CI = ClassInfo<LA, LB, LC, LD>;
+    constexpr int NA = CI::NA, NB = CI::NB, NC = CI::NC, ND = CI::ND, NCOMP = CI::NCOMP, NR = CI::NR;
+    constexpr int NAB = NA + NB, NSP = MODE == kDigestUHF ? 2 : 1, NROW = NAB * NSP, NANB = NA * NB;
+    extern __shared__ double rows[];   // bra b's row r at rows[(b * NROW + r) * nc]
+    __shared__ T tab_lds[rys_table_size<NR, T>()];
+    __shared__ double jred[kRowWaves][NBR * NANB];
+    __shared__ unsigned long long cred[kRowWaves][2];
+    __shared__ PairHdr sP1[NBR];
+    __shared__ T sdab[NBR][NANB];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the block's group (uniform): largest g with rpre[g] <= blockIdx.x
+    const long long blk = blockIdx.x;
+    const int ngrp = (a.nrbra + NBR - 1) / NBR;
+    int lo = 0, hi = ngrp - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.rpre[mid] <= blk) lo = mid; else hi = mid - 1;
+    }
+    const int g0 = __builtin_amdgcn_readfirstlane(lo * NBR);
+    const int nb = min(NBR, a.nrbra - g0);
+    long long cnt[NBR];
+    int pab[NBR];
+    long long cmax = 0;
+#pragma unroll
+    for (int b = 0; b < NBR; ++b) {
+        const int e = a.rbra[g0 + (b < nb ? b : 0)];
+        cnt[b] = b < nb ? a.prefix[e + 1] - a.prefix[e] : 0;
+        pab[b] = __builtin_amdgcn_readfirstlane(entry_bra(a, e));
+        cmax = max(cmax, cnt[b]);
+    }
+    const int entry0 = a.rbra[g0];
+    const long long seg = blk - a.rpre[lo];
+    const long long k0 = seg * a.rseg, k1 = min(cmax, k0 + (long long)a.rseg);
+    const MolView mv = mol_view(a, entry0);
+    const int n = mv.n, nc = a.ncols;
+    const int ket0 = entry_ket(a, entry0);
+    if (threadIdx.x < nb) sP1[threadIdx.x] = a.pairs[entry_bra(a, a.rbra[g0 + threadIdx.x])];
+    for (int i = threadIdx.x; i < NBR * NROW * nc; i += blockDim.x) rows[i] = 0.0;
+    const T* tab = rys_table_of<NR, T>(a);
+    for (int i = threadIdx.x; i < rys_table_size<NR, T>(); i += blockDim.x) tab_lds[i] = tab[i];
+    tab = tab_lds;
+    __syncthreads();
+    for (int t = threadIdx.x; t < nb * NANB; t += blockDim.x) {
+        const int b = t / NANB, ij = t - b * NANB;
+        sdab[b][ij] = (T)mv.D0[(long)(sP1[b].f1 + ij / NB) * n + sP1[b].f2 + ij % NB];
+    }
+    __syncthreads();
+    constexpr T jf = (MODE == kDigestRHF) ? 4 : 2;
+    // the bra Coulomb sums: reduced over the wave per ket pass and bra (in converged control
+    // flow), lane 0 keeping the wave's running totals (registers per bra, not per lane)
+    for (int t = threadIdx.x; t < kRowWaves * NBR * NANB; t += blockDim.x) (&jred[0][0])[t] = 0.0;
+    __syncthreads();
+    unsigned long long nprim = 0, nshell = 0;
+    for (long long base = k0; base < k1; base += blockDim.x) {
+        const long long jk = base + threadIdx.x;
+        bool act[NBR];
+        bool any = false;
+        // lanes past the segment read a valid record (the segment's first ket), unused: the
+        // wave stays converged for its Coulomb sums
+        const int pcd = ket0 + (int)(jk < k1 ? jk : k0);
+        const PairHdr P2 = a.pairs[pcd];
+        if (jk < k1) {
+            const double qcd = a.pairQ[pcd];
+#pragma unroll
+            for (int b = 0; b < NBR; ++b) {
+                act[b] = b < nb && jk < cnt[b] && !screened(a, mv, sP1[b], P2, a.pairQ[pab[b]] * qcd);
+                any = any || act[b];
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < NBR; ++b) act[b] = false;
+        }
+        if (__ballot(any) == 0ULL) continue;   // the whole wave: the Coulomb sums below are wave-wide
+        T e[NBR][NCOMP];
+#pragma unroll
+        for (int b = 0; b < NBR; ++b)
+#pragma unroll
+            for (int c = 0; c < NCOMP; ++c) e[b][c] = 0;
+        const T cd[3] = {(T)P2.AB[0], (T)P2.AB[1], (T)P2.AB[2]};
+        // ket primitives outside: each record is loaded once for all the group's bras
+        const int npq = any ? P2.nprim : 0;
+        for (int iq = 0; iq < npq; ++iq) {
+            const PrimPair qq = a.prims[P2.prim0 + iq];
+            const T QC[3] = {(T)(qq.P[0] - P2.A[0]), (T)(qq.P[1] - P2.A[1]), (T)(qq.P[2] - P2.A[2])};
+#pragma unroll
+            for (int b = 0; b < NBR; ++b) {
+                if (!act[b]) continue;
+                const int p0 = __builtin_amdgcn_readfirstlane(sP1[b].prim0);
+                const int np1 = __builtin_amdgcn_readfirstlane(sP1[b].nprim);
+                const double A1[3] = {sP1[b].A[0], sP1[b].A[1], sP1[b].A[2]};
+                const T ab[3] = {(T)sP1[b].AB[0], (T)sP1[b].AB[1], (T)sP1[b].AB[2]};
+                for (int ip = 0; ip < np1; ++ip)
+                    if (prim_quartet<LA, LB, LC, LD, T>(a, A1, a.prims[p0 + ip], qq, QC, ab, cd, tab, e[b])) ++nprim;
+            }
+        }
+        const int fc = P2.f1, fd = P2.f2;
+        const int lc = a.colmap[fc], ld = a.colmap[fd];
+        T dcd[NC * ND];
+#pragma unroll
+        for (int k = 0; k < NC; ++k)
+#pragma unroll
+            for (int l = 0; l < ND; ++l) dcd[k * ND + l] = (T)mv.D0[(long)(fc + k) * n + fd + l];
+#pragma unroll
+        for (int b = 0; b < NBR; ++b) {
+            // Coulomb, bra outputs: this pass's sum over the wave (every lane takes part)
+            double jb[NANB];
+#pragma unroll
+            for (int ij = 0; ij < NANB; ++ij) jb[ij] = 0.0;
+            if (act[b]) {
+                const T f = (sP1[b].same ? (T)0.5 : (T)1) * (P2.same ? (T)0.5 : (T)1) * (pab[b] == pcd ? (T)0.5 : (T)1);
+                normalise_block<LA, LB, LC, LD, T>(e[b], f);
+#pragma unroll
+                for (int ij = 0; ij < NANB; ++ij) {
+                    T s = 0;
+#pragma unroll
+                    for (int kl = 0; kl < NC * ND; ++kl) s = fma(e[b][ij * NC * ND + kl], dcd[kl], s);
+                    jb[ij] = (double)(jf * s);
+                }
+            }
+#pragma unroll
+            for (int ij = 0; ij < NANB; ++ij) {
+                const double v = wave_sum(jb[ij]);
+                if (lane == 0) jred[wave][b * NANB + ij] += v;
+            }
+            if (!act[b]) continue;
+            ++nshell;
+            const int fa = sP1[b].f1, fb = sP1[b].f2;
+            // Coulomb, ket outputs: one atomic each
+#pragma unroll
+            for (int kl = 0; kl < NC * ND; ++kl) {
+                T s = 0;
+#pragma unroll
+                for (int ij = 0; ij < NANB; ++ij) s = fma(e[b][ij * NC * ND + kl], sdab[b][ij], s);
+                jket_add<MODE>(a, mv, P2, kl / ND, kl % ND, ND, (double)(jf * s));
+            }
+            // exchange into bra b's LDS rows: row i (a function i), row NA + j (b function j)
+#pragma unroll
+            for (int sp = 0; sp < NSP; ++sp) {
+                const double* DK = (MODE == kDigestUHF) ? (sp == 0 ? mv.D1 : mv.D2) : mv.D0;
+                double* R = rows + (b * NROW + sp * NAB) * nc;
+                T dbd[NB * ND], dbc[NB * NC], dad[NA * ND], dac[NA * NC];
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+#pragma unroll
+                    for (int l = 0; l < ND; ++l) dbd[j * ND + l] = (T)DK[(long)(fb + j) * n + fd + l];
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) dbc[j * NC + k] = (T)DK[(long)(fb + j) * n + fc + k];
+                }
+#pragma unroll
+                for (int i = 0; i < NA; ++i) {
+#pragma unroll
+                    for (int l = 0; l < ND; ++l) dad[i * ND + l] = (T)DK[(long)(fa + i) * n + fd + l];
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) dac[i * NC + k] = (T)DK[(long)(fa + i) * n + fc + k];
+                }
+#pragma unroll
+                for (int i = 0; i < NA; ++i) {
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) {
+                        T s = 0;
+#pragma unroll
+                        for (int j = 0; j < NB; ++j)
+#pragma unroll
+                            for (int l = 0; l < ND; ++l) s = fma(e[b][((i * NB + j) * NC + k) * ND + l], dbd[j * ND + l], s);
+                        atomicAdd(R + i * nc + lc + k, (double)(-s));
+                    }
+#pragma unroll
+                    for (int l = 0; l < ND; ++l) {
+                        T s = 0;
+#pragma unroll
+                        for (int j = 0; j < NB; ++j)
+#pragma unroll
+                            for (int k = 0; k < NC; ++k) s = fma(e[b][((i * NB + j) * NC + k) * ND + l], dbc[j * NC + k], s);
+                        atomicAdd(R + i * nc + ld + l, (double)(-s));
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) {
+                        T s = 0;
+#pragma unroll
+                        for (int i = 0; i < NA; ++i)
+#pragma unroll
+                            for (int l = 0; l < ND; ++l) s = fma(e[b][((i * NB + j) * NC + k) * ND + l], dad[i * ND + l], s);
+                        atomicAdd(R + (NA + j) * nc + lc + k, (double)(-s));
+                    }
+#pragma unroll
+                    for (int l = 0; l < ND; ++l) {
+                        T s = 0;
+#pragma unroll
+                        for (int i = 0; i < NA; ++i)
+#pragma unroll
+                            for (int k = 0; k < NC; ++k) s = fma(e[b][((i * NB + j) * NC + k) * ND + l], dac[i * NC + k], s);
+                        atomicAdd(R + (NA + j) * nc + ld + l, (double)(-s));
+                    }
+                }
+            }
+        }
+    }
+    // bra Coulomb outputs: the waves' totals (jred), summed over the four waves below
+    if (a.counter) {
+        int np = (int)nprim, ns = (int)nshell;
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            np += __shfl_xor(np, m, 64);
+            ns += __shfl_xor(ns, m, 64);
+        }
+        if (lane == 0) {
+            cred[wave][0] = (unsigned long long)np;
+            cred[wave][1] = (unsigned long long)ns;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < nb * NANB) {
+        const int b = threadIdx.x / NANB, ij = threadIdx.x - b * NANB;
+        double v = 0.0;
+#pragma unroll
+        for (int w = 0; w < kRowWaves; ++w) v += jred[w][threadIdx.x];
+        const long o = (long)(sP1[b].f1 + ij / NB) * n + sP1[b].f2 + ij % NB;
+        atomic_add_f64(mv.W0 + o, v);
+        if (MODE == kDigestUHF) atomic_add_f64(mv.W1 + o, v);
+    }
+    if (a.counter && threadIdx.x == 0) {
+        unsigned long long* slot = a.counter + 2 * (blockIdx.x & (kCounterSlots - 1));
+        unsigned long long np = 0, ns = 0;
+        for (int w = 0; w < kRowWaves; ++w) {
+            np += cred[w][0];
+            ns += cred[w][1];
+        }
+        atomicAdd(slot, np);
+        atomicAdd(slot + 1, ns);
+    }
+    // merged exchange rows: consecutive threads take consecutive columns of a row
+    for (int br = 0; br < nb * NROW; ++br) {
+        const int b = br / NROW, r = br - b * NROW;
+        const int sp = r / NAB, rr = r - sp * NAB;
+        const int grow = rr < NA ? sP1[b].f1 + rr : sP1[b].f2 + rr - NA;
+        double* W = (MODE == kDigestUHF && sp) ? mv.W1 : mv.W0;
+        for (int col = threadIdx.x; col < nc; col += blockDim.x) {
+            const double v = rows[br * nc + col];
+            if (v != 0.0) atomic_add_f64(W + (long)grow * n + a.colinv[col], v);
+        }
+    }
+}
+
 template <typename V>
 __device__ __forceinline__ V sel3(int ax, V a0, V a1, V a2) {
     return ax == 0 ? a0 : (ax == 1 ? a1 : a2);
@@ -963,6 +1280,22 @@ static hipError_t launch_rows_v(const EriArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+// bra-group launch (variants 4 and 5: groups of 2 and 4 bras)
+template <int LA, int LB, int LC, int LD, int MODE, typename T, int NBR>
+static hipError_t launch_rows_mb(const EriArgs& a, hipStream_t st) {
+    const size_t dyn = (size_t)NBR * rows_mb_count<LA, LB, MODE>() * a.ncols * sizeof(double);
+    static size_t attr = 0;   // dynamic LDS limit set for this instantiation
+    if (dyn > attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_eri_rows_mb<LA, LB, LC, LD, MODE, T, NBR>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+        if (e != hipSuccess) return e;
+        attr = dyn;
+    }
+    hipLaunchKernelGGL((k_eri_rows_mb<LA, LB, LC, LD, MODE, T, NBR>), dim3((unsigned)a.nrtask), dim3(kRowThreads), dyn,
+                       st, a);
+    return hipGetLastError();
+}
+
 // variant chosen by the host (eri_rows_variant): 1 all bra rows + table in LDS, 2 all bra
 // rows, 3 the first shell's rows only
 template <int LA, int LB, int LC, int LD, int MODE, typename T>
@@ -976,6 +1309,15 @@ static hipError_t launch_rows(const EriArgs& a, int variant, hipStream_t st) {
             case 1: return launch_rows_v<LA, LB, LC, LD, MODE, T, true, true>(a, st);
             case 2: return launch_rows_v<LA, LB, LC, LD, MODE, T, false, true>(a, st);
             case 3: return launch_rows_v<LA, LB, LC, LD, MODE, T, false, false>(a, st);
+            case 4:
+            case 5:
+                // bra groups: FP64 spaces (every bra's kets the same run) of the small classes
+                if constexpr (sizeof(T) == 8 && CI::NCOMP <= kRowsMbMaxComp2) {
+                    if (a.kstart || a.e_bra || a.dlds) return hipErrorInvalidValue;
+                    if (variant == 4) return launch_rows_mb<LA, LB, LC, LD, MODE, T, 2>(a, st);
+                    if constexpr (CI::NCOMP <= kRowsMbMaxComp4) return launch_rows_mb<LA, LB, LC, LD, MODE, T, 4>(a, st);
+                }
+                return hipErrorInvalidValue;
             default: return hipErrorInvalidValue;
         }
     } else {
@@ -1018,13 +1360,19 @@ static size_t rows_lds_bytes(int cb, int ck, int mode, int ncols, bool fp32, int
     const size_t all = (size_t)(na[cb] + nb[cb]) * sp * ncols * sizeof(double),
                  first = (size_t)na[cb] * sp * ncols * sizeof(double);
     const size_t dens = dlds ? all : 0;
+    static const int ncomp[6] = {1, 3, 9, 6, 18, 36};
+    const int nc = ncomp[cb] * ncomp[ck];
     switch (variant) {
         case 1: return all + dens + tab + reserve;
         case 2: return all + dens + reserve;
         case 3: return first + dens + reserve;
+        case 4: return (fp32 || dlds || nc > kRowsMbMaxComp2) ? 0 : 2 * all + tab + reserve;
+        case 5: return (fp32 || dlds || nc > kRowsMbMaxComp4) ? 0 : 4 * all + tab + reserve;
         default: return 0;
     }
 }
+
+int eri_rows_group(int variant) { return variant == 4 ? 2 : variant == 5 ? 4 : 1; }
 
 bool eri_rows_dlds_fits(int cb, int ck, int mode, int ncols, bool fp32, int variant) {
     const size_t b = rows_lds_bytes(cb, ck, mode, ncols, fp32, variant, true);
@@ -1033,9 +1381,12 @@ bool eri_rows_dlds_fits(int cb, int ck, int mode, int ncols, bool fp32, int vari
 
 int eri_rows_variant(int cb, int ck, int mode, int ncols, bool fp32, int force) {
     if (!eri_uses_wave_tasks(cb, ck)) return 0;
-    bool fits[4] = {true, false, false, false};
-    for (int v = 1; v <= 3; ++v) fits[v] = rows_lds_bytes(cb, ck, mode, ncols, fp32, v, false) <= kRowLdsBudget;
-    if (force >= 1 && force <= 3 && fits[force]) return force;
+    bool fits[6] = {true, false, false, false, false, false};
+    for (int v = 1; v <= 5; ++v) {
+        const size_t b = rows_lds_bytes(cb, ck, mode, ncols, fp32, v, false);
+        fits[v] = b > 0 && b <= kRowLdsBudget;
+    }
+    if (force >= 1 && force <= 5 && fits[force]) return force;
     for (int v = 1; v <= 3; ++v)
         if (fits[v]) return v;
     return 0;

@@ -18,13 +18,17 @@ bool eri_uses_wave_tasks(int bra_class, int ket_class);
 // a.nrtask / a.rseg describe the block tasks, a.colmap / a.colinv / a.ncols the LDS row
 // columns; variant from eri_rows_variant: 1 every bra row and the Rys table in LDS, 2 every
 // bra row, 3 the first bra shell's rows only, 0 none fits the LDS budget at ncols columns
-// (the class then runs as wave tasks only); force 1-3 picks that variant when it fits
+// (the class then runs as wave tasks only); force 1-3 picks that variant when it fits,
+// force 4 or 5 the bra-group kernel (groups of 2 or 4 bras, every bra row and the table in
+// LDS; FP64 spaces of the classes of at most 9 or 3 components)
 hipError_t eri_launch_rows(int mode, int bra_class, int ket_class, const EriArgs& a, int variant, hipStream_t st,
                            bool fp32);
 int eri_rows_variant(int bra_class, int ket_class, int mode, int ncols, bool fp32, int force);
 // whether that variant still fits the LDS budget with the bra's exchange density rows
 // staged as well (a.dlds = 1)
 bool eri_rows_dlds_fits(int bra_class, int ket_class, int mode, int ncols, bool fp32, int variant);
+// bras per row task of a variant (2 or 4 for the bra-group variants, else 1)
+int eri_rows_group(int variant);
 // W[f1 + k][f2 + l] += jp[jo + k nd + l] (and W1) for every pair of the array (pair blocks
 // are disjoint, no atomics); the block size of a pair follows from the next pair's jo
 hipError_t jp_scatter(int npairs, const PairHdr* pairs, long long njp, const double* jp, double* W0, double* W1, int n,

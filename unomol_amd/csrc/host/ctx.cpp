@@ -91,18 +91,24 @@ void build_prefix(uno_ctx* c, QuartetClass& q, double dmax, double fp32_thr, dou
 
 // wave tasks: bra i owns ceil(count_i / 64) consecutive tasks from wpre[i]; tblk[b] is
 // the bra of task 64 b, bracketing each wave's bra search
-// row-merged block tasks for the bras with at least min_kets kets (none if min_kets <= 0)
-void build_rows(QuartetSpace& s, long long min_kets, int seg) {
+// row-merged block tasks for the bras with at least min_kets kets (none if min_kets <= 0):
+// rbra lists those bras; a task is a group of `group` consecutive ones (the last group may
+// be smaller) and one segment of seg kets of the group's largest ket count, rpre[g] the
+// first task of group g
+void build_rows(QuartetSpace& s, long long min_kets, int seg, int group) {
     const int nb = (int)s.prefix.size() - 1;
     s.rbra.clear();
     s.rpre.assign(1, 0);
-    if (min_kets > 0)
-        for (int i = 0; i < nb; ++i) {
-            const long long cnt = s.prefix[i + 1] - s.prefix[i];
-            if (cnt < min_kets) continue;
-            s.rbra.push_back(i);
-            s.rpre.push_back(s.rpre.back() + (cnt + seg - 1) / seg);
+    if (min_kets > 0) {
+        for (int i = 0; i < nb; ++i)
+            if (s.prefix[i + 1] - s.prefix[i] >= min_kets) s.rbra.push_back(i);
+        for (size_t g = 0; g < s.rbra.size(); g += group) {
+            long long cmax = 0;
+            for (size_t k = g; k < std::min(s.rbra.size(), g + group); ++k)
+                cmax = std::max(cmax, s.prefix[s.rbra[k] + 1] - s.prefix[s.rbra[k]]);
+            s.rpre.push_back(s.rpre.back() + (cmax + seg - 1) / seg);
         }
+    }
     s.nrtask = s.rpre.back();
 }
 
@@ -519,7 +525,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
             r.variant = variant;
             r.dlds = dlds;
             if (r.tasks) {
-                build_rows(sp, variant ? min_kets : 0, kRowSeg);
+                build_rows(sp, variant ? min_kets : 0, kRowSeg, eri_rows_group(variant));
                 build_tasks(sp, variant ? min_kets : 0);
             } else {
                 build_blocks(sp, eri_quartets_per_block(LX.lcls, LY.lcls));
@@ -665,7 +671,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
         if (c->tuning && tasks && c->row_merge) {
             // time every candidate on this space (each one run three times, the fastest counts)
             std::vector<TuneChoice> cand{{0, 0, 0}};
-            for (int v = 1; v <= 3; ++v)
+            for (int v = 1; v <= 5; ++v)
                 if (eri_rows_variant(LX.lcls, LY.lcls, mode, c->run_ncols[q.Y], prec == 1, v) == v) {
                     const bool df = c->row_dens != 0 &&
                                     eri_rows_dlds_fits(LX.lcls, LY.lcls, mode, c->run_ncols[q.Y], prec == 1, v);
@@ -680,7 +686,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
             UNO_HIP(hipEventCreate(&t0));
             UNO_HIP(hipEventCreate(&t1));
             for (const TuneChoice& tc : cand) {
-                build_rows(sp, tc.variant ? tc.min_kets : 0, kRowSeg);
+                build_rows(sp, tc.variant ? tc.min_kets : 0, kRowSeg, eri_rows_group(tc.variant));
                 build_tasks(sp, tc.variant ? tc.min_kets : 0);
                 a.dlds = tc.dlds;
                 if (int rc = set_index()) return rc;

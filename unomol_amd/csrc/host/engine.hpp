@@ -98,9 +98,9 @@ struct QuartetClass {
 };
 
 // profiling slots: kernel kind (0 the wave-task / per-quartet kernels, 1-3 the row-merged
-// variants), then precision (FP64, FP32), then the class pair (6 X + Y):
-// slot = 72 kind + 36 prec + 6 X + Y
-constexpr int kProfKinds = 4;
+// variants, 4-5 the bra-group row kernels), then precision (FP64, FP32), then the class
+// pair (6 X + Y): slot = 72 kind + 36 prec + 6 X + Y
+constexpr int kProfKinds = 6;
 constexpr int kProfSlots = kProfKinds * 2 * kNumPairClasses * kNumPairClasses;
 
 constexpr int kNumBuckets = 6;
@@ -192,7 +192,7 @@ int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell);
 // quartet-space helpers of the Fock build (ctx.cpp)
 void build_prefix(uno_ctx* c, QuartetClass& q, double dmax, double fp32_thr, double tau);
 void build_tasks(QuartetSpace& s, long long row_min_kets = 0);
-void build_rows(QuartetSpace& s, long long min_kets, int seg);
+void build_rows(QuartetSpace& s, long long min_kets, int seg, int group = 1);
 void build_blocks(QuartetSpace& s, int qpb);
 EriArgs base_args(uno_ctx* c);
 // keeps c's tuned configurations for later contexts of the same layout
