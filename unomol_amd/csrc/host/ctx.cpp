@@ -5,6 +5,7 @@
 // build: every iteration recomputes the significant shell quartets on the GPU and
 // contracts them with the density on the fly.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <cmath>
 #include <numeric>
@@ -447,6 +448,9 @@ int oneint_device(uno_ctx* c, double* S, double* T, double* H) {
 }
 
 int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
+    using hclk = std::chrono::steady_clock;
+    const auto h0 = hclk::now();
+    auto ms_since = [](hclk::time_point t) { return std::chrono::duration<double, std::milli>(hclk::now() - t).count(); };
     hipStream_t st = c->st;
     const long nn = (long)c->n * c->n;
     UNO_HIP(k_fill(nn, 0.0, c->W0.p, st));
@@ -461,6 +465,8 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
         UNO_HIP(hipMemcpyAsync(&dmax, c->scal.p, sizeof(double), hipMemcpyDeviceToHost, st));
         UNO_HIP(hipStreamSynchronize(st));
     }
+    c->last_ms[0] = ms_since(h0);
+    const auto h1 = hclk::now();
     double total = 0.0, total32 = 0.0;
     std::vector<hipEvent_t> ev0, ev1;
     std::vector<int> evcls;
@@ -552,6 +558,8 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
         }
         recs.push_back(r);
     }
+    c->last_ms[1] = ms_since(h1);
+    const auto h2 = hclk::now();
     char* ds = nullptr;
     if (!c->tuning && !hs.empty()) {
         // the previous build's upload must have read the pinned copy before it is reused
@@ -559,19 +567,23 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
         if (c->stage_ev) UNO_HIP(hipEventSynchronize(c->stage_ev));
         else UNO_HIP(hipEventCreateWithFlags(&c->stage_ev, hipEventDisableTiming));
         if (c->stage_pin_cap < hs.size()) {
+            // grown to twice the need: a pinned allocation costs milliseconds, and the
+            // full builds' arrays grow as the density bound loosens
             if (c->stage_pin) UNO_HIP(hipHostFree(c->stage_pin));
             c->stage_pin = nullptr;
             c->stage_pin_cap = 0;
-            UNO_HIP(hipHostMalloc((void**)&c->stage_pin, hs.size()));
-            c->stage_pin_cap = hs.size();
+            UNO_HIP(hipHostMalloc((void**)&c->stage_pin, 2 * hs.size()));
+            c->stage_pin_cap = 2 * hs.size();
         }
         std::memcpy(c->stage_pin, hs.data(), hs.size());
-        UNO_HIP(c->stage_d.alloc(hs.size()));
+        if (c->stage_d.n < hs.size()) UNO_HIP(c->stage_d.alloc(2 * hs.size()));
         UNO_HIP(hipMemcpyAsync(c->stage_d.p, c->stage_pin, hs.size(), hipMemcpyHostToDevice, st));
         UNO_HIP(hipEventRecord(c->stage_ev, st));
         ds = c->stage_d.p;
     }
 
+    c->last_ms[2] = ms_since(h2);
+    const auto h3 = hclk::now();
     // Pass 2: the launches
     for (const SpaceRec& r : recs) {
         QuartetClass& q = *r.q;
@@ -715,6 +727,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
         }
         (prec ? total32 : total) += (double)sp.nq;
     }
+    c->last_ms[3] = ms_since(h3);
     // the kets' Coulomb outputs, accumulated per pair, into W
     UNO_HIP(jp_scatter((int)c->pairs.size(), c->d_pairs.p, c->njp, c->JP.p, c->W0.p, uhf ? c->W1.p : nullptr, c->n, st));
     c->last_quartets = total + total32;

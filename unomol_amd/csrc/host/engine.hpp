@@ -154,6 +154,9 @@ struct uno_ctx {
     long long njp = 0;
     uno::EigenWS* eig = nullptr;
     double last_quartets = 0.0, last_quartets32 = 0.0;
+    // host milliseconds of the last build: density bound read-back, quartet spaces and
+    // index arrays, staging upload, launch enqueue (the kernels run asynchronously)
+    double last_ms[4] = {0.0, 0.0, 0.0, 0.0};
     // profiling pass: per quartet class (bra X, ket Y) kernel milliseconds and the
     // primitive / shell quartets the kernels evaluated
     int profile = 0;

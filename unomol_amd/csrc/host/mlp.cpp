@@ -30,6 +30,7 @@ struct uno_mlp {
     std::vector<DBuf<float>> z;
     std::vector<DBuf<__bf16>> h;
     DBuf<float> y, dy, dz[2], part, tb, mb;
+    DBuf<float> xin;   // inference inputs, kept between calls (no allocation per call)
     DBuf<int> idx;
     ~uno_mlp() {
         if (st) (void)hipStreamDestroy(st);
@@ -240,9 +241,8 @@ int uno_mlp_forward(uno_mlp* m, long n, const float* X, float* Y) {
     const int in = m->dims[0], out = m->dims.back();
     const int chunk = 4096;
     if (int rc = ensure_batch(m, chunk)) return rc;
-    DBuf<float> dX;
+    DBuf<float>& dX = m->xin;
     UNO_HIP(dX.upload(X, (size_t)n * in, m->st));
-    std::vector<float> hy((size_t)chunk * out);
     for (long r0 = 0; r0 < n; r0 += chunk) {
         const int cnt = (int)std::min<long>(chunk, n - r0);
         const int B = round_up(cnt, 64);

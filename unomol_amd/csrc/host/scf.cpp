@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <future>
 #include "engine.hpp"
 #include "../hip/kernels.hpp"
 
@@ -77,6 +78,42 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
         UNO_HIP(p->alloc(nn));
     if (o.incremental)
         for (int s = 0; s < ns; ++s) UNO_HIP(b.Pref[s].alloc(nn));
+    // The learned guess's host work (pair descriptors, the network on its own stream, the
+    // blocks assembled into a dense symmetric matrix) runs on a worker thread while this
+    // thread builds the one-electron matrices and X on the device.  The future is declared
+    // after everything the task touches, so an early return waits for it first.
+    struct GuessJob {
+        std::vector<double> D;
+        std::string err;
+        long np = 0;
+        double feat_s = 0.0, net_s = 0.0, asm_s = 0.0;
+    } gj;
+    std::future<int> guess_job;
+    if (o.guess == 2)
+        guess_job = std::async(std::launch::async, [&gj, c, n]() -> int {
+            const auto g0 = clk::now();
+            std::vector<float> X, Y;
+            std::vector<int> pairs;
+            if (int rc = guess_features(c->B, X, pairs, gj.err)) return rc;
+            gj.feat_s = secs(g0, clk::now());
+            gj.np = (long)pairs.size() / 2;
+            Y.assign((size_t)gj.np * kGuessOutDim, 0.0f);
+            if (gj.np > 0)
+                if (int rc = uno_mlp_forward(c->guess_model, gj.np, X.data(), Y.data())) {
+                    gj.err = get_error();
+                    return rc;
+                }
+            gj.net_s = secs(g0, clk::now()) - gj.feat_s;
+            if (int rc = guess_assemble(c->B, pairs, Y.data(), gj.D, gj.err)) return rc;
+            std::vector<double>& D = gj.D;
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < i; ++j) {
+                    const double v = 0.5 * (D[(size_t)i * n + j] + D[(size_t)j * n + i]);
+                    D[(size_t)i * n + j] = D[(size_t)j * n + i] = v;
+                }
+            gj.asm_s = secs(g0, clk::now()) - gj.feat_s - gj.net_s;
+            return UNO_OK;
+        });
     // one-electron matrices on the device, once
     if (int rc = oneint_device(c, b.S.p, b.T.p, b.H.p)) return rc;
     if (o.field[0] != 0.0 || o.field[1] != 0.0 || o.field[2] != 0.0) {
@@ -102,10 +139,13 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
             const double dx = a[0] - bb[0], dy = a[1] - bb[1], dz = a[2] - bb[2];
             nuc += c->B.q[i] * c->B.q[j] / std::sqrt(dx * dx + dy * dy + dz * dz);
         }
+    const double s_oneint = secs(t_start, clk::now());
     // X = U s^-1/2 keeping every vector (RHF.hpp:214-233)
     std::vector<double> ev(n);
     UNO_HIP(hipMemcpyAsync(b.Tm.p, b.S.p, nn * sizeof(double), hipMemcpyDeviceToDevice, st));
+    const auto tx = clk::now();
     if (int rc = eigh_device(c, n, b.Tm.p, b.Cw.p, ev.data())) return rc;
+    if (o.verbose > 1) std::fprintf(stderr, "[scf] eigensolve of S %.2f ms\n", 1e3 * secs(tx, clk::now()));
     for (int j = 0; j < n; ++j) {
         double f = ev[j];
         if (std::fabs(f) < 1e-7 && f < 0.0) f = -f;
@@ -116,6 +156,9 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
     // column j of U scaled by s_j^-1/2 on the device
     UNO_HIP(c->pk.upload(ev.data(), n, st));
     UNO_HIP(k_scale_cols(n, b.Cw.p, c->pk.p, b.X.p, st));
+    if (o.verbose > 1)
+        std::fprintf(stderr, "[scf] setup: buffers and one-electron matrices %.2f ms, X %.2f ms\n", 1e3 * s_oneint,
+                     1e3 * (secs(t_start, clk::now()) - s_oneint));
 
     double* P[2] = {b.P[0].p, b.P[1].p};
     std::vector<double> evals(2 * (size_t)n, 0.0);
@@ -127,7 +170,9 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
         const auto t0 = clk::now();
         UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, Fd, n, b.X.p, n, 0.0, b.Tm.p, n, st));       // F X
         UNO_HIP(gemm_f64(true, false, n, n, n, 1.0, b.X.p, n, b.Tm.p, n, 0.0, b.Fp.p, n, st));   // X^T F X
+        const auto te = clk::now();
         if (int rc = eigh_device(c, n, b.Fp.p, b.Cw.p, evals.data() + (size_t)s * n)) return rc;
+        if (o.verbose > 1) std::fprintf(stderr, "[scf] eigensolve %.2f ms\n", 1e3 * secs(te, clk::now()));
         UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, b.X.p, n, b.Cw.p, n, 0.0, b.C[s].p, n, st));  // C = X W
         if (write_P)
             UNO_HIP(gemm_f64(false, true, n, n, no, 1.0, b.C[s].p, n, b.C[s].p, n, 0.0, P[s], n, st)); // C_occ C_occ^T
@@ -257,21 +302,15 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
         // replaced by the core-Hamiltonian guess; with guess_check the core-H guess is
         // formed as well and the lower iteration-0 energy wins.
         const auto t0 = clk::now();
-        std::vector<float> X, Y;
-        std::vector<int> pairs;
-        std::string err;
-        if (int rc = guess_features(c->B, X, pairs, err)) { set_error(err); return rc; }
-        const long np = (long)pairs.size() / 2;
-        Y.assign((size_t)np * kGuessOutDim, 0.0f);
-        if (np > 0)
-            if (int rc = uno_mlp_forward(c->guess_model, np, X.data(), Y.data())) return rc;
-        std::vector<double> D;
-        if (int rc = guess_assemble(c->B, pairs, Y.data(), D, err)) { set_error(err); return rc; }
-        for (int i = 0; i < n; ++i)
-            for (int j = 0; j < i; ++j) {
-                const double v = 0.5 * (D[(size_t)i * n + j] + D[(size_t)j * n + i]);
-                D[(size_t)i * n + j] = D[(size_t)j * n + i] = v;
-            }
+        if (int rc = guess_job.get()) {
+            set_error(gj.err);
+            return rc;
+        }
+        std::vector<double>& D = gj.D;
+        if (o.verbose > 1)
+            std::fprintf(stderr, "[scf] guess (worker thread): %ld pairs, descriptors %.2f ms, network %.2f ms, "
+                                 "assembly %.2f ms; waited %.2f ms for it\n", gj.np, 1e3 * gj.feat_s, 1e3 * gj.net_s,
+                         1e3 * gj.asm_s, 1e3 * secs(t0, clk::now()));
         bool ok = true;
         for (double v : D) ok = ok && std::isfinite(v);
         for (int s = 0; s < ns && ok; ++s) {
@@ -384,9 +423,11 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
         const double build_s = secs(t0, clk::now());
         fock_s += build_s;
         if (o.verbose > 1)
-            std::fprintf(stderr, "[scf] build %d: %s %s, %.2f ms, %.3e shell quartets (%.3e FP32)\n", r.nfock + 1,
-                         full ? "full" : "incremental", mixed ? "mixed" : "FP64", 1e3 * build_s, c->last_quartets,
-                         c->last_quartets32);
+            std::fprintf(stderr,
+                         "[scf] build %d: %s %s, %.2f ms (host: bound %.2f, spaces %.2f, staging %.2f, enqueue %.2f), "
+                         "%.3e shell quartets (%.3e FP32)\n",
+                         r.nfock + 1, full ? "full" : "incremental", mixed ? "mixed" : "FP64", 1e3 * build_s, c->last_ms[0],
+                         c->last_ms[1], c->last_ms[2], c->last_ms[3], c->last_quartets, c->last_quartets32);
         r.quartets += c->last_quartets;
         ++r.nfock;
         energy = uhf ? hs[0] + 0.5 * hs[1] : 2.0 * hs[0] + hs[1];
