@@ -65,7 +65,10 @@ int uno_ctx_destroy(uno_ctx* ctx);
  * Qab*Qcd*max|D|, default 1e-13), "schwarz_mixed" (the threshold of uno_scf_run's
  * mixed-precision phase, never below "schwarz"; default 3e-9: that phase ends with a
  * full FP64 rebuild at "schwarz"), "mixed_switch" (|dE| in Eh ending that phase; default
- * 3e-6), "rebuild_every" (incremental builds between full rebuilds; default 10),
+ * 3e-6), "x_newton" (X = S^-1/2 by Newton-Schulz instead of the canonical U s^-1/2: -1,
+ * the default, from 256 functions on; 0 never; 1 always; the canonical X with its checks
+ * takes over when the iteration does not converge; uno_ctx_get "x_newton_steps" reads the
+ * last SCF's step count, -1 for the canonical X), "rebuild_every" (incremental builds between full rebuilds; default 10),
  * "fp32_threshold" (quartets whose bound is below it
  * run in FP32 when "precision" is 1; default 1e-6), "precision" (0 = FP64, 1 = mixed, for the
  * builds of uno_fock_rhf/uno_fock_uhf; uno_scf_run takes its own option),

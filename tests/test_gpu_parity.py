@@ -297,6 +297,33 @@ def test_scf_purification_matches_eigensolve(tag, uhf):
     t.set("purify", -1)
 
 
+@pytest.mark.parametrize("tag,uhf", [("B_w16_631gs", False), ("B_w16_631gs_cation", True), ("631.h2o", False)])
+def test_scf_newton_schulz_x_matches_canonical(tag, uhf):
+    """X = S^-1/2 by Newton-Schulz (x_newton = 1) against the reference's canonical
+    X = U s^-1/2 (x_newton = 0): the same energy (1e-10 Eh), orbital energies and density
+    (1e-9); the reference damping path, as the reference runs it."""
+    say("scf newton-schulz X vs canonical X " + tag)
+    path = os.path.join(R.GOLDEN, "decks", tag + ".dat") if tag.startswith("B_") else R.deck_path(tag)
+    b, t = _tints(path)
+    cls = unomol_amd.UnRestrictedHartreeFock if uhf else unomol_amd.RestrictedHartreeFock
+    out = []
+    for xn in (0, 1):
+        t.set("x_newton", xn)
+        hf = cls(b, t)
+        e = hf.findEnergy(accel=2, eps=1e-10, cflag=3, maxits=100)
+        assert hf.result.converged
+        steps = t.get("x_newton_steps")
+        assert (steps > 0) if xn else (steps == -1), (xn, steps)
+        out.append((e, hf.evals.copy(), hf.P.copy()))
+    t.set("x_newton", -1)
+    (e0, ev0, P0), (e1, ev1, P1) = out
+    say("newton-schulz X: |dE| %.2e, max |d eps| %.2e, max |dP| %.2e" %
+        (abs(e1 - e0), np.abs(ev1 - ev0).max(), np.abs(P1 - P0).max()))
+    assert abs(e1 - e0) < 1e-10
+    np.testing.assert_allclose(ev1, ev0, atol=1e-9)
+    assert np.abs(P1 - P0).max() < 1e-9
+
+
 def test_scf_diis_same_energy(tmp_path):
     say("scf diis water 6-31G*")
     path = R.deck_path("w2_631gs")

@@ -57,6 +57,8 @@ hipError_t k_shell_max(int n, int nsh, const int* sh_off, const int* sh_n, const
                        const double* D2, double* Dsh, hipStream_t st);
 // out[0] = max |x|
 hipError_t k_absmax(long len, const double* x, double* out, hipStream_t st);
+// A_ii += alpha (n x n)
+hipError_t k_shift_diag(int n, double alpha, double* A, hipStream_t st);
 // purification: Gershgorin bounds of F (lohi: 2n work, bounds: 2 doubles) and
 // X = (hi I - F) / (hi - lo); out[0] = tr(A)
 hipError_t sp2_init(int n, const double* F, double* X, double* lohi, double* bounds, hipStream_t st);

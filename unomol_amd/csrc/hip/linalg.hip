@@ -260,6 +260,16 @@ hipError_t k_absmax(long len, const double* x, double* out, hipStream_t st) {
     return hipGetLastError();
 }
 
+// A_ii += alpha (n x n)
+__global__ void k_shift_diag_(int n, double alpha, double* A) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) A[(long)i * n + i] += alpha;
+}
+hipError_t k_shift_diag(int n, double alpha, double* A, hipStream_t st) {
+    hipLaunchKernelGGL(k_shift_diag_, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, alpha, A);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // FP64 MFMA GEMM: 64x64 block tile, 4 waves each owning a 32x32 quarter made of
 // 2x2 tiles of v_mfma_f64_16x16x4_f64.  Operand layout of that instruction:

@@ -140,22 +140,84 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
             nuc += c->B.q[i] * c->B.q[j] / std::sqrt(dx * dx + dy * dy + dz * dz);
         }
     const double s_oneint = secs(t_start, clk::now());
-    // X = U s^-1/2 keeping every vector (RHF.hpp:214-233)
-    std::vector<double> ev(n);
-    UNO_HIP(hipMemcpyAsync(b.Tm.p, b.S.p, nn * sizeof(double), hipMemcpyDeviceToDevice, st));
-    const auto tx = clk::now();
-    if (int rc = eigh_device(c, n, b.Tm.p, b.Cw.p, ev.data())) return rc;
-    if (o.verbose > 1) std::fprintf(stderr, "[scf] eigensolve of S %.2f ms\n", 1e3 * secs(tx, clk::now()));
-    for (int j = 0; j < n; ++j) {
-        double f = ev[j];
-        if (std::fabs(f) < 1e-7 && f < 0.0) f = -f;
-        if (f == 0.0) { set_error("Zero Eigenvalue in Overlap Matrix"); return UNO_ESINGULAR; }
-        if (f < 0.0) { set_error("Negative Eigenvalue in Overlap Matrix"); return UNO_ESINGULAR; }
-        ev[j] = 1.0 / std::sqrt(f);
+    // X with X^T S X = I.  From kSp2MinN functions on (knob x_newton): the Lowdin
+    // S^-1/2 by the coupled Newton-Schulz iteration on S/c (c the Gershgorin bound of S, so
+    // the spectrum lies in (0, 1] when S is positive definite): Y0 = S/c, Z0 = I,
+    // T = (3I - ZY)/2, Y <- YT, Z <- TZ; Z -> (S/c)^-1/2 quadratically once |I - ZY| < 1,
+    // three FP64 MFMA GEMMs per step instead of an eigensolve.  Every quantity the SCF
+    // returns (energies, densities, orbital energies, MO coefficients up to the sign and
+    // rotations within degenerate levels) is the same for any X with X^T S X = I.  When the
+    // iteration does not reach its residual (S singular or indefinite, or too ill-
+    // conditioned), the canonical X of the reference below takes over, with its checks.
+    bool x_done = false;
+    c->last_x_steps = -1;
+    if (c->x_newton == 1 || (c->x_newton < 0 && n >= kSp2MinN)) {
+        DBuf<double> nsw;
+        UNO_HIP(nsw.alloc(2 * (size_t)n + 8));
+        UNO_HIP(sp2_init(n, b.S.p, b.Tm.p, nsw.p, nsw.p + 2 * (size_t)n, st));   // Gershgorin bounds of S
+        double bnd[2] = {0.0, 0.0};
+        UNO_HIP(hipMemcpyAsync(bnd, nsw.p + 2 * (size_t)n, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        UNO_HIP(hipStreamSynchronize(st));
+        const double cs = bnd[1];
+        if (cs > 0.0 && std::isfinite(cs)) {
+            double* Y = b.Tm.p;
+            double* Z = b.Cw.p;
+            double* Tt = b.Fp.p;
+            double* Yn = b.Etmp.p;
+            double* Zn = b.F.p;
+            UNO_HIP(k_axpby(nn, 1.0 / cs, b.S.p, 0.0, b.S.p, Y, st));
+            UNO_HIP(k_fill(nn, 0.0, Z, st));
+            UNO_HIP(k_shift_diag(n, 1.0, Z, st));
+            double res_prev = 1e300;
+            for (int it = 0; it < 80; ++it) {
+                UNO_HIP(gemm_f64(false, false, n, n, n, -0.5, Z, n, Y, n, 0.0, Tt, n, st));   // -ZY/2
+                UNO_HIP(k_shift_diag(n, 1.5, Tt, st));                                       // T = (3I - ZY)/2
+                // |I - ZY| = 2 |T - I|
+                UNO_HIP(k_shift_diag(n, -1.0, Tt, st));
+                UNO_HIP(k_absmax(nn, Tt, c->scal.p + 3, st));
+                UNO_HIP(k_shift_diag(n, 1.0, Tt, st));
+                double res = 0.0;
+                UNO_HIP(hipMemcpyAsync(&res, c->scal.p + 3, sizeof(double), hipMemcpyDeviceToHost, st));
+                UNO_HIP(hipStreamSynchronize(st));
+                res *= 2.0;
+                // the residual stays just below 1 for the first steps, while the small
+                // eigenvalues of S/c grow by about 1.5x per step; only a blow-up (S
+                // indefinite) ends the iteration early
+                if (!std::isfinite(res) || res > 1e3) break;
+                // converged: at rounding level, or no longer falling once below 1e-10
+                if (res < 1e-13 || (res < 1e-10 && res > 0.5 * res_prev)) {
+                    UNO_HIP(k_axpby(nn, 1.0 / std::sqrt(cs), Z, 0.0, Z, b.X.p, st));
+                    x_done = true;
+                    c->last_x_steps = it;
+                    if (o.verbose > 1) std::fprintf(stderr, "[scf] X by Newton-Schulz: %d steps, |I - ZY| %.1e\n", it, res);
+                    break;
+                }
+                res_prev = res;
+                UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, Y, n, Tt, n, 0.0, Yn, n, st));   // Y T
+                UNO_HIP(gemm_f64(false, false, n, n, n, 1.0, Tt, n, Z, n, 0.0, Zn, n, st));   // T Z
+                std::swap(Y, Yn);
+                std::swap(Z, Zn);
+            }
+        }
     }
-    // column j of U scaled by s_j^-1/2 on the device
-    UNO_HIP(c->pk.upload(ev.data(), n, st));
-    UNO_HIP(k_scale_cols(n, b.Cw.p, c->pk.p, b.X.p, st));
+    // X = U s^-1/2 keeping every vector (RHF.hpp:214-233)
+    if (!x_done) {
+        std::vector<double> ev(n);
+        UNO_HIP(hipMemcpyAsync(b.Tm.p, b.S.p, nn * sizeof(double), hipMemcpyDeviceToDevice, st));
+        const auto tx = clk::now();
+        if (int rc = eigh_device(c, n, b.Tm.p, b.Cw.p, ev.data())) return rc;
+        if (o.verbose > 1) std::fprintf(stderr, "[scf] eigensolve of S %.2f ms\n", 1e3 * secs(tx, clk::now()));
+        for (int j = 0; j < n; ++j) {
+            double f = ev[j];
+            if (std::fabs(f) < 1e-7 && f < 0.0) f = -f;
+            if (f == 0.0) { set_error("Zero Eigenvalue in Overlap Matrix"); return UNO_ESINGULAR; }
+            if (f < 0.0) { set_error("Negative Eigenvalue in Overlap Matrix"); return UNO_ESINGULAR; }
+            ev[j] = 1.0 / std::sqrt(f);
+        }
+        // column j of U scaled by s_j^-1/2 on the device
+        UNO_HIP(c->pk.upload(ev.data(), n, st));
+        UNO_HIP(k_scale_cols(n, b.Cw.p, c->pk.p, b.X.p, st));
+    }
     if (o.verbose > 1)
         std::fprintf(stderr, "[scf] setup: buffers and one-electron matrices %.2f ms, X %.2f ms\n", 1e3 * s_oneint,
                      1e3 * (secs(t_start, clk::now()) - s_oneint));
