@@ -68,7 +68,11 @@ int uno_ctx_destroy(uno_ctx* ctx);
  * 3e-6), "x_newton" (X = S^-1/2 by Newton-Schulz instead of the canonical U s^-1/2: -1,
  * the default, from 256 functions on; 0 never; 1 always; the canonical X with its checks
  * takes over when the iteration does not converge; uno_ctx_get "x_newton_steps" reads the
- * last SCF's step count, -1 for the canonical X), "rebuild_every" (incremental builds between full rebuilds; default 10),
+ * last SCF's step count, -1 for the canonical X), "rebuild_every" (incremental builds
+ * between full rebuilds; default 10), "deterministic" (1: Fock builds accumulate J and K
+ * in 128-bit fixed point with integer atomics, every bra as wave tasks, and uno_scf_run's
+ * dense products run without split K, so the same input gives bitwise the same G, energy,
+ * orbital energies and density in every run; default 0, FP64 atomics),
  * "fp32_threshold" (quartets whose bound is below it
  * run in FP32 when "precision" is 1; default 1e-6), "precision" (0 = FP64, 1 = mixed, for the
  * builds of uno_fock_rhf/uno_fock_uhf; uno_scf_run takes its own option),

@@ -489,6 +489,87 @@ def test_fock_rows_vs_waves_config_b(uhf):
         t.set("row_density_lds", -1)
 
 
+@pytest.mark.parametrize("uhf", [False, True])
+def test_fock_deterministic_bitwise(uhf):
+    """deterministic = 1: the Fock build accumulates in fixed point (two int64 words per
+    entry, integer atomics), so repeated builds of the same density give bitwise the same
+    G whatever order the waves run in.  Small deck: against the oracle's brute-force G
+    (1e-11).  16-water 6-31G* deck: three builds bitwise equal, FP64 and mixed precision,
+    and within 1e-11 (FP64) / 1e-5 (all-FP32 quartets) of the default build's scale."""
+    say("fock deterministic, uhf=%s" % uhf)
+    tag = "water_631gs_cation" if uhf else "631.h2o"
+    path = R.deck_path(tag)
+    deck = read_deck(path)
+    n = deck["norb"]
+    rng = np.random.default_rng(21)
+    A = rng.standard_normal((n, n)) * 0.1
+    Bm = rng.standard_normal((n, n)) * 0.1
+    PA, PB = _pack(A + A.T), _pack(Bm + Bm.T)
+    b, t = _tints(path, schwarz=1e-16)
+    t.set("deterministic", 1)
+    assert t.get("deterministic") == 1
+
+    def build(t, PA, PB):
+        if uhf:
+            GA, GB = np.zeros_like(PA), np.zeros_like(PB)
+            t.formGmatrix(PA, PB, GA, GB)
+            return np.concatenate([GA, GB])
+        G = np.zeros_like(PA)
+        t.formGmatrix(PA, G)
+        return G
+
+    G1 = build(t, PA, PB)
+    ref = _oracle_g(deck, PA, PB) if uhf else _oracle_g(deck, PA)
+    ref = np.concatenate(ref) if uhf else ref
+    assert np.abs(G1 - ref).max() < 1e-11
+    assert np.array_equal(build(t, PA, PB), G1)
+
+    path = os.path.join(R.GOLDEN, "decks", "B_w16_631gs.dat")
+    n = read_deck(path)["norb"]
+    rng = np.random.default_rng(22)
+    A = rng.standard_normal((n, n)) * 0.02
+    Bm = rng.standard_normal((n, n)) * 0.02
+    PA, PB = _pack(A + A.T + np.eye(n)), _pack(Bm + Bm.T + np.eye(n))
+    b, t = _tints(path)
+    for prec, thr, tol in ((0, 1e-6, 1e-11), (1, 1e30, 1e-5)):
+        t.set("precision", prec)
+        t.set("fp32_threshold", thr)
+        t.set("deterministic", 0)
+        Gdef = [build(t, PA, PB) for _ in range(3)]
+        scale = max(np.abs(Gdef[0]).max(), 1.0)
+        t.set("deterministic", 1)
+        Gdet = [build(t, PA, PB) for _ in range(3)]
+        say("precision %d: default builds differing from the first: %d of 2 (max %.1e); "
+            "deterministic vs default %.1e" %
+            (prec, sum(not np.array_equal(g, Gdef[0]) for g in Gdef[1:]),
+             max(np.abs(g - Gdef[0]).max() for g in Gdef[1:]), np.abs(Gdet[0] - Gdef[0]).max()))
+        assert all(np.array_equal(g, Gdet[0]) for g in Gdet[1:])
+        assert np.abs(Gdet[0] - Gdef[0]).max() < tol * scale
+
+
+def test_scf_deterministic_bitwise():
+    """A whole SCF with deterministic Fock builds (16-water 6-31G*, mixed precision,
+    incremental builds, purification, DIIS) run twice: bitwise the same energy, orbital
+    energies and density; and the default SCF's energy to 1e-10 Eh."""
+    say("scf deterministic, 16 waters")
+    path = os.path.join(R.GOLDEN, "decks", "B_w16_631gs.dat")
+    b, t = _tints(path)
+    kw = dict(accel=2, eps=1e-8, cflag=3, maxits=100, precision=1, incremental=1)
+    e_def = unomol_amd.RestrictedHartreeFock(b, t).findEnergy(**kw)
+    t.set("deterministic", 1)
+    out = []
+    for _ in range(2):
+        hf = unomol_amd.RestrictedHartreeFock(b, t)
+        e = hf.findEnergy(**kw)
+        assert hf.result.converged
+        out.append((e, hf.evals.copy(), hf.P.copy(), hf.result.iterations))
+    (e0, ev0, P0, it0), (e1, ev1, P1, it1) = out
+    say("deterministic SCF: %.12f (%d its) twice; default %.12f" % (e0, it0, e_def))
+    assert e0 == e1 and it0 == it1
+    assert np.array_equal(ev0, ev1) and np.array_equal(P0, P1)
+    assert abs(e0 - e_def) < 1e-10
+
+
 def test_autotune_keeps_fock_matrix():
     """The autotuner times the row-merge configurations of every quartet space and keeps the
     fastest: the Fock matrix built afterwards (RHF and UHF, FP64 and mixed) equals the

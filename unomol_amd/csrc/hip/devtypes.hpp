@@ -123,6 +123,11 @@ struct EriArgs {
     unsigned long long* queue;
     long long nqblk;
     const int* mol_active;    // per molecule: 0 skips its items (null: every molecule)
+    // deterministic accumulation (0: FP64 atomics): every W entry is two int64 words, the
+    // word at the entry's own address and one fix_lo entries further on, holding the sum in
+    // fixed point (integer additions give the same sum in any order); wave-task and
+    // wave kernels only
+    long long fix_lo;
 };
 
 }  // namespace uno

@@ -162,8 +162,8 @@ template <int MODE>
 __device__ __forceinline__ void jket_add(const EriArgs& a, const MolView& mv, const PairHdr& P2, int k, int l,
                                          int nd, double v) {
     const long o = (long)(P2.f1 + k) * mv.n + P2.f2 + l;
-    atomic_add_f64(a.jp ? a.jp + P2.jo + k * nd + l : mv.W0 + o, v);
-    if (MODE == kDigestUHF && !a.jp) atomic_add_f64(mv.W1 + o, v);
+    fock_add(a, a.jp ? a.jp + P2.jo + k * nd + l : mv.W0 + o, v);
+    if (MODE == kDigestUHF && !a.jp) fock_add(a, mv.W1 + o, v);
 }
 
 template <int LA, int LB, int LC, int LD, int MODE, typename T>
@@ -240,8 +240,8 @@ __device__ __forceinline__ void bra_task(const EriArgs& a, const long long t, co
                 for (int kl = 0; kl < NC * ND; ++kl) s = fma(e[(i * NB + j) * NC * ND + kl], dcd[kl], s);
                 const double tot = wave_sum((double)(jf * s));
                 if (lane == 0) {
-                    atomic_add_f64(mv.W0 + (long)(fa + i) * n + fb + j, tot);
-                    if (MODE == kDigestUHF) atomic_add_f64(mv.W1 + (long)(fa + i) * n + fb + j, tot);
+                    fock_add(a, mv.W0 + (long)(fa + i) * n + fb + j, tot);
+                    if (MODE == kDigestUHF) fock_add(a, mv.W1 + (long)(fa + i) * n + fb + j, tot);
                 }
             }
     }
@@ -291,7 +291,7 @@ __device__ __forceinline__ void bra_task(const EriArgs& a, const long long t, co
                 for (int j = 0; j < NB; ++j)
 #pragma unroll
                     for (int l = 0; l < ND; ++l) s = fma(e[((i * NB + j) * NC + k) * ND + l], dbd[j * ND + l], s);
-                atomic_add_f64(W + (long)(fa + i) * n + fc + k, (double)(-s));
+                fock_add(a, W + (long)(fa + i) * n + fc + k, (double)(-s));
             }
 #pragma unroll
         for (int i = 0; i < NA; ++i)
@@ -302,7 +302,7 @@ __device__ __forceinline__ void bra_task(const EriArgs& a, const long long t, co
                 for (int j = 0; j < NB; ++j)
 #pragma unroll
                     for (int k = 0; k < NC; ++k) s = fma(e[((i * NB + j) * NC + k) * ND + l], dbc[j * NC + k], s);
-                atomic_add_f64(W + (long)(fa + i) * n + fd + l, (double)(-s));
+                fock_add(a, W + (long)(fa + i) * n + fd + l, (double)(-s));
             }
 #pragma unroll
         for (int j = 0; j < NB; ++j)
@@ -313,7 +313,7 @@ __device__ __forceinline__ void bra_task(const EriArgs& a, const long long t, co
                 for (int i = 0; i < NA; ++i)
 #pragma unroll
                     for (int l = 0; l < ND; ++l) s = fma(e[((i * NB + j) * NC + k) * ND + l], dad[i * ND + l], s);
-                atomic_add_f64(W + (long)(fb + j) * n + fc + k, (double)(-s));
+                fock_add(a, W + (long)(fb + j) * n + fc + k, (double)(-s));
             }
 #pragma unroll
         for (int j = 0; j < NB; ++j)
@@ -324,7 +324,7 @@ __device__ __forceinline__ void bra_task(const EriArgs& a, const long long t, co
                 for (int i = 0; i < NA; ++i)
 #pragma unroll
                     for (int k = 0; k < NC; ++k) s = fma(e[((i * NB + j) * NC + k) * ND + l], dac[i * NC + k], s);
-                atomic_add_f64(W + (long)(fb + j) * n + fd + l, (double)(-s));
+                fock_add(a, W + (long)(fb + j) * n + fd + l, (double)(-s));
             }
     }}
 
@@ -1125,16 +1125,16 @@ __device__ __forceinline__ unsigned wave_one(const EriArgs& a, const long long g
                 for (int k = 0; k < NC; ++k)
                     for (int l = 0; l < ND; ++l)
                         s = fma(sE[((i * NB + j) * NC + k) * ND + l], (T)mv.D0[(long)(fc + k) * n + fd + l], s);
-                atomic_add_f64(mv.W0 + (long)(fa + i) * n + fb + j, (double)(jf * s));
-                if (MODE == kDigestUHF) atomic_add_f64(mv.W1 + (long)(fa + i) * n + fb + j, (double)(jf * s));
+                fock_add(a, mv.W0 + (long)(fa + i) * n + fb + j, (double)(jf * s));
+                if (MODE == kDigestUHF) fock_add(a, mv.W1 + (long)(fa + i) * n + fb + j, (double)(jf * s));
             } else {
                 const int kl = o - NA * NB, k = kl / ND, l = kl % ND;
                 for (int i = 0; i < NA; ++i)
                     for (int j = 0; j < NB; ++j)
                         s = fma(sE[((i * NB + j) * NC + k) * ND + l], (T)mv.D0[(long)(fa + i) * n + fb + j], s);
                 // straight into W (a wave's lanes already cover the ket block's contiguous rows)
-                atomic_add_f64(mv.W0 + (long)(fc + k) * n + fd + l, (double)(jf * s));
-                if (MODE == kDigestUHF) atomic_add_f64(mv.W1 + (long)(fc + k) * n + fd + l, (double)(jf * s));
+                fock_add(a, mv.W0 + (long)(fc + k) * n + fd + l, (double)(jf * s));
+                if (MODE == kDigestUHF) fock_add(a, mv.W1 + (long)(fc + k) * n + fd + l, (double)(jf * s));
             }
         }
         constexpr int NK = NA * NC + NA * ND + NB * NC + NB * ND;
@@ -1148,25 +1148,25 @@ __device__ __forceinline__ unsigned wave_one(const EriArgs& a, const long long g
                     for (int j = 0; j < NB; ++j)
                         for (int l = 0; l < ND; ++l)
                             s = fma(sE[((i * NB + j) * NC + k) * ND + l], (T)DK[(long)(fb + j) * n + fd + l], s);
-                    atomic_add_f64(W + (long)(fa + i) * n + fc + k, (double)(-s));
+                    fock_add(a, W + (long)(fa + i) * n + fc + k, (double)(-s));
                 } else if (o < NA * NC + NA * ND) {      // (a,d) <- D_bc
                     const int q = o - NA * NC, i = q / ND, l = q % ND;
                     for (int j = 0; j < NB; ++j)
                         for (int k = 0; k < NC; ++k)
                             s = fma(sE[((i * NB + j) * NC + k) * ND + l], (T)DK[(long)(fb + j) * n + fc + k], s);
-                    atomic_add_f64(W + (long)(fa + i) * n + fd + l, (double)(-s));
+                    fock_add(a, W + (long)(fa + i) * n + fd + l, (double)(-s));
                 } else if (o < NA * NC + NA * ND + NB * NC) {   // (b,c) <- D_ad
                     const int q = o - NA * NC - NA * ND, j = q / NC, k = q % NC;
                     for (int i = 0; i < NA; ++i)
                         for (int l = 0; l < ND; ++l)
                             s = fma(sE[((i * NB + j) * NC + k) * ND + l], (T)DK[(long)(fa + i) * n + fd + l], s);
-                    atomic_add_f64(W + (long)(fb + j) * n + fc + k, (double)(-s));
+                    fock_add(a, W + (long)(fb + j) * n + fc + k, (double)(-s));
                 } else {                                 // (b,d) <- D_ac
                     const int q = o - NA * NC - NA * ND - NB * NC, j = q / ND, l = q % ND;
                     for (int i = 0; i < NA; ++i)
                         for (int k = 0; k < NC; ++k)
                             s = fma(sE[((i * NB + j) * NC + k) * ND + l], (T)DK[(long)(fa + i) * n + fc + k], s);
-                    atomic_add_f64(W + (long)(fb + j) * n + fd + l, (double)(-s));
+                    fock_add(a, W + (long)(fb + j) * n + fd + l, (double)(-s));
                 }
             }
         }
@@ -1302,8 +1302,9 @@ template <int LA, int LB, int LC, int LD, int MODE, typename T>
 static hipError_t launch_rows(const EriArgs& a, int variant, hipStream_t st) {
     using CI = ClassInfo<LA, LB, LC, LD>;
     if constexpr (CI::NCOMP <= kLaneMaxComp && MODE <= kDigestUHF) {
+        // the row kernels merge in LDS with FP64 atomics: no deterministic form
         if (a.nrtask <= 0 || !a.rbra || !a.rpre || a.rseg <= 0 || !a.colmap || !a.colinv || a.ncols <= 0 ||
-            a.ncols > a.n)
+            a.ncols > a.n || a.fix_lo)
             return hipErrorInvalidValue;
         switch (variant) {
             case 1: return launch_rows_v<LA, LB, LC, LD, MODE, T, true, true>(a, st);
@@ -1464,6 +1465,22 @@ __global__ __launch_bounds__(256) void k_jp_scatter(int np, const PairHdr* __res
         W0[o] += v;
         if (W1) W1[o] += v;
     }
+}
+
+// fixed-point W (two int64 words per entry, fix_lo entries apart) to FP64, in place
+__global__ __launch_bounds__(256) void k_fixed_to_f64(long long nn, double* W, long long lo_off) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const long long* w = reinterpret_cast<const long long*>(W);
+    const long long hi = w[i], lo = w[i + lo_off];
+    W[i] = (double)hi * (1.0 / kFixHiScale) + (double)lo * (1.0 / kFixLoScale);
+}
+
+hipError_t fixed_to_f64(long long nn, double* W, long long lo_off, hipStream_t st) {
+    if (nn <= 0) return hipSuccess;
+    if (lo_off < nn) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_fixed_to_f64, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, nn, W, lo_off);
+    return hipGetLastError();
 }
 
 hipError_t jp_scatter(int npairs, const PairHdr* pairs, long long njp, const double* jp, double* W0, double* W1, int n,

@@ -274,6 +274,27 @@ __device__ __forceinline__ bool quartet_of(const EriArgs& a, long long g, int& p
 
 __device__ __forceinline__ void atomic_add_f64(double* p, double v) { unsafeAtomicAdd(p, v); }
 
+// Fixed-point accumulation: v = hi 2^-32 + lo 2^-72 + (rounding below 2^-73), hi and lo
+// added to two int64 words.  Integer addition is associative, so the sum does not depend
+// on the order in which the contributions arrive.  hi holds |W| up to 2^31; lo holds the
+// remainders (|lo| <= 2^39 each) of up to 2^24 contributions per entry.
+constexpr double kFixHiScale = 0x1p32;
+constexpr double kFixLoScale = 0x1p72;
+__device__ __forceinline__ void fixed_add(double* p, long long lo_off, double v) {
+    const double h = rint(v * kFixHiScale);
+    const double r = fma(-h, 1.0 / kFixHiScale, v);   // exact: |r| <= 2^-33, same grid as v
+    const long long hi = (long long)h;
+    const long long lo = (long long)rint(r * kFixLoScale);
+    unsigned long long* w = reinterpret_cast<unsigned long long*>(p);
+    if (hi) atomicAdd(w, (unsigned long long)hi);
+    if (lo) atomicAdd(w + lo_off, (unsigned long long)lo);
+}
+// a Fock-matrix contribution of the wave-task and wave kernels
+__device__ __forceinline__ void fock_add(const EriArgs& a, double* p, double v) {
+    if (a.fix_lo) fixed_add(p, a.fix_lo, v);
+    else unsafeAtomicAdd(p, v);
+}
+
 // density screen: Qab Qcd max(4|D_ab|, 4|D_cd|, |D_ac|, |D_ad|, |D_bc|, |D_bd|) < tau
 __device__ __forceinline__ bool screened(const EriArgs& a, const MolView& v, const PairHdr& P1, const PairHdr& P2,
                                          double q12) {

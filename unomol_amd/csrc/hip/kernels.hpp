@@ -33,6 +33,9 @@ int eri_rows_group(int variant);
 // are disjoint, no atomics); the block size of a pair follows from the next pair's jo
 hipError_t jp_scatter(int npairs, const PairHdr* pairs, long long njp, const double* jp, double* W0, double* W1, int n,
                       hipStream_t st);
+// deterministic builds: W accumulated in fixed point (entry i: int64 words i and i + lo_off)
+// back to FP64 in place, first nn entries
+hipError_t fixed_to_f64(long long nn, double* W, long long lo_off, hipStream_t st);
 hipError_t rys_eval_launch(int n, int npts, const double* x, double* u, double* w, hipStream_t st, bool fp32 = false,
                            bool ref2 = false);
 
@@ -76,6 +79,16 @@ hipError_t sp2_state_read(const void* states, int k, int* cur, int* done, int* s
 // C = alpha op(A) op(B) + beta C  (FP64 MFMA 16x16x4)
 hipError_t gemm_f64(bool ta, bool tb, int m, int n, int k, double alpha, const double* A, int lda, const double* B,
                     int ldb, double beta, double* C, int ldc, hipStream_t st);
+// 1: this thread's gemm_f64 calls give bitwise-repeatable products (no split K, whose
+// chunks are added with atomics); returns the previous setting
+int gemm_deterministic(int on);
+struct GemmDetScope {
+    int prev;
+    explicit GemmDetScope(bool on) : prev(gemm_deterministic(on ? 1 : 0)) {}
+    ~GemmDetScope() { gemm_deterministic(prev); }
+    GemmDetScope(const GemmDetScope&) = delete;
+    GemmDetScope& operator=(const GemmDetScope&) = delete;
+};
 
 // eigensolver pieces (linalg.hip)
 struct EigWork {

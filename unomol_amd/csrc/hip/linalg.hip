@@ -385,13 +385,22 @@ __global__ void k_scale_c(int M, int N, double beta, double* C, int ldc) {
     *c = beta == 0.0 ? 0.0 : beta * *c;
 }
 
+// split K adds the chunks' products with FP64 atomics, in whatever order the blocks
+// finish; a thread that asks for deterministic products runs one block per tile instead
+static thread_local int t_gemm_det = 0;
+int gemm_deterministic(int on) {
+    const int prev = t_gemm_det;
+    t_gemm_det = on;
+    return prev;
+}
+
 hipError_t gemm_f64(bool ta, bool tb, int m, int n, int k, double alpha, const double* A, int lda, const double* B,
                     int ldb, double beta, double* C, int ldc, hipStream_t st) {
     // few output tiles and a long K (the WY back-transform's 32-row products): split K so
     // that the launch has about 512 blocks, each chunk at least 64 long
     const long tiles = (long)((n + 63) / 64) * ((m + 63) / 64);
     int ksplit = 1, kchunk = k;
-    if (tiles < 128 && k >= 256) {
+    if (tiles < 128 && k >= 256 && !t_gemm_det) {
         ksplit = (int)std::min<long>(512 / tiles, k / 64);
         if (ksplit > 1) {
             kchunk = ((k + ksplit - 1) / ksplit + kGemmKc - 1) / kGemmKc * kGemmKc;

@@ -310,6 +310,7 @@ int uno_ctx_set(uno_ctx* c, const char* key, double v) {
     else if (k == "rys2_exact") c->rys2_exact = (int)v;
     else if (k == "purify") c->purify = (int)v;
     else if (k == "x_newton") c->x_newton = (int)v;
+    else if (k == "deterministic") c->deterministic = v != 0.0;
     else { set_error("unknown option " + k); return UNO_EINVAL; }
     return UNO_OK;
 }
@@ -333,6 +334,7 @@ int uno_ctx_get(uno_ctx* c, const char* key, double* v) {
     else if (k == "purify") *v = c->purify;
     else if (k == "x_newton") *v = c->x_newton;
     else if (k == "x_newton_steps") *v = c->last_x_steps;
+    else if (k == "deterministic") *v = c->deterministic;
     else if (k == "last_quartets") *v = c->last_quartets;
     else if (k == "last_quartets32") *v = c->last_quartets32;
     else if (k == "pair_runs") *v = (double)c->cls.size();

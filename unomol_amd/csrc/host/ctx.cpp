@@ -453,8 +453,17 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
     auto ms_since = [](hclk::time_point t) { return std::chrono::duration<double, std::milli>(hclk::now() - t).count(); };
     hipStream_t st = c->st;
     const long nn = (long)c->n * c->n;
-    UNO_HIP(k_fill(nn, 0.0, c->W0.p, st));
-    if (uhf) UNO_HIP(k_fill(nn, 0.0, c->W1.p, st));
+    // deterministic builds: W holds two int64 words per entry (the second nn entries on),
+    // every bra runs as wave tasks and the kets' Coulomb outputs go straight to W
+    const bool det = c->deterministic != 0;
+    const bool rows_on = c->row_merge && !det;
+    const long wlen = det ? 2 * nn : nn;
+    if (det) {
+        UNO_HIP(c->W0.alloc(wlen));
+        UNO_HIP(c->W1.alloc(wlen));
+    }
+    UNO_HIP(k_fill(wlen, 0.0, c->W0.p, st));
+    if (uhf) UNO_HIP(k_fill(wlen, 0.0, c->W1.p, st));
     UNO_HIP(k_fill(c->njp, 0.0, c->JP.p, st));
     // shell-block density maxima for the per-quartet screen, and the global bound
     UNO_HIP(k_shell_max(c->n, c->nsh, c->d_shoff.p, c->d_shn.p, c->D0.p, uhf ? c->D1.p : nullptr,
@@ -510,13 +519,13 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
             if (sp.nq == 0) continue;
             SpaceRec r{&q, prec, 0, 0, eri_uses_wave_tasks(LX.lcls, LY.lcls), 0, 0, 0, 0, 0, 0, 0};
             // row-merge configuration: the tuned one for this space if any, else the defaults
-            int variant = r.tasks && c->row_merge
+            int variant = r.tasks && rows_on
                               ? eri_rows_variant(LX.lcls, LY.lcls, mode, c->run_ncols[q.Y], prec == 1, c->row_variant)
                               : 0;
             long long min_kets = std::max(1LL, c->row_min_kets);
             int dlds = variant && c->row_dens != 0 &&
                        eri_rows_dlds_fits(LX.lcls, LY.lcls, mode, c->run_ncols[q.Y], prec == 1, variant);
-            if (r.tasks && c->row_merge && !c->tuning) {
+            if (r.tasks && rows_on && !c->tuning) {
                 auto it = c->tune.find(std::make_tuple(qi, prec, uhf ? 1 : 0, mixed ? 1 : 0));
                 if (it != c->tune.end()) {
                     const int v = it->second.variant;
@@ -607,7 +616,8 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
         a.nquart = sp.nq;
         a.same_class = q.X == q.Y;
         a.dsh = c->Dsh.p;
-        a.jp = c->JP.p;
+        a.jp = det ? nullptr : c->JP.p;
+        a.fix_lo = det ? nn : 0;
         a.dmax_global = 4.0 * dmax;
         a.D0 = c->D0.p;
         a.W0 = c->W0.p;
@@ -680,7 +690,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
                 if (int rc = launch(0)) return rc;
             return UNO_OK;
         };
-        if (c->tuning && tasks && c->row_merge) {
+        if (c->tuning && tasks && rows_on) {
             // time every candidate on this space (each one run three times, the fastest counts)
             std::vector<TuneChoice> cand{{0, 0, 0}};
             for (int v = 1; v <= 5; ++v)
@@ -728,8 +738,12 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
         (prec ? total32 : total) += (double)sp.nq;
     }
     c->last_ms[3] = ms_since(h3);
+    if (det) {
+        UNO_HIP(fixed_to_f64(nn, c->W0.p, nn, st));
+        if (uhf) UNO_HIP(fixed_to_f64(nn, c->W1.p, nn, st));
+    }
     // the kets' Coulomb outputs, accumulated per pair, into W
-    UNO_HIP(jp_scatter((int)c->pairs.size(), c->d_pairs.p, c->njp, c->JP.p, c->W0.p, uhf ? c->W1.p : nullptr, c->n, st));
+    if (!det) UNO_HIP(jp_scatter((int)c->pairs.size(), c->d_pairs.p, c->njp, c->JP.p, c->W0.p, uhf ? c->W1.p : nullptr, c->n, st));
     c->last_quartets = total + total32;
     c->last_quartets32 = total32;
     if (c->profile) {

@@ -165,9 +165,13 @@ struct uno_ctx {
     long long row_min_kets = uno::kRowMinKets;
     int row_variant = 0;     // row-merged LDS variant: 0 the best that fits, 1-3 forced
     int rys2_exact = 0;      // 1: exact two-root Rys quadrature; 0: the reference's (default)
-    int purify = -1;
-    int x_newton = -1;   // X = S^-1/2 by Newton-Schulz: -1 from kSp2MinN functions on, 0 never, 1 always
-    int last_x_steps = -1;   // the last SCF's Newton-Schulz steps (-1: the canonical X)         // SCF densities by purification: -1 from 256 functions, 0 never, 1 always
+    int purify = -1;         // SCF densities by purification: -1 from 256 functions, 0 never, 1 always
+    int x_newton = -1;       // X = S^-1/2 by Newton-Schulz: -1 from kSp2MinN functions on, 0 never, 1 always
+    int last_x_steps = -1;   // the last SCF's Newton-Schulz steps (-1: the canonical X)
+    // 1: Fock builds accumulate in fixed point (bitwise the same G for the same input, in
+    // any launch order) and run every bra as wave tasks (the row kernels merge with FP64
+    // atomics); the SCF's dense products run without split K
+    int deterministic = 0;
     int row_dens = -1;       // exchange density rows in LDS: -1 when they fit, 0 never, 1 forced where they fit
     // autotuned row-merge configuration per quartet space (run-pair index, precision, UHF,
     // mixed build): LDS variant (0 = wave tasks only) and the ket count from which a bra is

@@ -49,6 +49,8 @@ constexpr double kSp2Idem = 1e-8;
 int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, double* evals_out, double* P_io,
             double* C_out, const double* dh_extra) {
     const auto t_start = clk::now();
+    // a deterministic context: the dense products repeat bitwise as well
+    const GemmDetScope gemm_det(c->deterministic != 0);
     std::memset(&r, 0, sizeof(r));
     const int n = c->n;
     const long nn = (long)n * n;
