@@ -942,6 +942,25 @@ __device__ __forceinline__ V sel3(int ax, V a0, V a1, V a2) {
 // one 64-lane wave (one block) per shell quartet, already resolved (and screened) to its
 // bra pair, ket pair, entry and bound
 // ---------------------------------------------------------------------------
+// per component c of a class (the wave kernel's order): its 2-D table index on each
+// axis (x | y << 8 | z << 16) and the number of x^2, y^2, z^2 d functions among its four
+// (<< 24), fixed at compile time
+template <int LA, int LB, int LC, int LD>
+struct WaveCompTab {
+    static_assert(LA <= 2 && LB <= 2 && LC <= 2 && LD <= 2, "the wave kernel's transfer uses binom_le2");
+    int v[ClassInfo<LA, LB, LC, LD>::NCOMP];
+    constexpr WaveCompTab() : v() {
+        using CI = ClassInfo<LA, LB, LC, LD>;
+        for (int c = 0; c < CI::NCOMP; ++c) {
+            const int l = c % CI::ND, k = (c / CI::ND) % CI::NC, j = (c / (CI::ND * CI::NC)) % CI::NB,
+                      i = c / (CI::ND * CI::NC * CI::NB);
+            const int nd2 = (cart_nrm(LA, i) != 1.0) + (cart_nrm(LB, j) != 1.0) + (cart_nrm(LC, k) != 1.0) +
+                            (cart_nrm(LD, l) != 1.0);
+            v[c] = CI::tix(i, j, k, l, 0) | CI::tix(i, j, k, l, 1) << 8 | CI::tix(i, j, k, l, 2) << 16 | nd2 << 24;
+        }
+    }
+};
+
 template <int LA, int LB, int LC, int LD, int MODE, typename T>
 __device__ __forceinline__ unsigned wave_one(const EriArgs& a, const long long g, const int pab, const int pcd,
                                              const int entry) {
@@ -965,35 +984,18 @@ __device__ __forceinline__ unsigned wave_one(const EriArgs& a, const long long g
     const PairHdr& P1 = sP[0];
     const PairHdr& P2 = sP[1];
     const MolView mv = mol_view(a, entry);
-    // each lane owns components lane + 64 m; precompute their table indices
+    // each lane owns components lane + 64 m: their table indices from the class's table
+    static constexpr WaveCompTab<LA, LB, LC, LD> kComp{};
     int ix[PER], iy[PER], iz[PER];
     T acc[PER];
 #pragma unroll
     for (int m = 0; m < PER; ++m) {
         acc[m] = 0;
         const int c = lane + 64 * m;
-        const int cc = c < NCOMP ? c : 0;
-        const int l = cc % ND, k = (cc / ND) % NC, j = (cc / (ND * NC)) % NB, i = cc / (ND * NC * NB);
-        int tx = 0, ty = 0, tz = 0;
-        // table index per axis: ((a*(LB+1)+b)*(LC+1)+c)*(LD+1)+d with a = power of comp i, ...
-        int pw[4][3];
-        for (int ax = 0; ax < 3; ++ax) {
-            pw[0][ax] = 0; pw[1][ax] = 0; pw[2][ax] = 0; pw[3][ax] = 0;
-        }
-        {
-            int idx = 0;
-            for (int x = LA; x >= 0; --x) for (int y = LA - x; y >= 0; --y) { if (idx == i) { pw[0][0] = x; pw[0][1] = y; pw[0][2] = LA - x - y; } ++idx; }
-            idx = 0;
-            for (int x = LB; x >= 0; --x) for (int y = LB - x; y >= 0; --y) { if (idx == j) { pw[1][0] = x; pw[1][1] = y; pw[1][2] = LB - x - y; } ++idx; }
-            idx = 0;
-            for (int x = LC; x >= 0; --x) for (int y = LC - x; y >= 0; --y) { if (idx == k) { pw[2][0] = x; pw[2][1] = y; pw[2][2] = LC - x - y; } ++idx; }
-            idx = 0;
-            for (int x = LD; x >= 0; --x) for (int y = LD - x; y >= 0; --y) { if (idx == l) { pw[3][0] = x; pw[3][1] = y; pw[3][2] = LD - x - y; } ++idx; }
-        }
-        tx = ((pw[0][0] * (LB + 1) + pw[1][0]) * (LC + 1) + pw[2][0]) * (LD + 1) + pw[3][0];
-        ty = ((pw[0][1] * (LB + 1) + pw[1][1]) * (LC + 1) + pw[2][1]) * (LD + 1) + pw[3][1];
-        tz = ((pw[0][2] * (LB + 1) + pw[1][2]) * (LC + 1) + pw[2][2]) * (LD + 1) + pw[3][2];
-        ix[m] = tx; iy[m] = ty; iz[m] = tz;
+        const int cw = kComp.v[c < NCOMP ? c : 0];
+        ix[m] = cw & 255;
+        iy[m] = (cw >> 8) & 255;
+        iz[m] = (cw >> 16) & 255;
     }
     unsigned long long nprimq = 0;
     for (int ip = 0; ip < P1.nprim; ++ip) {
@@ -1058,10 +1060,10 @@ __device__ __forceinline__ unsigned wave_one(const EriArgs& a, const long long g
                 for (int i = 0; i <= b; ++i) {
                     T cdp = 1, h = 0;
                     for (int j = 0; j <= d; ++j) {
-                        h = fma((T)binom_c(d, j) * cdp, G[(aa + b - i) * (F + 1) + c + d - j], h);
+                        h = fma((T)binom_le2(d, j) * cdp, G[(aa + b - i) * (F + 1) + c + d - j], h);
                         cdp *= cd;
                     }
-                    v = fma((T)binom_c(b, i) * abp, h, v);
+                    v = fma((T)binom_le2(b, i) * abp, h, v);
                     abp *= ab;
                 }
                 sI[ra][t2] = v;
@@ -1085,19 +1087,9 @@ __device__ __forceinline__ unsigned wave_one(const EriArgs& a, const long long g
     for (int m = 0; m < PER; ++m) {
         const int c = lane + 64 * m;
         if (c < NCOMP) {
-            const int l = c % ND, k = (c / ND) % NC, j = (c / (ND * NC)) % NB, i = c / (ND * NC * NB);
+            // component normalisation: 1/sqrt(3) per x^2, y^2 or z^2 d function among the four
             T cn = 1;
-            // component normalisation
-            {
-                int idx = 0;
-                for (int x = LA; x >= 0; --x) for (int y = LA - x; y >= 0; --y) { if (idx == i && (x == 2 || y == 2 || LA - x - y == 2)) cn *= 0.57735026918962576451; ++idx; }
-                idx = 0;
-                for (int x = LB; x >= 0; --x) for (int y = LB - x; y >= 0; --y) { if (idx == j && (x == 2 || y == 2 || LB - x - y == 2)) cn *= 0.57735026918962576451; ++idx; }
-                idx = 0;
-                for (int x = LC; x >= 0; --x) for (int y = LC - x; y >= 0; --y) { if (idx == k && (x == 2 || y == 2 || LC - x - y == 2)) cn *= 0.57735026918962576451; ++idx; }
-                idx = 0;
-                for (int x = LD; x >= 0; --x) for (int y = LD - x; y >= 0; --y) { if (idx == l && (x == 2 || y == 2 || LD - x - y == 2)) cn *= 0.57735026918962576451; ++idx; }
-            }
+            for (int q = kComp.v[c] >> 24; q > 0; --q) cn *= 0.57735026918962576451;
             sE[c] = acc[m] * f * cn;
         }
     }
