@@ -942,12 +942,70 @@ __device__ __forceinline__ V sel3(int ax, V a0, V a1, V a2) {
 // one 64-lane wave (one block) per shell quartet, already resolved (and screened) to its
 // bra pair, ket pair, entry and bound
 // ---------------------------------------------------------------------------
+// One (root, axis) of the wave kernel: the 2-D recurrence G[e][f] scaled by wz (the weight
+// times the prefactor on the z axis), the ket transfer H[e][c][d] = sum_j C(d,j) cd^j
+// G[e][c+d-j] and the bra transfer I[a][b][c][d] = sum_i C(b,i) ab^i H[a+b-i][c][d], all
+// unrolled at compile time, each I entry stored (to LDS) as it is formed
+template <int LA, int LB, int LC, int LD, typename T>
+__device__ __forceinline__ void wave_axis(T C, T Cp, T B00, T B1, T B1p, T ab, T cd, T wz, T* I) {
+    constexpr int E = LA + LB, F = LC + LD;
+    T G[E + 1][F + 1];
+    G[0][0] = 1;
+    if constexpr (E > 0) G[1][0] = C;
+#pragma unroll
+    for (int e = 1; e < E; ++e) G[e + 1][0] = (T)e * B1 * G[e - 1][0] + C * G[e][0];
+#pragma unroll
+    for (int f = 0; f < F; ++f)
+#pragma unroll
+        for (int e = 0; e <= E; ++e) {
+            T v = Cp * G[e][f];
+            if (f > 0) v = fma((T)f * B1p, G[e][f - 1], v);
+            if (e > 0) v = fma((T)e * B00, G[e - 1][f], v);
+            G[e][f + 1] = v;
+        }
+#pragma unroll
+    for (int e = 0; e <= E; ++e)
+#pragma unroll
+        for (int f = 0; f <= F; ++f) G[e][f] *= wz;
+    T cdp[LD + 1], abp[LB + 1];
+    cdp[0] = 1;
+#pragma unroll
+    for (int j = 1; j <= LD; ++j) cdp[j] = cdp[j - 1] * cd;
+    abp[0] = 1;
+#pragma unroll
+    for (int i = 1; i <= LB; ++i) abp[i] = abp[i - 1] * ab;
+    T H[E + 1][LC + 1][LD + 1];
+#pragma unroll
+    for (int e = 0; e <= E; ++e)
+#pragma unroll
+        for (int c = 0; c <= LC; ++c)
+#pragma unroll
+            for (int d = 0; d <= LD; ++d) {
+                T v = 0;
+#pragma unroll
+                for (int j = 0; j <= d; ++j) v = fma((T)binom_c(d, j) * cdp[j], G[e][c + d - j], v);
+                H[e][c][d] = v;
+            }
+#pragma unroll
+    for (int a = 0; a <= LA; ++a)
+#pragma unroll
+        for (int b = 0; b <= LB; ++b)
+#pragma unroll
+            for (int c = 0; c <= LC; ++c)
+#pragma unroll
+                for (int d = 0; d <= LD; ++d) {
+                    T v = 0;
+#pragma unroll
+                    for (int i = 0; i <= b; ++i) v = fma((T)binom_c(b, i) * abp[i], H[a + b - i][c][d], v);
+                    I[((a * (LB + 1) + b) * (LC + 1) + c) * (LD + 1) + d] = v;
+                }
+}
+
 // per component c of a class (the wave kernel's order): its 2-D table index on each
 // axis (x | y << 8 | z << 16) and the number of x^2, y^2, z^2 d functions among its four
 // (<< 24), fixed at compile time
 template <int LA, int LB, int LC, int LD>
 struct WaveCompTab {
-    static_assert(LA <= 2 && LB <= 2 && LC <= 2 && LD <= 2, "the wave kernel's transfer uses binom_le2");
     int v[ClassInfo<LA, LB, LC, LD>::NCOMP];
     constexpr WaveCompTab() : v() {
         using CI = ClassInfo<LA, LB, LC, LD>;
@@ -969,7 +1027,6 @@ __device__ __forceinline__ unsigned wave_one(const EriArgs& a, const long long g
     constexpr int E = CI::E, F = CI::F, TBL = CI::TBL;
     constexpr int PER = (NCOMP + 63) / 64;
     __shared__ T sUW[2 * NR];
-    __shared__ T sG[NR * 3][(E + 1) * (F + 1)];
     __shared__ T sI[NR * 3][TBL];
     __shared__ T sE[NCOMP];
     const int lane = threadIdx.x;
@@ -1017,7 +1074,8 @@ __device__ __forceinline__ unsigned wave_one(const EriArgs& a, const long long g
             }
             __syncthreads();
             if (lane < 3 * NR) {
-                // VRR for (root, axis) = (lane / 3, lane % 3)
+                // (root, axis) = (lane / 3, lane % 3): recurrence and transfer in registers,
+                // the transferred table into LDS
                 const int r = lane / 3, ax = lane % 3;
                 const T fff = sUW[r] * it;
                 const T B00 = (T)0.5 * fff, B1 = ((T)0.5 - B00 * q) * ip_, B1p = ((T)0.5 - B00 * p) * iq_;
@@ -1026,47 +1084,9 @@ __device__ __forceinline__ unsigned wave_one(const EriArgs& a, const long long g
                 const T pqa = sel3(ax, PQ[0], PQ[1], PQ[2]);
                 const T C = (T)sel3(ax, pp.P[0] - P1.A[0], pp.P[1] - P1.A[1], pp.P[2] - P1.A[2]) - q * pqa * fff;
                 const T Cp = (T)sel3(ax, qq.P[0] - P2.A[0], qq.P[1] - P2.A[1], qq.P[2] - P2.A[2]) + p * pqa * fff;
-                T* G = sG[lane];
-                T Gl[E + 1][F + 1];
-                Gl[0][0] = 1;
-                if (E > 0) Gl[1][0] = C;
-#pragma unroll
-                for (int e = 1; e < E; ++e) Gl[e + 1][0] = (T)e * B1 * Gl[e - 1][0] + C * Gl[e][0];
-#pragma unroll
-                for (int f = 0; f < F; ++f)
-#pragma unroll
-                    for (int e = 0; e <= E; ++e) {
-                        T v = Cp * Gl[e][f];
-                        if (f > 0) v = fma((T)f * B1p, Gl[e][f - 1], v);
-                        if (e > 0) v = fma((T)e * B00, Gl[e - 1][f], v);
-                        Gl[e][f + 1] = v;
-                    }
-                const T wz = (ax == 2) ? sUW[NR + r] : (T)1;   // weight x prefactor folded into z
-#pragma unroll
-                for (int e = 0; e <= E; ++e)
-#pragma unroll
-                    for (int f = 0; f <= F; ++f) G[e * (F + 1) + f] = Gl[e][f] * wz;
-            }
-            __syncthreads();
-            // binomial transfer, entries split over the lanes
-            for (int q = lane; q < NR * 3 * TBL; q += 64) {
-                const int ra = q / TBL, t2 = q % TBL;
-                const int d = t2 % (LD + 1), c = (t2 / (LD + 1)) % (LC + 1), b = (t2 / ((LD + 1) * (LC + 1))) % (LB + 1),
-                          aa = t2 / ((LD + 1) * (LC + 1) * (LB + 1));
-                const int ax = ra % 3;
                 const T ab = (T)sel3(ax, P1.AB[0], P1.AB[1], P1.AB[2]), cd = (T)sel3(ax, P2.AB[0], P2.AB[1], P2.AB[2]);
-                const T* G = sG[ra];
-                T v = 0, abp = 1;
-                for (int i = 0; i <= b; ++i) {
-                    T cdp = 1, h = 0;
-                    for (int j = 0; j <= d; ++j) {
-                        h = fma((T)binom_le2(d, j) * cdp, G[(aa + b - i) * (F + 1) + c + d - j], h);
-                        cdp *= cd;
-                    }
-                    v = fma((T)binom_le2(b, i) * abp, h, v);
-                    abp *= ab;
-                }
-                sI[ra][t2] = v;
+                const T wz = (ax == 2) ? sUW[NR + r] : (T)1;   // weight x prefactor folded into z
+                wave_axis<LA, LB, LC, LD, T>(C, Cp, B00, B1, B1p, ab, cd, wz, sI[lane]);
             }
             __syncthreads();
 #pragma unroll

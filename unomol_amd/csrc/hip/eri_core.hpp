@@ -54,9 +54,6 @@ __host__ __device__ constexpr double binom_c(int n, int k) {
     for (int i = 0; i < k; ++i) r = r * (n - i) / (i + 1);
     return r;
 }
-// the same for n <= 2 (the transfer of shells up to d) without a division: kernels whose
-// n and k are run-time values use it
-__host__ __device__ constexpr double binom_le2(int n, int k) { return (n == 2 && k == 1) ? 2.0 : 1.0; }
 
 constexpr double kTwoPi25 = 34.986836655249725;   // 2 pi^(5/2)
 
