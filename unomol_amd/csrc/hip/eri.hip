@@ -1043,16 +1043,14 @@ __device__ __forceinline__ unsigned wave_one(const EriArgs& a, const long long g
     const MolView mv = mol_view(a, entry);
     // each lane owns components lane + 64 m: their table indices from the class's table
     static constexpr WaveCompTab<LA, LB, LC, LD> kComp{};
-    int ix[PER], iy[PER], iz[PER];
+    // (kept packed, one register per component, unpacked where used)
+    int cix[PER];
     T acc[PER];
 #pragma unroll
     for (int m = 0; m < PER; ++m) {
         acc[m] = 0;
         const int c = lane + 64 * m;
-        const int cw = kComp.v[c < NCOMP ? c : 0];
-        ix[m] = cw & 255;
-        iy[m] = (cw >> 8) & 255;
-        iz[m] = (cw >> 16) & 255;
+        cix[m] = kComp.v[c < NCOMP ? c : 0];
     }
     unsigned long long nprimq = 0;
     for (int ip = 0; ip < P1.nprim; ++ip) {
@@ -1067,10 +1065,12 @@ __device__ __forceinline__ unsigned wave_one(const EriArgs& a, const long long g
             const T pref = (T)kTwoPi25 * (T)pp.K * (T)qq.K * ip_ * iq_ * rt;
             const T PQ[3] = {(T)(pp.P[0] - qq.P[0]), (T)(pp.P[1] - qq.P[1]), (T)(pp.P[2] - qq.P[2])};
             const T x = p * q * it * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-            if (lane == 0) {
-                T u[NR], w[NR];
-                rys_roots<NR, T>(x, u, w, rys_table_of<NR, T>(a));
-                for (int r = 0; r < NR; ++r) { sUW[r] = u[r]; sUW[NR + r] = w[r] * pref; }
+            if (lane < NR) {
+                // root lane
+                T u, w;
+                rys_root_one<NR, T>(x, lane, u, w, rys_table_of<NR, T>(a));
+                sUW[lane] = u;
+                sUW[NR + lane] = w * pref;
             }
             __syncthreads();
             if (lane < 3 * NR) {
@@ -1092,9 +1092,10 @@ __device__ __forceinline__ unsigned wave_one(const EriArgs& a, const long long g
 #pragma unroll
             for (int m = 0; m < PER; ++m) {
                 if (lane + 64 * m < NCOMP) {
+                    const int ix = cix[m] & 255, iy = (cix[m] >> 8) & 255, iz = (cix[m] >> 16) & 255;
                     T s = acc[m];
 #pragma unroll
-                    for (int r = 0; r < NR; ++r) s = fma(sI[3 * r][ix[m]] * sI[3 * r + 1][iy[m]], sI[3 * r + 2][iz[m]], s);
+                    for (int r = 0; r < NR; ++r) s = fma(sI[3 * r][ix] * sI[3 * r + 1][iy], sI[3 * r + 2][iz], s);
                     acc[m] = s;
                 }
             }

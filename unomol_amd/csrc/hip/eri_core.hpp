@@ -129,6 +129,37 @@ template <int NR, typename T>
 __device__ __forceinline__ void rys_roots(T x, T* u, T* w) {
     rys_roots<NR, T>(x, u, w, rys_table_global<NR, T>());
 }
+// root r alone (0 <= r < NR), the same series as rys_roots: kernels that give each root a
+// lane of its own
+template <int NR, typename T>
+__device__ __forceinline__ void rys_root_one(T x, int r, T& u, T& w, const T* tab) {
+    constexpr bool f32 = sizeof(T) == 4;
+    if (x >= (T)kRysAsym[NR]) {
+        const T isx = rsqrt_nr(x), ix = isx * isx;
+        constexpr int o = NR * (NR - 1) / 2;
+        if constexpr (f32) {
+            u = uno_rys_asym_u_f[o + r] * ix;
+            w = uno_rys_asym_w_f[o + r] * isx;
+        } else {
+            u = uno_rys_asym_u[o + r] * ix;
+            w = uno_rys_asym_w[o + r] * isx;
+        }
+        return;
+    }
+    constexpr int NC = f32 ? UNO_RYS_NCOEF_F : UNO_RYS_NCOEF;
+    const int k = (int)x;
+    const T s = (T)2 * (x - (T)k) - (T)1;
+    const T* cu = tab + k * (2 * NR * NC) + r * NC;
+    const T* cw = cu + NR * NC;
+    T vu = cu[NC - 1], vw = cw[NC - 1];
+#pragma unroll
+    for (int m = NC - 2; m >= 0; --m) {
+        vu = fma(vu, s, cu[m]);
+        vw = fma(vw, s, cw[m]);
+    }
+    u = vu;
+    w = vw;
+}
 // the table of NR roots a launch evaluates: the exact tables, except for two roots, where
 // the reference-compatible table is the default (same layout and size)
 template <int NR, typename T>
