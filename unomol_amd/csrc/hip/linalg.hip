@@ -243,20 +243,31 @@ hipError_t k_shell_max(int n, int nsh, const int* sh_off, const int* sh_n, const
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(1024) void k_absmax_(long len, const double* x, double* out) {
-    __shared__ double sh[1024];
-    double m = 0.0;
-    for (long i = threadIdx.x; i < len; i += 1024) m = fmax(m, fabs(x[i]));
+// max |x_i| over a long vector: |x_i| >= 0 orders as its bit pattern (a NaN above
+// everything, so it propagates), so each block's maximum pattern goes to out with an
+// integer atomicMax after a fill kernel zeroes out (no memset node under graph capture)
+__global__ __launch_bounds__(256) void k_absmax_part(long len, const double* __restrict__ x,
+                                                     unsigned long long* out) {
+    __shared__ unsigned long long sh[256];
+    unsigned long long m = 0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < len; i += (long)gridDim.x * 256) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(x[i]) & 0x7fffffffffffffffull;
+        m = b > m ? b : m;
+    }
     sh[threadIdx.x] = m;
     __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = sh[threadIdx.x] > sh[threadIdx.x + s] ? sh[threadIdx.x] : sh[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = sh[0];
+    if (threadIdx.x == 0 && sh[0]) atomicMax(out, sh[0]);
 }
 hipError_t k_absmax(long len, const double* x, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_absmax_, dim3(1), dim3(1024), 0, st, len, x, out);
+    if (hipError_t e = k_fill(1, 0.0, out, st)) return e;
+    if (len <= 0) return hipSuccess;
+    const long blocks = std::min<long>((len + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_absmax_part, dim3((unsigned)blocks), dim3(256), 0, st, len, x,
+                       reinterpret_cast<unsigned long long*>(out));
     return hipGetLastError();
 }
 

@@ -79,10 +79,17 @@ struct Herm {
     }
 };
 
+// lanes per shell pair: each sums the nuclear attraction over every kOiSplit-th centre,
+// the partial sums then reduced over the group in a fixed order (the centre loop is the
+// kernel's work; one lane per pair left most of the GPU idle)
+constexpr int kOiSplit = 8;
+
 template <int LA, int LB>
 __global__ __launch_bounds__(128) void k_oneint(int npair, const int2* __restrict__ pairs, OneIntArgs a) {
     constexpr int NA = nc1(LA), NB = nc1(LB), L = LA + LB;
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    // groups of kOiSplit consecutive lanes: a group is wholly in or out of range
+    const int g = (int)(gt / kOiSplit), sub = (int)(gt % kOiSplit);
     if (g >= npair) return;
     const int sa = pairs[g].x, sb = pairs[g].y;
     const OneIntShell A = a.shells[sa], B = a.shells[sb];
@@ -116,7 +123,7 @@ __global__ __launch_bounds__(128) void k_oneint(int npair, const int2* __restric
 #pragma unroll
                     for (int z = 0; z <= L; ++z) V[x][y][z] = 0.0;
             const double pref = 2.0 * M_PI * ip * K;
-            for (int c = 0; c < a.ncen; ++c) {
+            for (int c = sub; c < a.ncen; c += kOiSplit) {
                 if (c == a.skip) continue;   // OneElectronInts.cpp:43
                 const double pc[3] = {P[0] - a.xyz[3 * c], P[1] - a.xyz[3 * c + 1], P[2] - a.xyz[3 * c + 2]};
                 double f[L + 1];
@@ -182,6 +189,12 @@ __global__ __launch_bounds__(128) void k_oneint(int npair, const int2* __restric
                     v[o] += nf * vs;
                 }
         }
+    // the group's partial nuclear sums (every lane of the group holds the total after)
+#pragma unroll
+    for (int o = 0; o < NA * NB; ++o)
+#pragma unroll
+        for (int m = 1; m < kOiSplit; m <<= 1) v[o] += __shfl_xor(v[o], m, kOiSplit);
+    if (sub) return;
     const int n = a.n;
 #pragma unroll
     for (int ia = 0; ia < NA; ++ia)
@@ -202,7 +215,7 @@ __global__ __launch_bounds__(128) void k_oneint(int npair, const int2* __restric
 
 hipError_t oneint_launch(int la, int lb, int npair, const int2* pairs, const OneIntArgs& a, hipStream_t st) {
     if (npair <= 0) return hipSuccess;
-    const dim3 grid((npair + 127) / 128), block(128);
+    const dim3 grid((unsigned)(((long)npair * kOiSplit + 127) / 128)), block(128);
 #define UNO_O(LA_, LB_)                                                                          \
     if (la == LA_ && lb == LB_) {                                                                \
         hipLaunchKernelGGL((k_oneint<LA_, LB_>), grid, block, 0, st, npair, pairs, a);           \

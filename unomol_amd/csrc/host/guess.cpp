@@ -270,7 +270,10 @@ int guess_features(const Basis& B, std::vector<float>& X, std::vector<int>& pair
     pairs_within_cut(B, pp);
     X.assign(pp.size() * kGuessIn, 0.0f);
     pairs.assign(2 * pp.size(), 0);
-    for (size_t k = 0; k < pp.size(); ++k) {
+    // pairs are independent (each writes its own row of X)
+    const long npp = (long)pp.size();
+#pragma omp parallel for schedule(dynamic, 32)
+    for (long k = 0; k < npp; ++k) {
         descriptor(B, F, pp[k].first, pp[k].second, &X[k * kGuessIn]);
         pairs[2 * k] = pp[k].first;
         pairs[2 * k + 1] = pp[k].second;
@@ -285,9 +288,14 @@ int guess_assemble(const Basis& B, const std::vector<int>& pairs, const float* Y
     if (!build_frames(B, F, err)) return UNO_EINVAL;
     const int n = B.norb;
     P.assign((size_t)n * n, 0.0);
+    const long np = (long)(pairs.size() / 2);
+    // the pairs' blocks of P are disjoint (a pair (a, b) with a < b writes the (a, b) and
+    // (b, a) blocks, (a, a) the diagonal block), so the pairs run in parallel
+#pragma omp parallel
+    {
     std::vector<double> Ma, Mb, T;
-    const size_t np = pairs.size() / 2;
-    for (size_t k = 0; k < np; ++k) {
+#pragma omp for schedule(dynamic, 16)
+    for (long k = 0; k < np; ++k) {
         const int a = pairs[2 * k], b = pairs[2 * k + 1];
         double Qt[9];
         transpose3(F.R[a].data(), Qt);
@@ -316,6 +324,7 @@ int guess_assemble(const Basis& B, const std::vector<int>& pairs, const float* Y
                     P[(size_t)fb[j] * n + fa[i]] = s;
                 }
             }
+    }
     }
     return UNO_OK;
 }

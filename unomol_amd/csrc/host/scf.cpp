@@ -141,6 +141,9 @@ int scf_run(uno_ctx* c, int nelec, const uno_scf_opts& o, uno_scf_result& r, dou
             const double dx = a[0] - bb[0], dy = a[1] - bb[1], dz = a[2] - bb[2];
             nuc += c->B.q[i] * c->B.q[j] / std::sqrt(dx * dx + dy * dy + dz * dz);
         }
+    // (verbose 2: wait for the one-electron kernels, so that the time printed below
+    // separates them from X)
+    if (o.verbose > 1) UNO_HIP(hipStreamSynchronize(st));
     const double s_oneint = secs(t_start, clk::now());
     // X with X^T S X = I.  From kSp2MinN functions on (knob x_newton): the Lowdin
     // S^-1/2 by the coupled Newton-Schulz iteration on S/c (c the Gershgorin bound of S, so
