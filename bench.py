@@ -497,6 +497,27 @@ def main():
     log("errors vs FP64: dE %.2e, max |d eps| %.2e (occupied %.2e), max |dP| %.2e" %
         (err["energy"], err["orbital_energy_max_abs"], err["occupied_orbital_energy_max_abs"],
          err["density_max_abs"]))
+    # deterministic Fock builds (fixed-point J/K accumulation, wave tasks only) beside the
+    # default FP64 build on the FP64 run's converged density: wall time per build (packed P
+    # up, packed G down included), bitwise repeat, largest difference from the default
+    def timed_build(ctx):
+        G = np.zeros_like(P64)
+        t0 = time.perf_counter()
+        ctx.formGmatrix(P64, G)
+        return G, 1e3 * (time.perf_counter() - t0)
+
+    timed_build(t64)
+    Gdef, ms_def = timed_build(t64)
+    t64.set("deterministic", 1)
+    timed_build(t64)
+    Gdet, ms_det = timed_build(t64)
+    Gdet2, _ = timed_build(t64)
+    t64.set("deterministic", 0)
+    det = {"build_ms_default": round(ms_def, 2), "build_ms_deterministic": round(ms_det, 2),
+           "repeat_bitwise": bool(np.array_equal(Gdet, Gdet2)),
+           "max_abs_diff_vs_default": float(np.abs(Gdet - Gdef).max())}
+    log("deterministic build: %.1f ms (default %.1f ms), repeat bitwise %s, max |dG| %.1e" %
+        (ms_det, ms_def, det["repeat_bitwise"], det["max_abs_diff_vs_default"]))
     # streaming copy of 1 GiB: the HBM reference point for the traffic counters
     cms = C.c_double()
     N.check(N.lib().uno_copy_probe(C.c_ulonglong(1 << 30), 1, C.byref(cms)))
@@ -542,6 +563,7 @@ def main():
         "ablation": ablation,
         "guess_model": model_info,
         "autotune": tune,
+        "deterministic_fock": det,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "config_b": cfgb,
