@@ -211,7 +211,9 @@ int uno_autotune(uno_ctx* ctx, const double* P, const double* PB, int mixed, dou
  * profiling mode in which every Fock build records, per quartet class (bra pair class
  * X, ket pair class Y, index 6*X+Y; pair classes ss ps pp ds dp dd), the kernel time
  * (HIP events, ms), launches, and the primitive and shell quartets evaluated.
- * enable=0 turns it off; stats are read with enable=-1.  Arrays hold 432 entries,
+ * enable=0 turns it off (and clears the stats); stats are read with enable=-1, which
+ * waits for the context's stream: builds record their events and counters without
+ * synchronising, and the read folds them in.  Arrays hold 432 entries,
  * index 72*kind + 36*prec + 6*X + Y: kind 0 the per-wave-task (or per-quartet) kernels,
  * kinds 1-3 the row-merged kernels (1: every bra row and the Rys table in LDS, 2: every
  * bra row, 3: the first bra shell's rows), kinds 4-5 the bra-group row kernels (groups

@@ -513,7 +513,9 @@ int uno_profile(uno_ctx* c, int enable, double* ms, double* launches, double* pr
     const int ncls = kProfSlots;
     if (enable >= 0) {
         c->profile = enable;
-        for (int i = 0; i < ncls; ++i) c->prof_ms[i] = c->prof_launch[i] = c->prof_prim[i] = c->prof_shell[i] = 0.0;
+        if (int rc = profile_reset(c, enable != 0)) return rc;
+    } else if (int rc = profile_collect(c)) {
+        return rc;
     }
     for (int i = 0; i < ncls; ++i) {
         if (ms) ms[i] = c->prof_ms[i];

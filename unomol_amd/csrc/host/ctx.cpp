@@ -477,12 +477,6 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
     c->last_ms[0] = ms_since(h0);
     const auto h1 = hclk::now();
     double total = 0.0, total32 = 0.0;
-    std::vector<hipEvent_t> ev0, ev1;
-    std::vector<int> evcls;
-    if (c->profile) {
-        UNO_HIP(c->d_counter.alloc(2 * kCounterSlots * kProfSlots));
-        UNO_HIP(hipMemsetAsync(c->d_counter.p, 0, 2 * kCounterSlots * kProfSlots * sizeof(unsigned long long), st));
-    }
     const int mode = uhf ? kDigestUHF : kDigestRHF;
 
     // Pass 1 (host): every non-empty space's configuration and index arrays, appended to one
@@ -666,16 +660,22 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
         // one launch per kernel kind, each with its own profiling slot
         auto launch = [&](int kind) -> int {
             const int slot = kind * 2 * kNumPairClasses * kNumPairClasses + ci;
-            hipEvent_t e0 = nullptr, e1 = nullptr;
-            const bool prof = c->profile && !c->tuning;
+            const bool prof = c->profile && !c->tuning && c->d_counter.p;
+            hipEvent_t e1 = nullptr;
             if (prof) {
                 a.counter = c->d_counter.p + 2 * kCounterSlots * slot;
-                UNO_HIP(hipEventCreate(&e0));
-                UNO_HIP(hipEventCreate(&e1));
-                ev0.push_back(e0);
-                ev1.push_back(e1);
-                evcls.push_back(slot);
-                UNO_HIP(hipEventRecord(e0, st));
+                const size_t k = c->prof_pending++;
+                while (c->prof_ev.size() < 2 * (k + 1)) {
+                    hipEvent_t e = nullptr;
+                    UNO_HIP(hipEventCreate(&e));
+                    c->prof_ev.push_back(e);
+                }
+                if (c->prof_evslot.size() < k + 1) c->prof_evslot.resize(k + 1);
+                c->prof_evslot[k] = slot;
+                e1 = c->prof_ev[2 * k + 1];
+                UNO_HIP(hipEventRecord(c->prof_ev[2 * k], st));
+            } else {
+                a.counter = nullptr;
             }
             if (kind) UNO_HIP(eri_launch_rows(mode, LX.lcls, LY.lcls, a, kind, st, prec == 1));
             else UNO_HIP(eri_launch(mode, LX.lcls, LY.lcls, a, st, prec == 1));
@@ -746,23 +746,42 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
     if (!det) UNO_HIP(jp_scatter((int)c->pairs.size(), c->d_pairs.p, c->njp, c->JP.p, c->W0.p, uhf ? c->W1.p : nullptr, c->n, st));
     c->last_quartets = total + total32;
     c->last_quartets32 = total32;
-    if (c->profile) {
-        UNO_HIP(hipStreamSynchronize(st));
-        std::vector<unsigned long long> cnt(2 * kCounterSlots * kProfSlots);
-        UNO_HIP(hipMemcpy(cnt.data(), c->d_counter.p, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-        for (size_t k = 0; k < ev0.size(); ++k) {
-            float ms = 0.f;
-            UNO_HIP(hipEventElapsedTime(&ms, ev0[k], ev1[k]));
-            c->prof_ms[evcls[k]] += ms;
-            c->prof_launch[evcls[k]] += 1.0;
-            (void)hipEventDestroy(ev0[k]);
-            (void)hipEventDestroy(ev1[k]);
+    return UNO_OK;
+}
+
+int profile_reset(uno_ctx* c, bool enable) {
+    for (int i = 0; i < kProfSlots; ++i) c->prof_ms[i] = c->prof_launch[i] = c->prof_prim[i] = c->prof_shell[i] = 0.0;
+    c->prof_pending = 0;   // pairs of earlier builds are dropped (their events are re-recorded)
+    if (enable) {
+        // the counters accumulate on the device from here until the next reset
+        UNO_HIP(c->d_counter.alloc(2 * kCounterSlots * kProfSlots));
+        // (all-zero bits: a fill of 0.0 clears the 64-bit counters)
+        UNO_HIP(k_fill(2 * kCounterSlots * kProfSlots, 0.0, reinterpret_cast<double*>(c->d_counter.p), c->st));
+    }
+    return UNO_OK;
+}
+
+int profile_collect(uno_ctx* c) {
+    if (!c->d_counter.p) return UNO_OK;
+    UNO_HIP(hipStreamSynchronize(c->st));
+    for (size_t k = 0; k < c->prof_pending; ++k) {
+        float ms = 0.f;
+        UNO_HIP(hipEventElapsedTime(&ms, c->prof_ev[2 * k], c->prof_ev[2 * k + 1]));
+        c->prof_ms[c->prof_evslot[k]] += ms;
+        c->prof_launch[c->prof_evslot[k]] += 1.0;
+    }
+    c->prof_pending = 0;
+    std::vector<unsigned long long> cnt(2 * kCounterSlots * kProfSlots);
+    UNO_HIP(hipMemcpy(cnt.data(), c->d_counter.p, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    // device totals since the last reset
+    for (int ci = 0; ci < kProfSlots; ++ci) {
+        double pr = 0.0, sh = 0.0;
+        for (int sl = 0; sl < kCounterSlots; ++sl) {
+            pr += (double)cnt[2 * (kCounterSlots * ci + sl)];
+            sh += (double)cnt[2 * (kCounterSlots * ci + sl) + 1];
         }
-        for (int ci = 0; ci < kProfSlots; ++ci)
-            for (int sl = 0; sl < kCounterSlots; ++sl) {
-                c->prof_prim[ci] += (double)cnt[2 * (kCounterSlots * ci + sl)];
-                c->prof_shell[ci] += (double)cnt[2 * (kCounterSlots * ci + sl) + 1];
-            }
+        c->prof_prim[ci] = pr;
+        c->prof_shell[ci] = sh;
     }
     return UNO_OK;
 }

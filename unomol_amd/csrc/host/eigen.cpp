@@ -164,5 +164,7 @@ uno_ctx::~uno_ctx() {
     if (stage_ev) (void)hipEventSynchronize(stage_ev);
     if (stage_pin) (void)hipHostFree(stage_pin);
     if (stage_ev) (void)hipEventDestroy(stage_ev);
+    if (st) (void)hipStreamSynchronize(st);
+    for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
     if (st) (void)hipStreamDestroy(st);
 }

@@ -184,6 +184,13 @@ struct uno_ctx {
     double prof_shell[uno::kProfSlots] = {};
     double prof_launch[uno::kProfSlots] = {};
     uno::DBuf<unsigned long long> d_counter;
+    // profiled launches are bracketed by event pairs from a reusable pool (pair k: events
+    // 2k, 2k+1; prof_evslot[k] its slot); builds leave them pending, and profile_collect
+    // reads them (and the device counters) when the results are asked for, so that a
+    // profiled SCF runs without a synchronisation per build
+    std::vector<hipEvent_t> prof_ev;
+    std::vector<int> prof_evslot;
+    size_t prof_pending = 0;
     uno_mlp* guess_model = nullptr;   // not owned: the network of guess = 2
     int skip_center = -1;             // centre left out of the nuclear attraction (DPM positron)
     // one-electron data on the device: shells, primitives, centres and the shell pairs
@@ -206,6 +213,10 @@ void build_blocks(QuartetSpace& s, int qpb);
 EriArgs base_args(uno_ctx* c);
 // keeps c's tuned configurations for later contexts of the same layout
 void remember_tuning(uno_ctx* c);
+// profiling: reset (and, when enabling, zero the device counters), or fold the pending
+// event pairs and the counters into the context's per-slot totals
+int profile_reset(uno_ctx* c, bool enable);
+int profile_collect(uno_ctx* c);
 // dense S, T, H = T + V (n x n, device) on the context's stream
 int oneint_device(uno_ctx* c, double* S, double* T, double* H);
 // Fock build from dense device densities already in c->D0 (and D1, D2 for UHF).
