@@ -96,9 +96,12 @@ int uno_ctx_destroy(uno_ctx* ctx);
  * -1, the default, purification from 256 functions). */
 int uno_ctx_set(uno_ctx* ctx, const char* key, double value);
 /* Reads the knobs above and, read-only: "last_quartets" (shell quartets evaluated by the
- * last Fock build), "last_quartets32" (of which in FP32), "pairs" (significant shell pairs), "pair_runs" (runs of the pair
- * array: one per angular pair class and primitive-count bucket, at most 36) and
- * "quartet_launches" (ERI launches per Fock build). */
+ * last Fock build), "last_quartets32" (of which in FP32), "pairs" (significant shell
+ * pairs), "pair_runs" (runs of the pair array: one per angular pair class and
+ * primitive-count bucket, at most 36), "quartet_launches" (ERI launches per Fock build),
+ * and the context build's host milliseconds: "ctx_ms_pairs" (pair and primitive records
+ * and their upload), "ctx_ms_schwarz" (Schwarz bounds on the device), "ctx_ms_runs"
+ * (sorted runs, applied on the device) and "ctx_ms_rest". */
 int uno_ctx_get(uno_ctx* ctx, const char* key, double* value);
 int uno_ctx_norb(const uno_ctx* ctx);
 

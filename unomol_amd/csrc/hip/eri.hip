@@ -1480,6 +1480,31 @@ __global__ __launch_bounds__(256) void k_jp_scatter(int np, const PairHdr* __res
     }
 }
 
+// the pair and primitive arrays reordered on the device: new pair k is old pair order[k],
+// its primitive records moved to prim0[k] and its Coulomb block offset set to jo[k]
+__global__ __launch_bounds__(256) void k_permute_pairs(int np, const int* __restrict__ order,
+                                                       const int* __restrict__ prim0, const int* __restrict__ jo,
+                                                       const PairHdr* __restrict__ oldp,
+                                                       const PrimPair* __restrict__ oldq, PairHdr* __restrict__ newp,
+                                                       PrimPair* __restrict__ newq) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= np) return;
+    PairHdr h = oldp[order[k]];
+    const int p0 = h.prim0;
+    h.prim0 = prim0[k];
+    h.jo = jo[k];
+    newp[k] = h;
+    for (int q = 0; q < h.nprim; ++q) newq[h.prim0 + q] = oldq[p0 + q];
+}
+
+hipError_t permute_pairs(int np, const int* order, const int* prim0, const int* jo, const PairHdr* oldp,
+                         const PrimPair* oldq, PairHdr* newp, PrimPair* newq, hipStream_t st) {
+    if (np <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_permute_pairs, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, np, order, prim0, jo,
+                       oldp, oldq, newp, newq);
+    return hipGetLastError();
+}
+
 // fixed-point W (two int64 words per entry, fix_lo entries apart) to FP64, in place
 __global__ __launch_bounds__(256) void k_fixed_to_f64(long long nn, double* W, long long lo_off) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;

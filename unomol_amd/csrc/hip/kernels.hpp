@@ -33,6 +33,10 @@ int eri_rows_group(int variant);
 // are disjoint, no atomics); the block size of a pair follows from the next pair's jo
 hipError_t jp_scatter(int npairs, const PairHdr* pairs, long long njp, const double* jp, double* W0, double* W1, int n,
                       hipStream_t st);
+// new pair k = old pair order[k], its primitives at prim0[k], Coulomb block at jo[k]
+// (device arrays; new ones must not overlap the old)
+hipError_t permute_pairs(int np, const int* order, const int* prim0, const int* jo, const PairHdr* oldp,
+                         const PrimPair* oldq, PairHdr* newp, PrimPair* newq, hipStream_t st);
 // deterministic builds: W accumulated in fixed point (entry i: int64 words i and i + lo_off)
 // back to FP64 in place, first nn entries
 hipError_t fixed_to_f64(long long nn, double* W, long long lo_off, hipStream_t st);

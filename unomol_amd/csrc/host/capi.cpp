@@ -335,6 +335,10 @@ int uno_ctx_get(uno_ctx* c, const char* key, double* v) {
     else if (k == "x_newton") *v = c->x_newton;
     else if (k == "x_newton_steps") *v = c->last_x_steps;
     else if (k == "deterministic") *v = c->deterministic;
+    else if (k == "ctx_ms_pairs") *v = c->ctx_ms[0];
+    else if (k == "ctx_ms_schwarz") *v = c->ctx_ms[1];
+    else if (k == "ctx_ms_runs") *v = c->ctx_ms[2];
+    else if (k == "ctx_ms_rest") *v = c->ctx_ms[3];
     else if (k == "last_quartets") *v = c->last_quartets;
     else if (k == "last_quartets32") *v = c->last_quartets32;
     else if (k == "pair_runs") *v = (double)c->cls.size();

@@ -6,6 +6,7 @@
 // contracts them with the density on the fly.
 #include <algorithm>
 #include <chrono>
+#include <cstdint>
 #include <cstring>
 #include <cmath>
 #include <numeric>
@@ -177,6 +178,9 @@ EriArgs base_args(uno_ctx* c) {
 }
 
 int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
+    using bclk = std::chrono::steady_clock;
+    auto since = [](bclk::time_point t) { return std::chrono::duration<double, std::milli>(bclk::now() - t).count(); };
+    const auto b0 = bclk::now();
     if (int rc = ensure_device()) return rc;
     std::string err;
     if (!make_basis(b, c->B, err)) {
@@ -203,7 +207,15 @@ int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
     c->pairs.clear();
     c->prims.clear();
     c->pair_of.assign((size_t)nsh * nsh, -1);
-    for (int a = 0; a < nsh; ++a)
+    // shell pairs (a, b <= a), built per first shell a in parallel into lists of their own,
+    // then laid out in the serial order (a, then b, ascending) at prefix-sum offsets
+    std::vector<std::vector<PairHdr>> hpa(nsh);
+    std::vector<std::vector<PrimPair>> ppa(nsh);
+    std::vector<std::vector<int>> bpa(nsh);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int a = 0; a < nsh; ++a) {
+        std::vector<PairHdr>& hv = hpa[a];
+        std::vector<PrimPair>& pv = ppa[a];
         for (int bsh = 0; bsh <= a; ++bsh) {
             const int s1 = (B.sh[a].l >= B.sh[bsh].l) ? a : bsh;
             const int s2 = (s1 == a) ? bsh : a;
@@ -220,7 +232,7 @@ int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
             }
             h.f1 = B.off[s1];
             h.f2 = B.off[s2];
-            h.prim0 = (int)c->prims.size();
+            h.prim0 = (int)pv.size();   // within this shell's list; made global below
             h.same = (a == bsh);
             h.sh1 = s1;
             h.sh2 = s2;
@@ -241,17 +253,43 @@ int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
                     for (int x = 0; x < 3; ++x) pp.P[x] = (al * A[x] + be * Bc[x]) * ip;
                     pp.K = S1.c[i] * S2.c[j] * ((h.same && i != j) ? 2.0 : 1.0) * s;
                     pp.s = s;
-                    c->prims.push_back(pp);
+                    pv.push_back(pp);
                 }
             }
-            h.nprim = (int)c->prims.size() - h.prim0;
+            h.nprim = (int)pv.size() - h.prim0;
             if (h.nprim == 0) continue;
-            c->pair_of[(size_t)a * nsh + bsh] = (int)c->pairs.size();
-            c->pairs.push_back(h);
+            hv.push_back(h);
+            bpa[a].push_back(bsh);
         }
+    }
+    {
+        std::vector<long long> hoff(nsh + 1, 0), poff(nsh + 1, 0);
+        for (int a = 0; a < nsh; ++a) {
+            hoff[a + 1] = hoff[a] + (long long)hpa[a].size();
+            poff[a + 1] = poff[a] + (long long)ppa[a].size();
+        }
+        if (poff[nsh] > (long long)INT32_MAX) {
+            set_error("too many primitive pairs for 32-bit offsets");
+            return UNO_EINVAL;
+        }
+        c->pairs.resize((size_t)hoff[nsh]);
+        c->prims.resize((size_t)poff[nsh]);
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int a = 0; a < nsh; ++a) {
+            for (size_t k = 0; k < hpa[a].size(); ++k) {
+                PairHdr h = hpa[a][k];
+                h.prim0 += (int)poff[a];
+                c->pairs[(size_t)hoff[a] + k] = h;
+                c->pair_of[(size_t)a * nsh + bpa[a][k]] = (int)(hoff[a] + (long long)k);
+            }
+            std::copy(ppa[a].begin(), ppa[a].end(), c->prims.begin() + poff[a]);
+        }
+    }
     const int np = (int)c->pairs.size();
     UNO_HIP(c->d_pairs.upload(c->pairs.data(), np, st));
     UNO_HIP(c->d_prims.upload(c->prims.data(), c->prims.size(), st));
+    c->ctx_ms[0] = since(b0);
+    const auto b1 = bclk::now();
     // Schwarz bounds Q_ab = sqrt(max |(ij|ij)|) from diagonal quartets, per pair class
     c->pairQ.assign(np, 0.0);
     DBuf<double> dq;
@@ -281,6 +319,8 @@ int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
     // Reorder the pair and primitive arrays into runs of (angular class, primitive-count
     // bucket), each sorted by Schwarz bound: a launch then walks contiguous memory, and the
     // lanes of a wave see alike primitive loops.
+    c->ctx_ms[1] = since(b1);
+    const auto b2 = bclk::now();
     // run key of every pair, taken before the arrays are permuted
     std::vector<int> runkey(np);
     for (int p = 0; p < np; ++p) runkey[p] = pcls[p] * kNumBuckets + prim_bucket(c->pairs[p].nprim);
@@ -290,17 +330,34 @@ int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
         if (runkey[x] != runkey[y]) return runkey[x] < runkey[y];
         return c->pairQ[x] > c->pairQ[y];
     });
-    std::vector<PairHdr> pairs2;
-    std::vector<PrimPair> prims2;
-    std::vector<double> q2;
+    // new primitive offsets and pair-major Coulomb offsets (each pair's function block), in
+    // run order
+    static const int kPairComp[kNumPairClasses] = {1, 3, 9, 6, 18, 36};
+    std::vector<int> nprim0(np), njo(np);
+    {
+        long long tp = 0, jo = 0;
+        for (int k = 0; k < np; ++k) {
+            nprim0[k] = (int)tp;
+            tp += c->pairs[order[k]].nprim;
+            njo[k] = (int)jo;
+            jo += kPairComp[runkey[order[k]] / kNumBuckets];
+        }
+        c->njp = jo;
+        UNO_HIP(c->JP.alloc((size_t)std::max(jo, 1LL)));
+    }
+    std::vector<PairHdr> pairs2(np);
+    std::vector<PrimPair> prims2(c->prims.size());
+    std::vector<double> q2(np);
     std::vector<int> newidx(np);
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < np; ++k) {
         PairHdr h = c->pairs[order[k]];
         const int p0 = h.prim0;
-        h.prim0 = (int)prims2.size();
-        for (int q = 0; q < h.nprim; ++q) prims2.push_back(c->prims[p0 + q]);
-        pairs2.push_back(h);
-        q2.push_back(c->pairQ[order[k]]);
+        h.prim0 = nprim0[k];
+        h.jo = njo[k];
+        std::copy(c->prims.begin() + p0, c->prims.begin() + p0 + h.nprim, prims2.begin() + h.prim0);
+        pairs2[k] = h;
+        q2[k] = c->pairQ[order[k]];
         newidx[order[k]] = k;
     }
     for (int& p : c->pair_of)
@@ -308,19 +365,22 @@ int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
     c->pairs.swap(pairs2);
     c->prims.swap(prims2);
     c->pairQ.swap(q2);
-    // pair-major Coulomb accumulator: each pair's function block, in pair order
+    // the device copies: the same permutation applied to the arrays uploaded above (the
+    // three index arrays go up instead of the records themselves)
     {
-        static const int kPairComp[kNumPairClasses] = {1, 3, 9, 6, 18, 36};
-        long long jo = 0;
-        for (int k = 0; k < np; ++k) {
-            c->pairs[k].jo = (int)jo;
-            jo += kPairComp[runkey[order[k]] / kNumBuckets];
-        }
-        c->njp = jo;
-        UNO_HIP(c->JP.alloc((size_t)std::max(jo, 1LL)));
+        DBuf<int> dorder, dp0, djo;
+        DBuf<PairHdr> np_d;
+        DBuf<PrimPair> nq_d;
+        UNO_HIP(dorder.upload(order.data(), np, st));
+        UNO_HIP(dp0.upload(nprim0.data(), np, st));
+        UNO_HIP(djo.upload(njo.data(), np, st));
+        UNO_HIP(np_d.alloc(np));
+        UNO_HIP(nq_d.alloc(c->prims.size()));
+        UNO_HIP(permute_pairs(np, dorder.p, dp0.p, djo.p, c->d_pairs.p, c->d_prims.p, np_d.p, nq_d.p, st));
+        UNO_HIP(hipStreamSynchronize(st));
+        c->d_pairs = std::move(np_d);
+        c->d_prims = std::move(nq_d);
     }
-    UNO_HIP(c->d_pairs.upload(c->pairs.data(), np, st));
-    UNO_HIP(c->d_prims.upload(c->prims.data(), c->prims.size(), st));
     UNO_HIP(c->d_pairQ.upload(c->pairQ.data(), np, st));
     c->cls.clear();
     for (int k = 0; k < np;) {
@@ -344,6 +404,8 @@ int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
                 set_error("internal error: pair run is not sorted by Schwarz bound");
                 return UNO_EINVAL;
             }
+    c->ctx_ms[2] = since(b2);
+    const auto b3 = bclk::now();
     // columns of the row-merged launches' LDS rows: per ket run, the functions its pairs
     // touch (whole shells), in function order
     {
@@ -425,6 +487,7 @@ int ctx_build(uno_ctx* c, const uno_basis* b, int start_shell) {
     UNO_HIP(c->scal.alloc(16));
     UNO_HIP(c->red.alloc(kRedBlocks));
     UNO_HIP(hipStreamSynchronize(st));
+    c->ctx_ms[3] = since(b3);
     return UNO_OK;
 }
 

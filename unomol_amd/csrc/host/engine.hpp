@@ -157,6 +157,9 @@ struct uno_ctx {
     // host milliseconds of the last build: density bound read-back, quartet spaces and
     // index arrays, staging upload, launch enqueue (the kernels run asynchronously)
     double last_ms[4] = {0.0, 0.0, 0.0, 0.0};
+    // host milliseconds of the context build: pair records, Schwarz bounds (device),
+    // sorted runs and their upload, the rest (columns, one-electron data, buffers)
+    double ctx_ms[4] = {0.0, 0.0, 0.0, 0.0};
     // profiling pass: per quartet class (bra X, ket Y) kernel milliseconds and the
     // primitive / shell quartets the kernels evaluated
     int profile = 0;
