@@ -393,10 +393,12 @@ def main():
 
     ctx_s = [0.0]
 
-    def one_scf(opts, profile=False):
+    def one_scf(opts, profile=False, fp32_threshold=None):
         t0 = time.perf_counter()
         t = unomol_amd.TwoElectronInts(basis)
         ctx_s[0] = time.perf_counter() - t0
+        if fp32_threshold is not None:
+            t.set("fp32_threshold", fp32_threshold)
         if opts.get("guess") == 2:
             unomol_amd.set_guess_model(t, model)
         if profile:
@@ -440,6 +442,12 @@ def main():
             ablation[name] = {"scf_s": round(dta, 4), "iterations": hfa.result.iterations,
                               "energy_diff_vs_fp64": ea - e64}
             log("ablation %s: E=%.10f in %.2f s, %d iterations" % (name, ea, dta, hfa.result.iterations))
+        # the timed path with no FP32 quartets (fp32_threshold 0): what the FP32 split costs
+        # or saves with these kernels; the configured workload keeps its 1e-6 threshold
+        _, hfa, ea, dta = one_scf(fast_opts, fp32_threshold=0.0)
+        ablation["mixed+learned, no FP32 quartets"] = {"scf_s": round(dta, 4), "iterations": hfa.result.iterations,
+                                                       "energy_diff_vs_fp64": ea - e64}
+        log("ablation no FP32 quartets: E=%.10f in %.2f s, %d iterations" % (ea, dta, hfa.result.iterations))
     # the 16-water cluster: plain FP64 GPU against the reference program's measured run
     log("config b: 16-water 6-31G*, three FP64 iterations")
     b_s, b_e = run_config_b(unomol_amd)
