@@ -22,8 +22,10 @@ Prints ONE compact JSON line (stdout) with the metric, the errors of the timed p
 against the plain FP64 run (energy, orbital energies, density), an ablation (FP64 +
 learned guess, mixed + core-H guess), a roofline object for the dominant ERI kernel of
 the timed steps (the class/precision kernel with the largest summed time; per-launch
-time from HIP events recorded on the engine's stream around each launch inside the timed
-region; algorithmic FLOPs = unomol_amd.flopmodel's per-primitive-quartet and
+time from HIP events recorded around each launch, on the stream it runs on, inside the
+timed region -- a build's launches run on three streams, so these spans include time
+shared with other kernels; `one_stream` repeats the step outside the timed region with
+every launch on one stream for the kernel's isolated duration; algorithmic FLOPs = unomol_amd.flopmodel's per-primitive-quartet and
 per-shell-quartet operation counts of the kernel's own arithmetic times the quartets the
 kernel evaluated, counted by the kernel; HBM traffic per launch from the committed
 FETCH_SIZE/WRITE_SIZE passes when they cover the same kernel), the 16-water cluster's
@@ -155,7 +157,7 @@ def run_config_b(unomol_amd):
 
 
 def read_profile(N, t):
-    """Per-class kernel time (HIP events on the engine's stream), launches and evaluated
+    """Per-class kernel time (HIP events on the stream each launch runs on), launches and evaluated
     primitive / shell quartets accumulated since profiling was switched on."""
     ms, launches, prim, shell = (np.zeros(PROF_SLOTS) for _ in range(4))
     N.check(N.lib().uno_profile(t.ctx, -1, N.ptr(ms), N.ptr(launches), N.ptr(prim), N.ptr(shell)))
@@ -393,12 +395,14 @@ def main():
 
     ctx_s = [0.0]
 
-    def one_scf(opts, profile=False, fp32_threshold=None):
+    def one_scf(opts, profile=False, fp32_threshold=None, launch_streams=None):
         t0 = time.perf_counter()
         t = unomol_amd.TwoElectronInts(basis)
         ctx_s[0] = time.perf_counter() - t0
         if fp32_threshold is not None:
             t.set("fp32_threshold", fp32_threshold)
+        if launch_streams is not None:
+            t.set("launch_streams", launch_streams)
         if opts.get("guess") == 2:
             unomol_amd.set_guess_model(t, model)
         if profile:
@@ -430,6 +434,7 @@ def main():
     # the 16-water run, all before the timed steps so that the timed steps' launches are the
     # last of their kernels in the process
     ablation = {}
+    det = None
     if not args.fp64_only:
         t64, hf64, e64, dt64 = one_scf(fp64_opts)
         ctx64_s = ctx_s[0]
@@ -448,6 +453,27 @@ def main():
         ablation["mixed+learned, no FP32 quartets"] = {"scf_s": round(dta, 4), "iterations": hfa.result.iterations,
                                                        "energy_diff_vs_fp64": ea - e64}
         log("ablation no FP32 quartets: E=%.10f in %.2f s, %d iterations" % (ea, dta, hfa.result.iterations))
+        # deterministic Fock builds (fixed-point J/K accumulation, wave tasks only) beside the
+        # default FP64 build on the FP64 run's converged density: wall time per build (packed P
+        # up, packed G down included), bitwise repeat, largest difference from the default
+        def timed_build(ctx):
+            G = np.zeros_like(P64)
+            t0 = time.perf_counter()
+            ctx.formGmatrix(P64, G)
+            return G, 1e3 * (time.perf_counter() - t0)
+
+        timed_build(t64)
+        Gdef, ms_def = timed_build(t64)
+        t64.set("deterministic", 1)
+        timed_build(t64)
+        Gdet, ms_det = timed_build(t64)
+        Gdet2, _ = timed_build(t64)
+        t64.set("deterministic", 0)
+        det = {"build_ms_default": round(ms_def, 2), "build_ms_deterministic": round(ms_det, 2),
+               "repeat_bitwise": bool(np.array_equal(Gdet, Gdet2)),
+               "max_abs_diff_vs_default": float(np.abs(Gdet - Gdef).max())}
+        log("deterministic build: %.1f ms (default %.1f ms), repeat bitwise %s, max |dG| %.1e" %
+            (ms_det, ms_def, det["repeat_bitwise"], det["max_abs_diff_vs_default"]))
     # the 16-water cluster: plain FP64 GPU against the reference program's measured run
     log("config b: 16-water 6-31G*, three FP64 iterations")
     b_s, b_e = run_config_b(unomol_amd)
@@ -461,6 +487,11 @@ def main():
                 cfgb["speedup_vs_reference_" + k[:-2]] = round(refb[k] / b_s, 1)
     log("config b: %.3f s on the GPU%s" % (b_s, "" if not refb else "; reference serial %.1f s" %
                                                                        refb.get("serial_s", float("nan"))))
+    iso_classes, iso_step_s = None, None
+    if not args.no_profile and not args.fp64_only:
+        ti, _, _, iso_step_s = one_scf(step_opts, profile=True, launch_streams=1)
+        iso_classes = class_table(*read_profile(N, ti))
+        del ti
     for i in range(args.warmup):
         _, _, e, dt = one_scf(step_opts)
         log("warmup %d: E=%.10f in %.2f s" % (i, e, dt))
@@ -491,6 +522,28 @@ def main():
         # the per-class table is long: stderr only, one JSON line
         print("[bench] eri_classes " + json.dumps(classes), file=sys.stderr, flush=True)
 
+    # The timed steps spread each build's ERI launches over three streams, so a launch's
+    # events span the time it shared the GPU with the other streams' kernels.  Beside the
+    # timed figure: the same step with every launch on one stream (run before the timed
+    # region, so the timed launches stay the last of their kernels), the dominant kernel's
+    # duration and rate in isolation.
+    if roofline is not None and iso_classes is not None:
+        same = [c for c in iso_classes if kernel_name(c) == roofline["kernel"]]
+        if same:
+            ms1 = sum(c["ms"] for c in same)
+            n1 = sum(c["launches"] for c in same)
+            fl1 = sum(c["gflop"] for c in same) * 1e9
+            peak = roofline["peak"]
+            tot_fl = sum(c["gflop"] for c in iso_classes) * 1e9
+            tot_ms = sum(c["ms"] for c in iso_classes)
+            roofline["streams"] = 3
+            roofline["one_stream"] = {
+                "step_s": round(iso_step_s, 4), "launches": int(n1), "avg_launch_ms": round(ms1 / max(n1, 1), 4),
+                "achieved": round(fl1 / (ms1 * 1e-3) / 1e12, 4), "frac": round(fl1 / (ms1 * 1e-3) / 1e12 / peak, 5),
+                "all_eri_kernels": {"ms": round(tot_ms, 3), "tflops": round(tot_fl / (tot_ms * 1e-3) / 1e12, 4)}}
+            log("one stream: step %.3f s, dominant kernel %.4f ms per launch, %.3f TFLOP/s" %
+                (iso_step_s, roofline["one_stream"]["avg_launch_ms"], roofline["one_stream"]["achieved"]))
+
     if args.fp64_only:
         t64, hf64, e64, dt64 = t, hf, energy, dt
         ctx64_s = step_ctx_s
@@ -505,27 +558,6 @@ def main():
     log("errors vs FP64: dE %.2e, max |d eps| %.2e (occupied %.2e), max |dP| %.2e" %
         (err["energy"], err["orbital_energy_max_abs"], err["occupied_orbital_energy_max_abs"],
          err["density_max_abs"]))
-    # deterministic Fock builds (fixed-point J/K accumulation, wave tasks only) beside the
-    # default FP64 build on the FP64 run's converged density: wall time per build (packed P
-    # up, packed G down included), bitwise repeat, largest difference from the default
-    def timed_build(ctx):
-        G = np.zeros_like(P64)
-        t0 = time.perf_counter()
-        ctx.formGmatrix(P64, G)
-        return G, 1e3 * (time.perf_counter() - t0)
-
-    timed_build(t64)
-    Gdef, ms_def = timed_build(t64)
-    t64.set("deterministic", 1)
-    timed_build(t64)
-    Gdet, ms_det = timed_build(t64)
-    Gdet2, _ = timed_build(t64)
-    t64.set("deterministic", 0)
-    det = {"build_ms_default": round(ms_def, 2), "build_ms_deterministic": round(ms_det, 2),
-           "repeat_bitwise": bool(np.array_equal(Gdet, Gdet2)),
-           "max_abs_diff_vs_default": float(np.abs(Gdet - Gdef).max())}
-    log("deterministic build: %.1f ms (default %.1f ms), repeat bitwise %s, max |dG| %.1e" %
-        (ms_det, ms_def, det["repeat_bitwise"], det["max_abs_diff_vs_default"]))
     # streaming copy of 1 GiB: the HBM reference point for the traffic counters
     cms = C.c_double()
     N.check(N.lib().uno_copy_probe(C.c_ulonglong(1 << 30), 1, C.byref(cms)))

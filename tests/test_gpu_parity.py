@@ -570,6 +570,49 @@ def test_scf_deterministic_bitwise():
     assert abs(e0 - e_def) < 1e-10
 
 
+@pytest.mark.parametrize("uhf", [False, True])
+def test_fock_launch_streams_same_matrix(uhf):
+    """The ERI launches of a build spread over three streams (the default) against one
+    stream (16-water 6-31G*, FP64 and mixed builds): the same G to FP64 rounding of the
+    accumulation order, and bitwise the same in deterministic mode, whose fixed-point sums
+    are converted only after every stream's launches have finished."""
+    say("fock launch streams 3 vs 1, uhf=%s" % uhf)
+    path = os.path.join(R.GOLDEN, "decks", "B_w16_631gs.dat")
+    n = read_deck(path)["norb"]
+    rng = np.random.default_rng(29)
+    A = rng.standard_normal((n, n)) * 0.02
+    Bm = rng.standard_normal((n, n)) * 0.02
+    PA, PB = _pack(A + A.T + np.eye(n)), _pack(Bm + Bm.T + np.eye(n))
+    b, t = _tints(path)
+
+    def build():
+        if uhf:
+            GA, GB = np.zeros_like(PA), np.zeros_like(PB)
+            t.formGmatrix(PA, PB, GA, GB)
+            return np.concatenate([GA, GB])
+        G = np.zeros_like(PA)
+        t.formGmatrix(PA, G)
+        return G
+
+    assert t.get("launch_streams") == 3
+    for prec in (0, 1):
+        t.set("precision", prec)
+        for det in (0, 1):
+            t.set("deterministic", det)
+            t.set("launch_streams", 1)
+            G1 = build()
+            t.set("launch_streams", 3)
+            G3 = build()
+            scale = max(np.abs(G1).max(), 1.0)
+            say("precision %d deterministic %d: max |dG| %.2e" % (prec, det, np.abs(G3 - G1).max()))
+            if det:
+                assert np.array_equal(G3, G1)
+            else:
+                assert np.abs(G3 - G1).max() < 1e-12 * scale
+    t.set("deterministic", 0)
+    t.set("precision", 0)
+
+
 def test_autotune_keeps_fock_matrix():
     """The autotuner times the row-merge configurations of every quartet space and keeps the
     fastest: the Fock matrix built afterwards (RHF and UHF, FP64 and mixed) equals the

@@ -311,6 +311,7 @@ int uno_ctx_set(uno_ctx* c, const char* key, double v) {
     else if (k == "purify") c->purify = (int)v;
     else if (k == "x_newton") c->x_newton = (int)v;
     else if (k == "deterministic") c->deterministic = v != 0.0;
+    else if (k == "launch_streams") c->launch_streams = std::max(1, std::min(3, (int)v));
     else { set_error("unknown option " + k); return UNO_EINVAL; }
     return UNO_OK;
 }
@@ -335,6 +336,7 @@ int uno_ctx_get(uno_ctx* c, const char* key, double* v) {
     else if (k == "x_newton") *v = c->x_newton;
     else if (k == "x_newton_steps") *v = c->last_x_steps;
     else if (k == "deterministic") *v = c->deterministic;
+    else if (k == "launch_streams") *v = c->launch_streams;
     else if (k == "ctx_ms_pairs") *v = c->ctx_ms[0];
     else if (k == "ctx_ms_schwarz") *v = c->ctx_ms[1];
     else if (k == "ctx_ms_runs") *v = c->ctx_ms[2];

@@ -650,6 +650,20 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
 
     c->last_ms[2] = ms_since(h2);
     const auto h3 = hclk::now();
+    // worker streams (not while tuning: the tuner times each launch on the context's
+    // stream); they start after everything enqueued so far on the context's stream
+    const int nls = c->tuning ? 1 : std::max(1, std::min(3, c->launch_streams));
+    for (int w = 0; w + 1 < nls; ++w)
+        if (!c->wst[w]) {
+            UNO_HIP(hipStreamCreate(&c->wst[w]));
+            UNO_HIP(hipEventCreateWithFlags(&c->wjoin[w], hipEventDisableTiming));
+        }
+    if (nls > 1) {
+        if (!c->wfork) UNO_HIP(hipEventCreateWithFlags(&c->wfork, hipEventDisableTiming));
+        UNO_HIP(hipEventRecord(c->wfork, st));
+        for (int w = 0; w + 1 < nls; ++w) UNO_HIP(hipStreamWaitEvent(c->wst[w], c->wfork, 0));
+    }
+    int nlaunch = 0;
     // Pass 2: the launches
     for (const SpaceRec& r : recs) {
         QuartetClass& q = *r.q;
@@ -724,6 +738,8 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
         auto launch = [&](int kind) -> int {
             const int slot = kind * 2 * kNumPairClasses * kNumPairClasses + ci;
             const bool prof = c->profile && !c->tuning && c->d_counter.p;
+            const int li = nlaunch++ % nls;
+            const hipStream_t ls = li == 0 ? st : c->wst[li - 1];
             hipEvent_t e1 = nullptr;
             if (prof) {
                 a.counter = c->d_counter.p + 2 * kCounterSlots * slot;
@@ -736,13 +752,13 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
                 if (c->prof_evslot.size() < k + 1) c->prof_evslot.resize(k + 1);
                 c->prof_evslot[k] = slot;
                 e1 = c->prof_ev[2 * k + 1];
-                UNO_HIP(hipEventRecord(c->prof_ev[2 * k], st));
+                UNO_HIP(hipEventRecord(c->prof_ev[2 * k], ls));
             } else {
                 a.counter = nullptr;
             }
-            if (kind) UNO_HIP(eri_launch_rows(mode, LX.lcls, LY.lcls, a, kind, st, prec == 1));
-            else UNO_HIP(eri_launch(mode, LX.lcls, LY.lcls, a, st, prec == 1));
-            if (prof) UNO_HIP(hipEventRecord(e1, st));
+            if (kind) UNO_HIP(eri_launch_rows(mode, LX.lcls, LY.lcls, a, kind, ls, prec == 1));
+            else UNO_HIP(eri_launch(mode, LX.lcls, LY.lcls, a, ls, prec == 1));
+            if (prof) UNO_HIP(hipEventRecord(e1, ls));
             return UNO_OK;
         };
         // the space's launches: row-merged bras with LDS variant v, then the wave tasks
@@ -799,6 +815,11 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
             if (int rc = run(r.variant)) return rc;
         }
         (prec ? total32 : total) += (double)sp.nq;
+    }
+    // the context's stream continues once every worker's launches are done
+    for (int w = 0; w + 1 < nls; ++w) {
+        UNO_HIP(hipEventRecord(c->wjoin[w], c->wst[w]));
+        UNO_HIP(hipStreamWaitEvent(st, c->wjoin[w], 0));
     }
     c->last_ms[3] = ms_since(h3);
     if (det) {

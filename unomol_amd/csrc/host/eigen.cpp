@@ -166,5 +166,13 @@ uno_ctx::~uno_ctx() {
     if (stage_ev) (void)hipEventDestroy(stage_ev);
     if (st) (void)hipStreamSynchronize(st);
     for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
+    for (int w = 0; w < 2; ++w) {
+        if (wst[w]) {
+            (void)hipStreamSynchronize(wst[w]);
+            (void)hipStreamDestroy(wst[w]);
+        }
+        if (wjoin[w]) (void)hipEventDestroy(wjoin[w]);
+    }
+    if (wfork) (void)hipEventDestroy(wfork);
     if (st) (void)hipStreamDestroy(st);
 }

@@ -175,6 +175,13 @@ struct uno_ctx {
     // any launch order) and run every bra as wave tasks (the row kernels merge with FP64
     // atomics); the SCF's dense products run without split K
     int deterministic = 0;
+    // the ERI launches of a Fock build go round-robin over the context's stream and
+    // launch_streams - 1 worker streams (default priority), joined back before the build's
+    // last steps: the many small launches (FP32 tails, small quartet spaces) then run side
+    // by side instead of each leaving most of the GPU idle
+    int launch_streams = 3;
+    hipStream_t wst[2] = {nullptr, nullptr};
+    hipEvent_t wfork = nullptr, wjoin[2] = {nullptr, nullptr};
     int row_dens = -1;       // exchange density rows in LDS: -1 when they fit, 0 never, 1 forced where they fit
     // autotuned row-merge configuration per quartet space (run-pair index, precision, UHF,
     // mixed build): LDS variant (0 = wave tasks only) and the ket count from which a bra is
