@@ -73,9 +73,9 @@ int uno_ctx_destroy(uno_ctx* ctx);
  * in 128-bit fixed point with integer atomics, every bra as wave tasks, and uno_scf_run's
  * dense products run without split K, so the same input gives bitwise the same G, energy,
  * orbital energies and density in every run; default 0, FP64 atomics),
- * "launch_streams" (1-3: the streams a Fock build's ERI launches are spread over, the
- * context's own and up to two workers at the default priority, joined before the build
- * returns; default 3),
+ * "launch_streams" (1-4: the streams a Fock build's ERI launches are spread over, the
+ * context's own and up to three workers at the default priority, joined before the build
+ * returns; default 3, the fastest of 1-4 at 64 waters),
  * "fp32_threshold" (quartets whose bound is below it
  * run in FP32 when "precision" is 1; default 1e-6), "precision" (0 = FP64, 1 = mixed, for the
  * builds of uno_fock_rhf/uno_fock_uhf; uno_scf_run takes its own option),

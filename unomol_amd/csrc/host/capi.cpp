@@ -311,7 +311,7 @@ int uno_ctx_set(uno_ctx* c, const char* key, double v) {
     else if (k == "purify") c->purify = (int)v;
     else if (k == "x_newton") c->x_newton = (int)v;
     else if (k == "deterministic") c->deterministic = v != 0.0;
-    else if (k == "launch_streams") c->launch_streams = std::max(1, std::min(3, (int)v));
+    else if (k == "launch_streams") c->launch_streams = std::max(1, std::min(kMaxLaunchStreams, (int)v));
     else { set_error("unknown option " + k); return UNO_EINVAL; }
     return UNO_OK;
 }

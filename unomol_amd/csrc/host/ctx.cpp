@@ -652,7 +652,7 @@ int fock_build(uno_ctx* c, bool uhf, double dmax_hint, bool mixed) {
     const auto h3 = hclk::now();
     // worker streams (not while tuning: the tuner times each launch on the context's
     // stream); they start after everything enqueued so far on the context's stream
-    const int nls = c->tuning ? 1 : std::max(1, std::min(3, c->launch_streams));
+    const int nls = c->tuning ? 1 : std::max(1, std::min(kMaxLaunchStreams, c->launch_streams));
     for (int w = 0; w + 1 < nls; ++w)
         if (!c->wst[w]) {
             UNO_HIP(hipStreamCreate(&c->wst[w]));

@@ -166,7 +166,7 @@ uno_ctx::~uno_ctx() {
     if (stage_ev) (void)hipEventDestroy(stage_ev);
     if (st) (void)hipStreamSynchronize(st);
     for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
-    for (int w = 0; w < 2; ++w) {
+    for (int w = 0; w + 1 < uno::kMaxLaunchStreams; ++w) {
         if (wst[w]) {
             (void)hipStreamSynchronize(wst[w]);
             (void)hipStreamDestroy(wst[w]);

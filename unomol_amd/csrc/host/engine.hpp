@@ -104,6 +104,9 @@ constexpr int kProfKinds = 6;
 constexpr int kProfSlots = kProfKinds * 2 * kNumPairClasses * kNumPairClasses;
 
 constexpr int kNumBuckets = 6;
+// streams a Fock build's ERI launches may be spread over (HIP's default is four hardware
+// queues per process)
+constexpr int kMaxLaunchStreams = 4;
 inline int prim_bucket(int nprim) {
     return nprim <= 1 ? 0 : nprim <= 3 ? 1 : nprim <= 6 ? 2 : nprim <= 9 ? 3 : nprim <= 18 ? 4 : 5;
 }
@@ -180,8 +183,8 @@ struct uno_ctx {
     // last steps: the many small launches (FP32 tails, small quartet spaces) then run side
     // by side instead of each leaving most of the GPU idle
     int launch_streams = 3;
-    hipStream_t wst[2] = {nullptr, nullptr};
-    hipEvent_t wfork = nullptr, wjoin[2] = {nullptr, nullptr};
+    hipStream_t wst[uno::kMaxLaunchStreams - 1] = {};
+    hipEvent_t wfork = nullptr, wjoin[uno::kMaxLaunchStreams - 1] = {};
     int row_dens = -1;       // exchange density rows in LDS: -1 when they fit, 0 never, 1 forced where they fit
     // autotuned row-merge configuration per quartet space (run-pair index, precision, UHF,
     // mixed build): LDS variant (0 = wave tasks only) and the ket count from which a bra is
