@@ -323,8 +323,11 @@ int uno_batch_destroy(uno_batch* batch);
 int uno_batch_nmol(const uno_batch* batch);
 /* Options of the batched Fock build: "graph" (1: capture and replay, 0: plain launches),
  * "queue" (0: one block per task; 1: the one-quartet-per-wave classes take their quartet
- * blocks from a device work queue; 2: the lane classes' 64-ket tasks as well).  uno_batch_get also reads the
- * last run's "space_builds", "graph_builds", "graph_replays" and "launches_per_build". */
+ * blocks from a device work queue; 2: the lane classes' 64-ket tasks as well),
+ * "launch_streams" (1-4: streams the list launches are spread over, forked and joined
+ * inside the build, parallel branches of the captured graph; default 3).  uno_batch_get
+ * also reads the last run's "space_builds", "graph_builds", "graph_replays" and
+ * "launches_per_build". */
 int uno_batch_set(uno_batch* batch, const char* key, double value);
 int uno_batch_get(const uno_batch* batch, const char* key, double* value);
 /* The final density of molecule m after uno_batch_run, dense row-major N x N: the RHF half

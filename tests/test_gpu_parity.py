@@ -945,17 +945,20 @@ def test_batched_graph_and_queue_match_plain_launches(tmp_path):
     launches, against plain launches of the same static quartet lists (no graph, no queue):
     the same energies, iteration counts (to one) and densities on every molecule; the graph replays
     every build of a run and the second run of the same batch (a fresh capture) agrees with
-    the first."""
+    the first.  The group launches run on three streams (the default; a captured graph then
+    has parallel branches) and, in the last two runs, on one."""
     say("batched graph/queue vs plain: 16 mixed molecules")
     paths = _batch_decks(tmp_path, 16, 11)
     runs = []
-    for graph, queue in ((1, 1), (1, 1), (0, 0), (0, 2)):
+    for graph, queue, streams in ((1, 1, 3), (1, 1, 3), (0, 0, 3), (0, 2, 3), (1, 1, 1), (0, 0, 1)):
         batch = unomol_amd.Batch(paths) if not runs or graph == 0 and queue == 0 else batch
         batch.set("graph", graph)
         batch.set("queue", queue)
+        batch.set("launch_streams", streams)
+        assert batch.get("launch_streams") == streams
         e, res = batch.run(maxits=100, eps=1e-10, cflag=2, accel=2)
-        say("  graph %d queue %d: builds %d replays %d spaces %d launches/build %d" %
-            (graph, queue, batch.get("graph_builds"), batch.get("graph_replays"), batch.get("space_builds"),
+        say("  graph %d queue %d streams %d: builds %d replays %d spaces %d launches/build %d" %
+            (graph, queue, streams, batch.get("graph_builds"), batch.get("graph_replays"), batch.get("space_builds"),
              batch.get("launches_per_build")))
         assert all(r.converged for r in res)
         if graph:

@@ -102,6 +102,12 @@ struct uno_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int use_graph = 1, use_queue = 1;
+    // the groups' launches go round-robin over st and launch_streams - 1 worker streams
+    // (forked from st after the fills and joined back; under capture the graph then holds
+    // parallel branches)
+    int launch_streams = 3;
+    hipStream_t wst[kMaxLaunchStreams - 1] = {};
+    hipEvent_t wfork = nullptr, wjoin[kMaxLaunchStreams - 1] = {};
     int graph_builds = 0, graph_replays = 0, space_builds = 0;
     void drop_graph() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -113,6 +119,12 @@ struct uno_batch {
         drop_graph();
         for (BatchMol& m : mol)
             if (m.ctx) uno_ctx_destroy(m.ctx);
+        if (st) (void)hipStreamSynchronize(st);
+        for (int w = 0; w + 1 < kMaxLaunchStreams; ++w) {
+            if (wst[w]) (void)hipStreamDestroy(wst[w]);
+            if (wjoin[w]) (void)hipEventDestroy(wjoin[w]);
+        }
+        if (wfork) (void)hipEventDestroy(wfork);
         if (st) (void)hipStreamDestroy(st);
     }
 };
@@ -286,13 +298,32 @@ static int build_static(uno_batch* b, const std::vector<double>& dmax) {
 
 // the launches of one build: accumulators and queue counters cleared by fill kernels (no
 // memset nodes when captured), then one launch per static space
+// the worker streams and events, made before any capture
+static int ensure_workers(uno_batch* b) {
+    const int nls = std::max(1, std::min(kMaxLaunchStreams, b->launch_streams));
+    for (int w = 0; w + 1 < nls; ++w)
+        if (!b->wst[w]) {
+            UNO_HIP(hipStreamCreate(&b->wst[w]));
+            UNO_HIP(hipEventCreateWithFlags(&b->wjoin[w], hipEventDisableTiming));
+        }
+    if (nls > 1 && !b->wfork) UNO_HIP(hipEventCreateWithFlags(&b->wfork, hipEventDisableTiming));
+    return UNO_OK;
+}
+
 static int enqueue_fock(uno_batch* b) {
     hipStream_t st = b->st;
     UNO_HIP(k_fill(b->store, 0.0, b->W0.p, st));
     UNO_HIP(k_fill(b->store, 0.0, b->W1.p, st));
     if (b->use_queue && !b->groups.empty())
         UNO_HIP(k_fill((long)b->groups.size(), 0.0, reinterpret_cast<double*>(b->queues.p), st));
+    const int nls = std::max(1, std::min(kMaxLaunchStreams, b->launch_streams));
+    if (nls > 1) {
+        UNO_HIP(hipEventRecord(b->wfork, st));
+        for (int w = 0; w + 1 < nls; ++w) UNO_HIP(hipStreamWaitEvent(b->wst[w], b->wfork, 0));
+    }
     for (size_t k = 0; k < b->groups.size(); ++k) {
+        const int li = (int)(k % (size_t)nls);
+        const hipStream_t ls = li == 0 ? st : b->wst[li - 1];
         const uno_batch::Group& g = b->groups[k];
         EriArgs a{};
         a.pairs = b->pairs.p;
@@ -326,7 +357,11 @@ static int enqueue_fock(uno_batch* b) {
         const bool q = b->use_queue >= 2 || (b->use_queue == 1 && !g.tasks);
         a.queue = q ? b->queues.p + k : nullptr;
         a.mol_active = b->active_d.p;
-        UNO_HIP(eri_launch(g.uhf ? kDigestUHF : kDigestRHF, g.cx, g.cy, a, st));
+        UNO_HIP(eri_launch(g.uhf ? kDigestUHF : kDigestRHF, g.cx, g.cy, a, ls));
+    }
+    for (int w = 0; w + 1 < nls; ++w) {
+        UNO_HIP(hipEventRecord(b->wjoin[w], b->wst[w]));
+        UNO_HIP(hipStreamWaitEvent(st, b->wjoin[w], 0));
     }
     return UNO_OK;
 }
@@ -348,6 +383,7 @@ static int batch_fock(uno_batch* b, const std::vector<double>& dmax, double& qua
         if (act[m]) quartets += b->mol_quartets[m];   // before the kernels' density screen
     }
     UNO_HIP(hipMemcpyAsync(b->active_d.p, act.data(), nm * sizeof(int), hipMemcpyHostToDevice, st));
+    if (int rc = ensure_workers(b)) return rc;
     if (b->use_graph) {
         if (!b->gexec) {
             UNO_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -474,6 +510,10 @@ int uno_batch_set(uno_batch* b, const char* key, double v) {
     const std::string k(key);
     if (k == "graph") b->use_graph = (int)v;
     else if (k == "queue") b->use_queue = (int)v;
+    else if (k == "launch_streams") {
+        b->launch_streams = std::max(1, std::min(kMaxLaunchStreams, (int)v));
+        b->drop_graph();   // a captured build keeps the stream layout it was captured with
+    }
     else { set_error("unknown batch option " + k); return UNO_EINVAL; }
     b->drop_graph();
     return UNO_OK;
@@ -484,6 +524,7 @@ int uno_batch_get(const uno_batch* b, const char* key, double* v) {
     const std::string k(key);
     if (k == "graph") *v = b->use_graph;
     else if (k == "queue") *v = b->use_queue;
+    else if (k == "launch_streams") *v = b->launch_streams;
     else if (k == "space_builds") *v = b->space_builds;
     else if (k == "graph_builds") *v = b->graph_builds;
     else if (k == "graph_replays") *v = b->graph_replays;
